@@ -1,0 +1,277 @@
+/*
+ * spicier_hip.h — C ABI of the MI355X batched-sweep engine (libspicier_hip.so).
+ *
+ * The engine replaces the GPU backend behind spicier-batched-sweep's
+ * `BatchedLuSolver` trait and adds the nonlinear DC-operating-point sweep that
+ * the trait's callers assemble on the CPU today.  Every entry point takes plain
+ * pointers and sizes; no torch or HIP types cross this boundary.
+ *
+ * Conventions shared by every entry point:
+ *   - return value: spx_status (0 = success); a message is kept per thread and
+ *     read with spx_last_error().
+ *   - matrices are column-major per system and batch-major across systems,
+ *     exactly as `BatchedLuSolver::solve_batch` receives them
+ *     (crates/spicier-batched-sweep/src/solver.rs:108-125).
+ *   - "host" entry points take host pointers, copy in, run on the handle's
+ *     stream and copy out before returning; "_device" entry points take device
+ *     pointers (memory on the handle's device) and only enqueue work on the
+ *     handle's stream unless stated otherwise.
+ *   - calls on one handle are serialized by an internal mutex; a handle may be
+ *     shared between threads (the trait is `Send + Sync`).
+ */
+#ifndef SPICIER_HIP_H
+#define SPICIER_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Status codes: one per `BatchedSweepError` variant
+ * (crates/spicier-batched-sweep/src/error.rs:7-46). */
+typedef enum spx_status {
+    SPX_OK = 0,
+    SPX_ERR_INVALID_DIMENSION = 1, /* InvalidDimension */
+    SPX_ERR_BATCH_TOO_LARGE = 2,   /* BatchTooLarge{size,max} */
+    SPX_ERR_MEMORY_ALLOC = 3,      /* MemoryAlloc */
+    SPX_ERR_TRANSFER = 4,          /* Transfer */
+    SPX_ERR_BACKEND_INIT = 5,      /* BackendInit */
+    SPX_ERR_BACKEND = 6,           /* Backend */
+    SPX_ERR_NO_BACKEND = 7,        /* NoBackendAvailable */
+    SPX_ERR_INVALID_ARGUMENT = 8   /* argument outside the documented domain */
+} spx_status;
+
+/* Per-instance convergence status: `ConvergenceStatus`
+ * (crates/spicier-batched-sweep/src/convergence.rs:41-50). */
+typedef enum spx_conv_status {
+    SPX_ACTIVE = 0,
+    SPX_CONVERGED = 1,
+    SPX_FAILED = 2,
+    SPX_SINGULAR = 3
+} spx_conv_status;
+
+/* Bit flags OR-ed into spx_sweep_result.flags[i]. */
+enum {
+    SPX_FLAG_KCL_FAIL = 1,      /* Δx test passed but the FP64 KCL residual test did not */
+    SPX_FLAG_GMIN_STEPPING = 2, /* plain Newton failed; Gmin stepping was run */
+    SPX_FLAG_SOURCE_STEPPING = 4,/* Gmin stepping failed; source stepping was run */
+    SPX_FLAG_FP64_ESCALATED = 8,/* FP32+IR could not reach FP64 accuracy; FP64 LU used */
+    SPX_FLAG_WARM_START = 16    /* Newton started from a caller or surrogate guess */
+};
+
+/* ------------------------------------------------------------------------- */
+/* Handle: one device, one HIP stream, device memory owned by the library. */
+typedef struct spx_handle spx_handle;
+
+int spx_device_count(int32_t* count);
+/* Creates a handle on `device_id`.  Cheap after the first call per process:
+ * `parallel.rs:194` creates a solver per sweep point, so handles share a
+ * process-wide device context. */
+int spx_create(int32_t device_id, spx_handle** out);
+int spx_destroy(spx_handle* h);
+/* Message of the last failing call on this thread ("" if none). */
+const char* spx_last_error(void);
+/* Blocks until every call enqueued on the handle's stream has finished. */
+int spx_synchronize(spx_handle* h);
+/* The handle's stream as an opaque pointer (hipStream_t), for callers that
+ * order their own device work against the library's. */
+void* spx_stream(spx_handle* h);
+
+/* ------------------------------------------------------------------------- */
+/* Batched dense linear solve — the `BatchedLuSolver::solve_batch` drop-in
+ * (replaces CudaBatchedLuSolver::solve_batch,
+ *  crates/spicier-backend-cuda/src/batched_lu.rs:373-494, and matches
+ *  CpuBatchedSolver, crates/spicier-batched-sweep/src/solver.rs:327-385).
+ *
+ * matrices: batch*n*n doubles, column-major per system; rhs: batch*n;
+ * solutions: batch*n out.  Partial-pivot LU, first maximum on ties, a system
+ * is singular iff an exact zero pivot column occurs or the solution is not
+ * finite; singular systems get an all-zero solution and singular[i] = 1
+ * (singular may be NULL).  *num_singular receives the count (may be NULL).
+ * batch == 0 succeeds and touches nothing.  n must be 1..256. */
+int spx_solve_batch_f64(spx_handle* h, const double* matrices, const double* rhs,
+                        int64_t n, int64_t batch, double* solutions,
+                        uint8_t* singular, int64_t* num_singular);
+/* Same contract on device pointers; enqueued on the handle's stream. */
+int spx_solve_batch_f64_device(spx_handle* h, const double* d_matrices,
+                               const double* d_rhs, int64_t n, int64_t batch,
+                               double* d_solutions, uint8_t* d_singular);
+/* Largest batch one launch of the solve accepts (GpuBatchConfig
+ * max_batch_per_launch, solver.rs:6-41). Larger batches are split. */
+int64_t spx_max_batch_per_launch(void);
+
+/* ------------------------------------------------------------------------- */
+/* Circuit description: the device-resident "circuit program".  Element kinds
+ * and stamp conventions follow spicier-core MnaSystem
+ * (crates/spicier-core/src/mna.rs:87-169) and the device files cited below.
+ * Nodes are numbered as spicier NodeId: 0 is ground, node k>0 is unknown k-1;
+ * voltage sources get branch unknowns num_nodes + (order of appearance). */
+typedef enum spx_element_kind {
+    SPX_EL_RESISTOR = 0,  /* node[0..1]; param[0] = resistance Ω (passive.rs:53-58) */
+    SPX_EL_CAPACITOR = 1, /* node[0..1]; param[0] = C (F); open at DC, BE companion in transient */
+    SPX_EL_VSOURCE = 2,   /* node[0]=pos,node[1]=neg; param[0] = V (sources.rs:96-101) */
+    SPX_EL_ISOURCE = 3,   /* current param[0] flows node[0] -> node[1] (sources.rs:230-237) */
+    SPX_EL_DIODE = 4,     /* node[0]=anode,node[1]=cathode; param[0]=IS, param[1]=N (diode.rs:91-131) */
+    SPX_EL_NMOS = 5,      /* node[0..2]=d,g,s; param[0]=VTO,[1]=KP,[2]=LAMBDA,[3]=W,[4]=L (level1.rs:154-248) */
+    SPX_EL_PMOS = 6,      /* as NMOS, PMOS sign convention */
+    SPX_EL_VPULSE = 7     /* voltage source whose transient value is PULSE(param[0]=V1, [1]=V2, [2]=TD,
+                             [3]=TR, [4]=TF, [5]=PW, [6]=PER) (waveforms.rs:160-180); DC value V1 */
+} spx_element_kind;
+
+#define SPX_MAX_ELEMENT_PARAMS 7
+
+typedef struct spx_element {
+    int32_t kind;                           /* spx_element_kind */
+    int32_t node[3];                        /* NodeId numbers, 0 = ground */
+    int32_t param[SPX_MAX_ELEMENT_PARAMS];  /* parameter slot per quantity, -1 = unused */
+} spx_element;
+
+typedef struct spx_circuit {
+    int32_t num_nodes;            /* nodes excluding ground */
+    int32_t num_elements;
+    const spx_element* elements;  /* stamped in this order (netlist.rs:441-445) */
+    int32_t num_params;           /* P: per-instance parameter slots */
+} spx_circuit;
+
+/* Distribution of one parameter slot. */
+typedef enum spx_param_kind {
+    SPX_P_CONST = 0,      /* a */
+    SPX_P_GAUSS_REL = 1,  /* (a + z*b*a).clamp(c, d): ParameterVariation{nominal=a, sigma=b, min=c, max=d}
+                             (spicier-solver/src/sweep.rs:17-54,193) */
+    SPX_P_GAUSS = 2,      /* a + z*b (rng.rs:148-150 gaussian_scaled) */
+    SPX_P_UNIFORM = 3,    /* a + (b-a)*u */
+    SPX_P_LOGUNIFORM = 4  /* exp(ln a + (ln b - ln a)*u) */
+} spx_param_kind;
+
+typedef struct spx_param_spec {
+    int32_t kind;       /* spx_param_kind */
+    int32_t rng_index;  /* hash-RNG param_idx of this slot (rng.rs:51-56) */
+    double a, b, c, d;
+} spx_param_spec;
+
+/* How instance parameters are produced. */
+typedef enum spx_generator {
+    SPX_GEN_HASH = 0,     /* stateless hash RNG, (seed, sweep_offset+i, rng_index) (rng.rs:39-150) */
+    SPX_GEN_LCG = 1,      /* MonteCarloGenerator's sequential LCG + Box-Muller, reproduced with
+                             jump-ahead; every varied slot must be SPX_P_GAUSS_REL
+                             (spicier-solver/src/sweep.rs:170-200) */
+    SPX_GEN_CORNER = 2,   /* CornerGenerator: bit j of i -> varied slot j at c (min) or d (max)
+                             (sweep.rs:204-227) */
+    SPX_GEN_LINEAR = 3,   /* LinearSweepGenerator: c + t*(d-c), t=i/(B-1) (sweep.rs:231-266) */
+    SPX_GEN_EXPLICIT = 4  /* caller-supplied parameters [batch][P] */
+} spx_generator;
+
+typedef struct spx_sweep_spec {
+    int32_t generator;             /* spx_generator */
+    uint64_t seed;
+    int64_t batch;                 /* B instances */
+    int64_t sweep_offset;          /* instance i uses sweep index sweep_offset + i */
+    const spx_param_spec* params;  /* num_params entries */
+    const double* explicit_params; /* [batch][P] for SPX_GEN_EXPLICIT (host or device pointer
+                                      matching the entry point), else NULL */
+} spx_sweep_spec;
+
+/* Fills params[B][P] (sweep-major, as generate_gaussian_parameters,
+ * rng.rs:208-227).  Host and device variants. */
+int spx_sample_params(spx_handle* h, const spx_sweep_spec* spec, int32_t num_params,
+                      double* params);
+int spx_sample_params_device(spx_handle* h, const spx_sweep_spec* spec, int32_t num_params,
+                             double* d_params);
+
+/* ------------------------------------------------------------------------- */
+/* Newton configuration: ConvergenceCriteria, GminSteppingParams and
+ * SourceSteppingParams (crates/spicier-solver/src/newton.rs:11-34,174-231)
+ * plus the engine's precision / limiter / warm-start choices. */
+typedef enum spx_precision {
+    SPX_PREC_F64 = 0,    /* FP64 LU every Newton iteration */
+    SPX_PREC_F32_IR = 1  /* FP32 LU (reused across iterations) + FP64 iterative refinement */
+} spx_precision;
+
+typedef enum spx_limiter {
+    SPX_LIMIT_REFERENCE = 0, /* diode.rs:139-148 memoryless log limiter, as the reference stamps
+                                (its Newton fixed point above vcrit is not the Shockley solution) */
+    SPX_LIMIT_NONE = 1,      /* plain Shockley, no limiting */
+    SPX_LIMIT_PNJLIM = 2     /* SPICE pnjlim on each diode's junction voltage, vcrit from the
+                                device's own IS; an iteration that limited cannot converge */
+} spx_limiter;
+
+typedef enum spx_warm_start {
+    SPX_WARM_ZERO = 0,      /* zero initial guess (newton.rs:80-83) */
+    SPX_WARM_GIVEN = 1,     /* x0[B][n] supplied by the caller */
+    SPX_WARM_SURROGATE = 2  /* predicted by the handle's trained surrogate */
+} spx_warm_start;
+
+typedef struct spx_nr_config {
+    double v_abstol;            /* 1e-6 */
+    double v_reltol;            /* 1e-3 */
+    double i_abstol;            /* 1e-12 */
+    int32_t max_iterations;     /* 50 */
+    int32_t gmin_stepping;      /* 1: fall back to Gmin stepping if plain NR fails */
+    double gmin_initial;        /* 1e-3 */
+    double gmin_final;          /* 1e-12 */
+    double gmin_factor;         /* 10 */
+    int32_t gmin_max_attempts;  /* 5 */
+    int32_t source_stepping;    /* 1: fall back to source stepping if Gmin stepping fails */
+    double src_initial;         /* 0.1 */
+    double src_step;            /* 0.1 */
+    double src_min_step;        /* 0.01 */
+    int32_t src_max_attempts;   /* 5 */
+    int32_t precision;          /* spx_precision */
+    int32_t limiter;            /* spx_limiter */
+    int32_t warm_start;         /* spx_warm_start */
+    int32_t ir_max_steps;       /* FP32+IR: refinement steps per Newton solve before refactoring */
+    double ir_tol;              /* FP32+IR: stop when |dx_i| <= ir_tol*(|x_i|+1e-30) for all i */
+    /* KCL residual test of the final x, FP64: devices evaluated unlimited
+     * (Shockley / square law) at x; node row i passes if
+     * |sum_j J_ij x_j - b_i| <= kcl_reltol * max(|J_ij x_j|, |b_i|) + kcl_abstol. */
+    double kcl_reltol;          /* SPICE reltol of the KCL residual test (1e-3) */
+    double kcl_abstol;          /* abstol (A) of node rows (1e-12) */
+    double kcl_vntol;           /* vntol (V) of branch rows (1e-6) */
+} spx_nr_config;
+
+/* Fills *cfg with the defaults quoted above (FP64, reference limiter, zero start). */
+void spx_nr_config_default(spx_nr_config* cfg);
+
+/* Caller-owned result buffers (host for spx_sweep_dcop, device for
+ * spx_sweep_dcop_device).  Any pointer except x may be NULL. */
+typedef struct spx_sweep_result {
+    double* x;            /* [B][n] converged solution (node voltages, then branch currents) */
+    int32_t* status;      /* [B] spx_conv_status */
+    int32_t* iterations;  /* [B] Newton iterations performed (all stages) */
+    int32_t* flags;       /* [B] SPX_FLAG_* */
+    double* residual;     /* [B] max_i |KCL_i| / tol_i of the final solution (<= 1 passes) */
+    double* params;       /* [B][P] sampled parameters */
+    double* node_stats;   /* [n][5] count,sum,sumsq,min,max over converged instances
+                             (StatisticsAccumulator, statistics.rs:135-154) */
+} spx_sweep_result;
+
+/* Monte Carlo / corner / linear DC-operating-point sweep (the nonlinear
+ * counterpart of solve_batched_sweep_gpu, crates/spicier-batched-sweep/src/sweep.rs:101-217,
+ * with the per-instance semantics of solve_newton_raphson + Gmin + source
+ * stepping, crates/spicier-solver/src/newton.rs:70-459).  x0 is [B][n] when
+ * cfg->warm_start == SPX_WARM_GIVEN, else NULL. */
+int spx_sweep_dcop(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
+                   const spx_nr_config* cfg, const double* x0, spx_sweep_result* out);
+/* Device-pointer variant: spec->explicit_params, x0 and every result buffer
+ * are device pointers; returns once the work is enqueued. */
+int spx_sweep_dcop_device(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
+                          const spx_nr_config* cfg, const double* d_x0, spx_sweep_result* d_out);
+
+/* Unknown count n of a circuit (num_nodes + number of voltage sources). */
+int32_t spx_circuit_size(const spx_circuit* ckt);
+
+/* ------------------------------------------------------------------------- */
+/* Statistics reduction over a [B][n] buffer (device pointers), skipping
+ * instances whose mask byte is 0 (mask may be NULL): out[n][5] =
+ * count,sum,sumsq,min,max (StatisticsAccumulator::add/merge,
+ * crates/spicier-batched-sweep/src/statistics.rs:135-154). */
+int spx_stats_reduce_device(spx_handle* h, const double* d_values, const uint8_t* d_mask,
+                            int64_t batch, int32_t n, double* d_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPICIER_HIP_H */

@@ -1,0 +1,718 @@
+// oracle.cpp — CPU FP64 restatement of spicier's sweep / DC-OP path.
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the HIP engine.  Only tests/,
+// the smoke check and bench.py's cpu_baseline leg load this library.
+//
+// Every function follows the reference file:line named above it, keeping the
+// reference's operation order so that -ffp-contract=off reproduces its
+// rounding.  Where the reference delegates to a third-party crate (nalgebra
+// 0.33.2 LU, faer 0.24.0 sparse LU) the crate's published algorithm is
+// restated and the deviation noted.
+#include "oracle.h"
+
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- rng.rs
+
+}  // namespace
+
+extern "C" uint64_t orc_splitmix64(uint64_t x) {  // rng.rs:39-44
+  x = x + 0x9e3779b97f4a7c15ULL;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+  return x ^ (x >> 31);
+}
+
+extern "C" uint64_t orc_combine_indices(uint64_t seed, uint32_t s, uint32_t p) {  // rng.rs:51-56
+  return seed ^ ((uint64_t)s * 0x517cc1b727220a95ULL) ^ ((uint64_t)p * 0x5851f42d4c957f2dULL);
+}
+
+extern "C" double orc_uniform(uint64_t seed, uint32_t s, uint32_t p) {  // rng.rs:80-85
+  uint64_t h = orc_splitmix64(orc_combine_indices(seed, s, p));
+  return (double)(h >> 11) * (1.0 / (double)(1ULL << 53));
+}
+
+extern "C" float orc_uniform_f32(uint64_t seed, uint32_t s, uint32_t p) {  // rng.rs:91-95
+  uint64_t h = orc_splitmix64(orc_combine_indices(seed, s, p));
+  return (float)(h >> 40) * (1.0f / (float)(1ULL << 24));
+}
+
+extern "C" double orc_gaussian(uint64_t seed, uint32_t s, uint32_t p) {  // rng.rs:120-131
+  double u1 = orc_uniform(seed, s, p * 2u);
+  double u2 = orc_uniform(seed, s, p * 2u + 1u);
+  u1 = std::fmax(u1, 1e-10);
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kPi * u2);
+}
+
+extern "C" float orc_gaussian_f32(uint64_t seed, uint32_t s, uint32_t p) {  // rng.rs:136-143
+  float u1 = orc_uniform_f32(seed, s, p * 2u);
+  float u2 = orc_uniform_f32(seed, s, p * 2u + 1u);
+  u1 = std::fmax(u1, 1e-7f);
+  return std::sqrt(-2.0f * std::log(u1)) * std::cos(2.0f * 3.14159265358979323846f * u2);
+}
+
+extern "C" double orc_gaussian_scaled(uint64_t seed, uint32_t s, uint32_t p, double mean,
+                                      double sigma) {  // rng.rs:148-150
+  return mean + orc_gaussian(seed, s, p) * sigma;
+}
+
+extern "C" void orc_generate_gaussian_parameters(uint64_t seed, int64_t num_sweeps,
+                                                 int32_t num_params, const double* means,
+                                                 const double* sigmas, double* out) {
+  // rng.rs:208-227: sweep-major.
+  for (int64_t s = 0; s < num_sweeps; ++s)
+    for (int32_t p = 0; p < num_params; ++p)
+      out[s * num_params + p] =
+          orc_gaussian_scaled(seed, (uint32_t)s, (uint32_t)p, means[p], sigmas[p]);
+}
+
+namespace {
+
+// Rust f64::clamp(min, max).
+inline double rclamp(double v, double lo, double hi) {
+  if (v < lo) v = lo;
+  if (v > hi) v = hi;
+  return v;
+}
+
+// ------------------------------------------- spicier-solver/src/sweep.rs:170-200
+struct Lcg {
+  uint64_t s;
+  double next_random() {  // :175-177
+    s = s * 6364136223846793005ULL + 1ULL;
+    return (double)(s >> 33) / (double)(1ULL << 31);
+  }
+  double next_normal() {  // :181-185
+    double u1 = std::fmax(next_random(), 1e-10);
+    double u2 = next_random();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kPi * u2);
+  }
+};
+
+bool is_varied(const spx_param_spec& p) { return p.kind != SPX_P_CONST; }
+
+}  // namespace
+
+extern "C" void orc_monte_carlo_lcg(uint64_t seed, int64_t num_samples, int32_t num_vars,
+                                    const double* nominal, const double* sigma,
+                                    const double* vmin, const double* vmax, double* out) {
+  Lcg g{seed};
+  for (int64_t i = 0; i < num_samples; ++i)
+    for (int32_t v = 0; v < num_vars; ++v) {
+      double z = g.next_normal();
+      out[i * num_vars + v] = rclamp(nominal[v] + z * sigma[v] * nominal[v], vmin[v], vmax[v]);
+    }
+}
+
+extern "C" int orc_generate_params(const spx_sweep_spec* spec, int32_t P, double* out) {
+  const int64_t B = spec->batch;
+  if (B < 0 || P < 0) return SPX_ERR_INVALID_ARGUMENT;
+  const spx_param_spec* ps = spec->params;
+  int nvar = 0;
+  for (int p = 0; p < P; ++p) nvar += is_varied(ps[p]);
+  switch (spec->generator) {
+    case SPX_GEN_HASH:
+      for (int64_t i = 0; i < B; ++i) {
+        uint32_t s = (uint32_t)(spec->sweep_offset + i);
+        for (int p = 0; p < P; ++p) {
+          const spx_param_spec& q = ps[p];
+          uint32_t r = (uint32_t)q.rng_index;
+          double v;
+          switch (q.kind) {
+            case SPX_P_CONST: v = q.a; break;
+            case SPX_P_GAUSS_REL: {
+              double z = orc_gaussian(spec->seed, s, r);
+              v = rclamp(q.a + z * q.b * q.a, q.c, q.d);
+              break;
+            }
+            case SPX_P_GAUSS: v = orc_gaussian_scaled(spec->seed, s, r, q.a, q.b); break;
+            case SPX_P_UNIFORM: v = q.a + (q.b - q.a) * orc_uniform(spec->seed, s, r); break;
+            case SPX_P_LOGUNIFORM: {
+              double la = std::log(q.a), lb = std::log(q.b);
+              v = std::exp(la + (lb - la) * orc_uniform(spec->seed, s, r));
+              break;
+            }
+            default: return SPX_ERR_INVALID_ARGUMENT;
+          }
+          out[i * P + p] = v;
+        }
+      }
+      return SPX_OK;
+    case SPX_GEN_LCG: {
+      // The generator's state is sequential over (sample, variation); the
+      // sweep offset skips offset*nvar normals (two draws each).
+      Lcg g{spec->seed};
+      for (int64_t k = 0; k < 2 * nvar * spec->sweep_offset; ++k) g.next_random();
+      for (int64_t i = 0; i < B; ++i)
+        for (int p = 0; p < P; ++p) {
+          const spx_param_spec& q = ps[p];
+          if (q.kind == SPX_P_CONST) { out[i * P + p] = q.a; continue; }
+          if (q.kind != SPX_P_GAUSS_REL) return SPX_ERR_INVALID_ARGUMENT;
+          double z = g.next_normal();
+          out[i * P + p] = rclamp(q.a + z * q.b * q.a, q.c, q.d);
+        }
+      return SPX_OK;
+    }
+    case SPX_GEN_CORNER: {  // sweep.rs:204-227
+      if (nvar >= 63 || B != (int64_t(1) << nvar)) return SPX_ERR_INVALID_DIMENSION;
+      for (int64_t i = 0; i < B; ++i) {
+        int j = 0;
+        for (int p = 0; p < P; ++p) {
+          const spx_param_spec& q = ps[p];
+          if (!is_varied(q)) { out[i * P + p] = q.a; continue; }
+          out[i * P + p] = ((i >> j) & 1) == 0 ? q.c : q.d;
+          ++j;
+        }
+      }
+      return SPX_OK;
+    }
+    case SPX_GEN_LINEAR:  // sweep.rs:231-266
+      for (int64_t i = 0; i < B; ++i) {
+        double t = B > 1 ? (double)i / (double)(B - 1) : 0.5;
+        for (int p = 0; p < P; ++p) {
+          const spx_param_spec& q = ps[p];
+          out[i * P + p] = is_varied(q) ? q.c + t * (q.d - q.c) : q.a;
+        }
+      }
+      return SPX_OK;
+    case SPX_GEN_EXPLICIT:
+      if (!spec->explicit_params) return SPX_ERR_INVALID_ARGUMENT;
+      std::memcpy(out, spec->explicit_params, sizeof(double) * (size_t)(B * P));
+      return SPX_OK;
+    default:
+      return SPX_ERR_INVALID_ARGUMENT;
+  }
+}
+
+// ----------------------------------------------------------- dense LU
+// nalgebra 0.33.2 LU::new (partial pivoting, first max |a| on ties, a zero
+// pivot column is skipped) then LU::solve (permute, unit-lower forward
+// substitution, upper back substitution with a zero-diagonal check), as used by
+// CpuBatchedSolver (solver.rs:370-380) and solve_dense (linear.rs:506).
+// The multiplier is formed as a * (1/pivot) and the update as (-u)*l + a,
+// the order nalgebra's gauss_step / axpy use.
+extern "C" int orc_lu_solve(int32_t n, const double* a_cm, const double* b, double* x) {
+  std::vector<double> a(a_cm, a_cm + (size_t)n * n);  // column-major
+  auto A = [&](int r, int c) -> double& { return a[(size_t)c * n + r]; };
+  std::vector<double> y(b, b + n);
+  std::vector<int> perm_from(n);
+  for (int i = 0; i < n; ++i) {
+    int piv = i;
+    double best = std::fabs(A(i, i));
+    for (int r = i + 1; r < n; ++r) {
+      double v = std::fabs(A(r, i));
+      if (v > best) { best = v; piv = r; }
+    }
+    perm_from[i] = piv;
+    double diag = A(piv, i);
+    if (diag == 0.0) continue;
+    if (piv != i) {
+      for (int c = 0; c < n; ++c) std::swap(A(i, c), A(piv, c));
+    }
+    double inv = 1.0 / diag;
+    for (int r = i + 1; r < n; ++r) A(r, i) *= inv;
+    for (int c = i + 1; c < n; ++c) {
+      double u = A(i, c);
+      for (int r = i + 1; r < n; ++r) A(r, c) = (-u) * A(r, i) + A(r, c);
+    }
+  }
+  // apply the row transpositions to b in order
+  for (int i = 0; i < n; ++i)
+    if (perm_from[i] != i) std::swap(y[i], y[perm_from[i]]);
+  for (int i = 0; i < n; ++i) {
+    double coeff = y[i];
+    for (int r = i + 1; r < n; ++r) y[r] = (-coeff) * A(r, i) + y[r];
+  }
+  bool singular = false;
+  for (int i = n - 1; i >= 0; --i) {
+    double d = A(i, i);
+    if (d == 0.0) { singular = true; break; }
+    double coeff = y[i] / d;
+    y[i] = coeff;
+    for (int r = 0; r < i; ++r) y[r] = (-coeff) * A(r, i) + y[r];
+  }
+  if (!singular)
+    for (int i = 0; i < n; ++i)
+      if (!std::isfinite(y[i])) { singular = true; break; }
+  for (int i = 0; i < n; ++i) x[i] = singular ? 0.0 : y[i];
+  return singular ? 1 : 0;
+}
+
+extern "C" int orc_solve_batch(const double* m, const double* rhs, int64_t n, int64_t batch,
+                               double* sol, uint8_t* singular, int64_t* num_singular) {
+  if (n < 0 || batch < 0) return SPX_ERR_INVALID_DIMENSION;
+  int64_t ns = 0;
+  for (int64_t i = 0; i < batch; ++i) {
+    int s = orc_lu_solve((int32_t)n, m + i * n * n, rhs + i * n, sol + i * n);
+    if (singular) singular[i] = (uint8_t)s;
+    ns += s;
+  }
+  if (num_singular) *num_singular = ns;
+  return SPX_OK;
+}
+
+// ----------------------------------------------------------- devices
+extern "C" double orc_thermal_voltage(double temp_k) {  // diode.rs:41-45
+  const double k = 1.380649e-23, q = 1.602176634e-19;
+  return k * temp_k / q;
+}
+
+extern "C" double orc_diode_limit(double vd, double nvt) {  // diode.rs:139-148
+  double vcrit = nvt * std::log(nvt / (1.41421356237309504880 * 1e-14));
+  if (vd > vcrit) {
+    double arg = (vd - vcrit) / nvt;
+    return vcrit + nvt * (1.0 + std::log1p(arg));
+  }
+  return vd;
+}
+
+extern "C" void orc_diode_evaluate(double is, double n, double vd, int32_t limiter, double* id,
+                                   double* gd) {  // diode.rs:91-108
+  double vt = orc_thermal_voltage(300.15);
+  double nvt = n * vt;
+  double v = limiter == SPX_LIMIT_REFERENCE ? orc_diode_limit(vd, nvt) : vd;
+  double e = std::exp(v / nvt);
+  *id = is * (e - 1.0);
+  double g = is * e / nvt;
+  *gd = std::fmax(g, 1e-12);
+}
+
+extern "C" void orc_mosfet_evaluate(int32_t pmos, double vto, double kp, double lambda, double w,
+                                    double l, double vgs, double vds, double* ids, double* gds,
+                                    double* gm, int32_t* region) {  // level1.rs:154-191
+  if (pmos) { vgs = -vgs; vds = -vds; }
+  double vth = std::fabs(vto);
+  double beta = kp * 1.0 * w / l;  // level1.rs:68-70, nf = 1
+  double i, gd, g;
+  if (vgs < vth) {
+    *ids = 0.0; *gds = 1e-12; *gm = 0.0; *region = 0;
+    return;
+  } else if (vds < vgs - vth) {
+    double vov = vgs - vth;
+    i = beta * (vov * vds - 0.5 * vds * vds) * (1.0 + lambda * vds);
+    gd = beta * (vov - vds) * (1.0 + lambda * vds) + beta * (vov * vds - 0.5 * vds * vds) * lambda;
+    g = beta * vds * (1.0 + lambda * vds);
+    *region = 1;
+  } else {
+    double vov = vgs - vth;
+    i = 0.5 * beta * vov * vov * (1.0 + lambda * vds);
+    gd = 0.5 * beta * vov * vov * lambda;
+    g = beta * vov * (1.0 + lambda * vds);
+    *region = 2;
+  }
+  *ids = pmos ? -i : i;
+  *gds = std::fmax(gd, 1e-12);
+  *gm = g;
+}
+
+// ----------------------------------------------------------- MNA assembly
+extern "C" int32_t orc_circuit_size(const spx_circuit* c) {
+  int32_t nv = 0;
+  for (int e = 0; e < c->num_elements; ++e)
+    nv += (c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE);
+  return c->num_nodes + nv;
+}
+
+namespace {
+
+// Triplet MNA (mna.rs:13-170) assembled straight into a dense column-major
+// matrix: adding each triplet in push order onto zeros equals
+// to_dense_matrix (mna.rs:62-71).
+struct Mna {
+  int n, nn;
+  double* J;
+  double* rhs;
+  void add(int r, int c, double v) { J[(size_t)c * n + r] += v; }
+  void conductance(int i, int j, double g) {  // mna.rs:87-98
+    if (i >= 0) add(i, i, g);
+    if (j >= 0) add(j, j, g);
+    if (i >= 0 && j >= 0) { add(i, j, -g); add(j, i, -g); }
+  }
+  void current(int i, int j, double I) {  // mna.rs:103-115
+    if (i >= 0) rhs[i] += -I;
+    if (j >= 0) rhs[j] += I;
+  }
+  void vsource(int i, int j, int k, double V) {  // mna.rs:124-145
+    int row = nn + k;
+    if (i >= 0) { add(i, row, 1.0); add(row, i, 1.0); }
+    if (j >= 0) { add(j, row, -1.0); add(row, j, -1.0); }
+    rhs[row] = V;
+  }
+};
+
+inline int idx(int32_t node) { return node - 1; }  // NodeId -> unknown, ground -> -1
+inline double volt(const double* x, int32_t node) { return node > 0 ? x[node - 1] : 0.0; }
+inline double P(const double* params, int32_t slot) { return slot >= 0 ? params[slot] : 0.0; }
+
+}  // namespace
+
+namespace {
+
+// SPICE3 DEVpnjlim: limits the new junction voltage against the previous
+// iteration's when it is above vcrit and moved by more than 2*nvt.
+double pnjlim(double vnew, double vold, double vt, double vcrit, int* limited) {
+  if (vnew > vcrit && std::fabs(vnew - vold) > vt + vt) {
+    if (vold > 0) {
+      double arg = 1 + (vnew - vold) / vt;
+      vnew = arg > 0 ? vold + vt * std::log(arg) : vcrit;
+    } else {
+      vnew = vt * std::log(vnew / vt);
+    }
+    if (limited) *limited = 1;
+  }
+  return vnew;
+}
+
+void stamp_impl(const spx_circuit* c, const double* params, const double* x, double gmin,
+                double sf, int32_t limiter, double* J, double* rhs, double* vstate,
+                int* limited) {
+  const int n = orc_circuit_size(c);
+  std::memset(J, 0, sizeof(double) * (size_t)n * n);
+  std::memset(rhs, 0, sizeof(double) * (size_t)n);
+  Mna m{n, c->num_nodes, J, rhs};
+  int k = 0;  // voltage-source branch counter
+  for (int e = 0; e < c->num_elements; ++e) {
+    const spx_element& el = c->elements[e];
+    const int a = idx(el.node[0]), b = idx(el.node[1]);
+    switch (el.kind) {
+      case SPX_EL_RESISTOR:
+        m.conductance(a, b, 1.0 / P(params, el.param[0]));
+        break;
+      case SPX_EL_CAPACITOR:
+        break;  // open at DC (passive.rs:233-245)
+      case SPX_EL_VSOURCE:
+      case SPX_EL_VPULSE:
+        m.vsource(a, b, k++, P(params, el.param[0]) * sf);
+        break;
+      case SPX_EL_ISOURCE:
+        m.current(a, b, P(params, el.param[0]) * sf);
+        break;
+      case SPX_EL_DIODE: {  // diode.rs:113-131, 190-199
+        double vd = volt(x, el.node[0]) - volt(x, el.node[1]);
+        double is = P(params, el.param[0]), nn = P(params, el.param[1]);
+        double nvt = nn * orc_thermal_voltage(300.15);
+        double vl, id, gd;
+        if (limiter == SPX_LIMIT_PNJLIM) {
+          double vold = vstate ? vstate[e] : 0.0;
+          double vcrit = nvt * std::log(nvt / (1.41421356237309504880 * is));
+          vl = pnjlim(vd, vold, nvt, vcrit, limited);
+          if (vstate) vstate[e] = vl;
+          double ex = std::exp(vl / nvt);
+          id = is * (ex - 1.0);
+          gd = std::fmax(is * ex / nvt, 1e-12);
+        } else {
+          vl = limiter == SPX_LIMIT_REFERENCE ? orc_diode_limit(vd, nvt) : vd;
+          orc_diode_evaluate(is, nn, vl, limiter, &id, &gd);
+        }
+        double ieq = id - gd * vl;
+        m.conductance(a, b, gd);
+        m.current(a, b, ieq);
+        break;
+      }
+      case SPX_EL_NMOS:
+      case SPX_EL_PMOS: {  // level1.rs:202-248, 305-318
+        const int d = idx(el.node[0]), g = idx(el.node[1]), s = idx(el.node[2]);
+        double vg = volt(x, el.node[1]), vd = volt(x, el.node[0]), vs = volt(x, el.node[2]);
+        double vgs = vg - vs, vds = vd - vs;
+        double ids, gds, gm;
+        int32_t region;
+        orc_mosfet_evaluate(el.kind == SPX_EL_PMOS, P(params, el.param[0]), P(params, el.param[1]),
+                            P(params, el.param[2]), P(params, el.param[3]), P(params, el.param[4]),
+                            vgs, vds, &ids, &gds, &gm, &region);
+        m.conductance(d, s, gds);
+        if (d >= 0) {
+          if (g >= 0) m.add(d, g, gm);
+          if (s >= 0) m.add(d, s, -gm);
+        }
+        if (s >= 0) {
+          if (g >= 0) m.add(s, g, -gm);
+          m.add(s, s, gm);
+        }
+        double ieq = ids - gds * vds - gm * vgs;
+        m.current(d, s, ieq);
+        break;
+      }
+      default:
+        break;
+    }
+  }
+  if (gmin != 0.0)  // GminStamper (newton.rs:393-398) + stamp_gmin (mna.rs:165-169)
+    for (int i = 0; i < c->num_nodes; ++i) m.add(i, i, gmin);
+}
+
+}  // namespace
+
+extern "C" void orc_stamp(const spx_circuit* c, const double* params, const double* x,
+                          double gmin, double sf, int32_t limiter, double* J, double* rhs) {
+  stamp_impl(c, params, x, gmin, sf, limiter, J, rhs, nullptr, nullptr);
+}
+
+namespace {
+
+// check_convergence (newton.rs:134-159)
+bool converged_dx(const double* old, const double* nw, int nn, int n, const spx_nr_config* cfg) {
+  for (int i = 0; i < nn; ++i) {
+    double delta = std::fabs(nw[i] - old[i]);
+    double tol = cfg->v_reltol * std::fmax(std::fabs(nw[i]), std::fabs(old[i])) + cfg->v_abstol;
+    if (delta > tol) return false;
+  }
+  for (int i = nn; i < n; ++i) {
+    double delta = std::fabs(nw[i] - old[i]);
+    double tol = cfg->v_reltol * std::fmax(std::fabs(nw[i]), std::fabs(old[i])) + cfg->i_abstol;
+    if (delta > tol) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" int orc_newton(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
+                          double gmin, double sf, const double* x0, double* x,
+                          int32_t* iterations) {
+  const int n = orc_circuit_size(c);
+  std::vector<double> J((size_t)n * n), rhs(n), xn(n), sol(n, 0.0);
+  std::vector<double> vstate(c->num_elements, 0.0);  // pnjlim junction voltages
+  if (x0) std::memcpy(sol.data(), x0, sizeof(double) * n);
+  for (int it = 0; it < cfg->max_iterations; ++it) {
+    int limited = 0;
+    stamp_impl(c, params, sol.data(), gmin, sf, cfg->limiter, J.data(), rhs.data(),
+               vstate.data(), &limited);
+    if (orc_lu_solve(n, J.data(), rhs.data(), xn.data())) {
+      std::memcpy(x, sol.data(), sizeof(double) * n);
+      *iterations = it + 1;
+      return -1;
+    }
+    bool conv = converged_dx(sol.data(), xn.data(), c->num_nodes, n, cfg) && !limited;
+    sol = xn;
+    if (conv) {
+      std::memcpy(x, sol.data(), sizeof(double) * n);
+      *iterations = it + 1;
+      return 1;
+    }
+  }
+  std::memcpy(x, sol.data(), sizeof(double) * n);
+  *iterations = cfg->max_iterations;
+  return 0;
+}
+
+extern "C" double orc_kcl_residual(const spx_circuit* c, const double* params,
+                                   const spx_nr_config* cfg, const double* x, double gmin) {
+  // KCL at x with devices linearised, unlimited, at x: r = J(x) x - b(x)
+  // equals the sum of element currents leaving each node.  For each row,
+  // tol = reltol * max(|J_ij x_j|, |b_i|) + abstol (node rows, amperes) or
+  // + vntol (branch rows, volts).  Returns max |r_i| / tol_i.
+  const int n = orc_circuit_size(c);
+  std::vector<double> J((size_t)n * n), rhs(n);
+  stamp_impl(c, params, x, gmin, 1.0, SPX_LIMIT_NONE, J.data(), rhs.data(), nullptr, nullptr);
+  double worst = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double r = -rhs[i], mag = std::fabs(rhs[i]);
+    for (int j = 0; j < n; ++j) {
+      double t = J[(size_t)j * n + i] * x[j];
+      r += t;
+      mag = std::fmax(mag, std::fabs(t));
+    }
+    double tol = cfg->kcl_reltol * mag + (i < c->num_nodes ? cfg->kcl_abstol : cfg->kcl_vntol);
+    double q = std::fabs(r) / tol;
+    if (!(q <= worst)) worst = q;  // NaN propagates as failure
+  }
+  return worst;
+}
+
+extern "C" int orc_dcop(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
+                        const double* x0, double* x, int32_t* iterations, int32_t* flags,
+                        double* residual) {
+  const int n = orc_circuit_size(c);
+  int32_t total = 0, it = 0, fl = (cfg->warm_start != SPX_WARM_ZERO && x0) ? SPX_FLAG_WARM_START : 0;
+  double final_gmin = 0.0;
+  int st = orc_newton(c, params, cfg, 0.0, 1.0, x0, x, &it);
+  total += it;
+  int status = st == 1 ? SPX_CONVERGED : (st < 0 ? SPX_SINGULAR : SPX_FAILED);
+
+  // Gmin stepping (newton.rs:373-459).  Retries at one level restart from the
+  // same solution and are therefore identical replays of the first attempt;
+  // the first attempt decides the level.
+  if (status != SPX_CONVERGED && cfg->gmin_stepping) {
+    fl |= SPX_FLAG_GMIN_STEPPING;
+    std::vector<double> sol(n, 0.0), trial(n);
+    double g = cfg->gmin_initial;
+    bool ok = true;
+    int last = 0;
+    while (g >= cfg->gmin_final * 0.99) {
+      last = orc_newton(c, params, cfg, g, 1.0, sol.data(), trial.data(), &it);
+      total += it;
+      if (last != 1) { ok = false; break; }
+      sol = trial;
+      final_gmin = g;
+      if (g <= cfg->gmin_final) break;
+      g /= cfg->gmin_factor;
+      if (g < cfg->gmin_final) g = cfg->gmin_final;
+    }
+    if (ok) {
+      std::memcpy(x, sol.data(), sizeof(double) * n);
+      status = SPX_CONVERGED;
+    } else {
+      status = last < 0 ? SPX_SINGULAR : SPX_FAILED;
+      final_gmin = 0.0;
+    }
+  }
+
+  // Source stepping (newton.rs:261-356), same replay remark.
+  if (status != SPX_CONVERGED && cfg->source_stepping) {
+    fl |= SPX_FLAG_SOURCE_STEPPING;
+    std::vector<double> sol(n, 0.0), trial(n);
+    double factor = cfg->src_initial, step = cfg->src_step;
+    bool ok = true;
+    int last = 1;
+    while (factor <= 1.0 + 1e-9) {
+      double cur = std::fmin(factor, 1.0);
+      last = orc_newton(c, params, cfg, 0.0, cur, sol.data(), trial.data(), &it);
+      total += it;
+      if (last != 1) {
+        if (step > cfg->src_min_step) {
+          step /= 2.0;
+          factor -= step;
+          continue;
+        }
+        ok = false;
+        break;
+      }
+      sol = trial;
+      if (std::fabs(cur - 1.0) < 1e-9) break;
+      factor += step;
+      if (step < cfg->src_step) step = std::fmin(step * 1.5, cfg->src_step);
+    }
+    if (ok) {
+      std::memcpy(x, sol.data(), sizeof(double) * n);
+      status = SPX_CONVERGED;
+      final_gmin = 0.0;
+    } else {
+      status = last < 0 ? SPX_SINGULAR : SPX_FAILED;
+    }
+  }
+
+  double res = orc_kcl_residual(c, params, cfg, x, final_gmin);
+  if (status == SPX_CONVERGED && !(res <= 1.0)) fl |= SPX_FLAG_KCL_FAIL;
+  if (status == SPX_SINGULAR)
+    for (int i = 0; i < n; ++i) x[i] = 0.0;
+  if (iterations) *iterations = total;
+  if (flags) *flags = fl;
+  if (residual) *residual = res;
+  return status;
+}
+
+namespace {
+void run_pool(int64_t count, int nthreads, const std::function<void(int64_t)>& body) {
+  if (nthreads <= 1) {
+    for (int64_t i = 0; i < count; ++i) body(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; ++t)
+    th.emplace_back([&] {
+      for (;;) {
+        int64_t i = next.fetch_add(16);
+        if (i >= count) break;
+        int64_t e = std::min<int64_t>(count, i + 16);
+        for (; i < e; ++i) body(i);
+      }
+    });
+  for (auto& t : th) t.join();
+}
+}  // namespace
+
+extern "C" int orc_sweep_dcop_params(const spx_circuit* c, const double* params, int64_t B,
+                                     const spx_nr_config* cfg, const double* x0,
+                                     spx_sweep_result* out, int32_t nthreads) {
+  const int n = orc_circuit_size(c);
+  const int P = c->num_params;
+  run_pool(B, nthreads, [&](int64_t i) {
+    int32_t it = 0, fl = 0;
+    double res = 0.0;
+    int st = orc_dcop(c, params + i * P, cfg,
+                      (cfg->warm_start != SPX_WARM_ZERO && x0) ? x0 + i * n : nullptr,
+                      out->x + i * n, &it, &fl, &res);
+    if (out->status) out->status[i] = st;
+    if (out->iterations) out->iterations[i] = it;
+    if (out->flags) out->flags[i] = fl;
+    if (out->residual) out->residual[i] = res;
+  });
+  if (out->node_stats) {
+    std::vector<uint8_t> mask(B);
+    for (int64_t i = 0; i < B; ++i)
+      mask[i] = out->status ? out->status[i] == SPX_CONVERGED : 1;
+    orc_stats_accumulate(out->x, mask.data(), B, n, out->node_stats);
+  }
+  return SPX_OK;
+}
+
+extern "C" int orc_sweep_dcop(const spx_circuit* c, const spx_sweep_spec* spec,
+                              const spx_nr_config* cfg, const double* x0, spx_sweep_result* out,
+                              int32_t nthreads) {
+  const int P = c->num_params;
+  std::vector<double> local;
+  double* params = out->params;
+  if (!params) {
+    local.resize((size_t)spec->batch * P);
+    params = local.data();
+  }
+  int rc = orc_generate_params(spec, P, params);
+  if (rc) return rc;
+  return orc_sweep_dcop_params(c, params, spec->batch, cfg, x0, out, nthreads);
+}
+
+// ----------------------------------------------------------- statistics.rs
+extern "C" void orc_stats_accumulate(const double* v, const uint8_t* mask, int64_t B, int32_t n,
+                                     double* out) {
+  for (int j = 0; j < n; ++j) {
+    double cnt = 0, sum = 0, sq = 0, mn = INFINITY, mx = -INFINITY;
+    for (int64_t i = 0; i < B; ++i) {
+      if (mask && !mask[i]) continue;
+      double x = v[i * n + j];
+      cnt += 1;
+      sum += x;
+      sq += x * x;
+      mn = std::fmin(mn, x);
+      mx = std::fmax(mx, x);
+    }
+    out[j * 5 + 0] = cnt;
+    out[j * 5 + 1] = sum;
+    out[j * 5 + 2] = sq;
+    out[j * 5 + 3] = mn;
+    out[j * 5 + 4] = mx;
+  }
+}
+
+extern "C" void orc_stats_from_moments(int64_t count, double sum, double sum_sq, double* o) {
+  double mean = count > 0 ? sum / (double)count : 0.0;  // statistics.rs:54-74
+  double var = count > 0 ? (sum_sq / (double)count) - mean * mean : 0.0;
+  var = std::fmax(var, 0.0);
+  o[0] = mean;
+  o[1] = var;
+  o[2] = std::sqrt(var);
+}
+
+extern "C" void orc_sweep_statistics(const double* s, int64_t count, double* o) {
+  // spicier-solver/src/sweep.rs:109-136 (two-pass).
+  if (count == 0) { o[0] = o[1] = o[2] = o[3] = 0.0; return; }
+  double sum = 0.0;
+  for (int64_t i = 0; i < count; ++i) sum += s[i];
+  double mean = sum / (double)count, acc = 0.0;
+  for (int64_t i = 0; i < count; ++i) acc += (s[i] - mean) * (s[i] - mean);
+  double mn = INFINITY, mx = -INFINITY;
+  for (int64_t i = 0; i < count; ++i) { mn = std::fmin(mn, s[i]); mx = std::fmax(mx, s[i]); }
+  o[0] = mean;
+  o[1] = std::sqrt(acc / (double)count);
+  o[2] = mn;
+  o[3] = mx;
+}

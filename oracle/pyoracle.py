@@ -1,0 +1,148 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg — never by the engine.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from spicier_amd import _abi as A
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    L = C.CDLL(LIB)
+    u64, u32, i32, i64, d, vp = C.c_uint64, C.c_uint32, C.c_int32, C.c_int64, C.c_double, C.c_void_p
+    sig = {
+        "orc_splitmix64": (u64, [u64]),
+        "orc_combine_indices": (u64, [u64, u32, u32]),
+        "orc_uniform": (d, [u64, u32, u32]),
+        "orc_uniform_f32": (C.c_float, [u64, u32, u32]),
+        "orc_gaussian": (d, [u64, u32, u32]),
+        "orc_gaussian_f32": (C.c_float, [u64, u32, u32]),
+        "orc_gaussian_scaled": (d, [u64, u32, u32, d, d]),
+        "orc_generate_gaussian_parameters": (None, [u64, i64, i32, vp, vp, vp]),
+        "orc_monte_carlo_lcg": (None, [u64, i64, i32, vp, vp, vp, vp, vp]),
+        "orc_generate_params": (C.c_int, [C.POINTER(A.SpxSweepSpec), i32, vp]),
+        "orc_lu_solve": (C.c_int, [i32, vp, vp, vp]),
+        "orc_solve_batch": (C.c_int, [vp, vp, i64, i64, vp, vp, C.POINTER(C.c_int64)]),
+        "orc_thermal_voltage": (d, [d]),
+        "orc_diode_limit": (d, [d, d]),
+        "orc_diode_evaluate": (None, [d, d, d, i32, C.POINTER(d), C.POINTER(d)]),
+        "orc_mosfet_evaluate": (None, [i32, d, d, d, d, d, d, d, C.POINTER(d), C.POINTER(d),
+                                       C.POINTER(d), C.POINTER(i32)]),
+        "orc_circuit_size": (i32, [C.POINTER(A.SpxCircuit)]),
+        "orc_stamp": (None, [C.POINTER(A.SpxCircuit), vp, vp, d, d, i32, vp, vp]),
+        "orc_newton": (C.c_int, [C.POINTER(A.SpxCircuit), vp, C.POINTER(A.SpxNrConfig), d, d,
+                                 vp, vp, C.POINTER(i32)]),
+        "orc_kcl_residual": (d, [C.POINTER(A.SpxCircuit), vp, C.POINTER(A.SpxNrConfig), vp, d]),
+        "orc_dcop": (C.c_int, [C.POINTER(A.SpxCircuit), vp, C.POINTER(A.SpxNrConfig), vp, vp,
+                               C.POINTER(i32), C.POINTER(i32), C.POINTER(d)]),
+        "orc_sweep_dcop": (C.c_int, [C.POINTER(A.SpxCircuit), C.POINTER(A.SpxSweepSpec),
+                                     C.POINTER(A.SpxNrConfig), vp, C.POINTER(A.SpxSweepResult),
+                                     i32]),
+        "orc_sweep_dcop_params": (C.c_int, [C.POINTER(A.SpxCircuit), vp, i64,
+                                            C.POINTER(A.SpxNrConfig), vp,
+                                            C.POINTER(A.SpxSweepResult), i32]),
+        "orc_stats_accumulate": (None, [vp, vp, i64, i32, vp]),
+        "orc_stats_from_moments": (None, [i64, d, d, vp]),
+        "orc_sweep_statistics": (None, [vp, i64, vp]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(L, k)
+        f.restype = r
+        f.argtypes = a
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def lu_solve(a, b):
+    a = np.asfortranarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(len(b))
+    s = lib().orc_lu_solve(len(b), _p(a), _p(b), _p(x))
+    return x, bool(s)
+
+
+def solve_batch(matrices, rhs, n, batch):
+    m = np.ascontiguousarray(matrices, dtype=np.float64)
+    r = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.zeros(n * batch)
+    sing = np.zeros(batch, dtype=np.uint8)
+    ns = C.c_int64(0)
+    rc = lib().orc_solve_batch(_p(m), _p(r), n, batch, _p(x), _p(sing), C.byref(ns))
+    assert rc == 0
+    return x, sing, ns.value
+
+
+def generate_params(circuit, batch, seed=42, generator=A.SPX_GEN_HASH, sweep_offset=0):
+    spec, keep = circuit.sweep_spec(batch, seed, generator, sweep_offset)
+    out = np.zeros((batch, circuit.num_params))
+    rc = lib().orc_generate_params(C.byref(spec), circuit.num_params, _p(out))
+    if rc:
+        raise ValueError(f"orc_generate_params failed: {rc}")
+    return out
+
+
+def stamp(circuit, params, x, gmin=0.0, source_factor=1.0, limiter=A.SPX_LIMIT_REFERENCE):
+    ckt, keep = circuit.to_c()
+    n = circuit.size
+    J = np.zeros((n, n), order="F")
+    rhs = np.zeros(n)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lib().orc_stamp(C.byref(ckt), _p(params), _p(x), gmin, source_factor, limiter, _p(J), _p(rhs))
+    return J, rhs
+
+
+def sweep_dcop(circuit, params, cfg=None, x0=None, nthreads=1):
+    """Per-instance DC-OP over params [B][P]; returns dict of arrays."""
+    cfg = cfg or A.default_nr_config()
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    B = params.shape[0]
+    n = circuit.size
+    ckt, keep = circuit.to_c()
+    out = dict(x=np.zeros((B, n)), status=np.zeros(B, np.int32),
+               iterations=np.zeros(B, np.int32), flags=np.zeros(B, np.int32),
+               residual=np.zeros(B), node_stats=np.zeros((n, 5)))
+    res = A.SpxSweepResult(x=_p(out["x"]), status=_p(out["status"]),
+                           iterations=_p(out["iterations"]), flags=_p(out["flags"]),
+                           residual=_p(out["residual"]), params=None,
+                           node_stats=_p(out["node_stats"]))
+    if x0 is not None:
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    rc = lib().orc_sweep_dcop_params(C.byref(ckt), _p(params), B, C.byref(cfg), _p(x0),
+                                     C.byref(res), nthreads)
+    assert rc == 0
+    return out
+
+
+def stats_accumulate(values, mask=None):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    B, n = v.shape
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    out = np.zeros((n, 5))
+    lib().orc_stats_accumulate(_p(v), _p(m), B, n, _p(out))
+    return out

@@ -1,0 +1,157 @@
+"""ctypes mirror of include/spicier_hip.h and the loader of libspicier_hip.so.
+
+The structures here are the boundary format only; all computation happens in
+the HIP library.  ``load_library()`` raises if the library is missing — there
+is no CPU fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspicier_hip.so")
+
+# spx_status (error.rs:7-46)
+SPX_OK = 0
+SPX_ERR_INVALID_DIMENSION = 1
+SPX_ERR_BATCH_TOO_LARGE = 2
+SPX_ERR_MEMORY_ALLOC = 3
+SPX_ERR_TRANSFER = 4
+SPX_ERR_BACKEND_INIT = 5
+SPX_ERR_BACKEND = 6
+SPX_ERR_NO_BACKEND = 7
+SPX_ERR_INVALID_ARGUMENT = 8
+
+# spx_conv_status (convergence.rs:41-50)
+SPX_ACTIVE, SPX_CONVERGED, SPX_FAILED, SPX_SINGULAR = 0, 1, 2, 3
+
+SPX_FLAG_KCL_FAIL = 1
+SPX_FLAG_GMIN_STEPPING = 2
+SPX_FLAG_SOURCE_STEPPING = 4
+SPX_FLAG_FP64_ESCALATED = 8
+SPX_FLAG_WARM_START = 16
+
+# element kinds
+SPX_EL_RESISTOR, SPX_EL_CAPACITOR, SPX_EL_VSOURCE, SPX_EL_ISOURCE = 0, 1, 2, 3
+SPX_EL_DIODE, SPX_EL_NMOS, SPX_EL_PMOS, SPX_EL_VPULSE = 4, 5, 6, 7
+SPX_MAX_ELEMENT_PARAMS = 7
+
+# parameter kinds
+SPX_P_CONST, SPX_P_GAUSS_REL, SPX_P_GAUSS, SPX_P_UNIFORM, SPX_P_LOGUNIFORM = 0, 1, 2, 3, 4
+
+# generators
+SPX_GEN_HASH, SPX_GEN_LCG, SPX_GEN_CORNER, SPX_GEN_LINEAR, SPX_GEN_EXPLICIT = 0, 1, 2, 3, 4
+
+SPX_PREC_F64, SPX_PREC_F32_IR = 0, 1
+SPX_LIMIT_REFERENCE, SPX_LIMIT_NONE, SPX_LIMIT_PNJLIM = 0, 1, 2
+SPX_WARM_ZERO, SPX_WARM_GIVEN, SPX_WARM_SURROGATE = 0, 1, 2
+
+
+class SpxElement(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("node", C.c_int32 * 3),
+                ("param", C.c_int32 * SPX_MAX_ELEMENT_PARAMS)]
+
+
+class SpxCircuit(C.Structure):
+    _fields_ = [("num_nodes", C.c_int32), ("num_elements", C.c_int32),
+                ("elements", C.POINTER(SpxElement)), ("num_params", C.c_int32)]
+
+
+class SpxParamSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("rng_index", C.c_int32),
+                ("a", C.c_double), ("b", C.c_double), ("c", C.c_double), ("d", C.c_double)]
+
+
+class SpxSweepSpec(C.Structure):
+    _fields_ = [("generator", C.c_int32), ("seed", C.c_uint64), ("batch", C.c_int64),
+                ("sweep_offset", C.c_int64), ("params", C.POINTER(SpxParamSpec)),
+                ("explicit_params", C.c_void_p)]
+
+
+class SpxNrConfig(C.Structure):
+    _fields_ = [
+        ("v_abstol", C.c_double), ("v_reltol", C.c_double), ("i_abstol", C.c_double),
+        ("max_iterations", C.c_int32),
+        ("gmin_stepping", C.c_int32), ("gmin_initial", C.c_double), ("gmin_final", C.c_double),
+        ("gmin_factor", C.c_double), ("gmin_max_attempts", C.c_int32),
+        ("source_stepping", C.c_int32), ("src_initial", C.c_double), ("src_step", C.c_double),
+        ("src_min_step", C.c_double), ("src_max_attempts", C.c_int32),
+        ("precision", C.c_int32), ("limiter", C.c_int32), ("warm_start", C.c_int32),
+        ("ir_max_steps", C.c_int32), ("ir_tol", C.c_double),
+        ("kcl_reltol", C.c_double), ("kcl_abstol", C.c_double), ("kcl_vntol", C.c_double),
+    ]
+
+
+class SpxSweepResult(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("status", C.c_void_p), ("iterations", C.c_void_p),
+                ("flags", C.c_void_p), ("residual", C.c_void_p), ("params", C.c_void_p),
+                ("node_stats", C.c_void_p)]
+
+
+def default_nr_config(**overrides) -> SpxNrConfig:
+    """Defaults of ConvergenceCriteria / GminSteppingParams / SourceSteppingParams
+    (crates/spicier-solver/src/newton.rs:24-34,185-194,222-231)."""
+    cfg = SpxNrConfig(
+        v_abstol=1e-6, v_reltol=1e-3, i_abstol=1e-12, max_iterations=50,
+        gmin_stepping=1, gmin_initial=1e-3, gmin_final=1e-12, gmin_factor=10.0,
+        gmin_max_attempts=5,
+        source_stepping=1, src_initial=0.1, src_step=0.1, src_min_step=0.01, src_max_attempts=5,
+        precision=SPX_PREC_F64, limiter=SPX_LIMIT_REFERENCE, warm_start=SPX_WARM_ZERO,
+        ir_max_steps=8, ir_tol=1e-13,
+        kcl_reltol=1e-3, kcl_abstol=1e-12, kcl_vntol=1e-6)
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+_lib = None
+
+
+def load_library():
+    """Load libspicier_hip.so (built in-tree by ``__graft_entry__.build()``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build the HIP engine first (python -c "
+            "'import __graft_entry__; __graft_entry__.build()')")
+    lib = C.CDLL(LIB_PATH)
+    vp, i32, i64, u8p = C.c_void_p, C.c_int32, C.c_int64, C.c_void_p
+    sigs = {
+        "spx_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+        "spx_create": (C.c_int, [i32, C.POINTER(C.c_void_p)]),
+        "spx_destroy": (C.c_int, [vp]),
+        "spx_last_error": (C.c_char_p, []),
+        "spx_synchronize": (C.c_int, [vp]),
+        "spx_stream": (C.c_void_p, [vp]),
+        "spx_solve_batch_f64": (C.c_int, [vp, vp, vp, i64, i64, vp, u8p, C.POINTER(C.c_int64)]),
+        "spx_solve_batch_f64_device": (C.c_int, [vp, vp, vp, i64, i64, vp, u8p]),
+        "spx_max_batch_per_launch": (C.c_int64, []),
+        "spx_sample_params": (C.c_int, [vp, C.POINTER(SpxSweepSpec), i32, vp]),
+        "spx_sample_params_device": (C.c_int, [vp, C.POINTER(SpxSweepSpec), i32, vp]),
+        "spx_nr_config_default": (None, [C.POINTER(SpxNrConfig)]),
+        "spx_sweep_dcop": (C.c_int, [vp, C.POINTER(SpxCircuit), C.POINTER(SpxSweepSpec),
+                                     C.POINTER(SpxNrConfig), vp, C.POINTER(SpxSweepResult)]),
+        "spx_sweep_dcop_device": (C.c_int, [vp, C.POINTER(SpxCircuit), C.POINTER(SpxSweepSpec),
+                                            C.POINTER(SpxNrConfig), vp,
+                                            C.POINTER(SpxSweepResult)]),
+        "spx_circuit_size": (C.c_int32, [C.POINTER(SpxCircuit)]),
+        "spx_stats_reduce_device": (C.c_int, [vp, vp, vp, i64, i32, vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a) -> C.c_void_p:
+    """Raw data pointer of a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return C.c_void_p(a.data_ptr())
+    return C.c_void_p(a.ctypes.data)

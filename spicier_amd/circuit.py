@@ -1,0 +1,315 @@
+"""Circuit programs for the sweep engine, and the sweep workloads' netlists.
+
+A :class:`Circuit` is the host-side builder of ``spx_circuit`` (the
+device-resident circuit program of ``include/spicier_hip.h``): a list of
+elements whose values live in per-instance *parameter slots*, each slot with a
+distribution (``spx_param_spec``).  It plays the part the reference's
+``SweepStamperFactory`` closures play (crates/spicier-solver/src/sweep.rs:272-326),
+in a form that can cross a C ABI and be evaluated on the GPU.
+
+Element stamping follows spicier-core's MNA conventions
+(crates/spicier-core/src/mna.rs:87-169); node 0 is ground and node k>0 is
+unknown k-1, voltage sources add branch unknowns after the nodes in order of
+appearance (crates/spicier-core/src/netlist.rs:243-326).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as A
+
+
+@dataclass(frozen=True)
+class Param:
+    """Handle of a parameter slot."""
+    slot: int
+
+
+@dataclass
+class Circuit:
+    name: str = "circuit"
+    elements: list = field(default_factory=list)
+    specs: list = field(default_factory=list)   # (kind, rng_index, a, b, c, d)
+    node_names: dict = field(default_factory=lambda: {"0": 0})
+    slot_names: list = field(default_factory=list)
+
+    # ---------------------------------------------------------------- nodes
+    def node(self, name) -> int:
+        if isinstance(name, int):
+            return name
+        if name in ("0", "gnd", "GND"):
+            return 0
+        if name not in self.node_names:
+            self.node_names[name] = len(self.node_names)
+        return self.node_names[name]
+
+    @property
+    def num_nodes(self) -> int:
+        return len(self.node_names) - 1
+
+    @property
+    def num_vsources(self) -> int:
+        return sum(e[0] in (A.SPX_EL_VSOURCE, A.SPX_EL_VPULSE) for e in self.elements)
+
+    @property
+    def size(self) -> int:
+        return self.num_nodes + self.num_vsources
+
+    @property
+    def num_params(self) -> int:
+        return len(self.specs)
+
+    @property
+    def nonlinear(self) -> bool:
+        return any(e[0] in (A.SPX_EL_DIODE, A.SPX_EL_NMOS, A.SPX_EL_PMOS) for e in self.elements)
+
+    # ---------------------------------------------------------------- slots
+    def _slot(self, kind, a, b=0.0, c=0.0, d=0.0, name=None) -> Param:
+        s = len(self.specs)
+        self.specs.append((kind, s, float(a), float(b), float(c), float(d)))
+        self.slot_names.append(name or f"p{s}")
+        return Param(s)
+
+    def const(self, v, name=None) -> Param:
+        return self._slot(A.SPX_P_CONST, v, name=name)
+
+    def gauss_rel(self, nominal, sigma=0.05, vmin=None, vmax=None, name=None) -> Param:
+        """ParameterVariation semantics: (nom + z*sigma*nom).clamp(min, max), default
+        bounds +-10% (crates/spicier-solver/src/sweep.rs:31-40,193)."""
+        vmin = nominal * 0.9 if vmin is None else vmin
+        vmax = nominal * 1.1 if vmax is None else vmax
+        return self._slot(A.SPX_P_GAUSS_REL, nominal, sigma, vmin, vmax, name)
+
+    def gauss(self, mean, sigma, name=None) -> Param:
+        """mean + z*sigma (rng.rs:148-150); corner/linear sweeps use mean -+ 3 sigma."""
+        return self._slot(A.SPX_P_GAUSS, mean, sigma, mean - 3 * sigma, mean + 3 * sigma, name)
+
+    def uniform(self, lo, hi, name=None) -> Param:
+        return self._slot(A.SPX_P_UNIFORM, lo, hi, lo, hi, name)
+
+    def loguniform(self, lo, hi, name=None) -> Param:
+        return self._slot(A.SPX_P_LOGUNIFORM, lo, hi, lo, hi, name)
+
+    def _p(self, v) -> int:
+        if isinstance(v, Param):
+            return v.slot
+        return self.const(v).slot
+
+    # ------------------------------------------------------------- elements
+    def _add(self, kind, nodes, params):
+        n = [self.node(x) for x in nodes] + [0] * (3 - len(nodes))
+        p = [self._p(v) for v in params] + [-1] * (A.SPX_MAX_ELEMENT_PARAMS - len(params))
+        self.elements.append((kind, n, p))
+
+    def resistor(self, a, b, r):
+        self._add(A.SPX_EL_RESISTOR, (a, b), (r,))
+
+    def capacitor(self, a, b, c):
+        self._add(A.SPX_EL_CAPACITOR, (a, b), (c,))
+
+    def vsource(self, pos, neg, v):
+        self._add(A.SPX_EL_VSOURCE, (pos, neg), (v,))
+
+    def vpulse(self, pos, neg, v1, v2, td, tr, tf, pw, per):
+        self._add(A.SPX_EL_VPULSE, (pos, neg), (v1, v2, td, tr, tf, pw, per))
+
+    def isource(self, frm, to, i):
+        """Current i flows from `frm` through the source into `to`."""
+        self._add(A.SPX_EL_ISOURCE, (frm, to), (i,))
+
+    def diode(self, anode, cathode, is_=1e-14, n=1.0):
+        self._add(A.SPX_EL_DIODE, (anode, cathode), (is_, n))
+
+    def nmos(self, d, g, s, vto=0.7, kp=2e-5, lam=0.0, w=10e-6, l=1e-6):
+        self._add(A.SPX_EL_NMOS, (d, g, s), (vto, kp, lam, w, l))
+
+    def pmos(self, d, g, s, vto=-0.7, kp=1e-5, lam=0.0, w=10e-6, l=1e-6):
+        self._add(A.SPX_EL_PMOS, (d, g, s), (vto, kp, lam, w, l))
+
+    # ------------------------------------------------------------ C structs
+    def to_c(self):
+        """Returns (SpxCircuit, keepalive) — keep `keepalive` referenced while
+        the struct is in use."""
+        arr = (A.SpxElement * max(1, len(self.elements)))()
+        for i, (k, n, p) in enumerate(self.elements):
+            arr[i].kind = k
+            for j in range(3):
+                arr[i].node[j] = n[j]
+            for j in range(A.SPX_MAX_ELEMENT_PARAMS):
+                arr[i].param[j] = p[j]
+        ckt = A.SpxCircuit(num_nodes=self.num_nodes, num_elements=len(self.elements),
+                           elements=C.cast(arr, C.POINTER(A.SpxElement)),
+                           num_params=self.num_params)
+        return ckt, arr
+
+    def param_specs_c(self):
+        arr = (A.SpxParamSpec * max(1, len(self.specs)))()
+        for i, (k, r, a, b, c, d) in enumerate(self.specs):
+            arr[i] = A.SpxParamSpec(kind=k, rng_index=r, a=a, b=b, c=c, d=d)
+        return arr
+
+    def sweep_spec(self, batch, seed=42, generator=A.SPX_GEN_HASH, sweep_offset=0,
+                   explicit_params=None):
+        """Returns (SpxSweepSpec, keepalive)."""
+        specs = self.param_specs_c()
+        sp = A.SpxSweepSpec(generator=generator, seed=seed, batch=batch,
+                            sweep_offset=sweep_offset,
+                            params=C.cast(specs, C.POINTER(A.SpxParamSpec)),
+                            explicit_params=A.ptr(explicit_params))
+        return sp, (specs, explicit_params)
+
+    def varied_slots(self):
+        return [i for i, s in enumerate(self.specs) if s[0] != A.SPX_P_CONST]
+
+
+# --------------------------------------------------------------------------
+# Workload netlists
+# --------------------------------------------------------------------------
+
+def cs_amp(uniform_r=True) -> Circuit:
+    """Common-source NMOS amplifier, 4 nodes + 1 branch = 5 unknowns.
+
+    VDD=5 V; R1 100k VDD-G; R2 47k G-0; RD 10k VDD-D; RS 1k S-0; M1 D G S with
+    VTO~N(0.7,0.02) V, KP~N(110e-6, 5%) A/V^2, LAMBDA=0.02, W/L=10; resistors
+    uniform +-1% (the 1,024-instance Monte Carlo case).  Topology as the
+    reference's NMOS common-source validation circuit
+    (crates/spicier-parser/tests/ngspice_validation.rs:834-842).
+    With ``uniform_r=False`` every varied quantity is a ParameterVariation
+    (Gaussian relative), the only form MonteCarloGenerator draws.
+    """
+    c = Circuit("cs_amp")
+    rvals = (("R1", 100e3), ("R2", 47e3), ("RD", 10e3), ("RS", 1e3))
+    if uniform_r:
+        vto = c.gauss(0.7, 0.02, name="VTO")
+        kp = c.gauss(110e-6, 0.05 * 110e-6, name="KP")
+        r = {k: c.uniform(0.99 * v, 1.01 * v, name=k) for k, v in rvals}
+    else:
+        vto = c.gauss_rel(0.7, 0.02 / 0.7, name="VTO")
+        kp = c.gauss_rel(110e-6, 0.05, name="KP")
+        r = {k: c.gauss_rel(v, 0.01 / 3, name=k) for k, v in rvals}
+    c.vsource("vdd", "0", 5.0)
+    c.resistor("vdd", "g", r["R1"])
+    c.resistor("g", "0", r["R2"])
+    c.resistor("vdd", "d", r["RD"])
+    c.resistor("s", "0", r["RS"])
+    c.nmos("d", "g", "s", vto=vto, kp=kp, lam=0.02, w=10e-6, l=1e-6)
+    return c
+
+
+INV_STAGES = 12
+INV_RC_NODES = 16
+INV_VDD = 1.8
+
+
+def inverter_chain() -> Circuit:
+    """12-stage self-biased CMOS inverter chain with a 16-node RC load: 64 unknowns.
+
+    Stage k (k = 1..12) has nodes g_k (gate), o_k (output), n_k (NMOS source)
+    and p_k (PMOS source):
+      * NMOS  d=o_k g=g_k s=n_k, W/L = 10, KP_N in group (k-1)//3;
+      * PMOS  d=o_k g=g_k s=p_k, W/L = 22, KP_P in group (k-1)//3;
+      * R_S = 1k from n_k to ground; p_k reaches the ideal 1.8 V rail through
+        R_S = 1k, written as its Norton equivalent (1k to ground plus
+        1.8 mA injected into p_k), so the rail adds no unknown;
+      * feedback R_F = 100k o_k-g_k biases the stage at its trip point
+        (midrail); coupling R_C = 100k o_{k-1}-g_k, stage 1 driven through
+        R_C from an ideal 0.9 V (Norton again).
+    RC load: r_1..r_16 in a ladder from o_12, R_j~U(1,10) kOhm, C_j~U(1,10) fF
+    to ground (open at DC).
+    Varied: VTO_N[12]~N(0.7,0.03), VTO_P[12]~N(-0.7,0.03), KP_N[4]~N(110e-6,5%),
+    KP_P[4]~N(50e-6,5%) — slots 0..31, the 32 DC-relevant parameters — then
+    R_j (slots 32..47) and C_j (48..63).  LAMBDA = 0.02.
+    """
+    c = Circuit("inverter_chain")
+    vton = [c.gauss(0.7, 0.03, name=f"VTO_N{k}") for k in range(INV_STAGES)]
+    vtop = [c.gauss(-0.7, 0.03, name=f"VTO_P{k}") for k in range(INV_STAGES)]
+    kpn = [c.gauss(110e-6, 0.05 * 110e-6, name=f"KP_N{g}") for g in range(4)]
+    kpp = [c.gauss(50e-6, 0.05 * 50e-6, name=f"KP_P{g}") for g in range(4)]
+    rl = [c.uniform(1e3, 10e3, name=f"RL{j}") for j in range(INV_RC_NODES)]
+    cl = [c.uniform(1e-15, 10e-15, name=f"CL{j}") for j in range(INV_RC_NODES)]
+    rs, rf, rc = c.const(1e3, "RS"), c.const(100e3, "RF"), c.const(100e3, "RC")
+    i_rail = c.const(INV_VDD / 1e3, "I_RAIL")
+    i_in = c.const(0.9 / 100e3, "I_IN")
+    lam = c.const(0.02, "LAMBDA")
+    wn, wp, l = c.const(10e-6, "WN"), c.const(22e-6, "WP"), c.const(1e-6, "L")
+    # node numbering: stage k -> g, o, n, p; then the RC ladder
+    for k in range(1, INV_STAGES + 1):
+        for t in "gonp":
+            c.node(f"{t}{k}")
+    for j in range(1, INV_RC_NODES + 1):
+        c.node(f"r{j}")
+    for k in range(1, INV_STAGES + 1):
+        grp = (k - 1) // 3
+        c.nmos(f"o{k}", f"g{k}", f"n{k}", vto=vton[k - 1], kp=kpn[grp], lam=lam, w=wn, l=l)
+        c.pmos(f"o{k}", f"g{k}", f"p{k}", vto=vtop[k - 1], kp=kpp[grp], lam=lam, w=wp, l=l)
+        c.resistor(f"n{k}", "0", rs)
+        c.resistor(f"p{k}", "0", rs)
+        c.isource("0", f"p{k}", i_rail)
+        c.resistor(f"o{k}", f"g{k}", rf)
+        if k == 1:
+            c.resistor("g1", "0", rc)
+            c.isource("0", "g1", i_in)
+        else:
+            c.resistor(f"o{k - 1}", f"g{k}", rc)
+    prev = f"o{INV_STAGES}"
+    for j in range(1, INV_RC_NODES + 1):
+        c.resistor(prev, f"r{j}", rl[j - 1])
+        c.capacitor(f"r{j}", "0", cl[j - 1])
+        prev = f"r{j}"
+    assert c.size == 64
+    return c
+
+
+def diode_resistor(v=5.0, r=1e3, is_=1e-14, n=1.0) -> Circuit:
+    """V - R - D to ground (the reference's 5 V / 1 kOhm diode check,
+    crates/spicier-parser/tests/ngspice_validation.rs:705-768)."""
+    c = Circuit("diode_resistor")
+    c.vsource("in", "0", v)
+    c.resistor("in", "a", r)
+    c.diode("a", "0", is_, n)
+    return c
+
+
+def voltage_divider(r1=1e3, r2=1e3, v=10.0, sigma=0.05) -> Circuit:
+    """The reference's SimpleDividerFactory sweep fixture
+    (crates/spicier-batched-sweep/src/sweep.rs:236-284): V - R1 - out - R2 - 0,
+    R2 varied as a ParameterVariation."""
+    c = Circuit("voltage_divider")
+    r2p = c.gauss_rel(r2, sigma, name="R2")
+    c.vsource("in", "0", v)
+    c.resistor("in", "out", r1)
+    c.resistor("out", "0", r2p)
+    return c
+
+
+def nmos_cs_validation() -> Circuit:
+    """NMOS common-source circuit of ngspice_validation.rs:834-890 (VDD 5 V,
+    RD 1k, VG 2 V, default-ish L1 NMOS); the reference asserts V(d) in (3,5)."""
+    c = Circuit("nmos_cs_validation")
+    c.vsource("vdd", "0", 5.0)
+    c.vsource("g", "0", 2.0)
+    c.resistor("vdd", "d", 1e3)
+    c.nmos("d", "g", "0", vto=0.7, kp=2e-5, lam=0.0, w=10e-6, l=1e-6)
+    return c
+
+
+def nominal_values(circuit: Circuit) -> np.ndarray:
+    """Nominal value of every slot: a for constants and Gaussians, the
+    midpoint for uniform slots, the geometric mean for log-uniform ones."""
+    out = []
+    for kind, _, a, b, _, _ in circuit.specs:
+        if kind == A.SPX_P_UNIFORM:
+            out.append(0.5 * (a + b))
+        elif kind == A.SPX_P_LOGUNIFORM:
+            out.append(float(np.sqrt(a * b)))
+        else:
+            out.append(a)
+    return np.array(out, dtype=np.float64)
+
+
+def instance_params(circuit: Circuit, batch: int) -> np.ndarray:
+    """Parameter matrix [batch][P] with every slot at its nominal value."""
+    return np.tile(nominal_values(circuit), (batch, 1))
