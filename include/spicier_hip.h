@@ -246,6 +246,8 @@ typedef struct spx_sweep_result {
     double* params;       /* [B][P] sampled parameters */
     double* node_stats;   /* [n][5] count,sum,sumsq,min,max over converged instances
                              (StatisticsAccumulator, statistics.rs:135-154) */
+    int32_t* factorizations; /* [B] LU factorizations performed (FP64 + FP32) */
+    int32_t* refinements;    /* [B] FP64-residual refinement steps performed (FP32+IR) */
 } spx_sweep_result;
 
 /* Monte Carlo / corner / linear DC-operating-point sweep (the nonlinear

@@ -528,75 +528,101 @@ extern "C" double orc_kcl_residual(const spx_circuit* c, const double* params,
   return worst;
 }
 
+namespace {
+
+bool has_nonlinear(const spx_circuit* c) {
+  for (int e = 0; e < c->num_elements; ++e) {
+    int k = c->elements[e].kind;
+    if (k == SPX_EL_DIODE || k == SPX_EL_NMOS || k == SPX_EL_PMOS) return true;
+  }
+  return false;
+}
+
+}  // namespace
+
 extern "C" int orc_dcop(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
                         const double* x0, double* x, int32_t* iterations, int32_t* flags,
                         double* residual) {
   const int n = orc_circuit_size(c);
   int32_t total = 0, it = 0, fl = (cfg->warm_start != SPX_WARM_ZERO && x0) ? SPX_FLAG_WARM_START : 0;
   double final_gmin = 0.0;
-  int st = orc_newton(c, params, cfg, 0.0, 1.0, x0, x, &it);
-  total += it;
-  int status = st == 1 ? SPX_CONVERGED : (st < 0 ? SPX_SINGULAR : SPX_FAILED);
+  int status;
 
-  // Gmin stepping (newton.rs:373-459).  Retries at one level restart from the
-  // same solution and are therefore identical replays of the first attempt;
-  // the first attempt decides the level.
-  if (status != SPX_CONVERGED && cfg->gmin_stepping) {
-    fl |= SPX_FLAG_GMIN_STEPPING;
-    std::vector<double> sol(n, 0.0), trial(n);
-    double g = cfg->gmin_initial;
-    bool ok = true;
-    int last = 0;
-    while (g >= cfg->gmin_final * 0.99) {
-      last = orc_newton(c, params, cfg, g, 1.0, sol.data(), trial.data(), &it);
-      total += it;
-      if (last != 1) { ok = false; break; }
-      sol = trial;
-      final_gmin = g;
-      if (g <= cfg->gmin_final) break;
-      g /= cfg->gmin_factor;
-      if (g < cfg->gmin_final) g = cfg->gmin_final;
-    }
-    if (ok) {
-      std::memcpy(x, sol.data(), sizeof(double) * n);
-      status = SPX_CONVERGED;
-    } else {
-      status = last < 0 ? SPX_SINGULAR : SPX_FAILED;
-      final_gmin = 0.0;
-    }
-  }
+  if (!has_nonlinear(c)) {
+    // A linear circuit takes the reference sweep's single solve per point
+    // (spicier-batched-sweep/src/sweep.rs:159-217).
+    std::vector<double> J((size_t)n * n), rhs(n);
+    std::vector<double> zero(n, 0.0);
+    stamp_impl(c, params, zero.data(), 0.0, 1.0, cfg->limiter, J.data(), rhs.data(), nullptr,
+               nullptr);
+    status = orc_lu_solve(n, J.data(), rhs.data(), x) ? SPX_SINGULAR : SPX_CONVERGED;
+    total = 1;
+  } else {
+    int st = orc_newton(c, params, cfg, 0.0, 1.0, x0, x, &it);
+    total += it;
+    status = st == 1 ? SPX_CONVERGED : (st < 0 ? SPX_SINGULAR : SPX_FAILED);
 
-  // Source stepping (newton.rs:261-356), same replay remark.
-  if (status != SPX_CONVERGED && cfg->source_stepping) {
-    fl |= SPX_FLAG_SOURCE_STEPPING;
-    std::vector<double> sol(n, 0.0), trial(n);
-    double factor = cfg->src_initial, step = cfg->src_step;
-    bool ok = true;
-    int last = 1;
-    while (factor <= 1.0 + 1e-9) {
-      double cur = std::fmin(factor, 1.0);
-      last = orc_newton(c, params, cfg, 0.0, cur, sol.data(), trial.data(), &it);
-      total += it;
-      if (last != 1) {
-        if (step > cfg->src_min_step) {
-          step /= 2.0;
-          factor -= step;
-          continue;
-        }
-        ok = false;
-        break;
+    // Gmin stepping (newton.rs:373-459).  Retries at one level restart from the
+    // same solution and are therefore identical replays of the first attempt;
+    // the first attempt decides the level.  On failure the stage's solution
+    // (last converged level, zeros if none) is what it returns.
+    if (status != SPX_CONVERGED && cfg->gmin_stepping) {
+      fl |= SPX_FLAG_GMIN_STEPPING;
+      std::vector<double> sol(n, 0.0), trial(n);
+      double g = cfg->gmin_initial;
+      bool ok = true;
+      int last = 0;
+      while (g >= cfg->gmin_final * 0.99) {
+        last = orc_newton(c, params, cfg, g, 1.0, sol.data(), trial.data(), &it);
+        total += it;
+        if (last != 1) { ok = false; break; }
+        sol = trial;
+        final_gmin = g;
+        if (g <= cfg->gmin_final) break;
+        g /= cfg->gmin_factor;
+        if (g < cfg->gmin_final) g = cfg->gmin_final;
       }
-      sol = trial;
-      if (std::fabs(cur - 1.0) < 1e-9) break;
-      factor += step;
-      if (step < cfg->src_step) step = std::fmin(step * 1.5, cfg->src_step);
-    }
-    if (ok) {
       std::memcpy(x, sol.data(), sizeof(double) * n);
-      status = SPX_CONVERGED;
-      final_gmin = 0.0;
-    } else {
-      status = last < 0 ? SPX_SINGULAR : SPX_FAILED;
+      if (ok) {
+        status = SPX_CONVERGED;
+      } else {
+        status = last < 0 ? SPX_SINGULAR : SPX_FAILED;
+        final_gmin = 0.0;
+      }
+    }
+
+    // Source stepping (newton.rs:261-356), same replay remark.
+    if (status != SPX_CONVERGED && cfg->source_stepping) {
+      fl |= SPX_FLAG_SOURCE_STEPPING;
+      std::vector<double> sol(n, 0.0), trial(n);
+      double factor = cfg->src_initial, step = cfg->src_step;
+      bool ok = true;
+      int last = 1;
+      while (factor <= 1.0 + 1e-9) {
+        double cur = std::fmin(factor, 1.0);
+        last = orc_newton(c, params, cfg, 0.0, cur, sol.data(), trial.data(), &it);
+        total += it;
+        if (last != 1) {
+          if (step > cfg->src_min_step) {
+            step /= 2.0;
+            factor -= step;
+            continue;
+          }
+          ok = false;
+          break;
+        }
+        sol = trial;
+        if (std::fabs(cur - 1.0) < 1e-9) break;
+        factor += step;
+        if (step < cfg->src_step) step = std::fmin(step * 1.5, cfg->src_step);
+      }
+      std::memcpy(x, sol.data(), sizeof(double) * n);
+      if (ok) {
+        status = SPX_CONVERGED;
+        final_gmin = 0.0;
+      } else {
+        status = last < 0 ? SPX_SINGULAR : SPX_FAILED;
+      }
     }
   }
 

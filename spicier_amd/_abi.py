@@ -86,7 +86,8 @@ class SpxNrConfig(C.Structure):
 class SpxSweepResult(C.Structure):
     _fields_ = [("x", C.c_void_p), ("status", C.c_void_p), ("iterations", C.c_void_p),
                 ("flags", C.c_void_p), ("residual", C.c_void_p), ("params", C.c_void_p),
-                ("node_stats", C.c_void_p)]
+                ("node_stats", C.c_void_p), ("factorizations", C.c_void_p),
+                ("refinements", C.c_void_p)]
 
 
 def default_nr_config(**overrides) -> SpxNrConfig:

@@ -1,0 +1,704 @@
+// host_api.hip — the C ABI of libspicier_hip.so (include/spicier_hip.h).
+//
+// A handle owns one HIP stream on one device and a grow-only set of device
+// workspaces; calls on a handle are serialized by its mutex.  Host-pointer
+// entry points stage through the workspaces and synchronize; "_device"
+// entry points only enqueue on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/spicier_hip.h"
+#include "program.h"
+
+namespace spx {
+hipError_t launch_newton(const DevProgram& g, const DevNrCfg& cfg, const double* params,
+                         const double* x0, int64_t B, const DevOut& out, hipStream_t st);
+hipError_t launch_sample(const spx_param_spec* d_specs, int P, int gen, uint64_t seed, int64_t B,
+                         int64_t offset, int nvar, double* out, hipStream_t st);
+hipError_t launch_stats(const double* v, const uint8_t* mask, const int* status, int64_t B, int n,
+                        double* part, double* out, hipStream_t st);
+int64_t stats_num_blocks(int64_t B);
+size_t newton_smem_bytes(const DevProgram& g, int N);
+hipError_t launch_solve_batch(const double* M, const double* rhs, int n, int64_t B, double* x,
+                              uint8_t* sing, double* work, int64_t chunk, hipStream_t st);
+size_t solve_generic_workspace(int n, int64_t chunk);
+}  // namespace spx
+
+using namespace spx;
+
+struct spx_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::map<std::string, std::pair<void*, size_t>> ws;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define SPX_HIP(call, code)                                                               \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(code, std::string(#call) + ": " + hipGetErrorString(e_));               \
+  } while (0)
+
+// Grow-only named device buffer.
+int workspace(spx_handle* h, const char* name, size_t bytes, void** out) {
+  auto& slot = h->ws[name];
+  if (slot.second < bytes) {
+    if (slot.first) hipFree(slot.first);
+    slot = {nullptr, 0};
+    size_t want = std::max<size_t>(bytes, 256);
+    if (hipMalloc(&slot.first, want) != hipSuccess)
+      return fail(SPX_ERR_MEMORY_ALLOC, std::string("hipMalloc of ") + std::to_string(want) +
+                                            " bytes for " + name + " failed");
+    slot.second = want;
+  }
+  *out = slot.first;
+  return SPX_OK;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    hipGetDevice(&prev);
+    if (prev != dev) hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    hipGetDevice(&cur);
+    if (prev >= 0 && cur != prev) hipSetDevice(prev);
+  }
+};
+
+// ------------------------------------------------------------- circuit compile
+struct HostProgram {
+  int n = 0, nn = 0, P = 0, num_elems = 0, num_vals = 2, has_nonlinear = 0;
+  std::vector<int> elems, order, pos_rc, pos_off, contrib, rhs_off, rhs_contrib;
+  int max_row_nnz = 0;
+};
+
+int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n) {
+  if (!c || c->num_nodes < 0 || c->num_elements < 0 || c->num_params < 0 ||
+      (c->num_elements > 0 && !c->elements))
+    return fail(SPX_ERR_INVALID_ARGUMENT, "invalid circuit description");
+  const int nn = c->num_nodes, P = c->num_params;
+  int nv = 0;
+  for (int e = 0; e < c->num_elements; ++e)
+    nv += c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE;
+  const int n = nn + nv;
+  if (n < 1) return fail(SPX_ERR_INVALID_DIMENSION, "circuit has no unknowns");
+  if (n > max_n)
+    return fail(SPX_ERR_INVALID_DIMENSION,
+                "circuit has " + std::to_string(n) + " unknowns; the dense sweep path takes <= " +
+                    std::to_string(max_n));
+  hp.n = n;
+  hp.nn = nn;
+  hp.P = P;
+  hp.num_elems = c->num_elements;
+  std::map<std::pair<int, int>, std::vector<int>> pos;
+  std::vector<std::vector<int>> rhs(n);
+  auto add = [&](int r, int col, int vid, bool neg) { pos[{r, col}].push_back(vid << 1 | neg); };
+  auto cond = [&](int i, int j, int vid) {  // mna.rs:87-98
+    if (i >= 0) add(i, i, vid, false);
+    if (j >= 0) add(j, j, vid, false);
+    if (i >= 0 && j >= 0) { add(i, j, vid, true); add(j, i, vid, true); }
+  };
+  auto cur = [&](int i, int j, int vid) {  // mna.rs:103-115
+    if (i >= 0) rhs[i].push_back(vid << 1 | 1);
+    if (j >= 0) rhs[j].push_back(vid << 1);
+  };
+  int branch = 0;
+  for (int e = 0; e < c->num_elements; ++e) {
+    const spx_element& el = c->elements[e];
+    int u[3];
+    for (int k = 0; k < 3; ++k) {
+      if (el.node[k] < 0 || el.node[k] > nn)
+        return fail(SPX_ERR_INVALID_ARGUMENT, "element " + std::to_string(e) +
+                                                  " references node " + std::to_string(el.node[k]));
+      u[k] = el.node[k] - 1;
+    }
+    int need = 0;
+    switch (el.kind) {
+      case SPX_EL_RESISTOR: case SPX_EL_CAPACITOR: case SPX_EL_VSOURCE: case SPX_EL_ISOURCE:
+        need = 1; break;
+      case SPX_EL_DIODE: need = 2; break;
+      case SPX_EL_NMOS: case SPX_EL_PMOS: need = 5; break;
+      case SPX_EL_VPULSE: need = 7; break;
+      default:
+        return fail(SPX_ERR_INVALID_ARGUMENT, "element " + std::to_string(e) + " has unknown kind");
+    }
+    for (int k = 0; k < need; ++k)
+      if (el.param[k] < 0 || el.param[k] >= P)
+        return fail(SPX_ERR_INVALID_ARGUMENT, "element " + std::to_string(e) +
+                                                  " parameter slot out of range");
+    int vid = -1;
+    switch (el.kind) {
+      case SPX_EL_RESISTOR:
+        vid = hp.num_vals++;
+        cond(u[0], u[1], vid);
+        break;
+      case SPX_EL_CAPACITOR:
+        break;  // open at DC
+      case SPX_EL_VSOURCE:
+      case SPX_EL_VPULSE: {  // mna.rs:124-145
+        vid = hp.num_vals++;
+        const int row = nn + branch++;
+        if (u[0] >= 0) { add(u[0], row, kValOne, false); add(row, u[0], kValOne, false); }
+        if (u[1] >= 0) { add(u[1], row, kValOne, true); add(row, u[1], kValOne, true); }
+        rhs[row].clear();  // rhs[row] = voltage (assignment)
+        rhs[row].push_back(vid << 1);
+        break;
+      }
+      case SPX_EL_ISOURCE:
+        vid = hp.num_vals++;
+        cur(u[0], u[1], vid);
+        break;
+      case SPX_EL_DIODE:
+        vid = hp.num_vals;
+        hp.num_vals += 2;
+        cond(u[0], u[1], vid);
+        cur(u[0], u[1], vid + 1);
+        hp.has_nonlinear = 1;
+        break;
+      case SPX_EL_NMOS:
+      case SPX_EL_PMOS: {  // level1.rs:209-247
+        vid = hp.num_vals;
+        hp.num_vals += 3;
+        const int d = u[0], g = u[1], s = u[2];
+        cond(d, s, vid);
+        if (d >= 0) {
+          if (g >= 0) add(d, g, vid + 1, false);
+          if (s >= 0) add(d, s, vid + 1, true);
+        }
+        if (s >= 0) {
+          if (g >= 0) add(s, g, vid + 1, true);
+          add(s, s, vid + 1, false);
+        }
+        cur(d, s, vid + 2);
+        hp.has_nonlinear = 1;
+        break;
+      }
+    }
+    int rec[kElemInts] = {el.kind, u[0], u[1], u[2], el.param[0], el.param[1], el.param[2],
+                          el.param[3], el.param[4], el.param[5], el.param[6], vid};
+    hp.elems.insert(hp.elems.end(), rec, rec + kElemInts);
+  }
+  for (int i = 0; i < nn; ++i) add(i, i, kValGmin, false);  // GminStamper, newton.rs:393-398
+  std::vector<int> rownnz(n, 0);
+  hp.pos_off.push_back(0);
+  for (auto& kv : pos) {
+    hp.pos_rc.push_back(kv.first.first | (kv.first.second << 16));
+    hp.contrib.insert(hp.contrib.end(), kv.second.begin(), kv.second.end());
+    hp.pos_off.push_back((int)hp.contrib.size());
+    rownnz[kv.first.first]++;
+  }
+  hp.max_row_nnz = *std::max_element(rownnz.begin(), rownnz.end());
+  hp.rhs_off.push_back(0);
+  for (int r = 0; r < n; ++r) {
+    hp.rhs_contrib.insert(hp.rhs_contrib.end(), rhs[r].begin(), rhs[r].end());
+    hp.rhs_off.push_back((int)hp.rhs_contrib.size());
+  }
+  // evaluation order: nonlinear devices first, grouped by kind
+  auto rank = [](int k) {
+    switch (k) {
+      case SPX_EL_NMOS: case SPX_EL_PMOS: return 0;
+      case SPX_EL_DIODE: return 1;
+      case SPX_EL_RESISTOR: return 2;
+      default: return 3;
+    }
+  };
+  hp.order.resize(c->num_elements);
+  for (int e = 0; e < c->num_elements; ++e) hp.order[e] = e;
+  std::stable_sort(hp.order.begin(), hp.order.end(), [&](int a, int b) {
+    return rank(c->elements[a].kind) < rank(c->elements[b].kind);
+  });
+  return SPX_OK;
+}
+
+// Uploads the program into the handle's "program" workspace.
+int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g) {
+  std::vector<int> blob;
+  auto put = [&](const std::vector<int>& v) {
+    size_t off = blob.size();
+    blob.insert(blob.end(), v.begin(), v.end());
+    while (blob.size() % 4) blob.push_back(0);
+    return off;
+  };
+  size_t o_el = put(hp.elems), o_or = put(hp.order), o_rc = put(hp.pos_rc),
+         o_po = put(hp.pos_off), o_co = put(hp.contrib), o_ro = put(hp.rhs_off),
+         o_rh = put(hp.rhs_contrib);
+  void* d = nullptr;
+  int rc = workspace(h, "program", blob.size() * sizeof(int) + 16, &d);
+  if (rc) return rc;
+  SPX_HIP(hipMemcpyAsync(d, blob.data(), blob.size() * sizeof(int), hipMemcpyHostToDevice,
+                         h->stream),
+          SPX_ERR_TRANSFER);
+  // the host blob dies at return: wait for the copy
+  SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+  const int* base = static_cast<const int*>(d);
+  g.n = hp.n;
+  g.num_nodes = hp.nn;
+  g.P = hp.P;
+  g.num_elems = hp.num_elems;
+  g.num_vals = hp.num_vals;
+  g.num_pos = (int)hp.pos_rc.size();
+  g.has_nonlinear = hp.has_nonlinear;
+  g.max_row_nnz = hp.max_row_nnz;
+  g.elems = base + o_el;
+  g.order = base + o_or;
+  g.pos_rc = base + o_rc;
+  g.pos_off = base + o_po;
+  g.contrib = base + o_co;
+  g.rhs_off = base + o_ro;
+  g.rhs_contrib = base + o_rh;
+  return SPX_OK;
+}
+
+int count_varied(const spx_param_spec* ps, int P) {
+  int v = 0;
+  for (int p = 0; p < P; ++p) v += ps[p].kind != SPX_P_CONST;
+  return v;
+}
+
+int check_spec(const spx_sweep_spec* spec, int P) {
+  if (!spec) return fail(SPX_ERR_INVALID_ARGUMENT, "null sweep spec");
+  if (spec->batch < 0) return fail(SPX_ERR_INVALID_DIMENSION, "negative batch");
+  if (spec->batch > 0x7fffffffLL)
+    return fail(SPX_ERR_BATCH_TOO_LARGE, "batch " + std::to_string(spec->batch) + " exceeds " +
+                                             std::to_string(0x7fffffffLL));
+  if (P > 0 && !spec->params && spec->generator != SPX_GEN_EXPLICIT)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "null parameter specs");
+  if (spec->generator < SPX_GEN_HASH || spec->generator > SPX_GEN_EXPLICIT)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "unknown generator");
+  if (spec->generator == SPX_GEN_EXPLICIT && P > 0 && spec->batch > 0 && !spec->explicit_params)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "explicit generator without parameters");
+  if (spec->generator == SPX_GEN_LCG)
+    for (int p = 0; p < P; ++p)
+      if (spec->params[p].kind != SPX_P_CONST && spec->params[p].kind != SPX_P_GAUSS_REL)
+        return fail(SPX_ERR_INVALID_ARGUMENT,
+                    "the LCG generator draws ParameterVariation (Gaussian-relative) slots only");
+  if (spec->generator == SPX_GEN_CORNER && P > 0) {
+    int nv = count_varied(spec->params, P);
+    if (nv >= 62 || spec->batch != (int64_t(1) << nv))
+      return fail(SPX_ERR_INVALID_DIMENSION, "corner sweep needs batch == 2^(varied slots)");
+  }
+  return SPX_OK;
+}
+
+// Enqueue parameter generation into d_out [B][P].
+int sample_device(spx_handle* h, const spx_sweep_spec* spec, int P, double* d_out) {
+  const int64_t B = spec->batch;
+  if (B == 0 || P == 0) return SPX_OK;
+  if (spec->generator == SPX_GEN_EXPLICIT) {
+    SPX_HIP(hipMemcpyAsync(d_out, spec->explicit_params, sizeof(double) * B * P,
+                           hipMemcpyDeviceToDevice, h->stream),
+            SPX_ERR_TRANSFER);
+    return SPX_OK;
+  }
+  void* d_specs = nullptr;
+  int rc = workspace(h, "specs", sizeof(spx_param_spec) * P, &d_specs);
+  if (rc) return rc;
+  SPX_HIP(hipMemcpyAsync(d_specs, spec->params, sizeof(spx_param_spec) * P,
+                         hipMemcpyHostToDevice, h->stream),
+          SPX_ERR_TRANSFER);
+  SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+  SPX_HIP(launch_sample(static_cast<spx_param_spec*>(d_specs), P, spec->generator, spec->seed, B,
+                        spec->sweep_offset, count_varied(spec->params, P), d_out, h->stream),
+          SPX_ERR_BACKEND);
+  return SPX_OK;
+}
+
+DevNrCfg to_dev(const spx_nr_config* c) {
+  DevNrCfg d;
+  d.v_abstol = c->v_abstol;
+  d.v_reltol = c->v_reltol;
+  d.i_abstol = c->i_abstol;
+  d.max_iterations = c->max_iterations;
+  d.gmin_stepping = c->gmin_stepping;
+  d.gmin_initial = c->gmin_initial;
+  d.gmin_final = c->gmin_final;
+  d.gmin_factor = c->gmin_factor;
+  d.source_stepping = c->source_stepping;
+  d.src_initial = c->src_initial;
+  d.src_step = c->src_step;
+  d.src_min_step = c->src_min_step;
+  d.precision = c->precision;
+  d.limiter = c->limiter;
+  d.warm = c->warm_start;
+  d.ir_max_steps = c->ir_max_steps;
+  d.ir_tol = c->ir_tol;
+  d.kcl_reltol = c->kcl_reltol;
+  d.kcl_abstol = c->kcl_abstol;
+  d.kcl_vntol = c->kcl_vntol;
+  return d;
+}
+
+int check_cfg(const spx_nr_config* c) {
+  if (!c) return fail(SPX_ERR_INVALID_ARGUMENT, "null Newton configuration");
+  if (c->max_iterations < 1 || c->max_iterations > 100000)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "max_iterations must be 1..100000");
+  if (c->precision != SPX_PREC_F64 && c->precision != SPX_PREC_F32_IR)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "unknown precision");
+  if (c->limiter < SPX_LIMIT_REFERENCE || c->limiter > SPX_LIMIT_PNJLIM)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "unknown limiter");
+  if (c->warm_start != SPX_WARM_ZERO && c->warm_start != SPX_WARM_GIVEN)
+    return fail(SPX_ERR_INVALID_ARGUMENT,
+                "warm start must be zero or given here (the surrogate feeds x0 through "
+                "spx_sweep_dcop_device)");
+  if (c->precision == SPX_PREC_F32_IR && (c->ir_max_steps < 1 || c->ir_max_steps > 1000))
+    return fail(SPX_ERR_INVALID_ARGUMENT, "ir_max_steps must be 1..1000");
+  if (c->gmin_stepping && !(c->gmin_factor > 1.0 && c->gmin_final > 0.0))
+    return fail(SPX_ERR_INVALID_ARGUMENT, "Gmin stepping needs factor > 1 and final > 0");
+  if (c->source_stepping && !(c->src_min_step > 0.0 && c->src_step > 0.0))
+    return fail(SPX_ERR_INVALID_ARGUMENT, "source stepping needs positive steps");
+  return SPX_OK;
+}
+
+int sweep_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
+                        const spx_nr_config* cfg, const double* d_x0, spx_sweep_result* out) {
+  int rc;
+  if (!out || !out->x) return fail(SPX_ERR_INVALID_ARGUMENT, "result.x is required");
+  if ((rc = check_cfg(cfg))) return rc;
+  HostProgram hp;
+  if ((rc = compile_circuit(ckt, hp, 64))) return rc;
+  if ((rc = check_spec(spec, hp.P))) return rc;
+  if (cfg->warm_start == SPX_WARM_GIVEN && !d_x0)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "warm start given without x0");
+  const int64_t B = spec->batch;
+  if (B == 0) return SPX_OK;
+  DevProgram g;
+  if ((rc = upload_program(h, hp, g))) return rc;
+  double* params = out->params;
+  if (!params) {
+    void* p = nullptr;
+    if ((rc = workspace(h, "params", sizeof(double) * std::max<int64_t>(1, B * hp.P), &p)))
+      return rc;
+    params = static_cast<double*>(p);
+  }
+  if ((rc = sample_device(h, spec, hp.P, params))) return rc;
+  int* status = out->status;
+  if (!status && out->node_stats) {
+    void* p = nullptr;
+    if ((rc = workspace(h, "status", sizeof(int) * B, &p))) return rc;
+    status = static_cast<int*>(p);
+  }
+  DevOut o{out->x, status, out->iterations, out->flags, out->residual, out->factorizations,
+           out->refinements};
+  DevNrCfg dc = to_dev(cfg);
+  const double* x0 = cfg->warm_start == SPX_WARM_GIVEN ? d_x0 : nullptr;
+  SPX_HIP(launch_newton(g, dc, params, x0, B, o, h->stream), SPX_ERR_BACKEND);
+  if (out->node_stats) {
+    void* part = nullptr;
+    if ((rc = workspace(h, "stats_part", sizeof(double) * 5 * hp.n * stats_num_blocks(B), &part)))
+      return rc;
+    SPX_HIP(launch_stats(out->x, nullptr, status, B, hp.n, static_cast<double*>(part),
+                         out->node_stats, h->stream),
+            SPX_ERR_BACKEND);
+  }
+  return SPX_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+int spx_device_count(int32_t* count) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    if (count) *count = 0;
+    return fail(SPX_ERR_NO_BACKEND, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+  }
+  if (count) *count = c;
+  return SPX_OK;
+}
+
+int spx_create(int32_t device_id, spx_handle** out) {
+  if (!out) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle pointer");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return fail(SPX_ERR_NO_BACKEND, "no HIP device available");
+  if (device_id < 0 || device_id >= count)
+    return fail(SPX_ERR_BACKEND_INIT, "device " + std::to_string(device_id) + " out of range");
+  DeviceGuard dg(device_id);
+  auto* h = new spx_handle;
+  h->device = device_id;
+  e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(SPX_ERR_BACKEND_INIT, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = h;
+  return SPX_OK;
+}
+
+int spx_destroy(spx_handle* h) {
+  if (!h) return SPX_OK;
+  {
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    hipStreamSynchronize(h->stream);
+    for (auto& kv : h->ws)
+      if (kv.second.first) hipFree(kv.second.first);
+    hipStreamDestroy(h->stream);
+  }
+  delete h;
+  return SPX_OK;
+}
+
+const char* spx_last_error(void) { return g_err.c_str(); }
+
+int spx_synchronize(spx_handle* h) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  DeviceGuard dg(h->device);
+  SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_BACKEND);
+  return SPX_OK;
+}
+
+void* spx_stream(spx_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+int64_t spx_max_batch_per_launch(void) { return 0x7fffffffLL; }
+
+int spx_solve_batch_f64_device(spx_handle* h, const double* dM, const double* drhs, int64_t n,
+                               int64_t B, double* dx, uint8_t* dsing) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  if (B < 0 || n < 0) return fail(SPX_ERR_INVALID_DIMENSION, "negative size");
+  if (B == 0) return SPX_OK;
+  if (n < 1 || n > 4096)
+    return fail(SPX_ERR_INVALID_DIMENSION, "n must be 1..4096, got " + std::to_string(n));
+  if (B > spx_max_batch_per_launch())
+    return fail(SPX_ERR_BATCH_TOO_LARGE, "batch " + std::to_string(B) + " exceeds " +
+                                             std::to_string(spx_max_batch_per_launch()));
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  double* work = nullptr;
+  int64_t chunk = 1;
+  if (n > 64) {
+    chunk = std::max<int64_t>(1, std::min<int64_t>(B, (int64_t)(256ull << 20) /
+                                                          (int64_t)(sizeof(double) * n * n)));
+    void* w = nullptr;
+    int rc = workspace(h, "lu_generic", solve_generic_workspace((int)n, chunk), &w);
+    if (rc) return rc;
+    work = static_cast<double*>(w);
+  }
+  SPX_HIP(launch_solve_batch(dM, drhs, (int)n, B, dx, dsing, work, chunk, h->stream),
+          SPX_ERR_BACKEND);
+  return SPX_OK;
+}
+
+int spx_solve_batch_f64(spx_handle* h, const double* M, const double* rhs, int64_t n, int64_t B,
+                        double* x, uint8_t* sing, int64_t* num_singular) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  if (B < 0 || n < 0) return fail(SPX_ERR_INVALID_DIMENSION, "negative size");
+  if (num_singular) *num_singular = 0;
+  if (B == 0) return SPX_OK;  // batched_lu.rs:380-387
+  if (!M || !rhs || !x) return fail(SPX_ERR_INVALID_ARGUMENT, "null buffer");
+  if (n < 1 || n > 4096)
+    return fail(SPX_ERR_INVALID_DIMENSION, "n must be 1..4096, got " + std::to_string(n));
+  void *dM, *dr, *dx, *ds;
+  {
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    int rc;
+    if ((rc = workspace(h, "sb_M", sizeof(double) * B * n * n, &dM))) return rc;
+    if ((rc = workspace(h, "sb_rhs", sizeof(double) * B * n, &dr))) return rc;
+    if ((rc = workspace(h, "sb_x", sizeof(double) * B * n, &dx))) return rc;
+    if ((rc = workspace(h, "sb_sing", B, &ds))) return rc;
+    SPX_HIP(hipMemcpyAsync(dM, M, sizeof(double) * B * n * n, hipMemcpyHostToDevice, h->stream),
+            SPX_ERR_TRANSFER);
+    SPX_HIP(hipMemcpyAsync(dr, rhs, sizeof(double) * B * n, hipMemcpyHostToDevice, h->stream),
+            SPX_ERR_TRANSFER);
+  }
+  int rc = spx_solve_batch_f64_device(h, (double*)dM, (double*)dr, n, B, (double*)dx,
+                                      (uint8_t*)ds);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  std::vector<uint8_t> hs(B);
+  SPX_HIP(hipMemcpyAsync(x, dx, sizeof(double) * B * n, hipMemcpyDeviceToHost, h->stream),
+          SPX_ERR_TRANSFER);
+  SPX_HIP(hipMemcpyAsync(hs.data(), ds, B, hipMemcpyDeviceToHost, h->stream), SPX_ERR_TRANSFER);
+  SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+  int64_t ns = 0;
+  for (int64_t i = 0; i < B; ++i) ns += hs[i];
+  if (sing) std::memcpy(sing, hs.data(), B);
+  if (num_singular) *num_singular = ns;
+  return SPX_OK;
+}
+
+int spx_sample_params_device(spx_handle* h, const spx_sweep_spec* spec, int32_t P,
+                             double* d_params) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  int rc = check_spec(spec, P);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  return sample_device(h, spec, P, d_params);
+}
+
+int spx_sample_params(spx_handle* h, const spx_sweep_spec* spec, int32_t P, double* params) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  int rc = check_spec(spec, P);
+  if (rc) return rc;
+  const int64_t B = spec->batch;
+  if (B == 0 || P == 0) return SPX_OK;
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  void *d = nullptr, *de = nullptr;
+  if ((rc = workspace(h, "sp_out", sizeof(double) * B * P, &d))) return rc;
+  spx_sweep_spec local = *spec;
+  if (spec->generator == SPX_GEN_EXPLICIT) {
+    if ((rc = workspace(h, "sp_explicit", sizeof(double) * B * P, &de))) return rc;
+    SPX_HIP(hipMemcpyAsync(de, spec->explicit_params, sizeof(double) * B * P,
+                           hipMemcpyHostToDevice, h->stream),
+            SPX_ERR_TRANSFER);
+    local.explicit_params = static_cast<double*>(de);
+  }
+  if ((rc = sample_device(h, &local, P, static_cast<double*>(d)))) return rc;
+  SPX_HIP(hipMemcpyAsync(params, d, sizeof(double) * B * P, hipMemcpyDeviceToHost, h->stream),
+          SPX_ERR_TRANSFER);
+  SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+  return SPX_OK;
+}
+
+void spx_nr_config_default(spx_nr_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->v_abstol = 1e-6;
+  c->v_reltol = 1e-3;
+  c->i_abstol = 1e-12;
+  c->max_iterations = 50;
+  c->gmin_stepping = 1;
+  c->gmin_initial = 1e-3;
+  c->gmin_final = 1e-12;
+  c->gmin_factor = 10.0;
+  c->gmin_max_attempts = 5;
+  c->source_stepping = 1;
+  c->src_initial = 0.1;
+  c->src_step = 0.1;
+  c->src_min_step = 0.01;
+  c->src_max_attempts = 5;
+  c->precision = SPX_PREC_F64;
+  c->limiter = SPX_LIMIT_REFERENCE;
+  c->warm_start = SPX_WARM_ZERO;
+  c->ir_max_steps = 8;
+  c->ir_tol = 1e-13;
+  c->kcl_reltol = 1e-3;
+  c->kcl_abstol = 1e-12;
+  c->kcl_vntol = 1e-6;
+}
+
+int32_t spx_circuit_size(const spx_circuit* c) {
+  if (!c || !c->elements) return c ? c->num_nodes : -1;
+  int32_t nv = 0;
+  for (int e = 0; e < c->num_elements; ++e)
+    nv += c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE;
+  return c->num_nodes + nv;
+}
+
+int spx_sweep_dcop_device(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
+                          const spx_nr_config* cfg, const double* d_x0, spx_sweep_result* out) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  return sweep_device_locked(h, ckt, spec, cfg, d_x0, out);
+}
+
+int spx_sweep_dcop(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
+                   const spx_nr_config* cfg, const double* x0, spx_sweep_result* out) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  if (!out || !out->x) return fail(SPX_ERR_INVALID_ARGUMENT, "result.x is required");
+  int rc;
+  HostProgram hp;
+  if ((rc = compile_circuit(ckt, hp, 64))) return rc;
+  if ((rc = check_spec(spec, hp.P))) return rc;
+  if ((rc = check_cfg(cfg))) return rc;
+  const int64_t B = spec->batch;
+  if (B == 0) return SPX_OK;
+  const int n = hp.n, P = hp.P;
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  void *dx, *dst, *dit, *dfl, *dres, *dpar, *dstats = nullptr, *dx0 = nullptr, *dexp = nullptr;
+  void *dfac = nullptr, *dref = nullptr;
+  if ((rc = workspace(h, "sw_x", sizeof(double) * B * n, &dx))) return rc;
+  if ((rc = workspace(h, "sw_status", sizeof(int) * B, &dst))) return rc;
+  if ((rc = workspace(h, "sw_iter", sizeof(int) * B, &dit))) return rc;
+  if ((rc = workspace(h, "sw_flags", sizeof(int) * B, &dfl))) return rc;
+  if ((rc = workspace(h, "sw_res", sizeof(double) * B, &dres))) return rc;
+  if ((rc = workspace(h, "sw_params", sizeof(double) * std::max<int64_t>(1, B * P), &dpar)))
+    return rc;
+  if (out->node_stats && (rc = workspace(h, "sw_stats", sizeof(double) * 5 * n, &dstats)))
+    return rc;
+  if (out->factorizations && (rc = workspace(h, "sw_fac", sizeof(int) * B, &dfac))) return rc;
+  if (out->refinements && (rc = workspace(h, "sw_ref", sizeof(int) * B, &dref))) return rc;
+  if (cfg->warm_start == SPX_WARM_GIVEN) {
+    if (!x0) return fail(SPX_ERR_INVALID_ARGUMENT, "warm start given without x0");
+    if ((rc = workspace(h, "sw_x0", sizeof(double) * B * n, &dx0))) return rc;
+    SPX_HIP(hipMemcpyAsync(dx0, x0, sizeof(double) * B * n, hipMemcpyHostToDevice, h->stream),
+            SPX_ERR_TRANSFER);
+  }
+  spx_sweep_spec local = *spec;
+  if (spec->generator == SPX_GEN_EXPLICIT && P > 0) {
+    if ((rc = workspace(h, "sw_explicit", sizeof(double) * B * P, &dexp))) return rc;
+    SPX_HIP(hipMemcpyAsync(dexp, spec->explicit_params, sizeof(double) * B * P,
+                           hipMemcpyHostToDevice, h->stream),
+            SPX_ERR_TRANSFER);
+    local.explicit_params = static_cast<double*>(dexp);
+  }
+  spx_sweep_result dres_s{(double*)dx,   (int*)dst,      (int*)dit, (int*)dfl,
+                          (double*)dres, (double*)dpar,  (double*)dstats, (int*)dfac,
+                          (int*)dref};
+  if ((rc = sweep_device_locked(h, ckt, &local, cfg, (double*)dx0, &dres_s))) return rc;
+  auto d2h = [&](void* dst_h, const void* src_d, size_t bytes) -> int {
+    if (!dst_h) return SPX_OK;
+    SPX_HIP(hipMemcpyAsync(dst_h, src_d, bytes, hipMemcpyDeviceToHost, h->stream),
+            SPX_ERR_TRANSFER);
+    return SPX_OK;
+  };
+  if ((rc = d2h(out->x, dx, sizeof(double) * B * n))) return rc;
+  if ((rc = d2h(out->status, dst, sizeof(int) * B))) return rc;
+  if ((rc = d2h(out->iterations, dit, sizeof(int) * B))) return rc;
+  if ((rc = d2h(out->flags, dfl, sizeof(int) * B))) return rc;
+  if ((rc = d2h(out->residual, dres, sizeof(double) * B))) return rc;
+  if ((rc = d2h(out->params, dpar, sizeof(double) * B * P))) return rc;
+  if (out->node_stats && (rc = d2h(out->node_stats, dstats, sizeof(double) * 5 * n))) return rc;
+  if ((rc = d2h(out->factorizations, dfac, sizeof(int) * B))) return rc;
+  if ((rc = d2h(out->refinements, dref, sizeof(int) * B))) return rc;
+  SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+  return SPX_OK;
+}
+
+int spx_stats_reduce_device(spx_handle* h, const double* d_values, const uint8_t* d_mask,
+                            int64_t B, int32_t n, double* d_out) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  if (B < 0 || n < 1) return fail(SPX_ERR_INVALID_DIMENSION, "bad statistics shape");
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  void* part = nullptr;
+  int rc = workspace(h, "stats_part2", sizeof(double) * 5 * n * stats_num_blocks(B), &part);
+  if (rc) return rc;
+  SPX_HIP(launch_stats(d_values, d_mask, nullptr, B, n, static_cast<double*>(part), d_out,
+                       h->stream),
+          SPX_ERR_BACKEND);
+  return SPX_OK;
+}
+
+}  // extern "C"

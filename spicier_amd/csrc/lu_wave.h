@@ -1,0 +1,131 @@
+// lu_wave.h — partial-pivot LU of one n x n system (n <= N <= 64) held by one
+// 64-wide wavefront, row `lane` in lane `lane`'s registers.
+//
+// Layout: a[j] of lane r is A[r][j]; j is a compile-time index so the row
+// stays in VGPRs (N doubles = 2N VGPRs).  Rows are never moved: each lane
+// keeps `pos`, its row's current position in the partially pivoted matrix,
+// and a swap of positions k <-> p reproduces the physical row swap of the
+// reference's LU.  The pivot of step k is the largest |A[.][k]| over rows at
+// positions >= k, smallest position on ties — nalgebra's icamax rule (first
+// maximum), which CpuBatchedSolver inherits (spicier-batched-sweep
+// src/solver.rs:370-380).  The pivot row is broadcast to the wave with
+// v_readlane (one SGPR pair per element), multipliers are formed as
+// a * (1/pivot) and updates as fma(-u, l, a).
+//
+// Lanes >= n carry identity rows (a[j] = [j == lane]) so that padding never
+// wins a pivot search and solves to zero.
+#pragma once
+
+#include <type_traits>
+
+#include "device_common.h"
+
+namespace spx {
+
+// Compile-time loop: f(integral_constant<int, I>) for I in [Begin, End).  The
+// index is a constant expression in the body, so register arrays indexed by
+// it are split into scalars before any loop optimisation runs.
+template <int Begin, int End, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (Begin < End) {
+    f(std::integral_constant<int, Begin>{});
+    static_for<Begin + 1, End>(f);
+  }
+}
+
+// Same, counting down from Hi-1 to Lo.
+template <int Lo, int Hi, typename F>
+__device__ __forceinline__ void static_for_down(F&& f) {
+  if constexpr (Lo < Hi) {
+    f(std::integral_constant<int, Hi - 1>{});
+    static_for_down<Lo, Hi - 1>(f);
+  }
+}
+
+__device__ __forceinline__ double bcast(double v, int lane) { return readlane_f64(v, lane); }
+__device__ __forceinline__ float bcast(float v, int lane) { return readlane_f32(v, lane); }
+
+template <typename T>
+__device__ __forceinline__ T tabs(T v) { return v < T(0) ? -v : v; }
+
+template <int N, typename T>
+struct RowLU {
+  T a[N];
+  int pos;
+  int piv[N];  // lane holding the pivot row of step k (wave-uniform)
+  T dinv;      // 1 / U[pos][pos] of this lane's row
+
+  // Factor in place.  Returns false if a pivot column is zero (or NaN): the
+  // reference's LU then reports the system singular.
+  __device__ __forceinline__ bool factor(int n, int lane) {
+    pos = lane;
+    dinv = T(0);
+    bool ok = true;
+    static_for<0, N>([&](auto K) __attribute__((always_inline)) {
+      constexpr int k = K;
+      if (!ok || k >= n) return;
+      // argmax |a[k]| over rows at positions >= k, smallest position on ties
+      T v = pos >= k ? tabs(a[k]) : T(-1);
+      int bp = pos;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        T ov = __shfl_xor(v, off, kWave);
+        int op = __shfl_xor(bp, off, kWave);
+        bool take = (ov > v) || (ov == v && op < bp);
+        v = take ? ov : v;
+        bp = take ? op : bp;
+      }
+      bp = __builtin_amdgcn_readfirstlane(bp);
+      if (!(bcast(v, 0) > T(0))) { ok = false; return; }
+      uint64_t m = __ballot(pos == bp);
+      int p = __builtin_amdgcn_readfirstlane((int)__ffsll((unsigned long long)m) - 1);
+      piv[k] = p;
+      if (pos == k) pos = bp;  // row at position k moves to the pivot's position
+      if (lane == p) pos = k;
+      T pv = bcast(a[k], p);
+      T inv = T(1) / pv;
+      if (lane == p) dinv = inv;
+      T l = pos > k ? a[k] * inv : T(0);
+      if (pos > k) a[k] = l;
+      static_for<k + 1, N>([&](auto J) __attribute__((always_inline)) {
+        constexpr int j = J;
+        if (j < n) {
+          T u = bcast(a[j], p);
+          a[j] = fma(-u, l, a[j]);
+        }
+      });
+    });
+    return ok;
+  }
+
+  // Solve with the stored factors for right-hand side b (b of lane r is
+  // b[r]).  Returns x[lane] in each lane < n.  Forward substitution replays
+  // the elimination on b; back substitution divides by the pivot (exact
+  // division when `divide`, multiplication by the stored inverse otherwise).
+  template <bool divide>
+  __device__ __forceinline__ T solve(T b, int n, int lane) const {
+    static_for<0, N>([&](auto K) __attribute__((always_inline)) {
+      constexpr int k = K;
+      if (k < n) {
+        T bp = bcast(b, piv[k]);
+        T l = pos > k ? a[k] : T(0);
+        b = fma(-bp, l, b);
+      }
+    });
+    T x = T(0);
+    static_for_down<0, N>([&](auto K) __attribute__((always_inline)) {
+      constexpr int k = K;
+      if (k < n) {
+        int p = piv[k];
+        T bk = bcast(b, p);
+        T xk = divide ? bk / bcast(a[k], p) : bk * bcast(dinv, p);
+        if (lane == k) x = xk;
+        T c = pos < k ? xk : T(0);
+        b = fma(-c, a[k], b);
+      }
+    });
+    return x;
+  }
+};
+
+}  // namespace spx

@@ -1,0 +1,422 @@
+// newton_kernel.h — the fused Newton DC-operating-point kernel template,
+// instantiated per (system size bucket N, precision) in newton_*.hip.
+//
+// One 64-wide wavefront per circuit instance (one workgroup), so each
+// instance's convergence mask is its own control flow — a converged
+// instance's wave retires and the dispatcher backfills the CU with the next
+// instance.  Per Newton iteration, all on-chip:
+//   1. device evaluation: lanes over elements (diode / MOSFET companion
+//      models in FP64, node voltages read from LDS);
+//   2. assembly: lanes over the shared sparsity pattern, each summing its
+//      entry's contributions in the reference's triplet order into a dense
+//      LDS matrix (entries outside the pattern stay zero from the start);
+//   3. linear solve: row-per-lane LU in VGPRs (FP64), or FP32 factors reused
+//      across Newton iterations with FP64-residual iterative refinement;
+//   4. Δx convergence test (newton.rs:134-159) as a wave vote.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "lu_wave.h"
+#include "program.h"
+
+namespace spx {
+
+inline size_t newton_smem_doubles(const DevProgram& g, int N) {
+  return (size_t)N * (N + 2) + 3 * N + g.num_vals + g.P + g.num_elems;
+}
+
+template <int N, bool IR>
+struct Instance {
+  const DevProgram& g;
+  const DevNrCfg& cfg;
+  int lane;
+  double* J;      // N x (N+2) dense, row-major, stride N+2
+  double* rhs;    // N
+  double* xs;     // N current x
+  double* xw;     // N scratch x (IR iterate / saved level solution)
+  double* vals;   // num_vals
+  double* pv;     // P parameters
+  double* vst;    // num_elems pnjlim junction voltages
+  int nfact = 0;  // LU factorizations performed
+  int nref = 0;   // refinement steps performed
+  static constexpr int S = N + 2;
+
+  __device__ double X(int u) const { return u >= 0 ? xs[u] : 0.0; }
+
+  // device evaluation + pattern assembly at xs; returns true if a pnjlim
+  // limiter acted on any diode
+  __device__ __forceinline__ bool assemble(double gmin, double sf, int limiter) {
+    bool lim = false;
+    for (int i = lane; i < g.num_elems; i += kWave) {
+      const int e = g.order[i];
+      const int* el = g.elems + e * kElemInts;
+      const int kind = el[0], vid = el[11];
+      switch (kind) {
+        case SPX_EL_RESISTOR:
+          vals[vid] = 1.0 / pv[el[4]];
+          break;
+        case SPX_EL_VSOURCE:
+        case SPX_EL_VPULSE:
+        case SPX_EL_ISOURCE:
+          vals[vid] = pv[el[4]] * sf;
+          break;
+        case SPX_EL_DIODE: {
+          double vd = X(el[1]) - X(el[2]);
+          double gd, ieq;
+          diode_companion(pv[el[4]], pv[el[5]], vd, limiter, vst[e], lim, gd, ieq);
+          vals[vid] = gd;
+          vals[vid + 1] = ieq;
+          break;
+        }
+        case SPX_EL_NMOS:
+        case SPX_EL_PMOS: {
+          double vd = X(el[1]), vg = X(el[2]), vs = X(el[3]);
+          double gds, gm, ieq;
+          mosfet_companion(kind == SPX_EL_PMOS, pv[el[4]], pv[el[5]], pv[el[6]], pv[el[7]],
+                           pv[el[8]], vg - vs, vd - vs, gds, gm, ieq);
+          vals[vid] = gds;
+          vals[vid + 1] = gm;
+          vals[vid + 2] = ieq;
+          break;
+        }
+        default:
+          break;
+      }
+    }
+    if (lane == 0) vals[kValGmin] = gmin;
+    __syncthreads();
+    for (int q = lane; q < g.num_pos; q += kWave) {
+      double sum = 0.0;
+      const int c1 = g.pos_off[q + 1];
+      for (int c = g.pos_off[q]; c < c1; ++c) {
+        const int t = g.contrib[c];
+        const double v = vals[t >> 1];
+        sum = (t & 1) ? sum - v : sum + v;
+      }
+      const int rc = g.pos_rc[q];
+      J[(rc & 0xffff) * S + (rc >> 16)] = sum;
+    }
+    for (int r = lane; r < g.n; r += kWave) {
+      double sum = 0.0;
+      const int c1 = g.rhs_off[r + 1];
+      for (int c = g.rhs_off[r]; c < c1; ++c) {
+        const int t = g.rhs_contrib[c];
+        const double v = vals[t >> 1];
+        sum = (t & 1) ? sum - v : sum + v;
+      }
+      rhs[r] = sum;
+    }
+    __syncthreads();
+    return __any(lim);
+  }
+
+  template <typename T>
+  __device__ __forceinline__ void load_rows(T (&a)[N]) const {
+    const int n = g.n;
+    static_for<0, N>([&](auto JJ) __attribute__((always_inline)) {
+      constexpr int j = JJ;
+      a[j] = lane < n ? (T)J[lane * S + j] : (j == lane ? T(1) : T(0));
+    });
+  }
+
+  // x_new = J^{-1} rhs in FP64.  Returns false if singular.
+  __device__ __forceinline__ bool solve_f64(double& xnew) {
+    ++nfact;
+    RowLU<N, double> lu;
+    load_rows(lu.a);
+    if (!lu.factor(g.n, lane)) return false;
+    xnew = lu.template solve<true>(lane < g.n ? rhs[lane] : 0.0, g.n, lane);
+    return __all(lane >= g.n || isfinite(xnew));
+  }
+
+  // FP32 factors + FP64 iterative refinement, starting from the current x.
+  // `fresh` says the factors were computed from this J; returns 0 converged,
+  // 1 needs refactoring (stale factors did not converge), 2 failed.
+  __device__ __forceinline__ int refine(const RowLU<N, float>& lu, bool fresh, double& xnew) {
+    const int n = g.n;
+    const double* jrow = J + (lane < n ? lane : 0) * S;
+    const double b = lane < n ? rhs[lane] : 0.0;
+    double x = lane < n ? xs[lane] : 0.0;
+    const double floor_abs = lane < g.num_nodes ? cfg.v_abstol * 1e-9 : cfg.i_abstol * 1e-9;
+    double prev = INFINITY;
+    for (int s = 0; s < cfg.ir_max_steps; ++s) {
+      if (lane < n) xw[lane] = x;
+      __syncthreads();
+      double r = b;
+      if (lane < n)
+        for (int j = 0; j < n; ++j) r = fma(-jrow[j], xw[j], r);
+      __syncthreads();
+      float d = lu.template solve<false>((float)r, n, lane);
+      ++nref;
+      x += (double)d;
+      double ad = lane < n ? fabs((double)d) : 0.0;
+      bool ok = !(ad > cfg.ir_tol * fabs(x) + floor_abs);
+      if (__all(ok)) {
+        xnew = x;
+        return __all(lane >= n || isfinite(x)) ? 0 : 2;
+      }
+      double m = wave_max(ad);
+      if (!(m < 0.5 * prev) && s > 0) break;  // stagnating or diverging
+      prev = m;
+    }
+    return fresh ? 2 : 1;
+  }
+
+  // solve_newton_raphson at fixed gmin / source factor from xs; leaves the
+  // final x in xs.  Returns 1 converged, 0 not converged, -1 singular.
+  __device__ __forceinline__ int newton(double gmin, double sf, int& iters, int& fp64_escalations) {
+    const int n = g.n, nn = g.num_nodes;
+    for (int i = lane; i < g.num_elems; i += kWave) vst[i] = 0.0;
+    __syncthreads();
+    RowLU<N, float> lu32;
+    bool have32 = false;
+    for (int it = 0; it < cfg.max_iterations; ++it) {
+      const bool lim = assemble(gmin, sf, cfg.limiter);
+      double xnew = 0.0;
+      bool ok = true;
+      if constexpr (IR) {
+        int rc = 1;
+        for (int attempt = 0; attempt < 2 && rc == 1; ++attempt) {
+          bool fresh = !have32 || attempt > 0;
+          if (fresh) {
+            ++nfact;
+            load_rows(lu32.a);
+            have32 = lu32.factor(n, lane);
+            if (!have32) { rc = 2; break; }
+          }
+          rc = refine(lu32, fresh, xnew);
+        }
+        if (rc != 0) {  // FP32 cannot resolve this system: FP64 for this iteration
+          ++fp64_escalations;
+          have32 = false;
+          ok = solve_f64(xnew);
+        }
+      } else {
+        ok = solve_f64(xnew);
+      }
+      if (!ok) {
+        iters = it + 1;
+        return -1;
+      }
+      const double xo = lane < n ? xs[lane] : 0.0;
+      bool pass = true;
+      if (lane < n) {  // check_convergence, newton.rs:134-159
+        double delta = fabs(xnew - xo);
+        double tol = cfg.v_reltol * fmax(fabs(xnew), fabs(xo)) +
+                     (lane < nn ? cfg.v_abstol : cfg.i_abstol);
+        pass = !(delta > tol);
+      }
+      const bool conv = __all(pass) && !lim;
+      __syncthreads();
+      if (lane < n) xs[lane] = xnew;
+      __syncthreads();
+      if (conv) {
+        iters = it + 1;
+        return 1;
+      }
+    }
+    iters = cfg.max_iterations;
+    return 0;
+  }
+
+  __device__ __forceinline__ void save_x() {
+    if (lane < g.n) xw[lane] = xs[lane];
+    __syncthreads();
+  }
+  __device__ __forceinline__ void restore_x() {
+    if (lane < g.n) xs[lane] = xw[lane];
+    __syncthreads();
+  }
+  __device__ __forceinline__ void zero_x() {
+    if (lane < g.n) xs[lane] = 0.0;
+    __syncthreads();
+  }
+
+  // max_i |KCL_i| / tol_i at xs, devices unlimited (include/spicier_hip.h)
+  __device__ __forceinline__ double kcl(double gmin) {
+    assemble(gmin, 1.0, SPX_LIMIT_NONE);
+    double q = 0.0;
+    if (lane < g.n) {
+      double r = -rhs[lane], mag = fabs(rhs[lane]);
+      for (int j = 0; j < g.n; ++j) {
+        double t = J[lane * S + j] * xs[j];
+        r += t;
+        mag = fmax(mag, fabs(t));
+      }
+      double tol = cfg.kcl_reltol * mag + (lane < g.num_nodes ? cfg.kcl_abstol : cfg.kcl_vntol);
+      q = fabs(r) / tol;
+      if (!(q == q)) q = INFINITY;
+    }
+    return wave_max(q);
+  }
+};
+
+template <int N, bool IR>
+__global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
+                                                     const double* __restrict__ params,
+                                                     const double* __restrict__ x0, int64_t B,
+                                                     DevOut out) {
+  extern __shared__ double smem[];
+  const int64_t inst = blockIdx.x;
+  const int lane = threadIdx.x;
+  constexpr int S = N + 2;
+  double* J = smem;
+  double* rhs = J + N * S;
+  double* xs = rhs + N;
+  double* xw = xs + N;
+  double* vals = xw + N;
+  double* pv = vals + g.num_vals;
+  double* vst = pv + g.P;
+  Instance<N, IR> I{g, cfg, lane, J, rhs, xs, xw, vals, pv, vst};
+  const int n = g.n;
+
+  for (int i = lane; i < N * S; i += kWave) J[i] = 0.0;
+  for (int p = lane; p < g.P; p += kWave) pv[p] = params[inst * g.P + p];
+  if (lane == 0) { vals[kValOne] = 1.0; vals[kValGmin] = 0.0; }
+  if (lane < N) xs[lane] = (x0 && lane < n) ? x0[inst * n + lane] : 0.0;
+  __syncthreads();
+
+  int total = 0, it = 0, esc = 0, status = SPX_FAILED;
+  int flags = x0 ? SPX_FLAG_WARM_START : 0;
+  double final_gmin = 0.0;
+
+  if (!g.has_nonlinear) {
+    // linear circuit: one solve (sweep.rs:159-217)
+    I.assemble(0.0, 1.0, cfg.limiter);
+    double xn = 0.0;
+    bool ok = I.solve_f64(xn);
+    if (lane < n) xs[lane] = ok ? xn : 0.0;
+    __syncthreads();
+    status = ok ? SPX_CONVERGED : SPX_SINGULAR;
+    total = 1;
+  } else {
+    // Plain NR, then Gmin stepping (newton.rs:373-459), then source stepping
+    // (newton.rs:261-356), as one state machine so the Newton loop has a
+    // single, inlined call site.  Retries at one level restart from the same
+    // solution and so replay the first attempt exactly; the first attempt
+    // decides the level.  A stage that fails returns its last converged
+    // level's solution (zeros if none).
+    enum { kPlain, kGmin, kSource, kDone };
+    int stage = kPlain;
+    double gm = 0.0, factor = 0.0, step = 0.0;
+    auto enter_gmin = [&]() -> int {
+      flags |= SPX_FLAG_GMIN_STEPPING;
+      I.zero_x();
+      I.save_x();
+      gm = cfg.gmin_initial;
+      if (gm >= cfg.gmin_final * 0.99) return kGmin;
+      status = SPX_CONVERGED;  // no level to run: the stage returns converged zeros
+      return kDone;
+    };
+    auto enter_source = [&]() -> int {
+      flags |= SPX_FLAG_SOURCE_STEPPING;
+      I.zero_x();
+      I.save_x();
+      factor = cfg.src_initial;
+      step = cfg.src_step;
+      if (factor <= 1.0 + 1e-9) return kSource;
+      status = SPX_CONVERGED;
+      final_gmin = 0.0;
+      return kDone;
+    };
+    auto after_failure = [&]() -> int {
+      if (stage == kPlain && cfg.gmin_stepping) return enter_gmin();
+      if (stage != kSource && cfg.source_stepping) return enter_source();
+      return kDone;
+    };
+    while (stage != kDone) {
+      double lg = 0.0, lsf = 1.0;
+      if (stage == kGmin) {
+        lg = gm;
+      } else if (stage == kSource) {
+        lsf = fmin(factor, 1.0);
+        I.restore_x();
+      }
+      const int st = I.newton(lg, lsf, it, esc);
+      total += it;
+      if (stage == kPlain) {
+        status = st == 1 ? SPX_CONVERGED : (st < 0 ? SPX_SINGULAR : SPX_FAILED);
+        stage = status == SPX_CONVERGED ? kDone : after_failure();
+      } else if (stage == kGmin) {
+        if (st != 1) {
+          I.restore_x();
+          status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
+          final_gmin = 0.0;
+          stage = after_failure();
+        } else {
+          I.save_x();
+          final_gmin = gm;
+          bool more = gm > cfg.gmin_final;
+          if (more) {
+            gm /= cfg.gmin_factor;
+            if (gm < cfg.gmin_final) gm = cfg.gmin_final;
+            more = gm >= cfg.gmin_final * 0.99;
+          }
+          if (!more) {
+            I.restore_x();
+            status = SPX_CONVERGED;
+            stage = kDone;
+          }
+        }
+      } else {  // kSource
+        const double cur = fmin(factor, 1.0);
+        bool finished = false;
+        if (st != 1) {
+          if (step > cfg.src_min_step) {
+            step /= 2.0;
+            factor -= step;
+            finished = !(factor <= 1.0 + 1e-9);
+          } else {
+            I.restore_x();
+            status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
+            stage = kDone;
+          }
+        } else {
+          I.save_x();
+          if (fabs(cur - 1.0) < 1e-9) {
+            finished = true;
+          } else {
+            factor += step;
+            if (step < cfg.src_step) step = fmin(step * 1.5, cfg.src_step);
+            finished = !(factor <= 1.0 + 1e-9);
+          }
+        }
+        if (finished) {
+          I.restore_x();
+          status = SPX_CONVERGED;
+          final_gmin = 0.0;
+          stage = kDone;
+        }
+      }
+    }
+  }
+  if (esc) flags |= SPX_FLAG_FP64_ESCALATED;
+  double res = I.kcl(final_gmin);
+  if (status == SPX_CONVERGED && !(res <= 1.0)) flags |= SPX_FLAG_KCL_FAIL;
+  if (lane < n) out.x[inst * n + lane] = status == SPX_SINGULAR ? 0.0 : xs[lane];
+  if (lane == 0) {
+    if (out.status) out.status[inst] = status;
+    if (out.iterations) out.iterations[inst] = total;
+    if (out.flags) out.flags[inst] = flags;
+    if (out.residual) out.residual[inst] = res;
+    if (out.factorizations) out.factorizations[inst] = I.nfact;
+    if (out.refinements) out.refinements[inst] = I.nref;
+  }
+}
+
+
+template <int N, bool IR>
+hipError_t launch_newton_impl(const DevProgram& g, const DevNrCfg& cfg, const double* params,
+                              const double* x0, int64_t B, const DevOut& out, hipStream_t st) {
+  size_t sm = sizeof(double) * newton_smem_doubles(g, N);
+  hipError_t e = hipFuncSetAttribute((const void*)k_newton_dense<N, IR>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_newton_dense<N, IR>), dim3((unsigned)B), dim3(kWave), sm, st, g, cfg,
+                     params, x0, B, out);
+  return hipGetLastError();
+}
+
+}  // namespace spx

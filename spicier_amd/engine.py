@@ -1,0 +1,420 @@
+"""Host-side mirror of spicier-batched-sweep's backend interface, over the C ABI.
+
+Names, argument meaning and error behaviour follow the reference:
+  * ``BatchedLuSolver`` / ``BatchedSolveResult`` / ``GpuBatchConfig`` /
+    ``BackendType`` / ``BackendSelector``
+    (crates/spicier-batched-sweep/src/solver.rs:1-312),
+  * the ``BatchedSweepError`` family (crates/spicier-batched-sweep/src/error.rs:7-46),
+  * ``ConvergenceCriteria`` + Gmin / source stepping parameters
+    (crates/spicier-solver/src/newton.rs:11-34,174-231),
+  * ``ConvergenceStatus`` (crates/spicier-batched-sweep/src/convergence.rs:41-50),
+  * ``SweepStatistics`` (crates/spicier-batched-sweep/src/statistics.rs:35-74).
+Every computation runs in libspicier_hip.so; there is no CPU fallback: a
+missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import threading
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi as A
+
+# ---------------------------------------------------------------- errors
+
+
+class BatchedSweepError(RuntimeError):
+    """Base of the reference's error enum (error.rs:7-46)."""
+
+
+class NoBackendAvailable(BatchedSweepError):
+    pass
+
+
+class BackendInit(BatchedSweepError):
+    pass
+
+
+class MemoryAlloc(BatchedSweepError):
+    pass
+
+
+class Transfer(BatchedSweepError):
+    pass
+
+
+class InvalidDimension(BatchedSweepError):
+    pass
+
+
+class BatchTooLarge(BatchedSweepError):
+    pass
+
+
+class Backend(BatchedSweepError):
+    pass
+
+
+class InvalidArgument(BatchedSweepError):
+    pass
+
+
+_ERRORS = {
+    A.SPX_ERR_INVALID_DIMENSION: InvalidDimension, A.SPX_ERR_BATCH_TOO_LARGE: BatchTooLarge,
+    A.SPX_ERR_MEMORY_ALLOC: MemoryAlloc, A.SPX_ERR_TRANSFER: Transfer,
+    A.SPX_ERR_BACKEND_INIT: BackendInit, A.SPX_ERR_BACKEND: Backend,
+    A.SPX_ERR_NO_BACKEND: NoBackendAvailable, A.SPX_ERR_INVALID_ARGUMENT: InvalidArgument,
+}
+
+
+def _check(rc: int):
+    if rc != A.SPX_OK:
+        msg = A.load_library().spx_last_error().decode(errors="replace")
+        raise _ERRORS.get(rc, BatchedSweepError)(msg)
+
+
+# ---------------------------------------------------------------- config / types
+MAX_BATCH_SIZE = 65535     # solver.rs:6
+MIN_BATCH_SIZE = 16        # solver.rs:9
+MIN_MATRIX_SIZE = 32       # solver.rs:12
+
+
+@dataclass
+class GpuBatchConfig:
+    """solver.rs:14-41."""
+    min_batch_size: int = MIN_BATCH_SIZE
+    min_matrix_size: int = MIN_MATRIX_SIZE
+    max_batch_per_launch: int = MAX_BATCH_SIZE
+
+    def should_use_gpu(self, matrix_size: int, batch_size: int) -> bool:
+        return matrix_size >= self.min_matrix_size and batch_size >= self.min_batch_size
+
+
+class BackendType(enum.Enum):
+    """solver.rs:79-92 plus the HIP backend this engine adds."""
+    Cuda = "CUDA"
+    Metal = "Metal"
+    Mps = "MPS"
+    Accelerate = "Accelerate"
+    Faer = "Faer"
+    Cpu = "CPU"
+    Hip = "HIP"
+
+    def __str__(self):
+        return self.value
+
+
+class ConvergenceStatus(enum.IntEnum):
+    Active = A.SPX_ACTIVE
+    Converged = A.SPX_CONVERGED
+    Failed = A.SPX_FAILED
+    Singular = A.SPX_SINGULAR
+
+
+@dataclass
+class BatchedSolveResult:
+    """solver.rs:43-75."""
+    solutions: np.ndarray
+    singular_indices: list
+    n: int
+    batch_size: int
+
+    def solution(self, index: int):
+        if index >= self.batch_size:
+            return None
+        return self.solutions[index * self.n:(index + 1) * self.n]
+
+    def is_singular(self, index: int) -> bool:
+        return index in self.singular_indices
+
+    def num_solved(self) -> int:
+        return self.batch_size - len(self.singular_indices)
+
+
+# ---------------------------------------------------------------- device handles
+class Device:
+    """One spx handle (device + HIP stream).  Handles are cached per device so
+    creating a solver per sweep point (parallel.rs:194) stays cheap."""
+
+    _cache: dict = {}
+    _lock = threading.Lock()
+
+    def __init__(self, device_id: int = 0):
+        lib = A.load_library()
+        h = C.c_void_p()
+        _check(lib.spx_create(device_id, C.byref(h)))
+        self.lib = lib
+        self.handle = h
+        self.device_id = device_id
+
+    @classmethod
+    def get(cls, device_id: int = 0) -> "Device":
+        with cls._lock:
+            d = cls._cache.get(device_id)
+            if d is None:
+                d = cls(device_id)
+                cls._cache[device_id] = d
+            return d
+
+    @staticmethod
+    def count() -> int:
+        lib = A.load_library()
+        c = C.c_int32(0)
+        rc = lib.spx_device_count(C.byref(c))
+        return c.value if rc == A.SPX_OK else 0
+
+    def synchronize(self):
+        _check(self.lib.spx_synchronize(self.handle))
+
+    @property
+    def stream(self) -> int:
+        return self.lib.spx_stream(self.handle)
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                self.lib.spx_destroy(self.handle)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- the trait
+class BatchedLuSolver:
+    """The reference's `BatchedLuSolver` trait (solver.rs:108-135)."""
+
+    def solve_batch(self, matrices, rhs, n: int, batch_size: int) -> BatchedSolveResult:
+        raise NotImplementedError
+
+    def should_use_gpu(self, matrix_size: int, batch_size: int) -> bool:
+        raise NotImplementedError
+
+    def backend_type(self) -> BackendType:
+        raise NotImplementedError
+
+    def config(self) -> GpuBatchConfig:
+        raise NotImplementedError
+
+
+class HipBatchedSolver(BatchedLuSolver):
+    """The HIP drop-in for CudaBatchedSolver (spicier-batched-sweep/src/cuda.rs:12-87):
+    matrices column-major per system, singular systems -> zeros + index."""
+
+    def __init__(self, config: GpuBatchConfig | None = None, device_id: int = 0):
+        self._config = config or GpuBatchConfig()
+        self.device = Device.get(device_id)
+
+    def solve_batch(self, matrices, rhs, n: int, batch_size: int) -> BatchedSolveResult:
+        m = np.ascontiguousarray(matrices, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(rhs, dtype=np.float64).reshape(-1)
+        if m.size != batch_size * n * n:
+            raise InvalidDimension(f"Expected {batch_size * n * n} matrix elements, got {m.size}")
+        if b.size != batch_size * n:
+            raise InvalidDimension(f"Expected {batch_size * n} RHS elements, got {b.size}")
+        x = np.zeros(batch_size * n)
+        sing = np.zeros(max(batch_size, 1), dtype=np.uint8)
+        ns = C.c_int64(0)
+        _check(self.device.lib.spx_solve_batch_f64(self.device.handle, A.ptr(m), A.ptr(b), n,
+                                                   batch_size, A.ptr(x), A.ptr(sing),
+                                                   C.byref(ns)))
+        idx = [int(i) for i in np.nonzero(sing[:batch_size])[0]]
+        return BatchedSolveResult(x, idx, n, batch_size)
+
+    def solve_batch_device(self, d_matrices, d_rhs, n: int, batch_size: int, d_x, d_sing=None):
+        """Device-pointer variant on torch tensors / raw pointers (stream-ordered)."""
+        _check(self.device.lib.spx_solve_batch_f64_device(
+            self.device.handle, A.ptr(d_matrices), A.ptr(d_rhs), n, batch_size, A.ptr(d_x),
+            A.ptr(d_sing)))
+
+    def should_use_gpu(self, matrix_size: int, batch_size: int) -> bool:
+        return self._config.should_use_gpu(matrix_size, batch_size)
+
+    def backend_type(self) -> BackendType:
+        return BackendType.Hip
+
+    def config(self) -> GpuBatchConfig:
+        return self._config
+
+
+@dataclass
+class BackendSelector:
+    """solver.rs:139-312, with HIP as the only backend this engine ships.  The
+    reference falls through to its CPU solver when no GPU is present; this
+    engine has no CPU path and raises NoBackendAvailable instead."""
+    preferred: BackendType | None = None
+    config: GpuBatchConfig = field(default_factory=GpuBatchConfig)
+
+    @classmethod
+    def auto(cls) -> "BackendSelector":
+        return cls()
+
+    @classmethod
+    def prefer_hip(cls) -> "BackendSelector":
+        return cls(BackendType.Hip)
+
+    def create_solver(self) -> BatchedLuSolver:
+        if self.preferred not in (None, BackendType.Hip):
+            raise NoBackendAvailable(f"backend {self.preferred} is not part of this engine")
+        if Device.count() == 0:
+            raise NoBackendAvailable("No GPU backend available")
+        return HipBatchedSolver(self.config)
+
+
+# ---------------------------------------------------------------- Newton config
+@dataclass
+class ConvergenceCriteria:
+    """newton.rs:11-34."""
+    v_abstol: float = 1e-6
+    v_reltol: float = 1e-3
+    i_abstol: float = 1e-12
+    max_iterations: int = 50
+    gmin: float = 1e-12
+
+
+@dataclass
+class GminSteppingParams:
+    """newton.rs:174-194."""
+    initial_gmin: float = 1e-3
+    final_gmin: float = 1e-12
+    reduction_factor: float = 10.0
+    max_attempts_per_level: int = 5
+
+
+@dataclass
+class SourceSteppingParams:
+    """newton.rs:211-231."""
+    initial_factor: float = 0.1
+    factor_step: float = 0.1
+    max_attempts_per_level: int = 5
+    min_factor_step: float = 0.01
+
+
+class Precision(enum.IntEnum):
+    F64 = A.SPX_PREC_F64
+    F32_IR = A.SPX_PREC_F32_IR
+
+
+class Limiter(enum.IntEnum):
+    Reference = A.SPX_LIMIT_REFERENCE
+    NoLimit = A.SPX_LIMIT_NONE
+    Pnjlim = A.SPX_LIMIT_PNJLIM
+
+
+def nr_config(criteria: ConvergenceCriteria | None = None,
+              gmin: GminSteppingParams | None = None,
+              source: SourceSteppingParams | None = None,
+              precision: Precision = Precision.F64, limiter: Limiter = Limiter.Reference,
+              gmin_stepping: bool = True, source_stepping: bool = True,
+              ir_max_steps: int = 8, ir_tol: float = 1e-13) -> A.SpxNrConfig:
+    c = criteria or ConvergenceCriteria()
+    g = gmin or GminSteppingParams()
+    s = source or SourceSteppingParams()
+    return A.default_nr_config(
+        v_abstol=c.v_abstol, v_reltol=c.v_reltol, i_abstol=c.i_abstol,
+        max_iterations=c.max_iterations,
+        gmin_stepping=int(gmin_stepping), gmin_initial=g.initial_gmin, gmin_final=g.final_gmin,
+        gmin_factor=g.reduction_factor, gmin_max_attempts=g.max_attempts_per_level,
+        source_stepping=int(source_stepping), src_initial=s.initial_factor,
+        src_step=s.factor_step, src_min_step=s.min_factor_step,
+        src_max_attempts=s.max_attempts_per_level,
+        precision=int(precision), limiter=int(limiter), ir_max_steps=ir_max_steps,
+        ir_tol=ir_tol)
+
+
+# ---------------------------------------------------------------- statistics
+@dataclass
+class SweepStatistics:
+    """statistics.rs:35-74 (from_moments: population variance E[x^2]-mean^2)."""
+    count: int
+    min: float
+    max: float
+    mean: float
+    variance: float
+    std_dev: float
+    sum: float
+
+    @classmethod
+    def from_moments(cls, count, s, sq, mn, mx):
+        count = int(count)
+        mean = s / count if count > 0 else 0.0
+        var = max((sq / count) - mean * mean, 0.0) if count > 0 else 0.0
+        return cls(count, mn, mx, mean, var, var ** 0.5, s)
+
+
+# ---------------------------------------------------------------- DC-OP sweep
+@dataclass
+class DcopSweepResult:
+    """Nonlinear counterpart of GpuBatchedSweepResult (sweep.rs:17-55)."""
+    solutions: np.ndarray      # [B, n]
+    status: np.ndarray         # [B] ConvergenceStatus codes
+    iterations: np.ndarray     # [B]
+    flags: np.ndarray          # [B] SPX_FLAG_*
+    residual: np.ndarray       # [B] max |KCL| / tol
+    params: np.ndarray         # [B, P]
+    node_stats: np.ndarray     # [n, 5] count,sum,sumsq,min,max over converged
+    factorizations: np.ndarray = None  # [B] LU factorizations performed
+    refinements: np.ndarray = None     # [B] FP64-residual refinement steps
+    used_gpu: bool = True
+    backend_used: BackendType = BackendType.Hip
+
+    @property
+    def total_count(self) -> int:
+        return len(self.status)
+
+    @property
+    def converged_count(self) -> int:
+        return int(np.count_nonzero(self.status == A.SPX_CONVERGED))
+
+    @property
+    def singular_indices(self):
+        return [int(i) for i in np.nonzero(self.status == A.SPX_SINGULAR)[0]]
+
+    def accepted_mask(self) -> np.ndarray:
+        """Converged and passing the FP64 KCL residual test."""
+        return (self.status == A.SPX_CONVERGED) & ((self.flags & A.SPX_FLAG_KCL_FAIL) == 0)
+
+    def node_voltages(self, node_index: int) -> np.ndarray:
+        return self.solutions[:, node_index]
+
+    def statistics(self, node_index: int) -> SweepStatistics:
+        return SweepStatistics.from_moments(*self.node_stats[node_index])
+
+
+def sweep_dcop(circuit, batch: int, seed: int = 42, generator: int = A.SPX_GEN_HASH,
+               sweep_offset: int = 0, config: A.SpxNrConfig | None = None, x0=None,
+               explicit_params=None, device_id: int = 0) -> DcopSweepResult:
+    """Monte Carlo / corner / linear DC-operating-point sweep on the GPU
+    (host buffers in and out)."""
+    dev = Device.get(device_id)
+    cfg = config or nr_config()
+    ckt, keep = circuit.to_c()
+    if explicit_params is not None:
+        explicit_params = np.ascontiguousarray(explicit_params, dtype=np.float64)
+        generator = A.SPX_GEN_EXPLICIT
+    spec, keep2 = circuit.sweep_spec(batch, seed, generator, sweep_offset, explicit_params)
+    n, P = circuit.size, circuit.num_params
+    out = DcopSweepResult(np.zeros((batch, n)), np.zeros(batch, np.int32),
+                          np.zeros(batch, np.int32), np.zeros(batch, np.int32),
+                          np.zeros(batch), np.zeros((batch, P)), np.zeros((n, 5)),
+                          np.zeros(batch, np.int32), np.zeros(batch, np.int32))
+    res = A.SpxSweepResult(A.ptr(out.solutions), A.ptr(out.status), A.ptr(out.iterations),
+                           A.ptr(out.flags), A.ptr(out.residual), A.ptr(out.params),
+                           A.ptr(out.node_stats), A.ptr(out.factorizations),
+                           A.ptr(out.refinements))
+    if x0 is not None:
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        cfg.warm_start = A.SPX_WARM_GIVEN
+    _check(dev.lib.spx_sweep_dcop(dev.handle, C.byref(ckt), C.byref(spec), C.byref(cfg),
+                                  A.ptr(x0), C.byref(res)))
+    return out
+
+
+def sample_params(circuit, batch: int, seed: int = 42, generator: int = A.SPX_GEN_HASH,
+                  sweep_offset: int = 0, device_id: int = 0) -> np.ndarray:
+    dev = Device.get(device_id)
+    spec, keep = circuit.sweep_spec(batch, seed, generator, sweep_offset)
+    out = np.zeros((batch, circuit.num_params))
+    _check(dev.lib.spx_sample_params(dev.handle, C.byref(spec), circuit.num_params, A.ptr(out)))
+    return out

@@ -344,11 +344,16 @@ def test_symmetric_inverter_trips_at_midrail(oracle):
 
 
 def test_convergence_check_semantics(oracle):
-    # newton.rs:134-159 / 637-648: the Newton loop stops on the first
-    # iteration whose update passes; a linear circuit therefore takes 2.
+    # A linear circuit takes the sweep's single solve (sweep.rs:159-217);
+    # Newton (newton.rs:134-159) stops on the first iteration whose update
+    # passes the Δx test, so it needs one more than the last big update.
     c = CK.voltage_divider()
     out = oracle.sweep_dcop(c, CK.instance_params(c, 1))
-    assert out["iterations"][0] == 2 and abs(out["x"][0, 1] - 5.0) < 1e-12
+    assert out["iterations"][0] == 1 and abs(out["x"][0, 1] - 5.0) < 1e-12
+    c = CK.nmos_cs_validation()
+    o = oracle.sweep_dcop(c, CK.instance_params(c, 1))
+    t = oracle.sweep_dcop(c, CK.instance_params(c, 1), A.default_nr_config(max_iterations=o["iterations"][0] - 1, gmin_stepping=0, source_stepping=0))
+    assert o["status"][0] == A.SPX_CONVERGED and t["status"][0] == A.SPX_FAILED
 
 
 def test_gmin_and_source_stepping_reach_the_same_point(oracle):
