@@ -223,7 +223,8 @@ typedef struct spx_nr_config {
     int32_t limiter;            /* spx_limiter */
     int32_t warm_start;         /* spx_warm_start */
     int32_t ir_max_steps;       /* FP32+IR: refinement steps per Newton solve before refactoring */
-    double ir_tol;              /* FP32+IR: stop when |dx_i| <= ir_tol*(|x_i|+1e-30) for all i */
+    double ir_tol;              /* FP32+IR: stop when ||b - J x||inf <= ir_tol ||J||inf ||x||inf
+                                   (LAPACK dsgesv's test); 0 = sqrt(n) * 2^-53 */
     /* KCL residual test of the final x, FP64: devices evaluated unlimited
      * (Shockley / square law) at x; node row i passes if
      * |sum_j J_ij x_j - b_i| <= kcl_reltol * max(|J_ij x_j|, |b_i|) + kcl_abstol. */

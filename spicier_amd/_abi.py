@@ -99,7 +99,7 @@ def default_nr_config(**overrides) -> SpxNrConfig:
         gmin_max_attempts=5,
         source_stepping=1, src_initial=0.1, src_step=0.1, src_min_step=0.01, src_max_attempts=5,
         precision=SPX_PREC_F64, limiter=SPX_LIMIT_REFERENCE, warm_start=SPX_WARM_ZERO,
-        ir_max_steps=8, ir_tol=1e-13,
+        ir_max_steps=8, ir_tol=0.0,
         kcl_reltol=1e-3, kcl_abstol=1e-12, kcl_vntol=1e-6)
     for k, v in overrides.items():
         setattr(cfg, k, v)

@@ -28,8 +28,17 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def _compile(src):
+def _headers():
+    return [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")] + [
+        os.path.join(os.path.dirname(HERE), "include", "spicier_hip.h")]
+
+
+def _compile(src, incremental=False):
     obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
+    if incremental and os.path.exists(obj):
+        newest = max(os.path.getmtime(p) for p in [src, *_headers()])
+        if os.path.getmtime(obj) >= newest:
+            return obj
     cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -37,11 +46,13 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False) -> str:
+def build(verbose=False, incremental=False) -> str:
+    """Compiles every HIP source (all of them unless `incremental`, which skips
+    objects newer than their source and every header) and links the library."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, incremental), srcs))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -54,4 +65,5 @@ def build(verbose=False) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    import sys
+    build(verbose=True, incremental="--incremental" in sys.argv)

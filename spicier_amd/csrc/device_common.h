@@ -170,6 +170,76 @@ __device__ __forceinline__ float readlane_f32(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// One DPP max step on a 64-bit key (two 32-bit moves; lanes whose source is
+// outside the pattern read 0, the identity of an unsigned max).
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint64_t dpp_max_u64(uint64_t v) {
+  uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, Ctrl, RowMask, 0xf, false);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), Ctrl, RowMask, 0xf,
+                                                      false);
+  uint64_t o = ((uint64_t)hi << 32) | lo;
+  return o > v ? o : v;
+}
+
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint32_t dpp_max_u32(uint32_t v) {
+  uint32_t o = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, Ctrl, RowMask, 0xf, false);
+  return o > v ? o : v;
+}
+
+// Wave-wide unsigned max with DPP (quad_perm [1,0,3,2], [2,3,0,1], row_shr:4,
+// row_shr:8, row_bcast:15, row_bcast:31 — the GFX9 reduction ladder), the
+// result read from lane 63 so it is wave-uniform.  No LDS round trips.
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+  v = dpp_max_u64<0xb1, 0xf>(v);
+  v = dpp_max_u64<0x4e, 0xf>(v);
+  v = dpp_max_u64<0x114, 0xf>(v);
+  v = dpp_max_u64<0x118, 0xf>(v);
+  v = dpp_max_u64<0x142, 0xa>(v);
+  v = dpp_max_u64<0x143, 0xc>(v);
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  v = dpp_max_u32<0xb1, 0xf>(v);
+  v = dpp_max_u32<0x4e, 0xf>(v);
+  v = dpp_max_u32<0x114, 0xf>(v);
+  v = dpp_max_u32<0x118, 0xf>(v);
+  v = dpp_max_u32<0x142, 0xa>(v);
+  v = dpp_max_u32<0x143, 0xc>(v);
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Lane holding the partial-pivot row: the largest |a| over eligible lanes,
+// the smallest `pos` among exact ties (nalgebra's first-maximum rule).  For
+// non-negative IEEE values the bit pattern orders like the value, so the max
+// runs on integer keys.  Returns -1 if every eligible |a| is zero.
+__device__ __forceinline__ int pivot_lane(double absval, bool eligible, int pos) {
+  uint64_t key = eligible ? (uint64_t)__double_as_longlong(absval) : 0ull;
+  uint64_t mx = wave_max_u64(key);
+  if (mx == 0) return -1;
+  uint64_t m = __ballot(eligible && key == mx);
+  if (__popcll(m) != 1) {  // exact tie: smallest position wins
+    uint32_t best = wave_max_u32(((m >> __lane_id()) & 1) ? (uint32_t)(64 - pos) : 0u);
+    m = __ballot(pos == (int)(64 - best));
+  }
+  return (int)__ffsll((unsigned long long)m) - 1;
+}
+
+__device__ __forceinline__ int pivot_lane(float absval, bool eligible, int pos) {
+  uint32_t key = eligible ? (uint32_t)__float_as_uint(absval) : 0u;
+  uint32_t mx = wave_max_u32(key);
+  if (mx == 0) return -1;
+  uint64_t m = __ballot(eligible && key == mx);
+  if (__popcll(m) != 1) {
+    uint32_t best = wave_max_u32(((m >> __lane_id()) & 1) ? (uint32_t)(64 - pos) : 0u);
+    m = __ballot(pos == (int)(64 - best));
+  }
+  return (int)__ffsll((unsigned long long)m) - 1;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));

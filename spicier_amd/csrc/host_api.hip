@@ -600,7 +600,7 @@ void spx_nr_config_default(spx_nr_config* c) {
   c->limiter = SPX_LIMIT_REFERENCE;
   c->warm_start = SPX_WARM_ZERO;
   c->ir_max_steps = 8;
-  c->ir_tol = 1e-13;
+  c->ir_tol = 0.0;
   c->kcl_reltol = 1e-3;
   c->kcl_abstol = 1e-12;
   c->kcl_vntol = 1e-6;

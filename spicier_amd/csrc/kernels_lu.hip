@@ -31,10 +31,10 @@ __global__ __launch_bounds__(64) void k_solve_rows(const double* __restrict__ M,
     lu.a[j] = (j < n && lane < n) ? A[(int64_t)j * n + lane] : (j == lane ? 1.0 : 0.0);
   });
   const double b = lane < n ? rhs[i * n + lane] : 0.0;
-  bool ok = lu.factor(n, lane);
+  bool ok = lu.factor(lane);
   double xv = 0.0;
   if (ok) {
-    xv = lu.template solve<true>(b, n, lane);
+    xv = lu.template solve<true>(b, lane);
     ok = __all(lane >= n || isfinite(xv));
   }
   if (lane < n) x[i * n + lane] = ok ? xv : 0.0;

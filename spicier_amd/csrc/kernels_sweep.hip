@@ -108,20 +108,34 @@ __global__ void k_stats_partial(const double* __restrict__ v, const uint8_t* __r
   }
 }
 
-// Pass 2: merge the partials of each node in block order
-// (StatisticsAccumulator::merge, statistics.rs:148-154).
-__global__ void k_stats_merge(const double* __restrict__ part, int nblocks, int n,
-                              double* __restrict__ out) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+// Pass 2: one workgroup per node merges the partials
+// (StatisticsAccumulator::merge, statistics.rs:148-154): each thread folds a
+// strided subset in order, then a fixed-shape tree combines the threads, so
+// the result is deterministic run to run.
+__global__ __launch_bounds__(256) void k_stats_merge(const double* __restrict__ part,
+                                                     int nblocks, int n,
+                                                     double* __restrict__ out) {
+  __shared__ double s[5][256];
+  const int j = blockIdx.x, t = threadIdx.x;
   double cnt = 0, sum = 0, sq = 0, mn = INFINITY, mx = -INFINITY;
-  for (int b = 0; b < nblocks; ++b) {
+  for (int b = t; b < nblocks; b += 256) {
     const double* p = part + ((int64_t)b * n + j) * 5;
     cnt += p[0]; sum += p[1]; sq += p[2];
     mn = fmin(mn, p[3]); mx = fmax(mx, p[4]);
   }
-  double* o = out + (int64_t)j * 5;
-  o[0] = cnt; o[1] = sum; o[2] = sq; o[3] = mn; o[4] = mx;
+  s[0][t] = cnt; s[1][t] = sum; s[2][t] = sq; s[3][t] = mn; s[4][t] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      s[0][t] += s[0][t + w];
+      s[1][t] += s[1][t + w];
+      s[2][t] += s[2][t + w];
+      s[3][t] = fmin(s[3][t], s[3][t + w]);
+      s[4][t] = fmax(s[4][t], s[4][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t < 5) out[(int64_t)j * 5 + t] = s[t][0];
 }
 
 // ------------------------------------------------------------- launchers
@@ -177,8 +191,7 @@ hipError_t launch_stats(const double* v, const uint8_t* mask, const int* status,
   int threads = n < 256 ? ((n + 63) / 64) * 64 : 256;
   hipLaunchKernelGGL(k_stats_partial, dim3((unsigned)nb), dim3(threads), 0, st, v, mask, status, B,
                      n, chunk, part);
-  hipLaunchKernelGGL(k_stats_merge, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part,
-                     (int)nb, n, out);
+  hipLaunchKernelGGL(k_stats_merge, dim3((unsigned)n), dim3(256), 0, st, part, (int)nb, n, out);
   return hipGetLastError();
 }
 

@@ -52,77 +52,64 @@ template <int N, typename T>
 struct RowLU {
   T a[N];
   int pos;
-  int piv[N];  // lane holding the pivot row of step k (wave-uniform)
   T dinv;      // 1 / U[pos][pos] of this lane's row
 
   // Factor in place.  Returns false if a pivot column is zero (or NaN): the
-  // reference's LU then reports the system singular.
-  __device__ __forceinline__ bool factor(int n, int lane) {
+  // reference's LU then reports the system singular.  Every step runs over
+  // all N columns with no per-column guard, so the update is straight-line
+  // code; rows and columns past n are identity padding, whose multipliers
+  // are zero.
+  __device__ __forceinline__ bool factor(int lane) {
     pos = lane;
     dinv = T(0);
     bool ok = true;
     static_for<0, N>([&](auto K) __attribute__((always_inline)) {
       constexpr int k = K;
-      if (!ok || k >= n) return;
+      if (!ok) return;
       // argmax |a[k]| over rows at positions >= k, smallest position on ties
-      T v = pos >= k ? tabs(a[k]) : T(-1);
-      int bp = pos;
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        T ov = __shfl_xor(v, off, kWave);
-        int op = __shfl_xor(bp, off, kWave);
-        bool take = (ov > v) || (ov == v && op < bp);
-        v = take ? ov : v;
-        bp = take ? op : bp;
-      }
-      bp = __builtin_amdgcn_readfirstlane(bp);
-      if (!(bcast(v, 0) > T(0))) { ok = false; return; }
-      uint64_t m = __ballot(pos == bp);
-      int p = __builtin_amdgcn_readfirstlane((int)__ffsll((unsigned long long)m) - 1);
-      piv[k] = p;
+      int p = pivot_lane(tabs(a[k]), pos >= k, pos);
+      p = __builtin_amdgcn_readfirstlane(p);
+      if (p < 0) { ok = false; return; }
+      const int bp = __builtin_amdgcn_readlane(pos, p);
       if (pos == k) pos = bp;  // row at position k moves to the pivot's position
       if (lane == p) pos = k;
-      T pv = bcast(a[k], p);
-      T inv = T(1) / pv;
+      const T pv = bcast(a[k], p);
+      const T inv = T(1) / pv;
       if (lane == p) dinv = inv;
-      T l = pos > k ? a[k] * inv : T(0);
+      const T l = pos > k ? a[k] * inv : T(0);
       if (pos > k) a[k] = l;
       static_for<k + 1, N>([&](auto J) __attribute__((always_inline)) {
         constexpr int j = J;
-        if (j < n) {
-          T u = bcast(a[j], p);
-          a[j] = fma(-u, l, a[j]);
-        }
+        a[j] = fma(-bcast(a[j], p), l, a[j]);
       });
     });
     return ok;
   }
 
+  // Lane whose row sits at position k (wave-uniform).
+  __device__ __forceinline__ int lane_at(int k) const {
+    return __builtin_amdgcn_readfirstlane((int)__ffsll((unsigned long long)__ballot(pos == k)) - 1);
+  }
+
   // Solve with the stored factors for right-hand side b (b of lane r is
-  // b[r]).  Returns x[lane] in each lane < n.  Forward substitution replays
-  // the elimination on b; back substitution divides by the pivot (exact
-  // division when `divide`, multiplication by the stored inverse otherwise).
+  // b[r]).  Returns x[lane] in each lane.  Forward substitution replays the
+  // elimination on b; back substitution divides by the pivot (exact division
+  // when `divide`, multiplication by the stored inverse otherwise).
   template <bool divide>
-  __device__ __forceinline__ T solve(T b, int n, int lane) const {
+  __device__ __forceinline__ T solve(T b, int lane) const {
     static_for<0, N>([&](auto K) __attribute__((always_inline)) {
       constexpr int k = K;
-      if (k < n) {
-        T bp = bcast(b, piv[k]);
-        T l = pos > k ? a[k] : T(0);
-        b = fma(-bp, l, b);
-      }
+      const T bp = bcast(b, lane_at(k));
+      b = fma(-bp, pos > k ? a[k] : T(0), b);
     });
     T x = T(0);
     static_for_down<0, N>([&](auto K) __attribute__((always_inline)) {
       constexpr int k = K;
-      if (k < n) {
-        int p = piv[k];
-        T bk = bcast(b, p);
-        T xk = divide ? bk / bcast(a[k], p) : bk * bcast(dinv, p);
-        if (lane == k) x = xk;
-        T c = pos < k ? xk : T(0);
-        b = fma(-c, a[k], b);
-      }
+      const int p = lane_at(k);
+      const T bk = bcast(b, p);
+      const T xk = divide ? bk / bcast(a[k], p) : bk * bcast(dinv, p);
+      if (lane == k) x = xk;
+      b = fma(-(pos < k ? xk : T(0)), a[k], b);
     });
     return x;
   }

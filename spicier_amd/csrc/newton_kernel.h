@@ -126,40 +126,48 @@ struct Instance {
     ++nfact;
     RowLU<N, double> lu;
     load_rows(lu.a);
-    if (!lu.factor(g.n, lane)) return false;
-    xnew = lu.template solve<true>(lane < g.n ? rhs[lane] : 0.0, g.n, lane);
+    if (!lu.factor(lane)) return false;
+    xnew = lu.template solve<true>(lane < g.n ? rhs[lane] : 0.0, lane);
     return __all(lane >= g.n || isfinite(xnew));
   }
 
   // FP32 factors + FP64 iterative refinement, starting from the current x.
-  // `fresh` says the factors were computed from this J; returns 0 converged,
-  // 1 needs refactoring (stale factors did not converge), 2 failed.
+  // Stops on LAPACK dsgesv's backward-error test
+  //   ||b - J x||_inf <= tol * ||J||_inf * ||x||_inf,  tol = ir_tol or sqrt(n)*2^-53,
+  // i.e. once x is as good as an FP64 factorisation would give.  `fresh` says
+  // the factors were computed from this J; returns 0 converged, 1 needs
+  // refactoring (stale factors stopped contracting), 2 failed.
   __device__ __forceinline__ int refine(const RowLU<N, float>& lu, bool fresh, double& xnew) {
     const int n = g.n;
     const double* jrow = J + (lane < n ? lane : 0) * S;
     const double b = lane < n ? rhs[lane] : 0.0;
     double x = lane < n ? xs[lane] : 0.0;
-    const double floor_abs = lane < g.num_nodes ? cfg.v_abstol * 1e-9 : cfg.i_abstol * 1e-9;
+    double rowsum = 0.0;
+    if (lane < n)
+      for (int j = 0; j < n; ++j) rowsum += fabs(jrow[j]);
+    const double anrm = wave_max(rowsum);
+    const double tol = cfg.ir_tol > 0.0 ? cfg.ir_tol : sqrt((double)n) * 1.1102230246251565e-16;
     double prev = INFINITY;
-    for (int s = 0; s < cfg.ir_max_steps; ++s) {
+    for (int s = 0; s <= cfg.ir_max_steps; ++s) {
       if (lane < n) xw[lane] = x;
       __syncthreads();
       double r = b;
       if (lane < n)
         for (int j = 0; j < n; ++j) r = fma(-jrow[j], xw[j], r);
       __syncthreads();
-      float d = lu.template solve<false>((float)r, n, lane);
-      ++nref;
-      x += (double)d;
-      double ad = lane < n ? fabs((double)d) : 0.0;
-      bool ok = !(ad > cfg.ir_tol * fabs(x) + floor_abs);
-      if (__all(ok)) {
+      const double rn = wave_max(lane < n ? fabs(r) : 0.0);
+      const double xn = wave_max(lane < n ? fabs(x) : 0.0);
+      if (!(rn > tol * anrm * xn)) {  // converged (also catches r == 0)
         xnew = x;
         return __all(lane >= n || isfinite(x)) ? 0 : 2;
       }
-      double m = wave_max(ad);
-      if (!(m < 0.5 * prev) && s > 0) break;  // stagnating or diverging
-      prev = m;
+      if (s == cfg.ir_max_steps) break;
+      float d = lu.template solve<false>((float)r, lane);
+      ++nref;
+      x += (double)d;
+      const double dn = wave_max(lane < n ? fabs((double)d) : 0.0);
+      if (!(dn < 0.5 * prev) && s > 0) break;  // stopped contracting
+      prev = dn;
     }
     return fresh ? 2 : 1;
   }
@@ -183,7 +191,7 @@ struct Instance {
           if (fresh) {
             ++nfact;
             load_rows(lu32.a);
-            have32 = lu32.factor(n, lane);
+            have32 = lu32.factor(lane);
             if (!have32) { rc = 2; break; }
           }
           rc = refine(lu32, fresh, xnew);

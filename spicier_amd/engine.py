@@ -307,7 +307,7 @@ def nr_config(criteria: ConvergenceCriteria | None = None,
               source: SourceSteppingParams | None = None,
               precision: Precision = Precision.F64, limiter: Limiter = Limiter.Reference,
               gmin_stepping: bool = True, source_stepping: bool = True,
-              ir_max_steps: int = 8, ir_tol: float = 1e-13) -> A.SpxNrConfig:
+              ir_max_steps: int = 8, ir_tol: float = 0.0) -> A.SpxNrConfig:
     c = criteria or ConvergenceCriteria()
     g = gmin or GminSteppingParams()
     s = source or SourceSteppingParams()

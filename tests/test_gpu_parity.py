@@ -108,7 +108,11 @@ def _parity(got, want, tag):
     assert np.all(ok[same_it]), (tag, np.argwhere(~ok[same_it])[:5])
     if (~same_it).any():
         assert np.all(close(got.solutions[~same_it], want["x"][~same_it], rtol=1e-3, atol=1e-6))
-    np.testing.assert_array_equal(got.flags, want["flags"], err_msg=tag)
+    # the oracle has no FP32 path, so its flags never carry FP64_ESCALATED
+    np.testing.assert_array_equal(got.flags & ~A.SPX_FLAG_FP64_ESCALATED, want["flags"],
+                                  err_msg=tag)
+    esc = np.count_nonzero(got.flags & A.SPX_FLAG_FP64_ESCALATED)
+    assert esc <= 0.01 * len(got.flags), (tag, "FP32+IR escalated to FP64 on", esc)
 
 
 @pytest.mark.parametrize("prec", [E.Precision.F64, E.Precision.F32_IR])
