@@ -8,9 +8,15 @@ stepping fallbacks), the FP64 KCL acceptance test and the per-node
 statistics, all on the device with nothing crossing PCIe.
 
 Forms timed in the same run:
-  fp64     plain FP64: FP64 LU every Newton iteration, zero initial guess;
-  fp32_ir  FP32 LU reused across Newton iterations + FP64-residual iterative
-           refinement (the headline form).
+  fp64               plain FP64: FP64 LU every Newton iteration, zero initial guess;
+  fp32_ir            FP32 LU reused across Newton iterations + FP64-residual
+                     iterative refinement, zero initial guess;
+  fp32_ir_surrogate  the same, warm-started by the MLP surrogate (the headline
+                     form; its inference is inside the timed step).  The
+                     surrogate is trained once before timing, by the library's
+                     own kernels, on 262,144 FP64 operating points whose
+                     parameters are drawn at sweep indices 2^20.. (disjoint
+                     from the benchmark's 0..65,535); training time is reported.
 "Converged" = Newton's Δx test passed AND the FP64 KCL residual test passed;
 every other instance is counted as flagged.
 
@@ -48,7 +54,9 @@ def main():
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64,fp32_ir")
+    ap.add_argument("--forms", default="fp64,fp32_ir,fp32_ir_surrogate")
+    ap.add_argument("--train", type=int, default=262144)
+    ap.add_argument("--epochs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the sweep shards by instance range with no "
@@ -97,9 +105,22 @@ def main():
     forms = {
         "fp64": E.nr_config(precision=E.Precision.F64),
         "fp32_ir": E.nr_config(precision=E.Precision.F32_IR),
+        "fp32_ir_surrogate": E.nr_config(precision=E.Precision.F32_IR),
     }
+    forms["fp32_ir_surrogate"].warm_start = A.SPX_WARM_SURROGATE
     wanted = [f for f in args.forms.split(",") if f in forms]
     report = {}
+    training = None
+    if "fp32_ir_surrogate" in wanted:
+        t0 = time.perf_counter()
+        sur = E.train_surrogate(ckt_py, list(range(32)), n_train=args.train, epochs=args.epochs,
+                                batch=1024, lr=2e-3)
+        dev.synchronize()
+        training = {"seconds": time.perf_counter() - t0, "points": args.train,
+                    "epochs": args.epochs, "loss_first": sur.loss_history[0],
+                    "loss_last": sur.loss_history[-1],
+                    "network": "32 -> 256 -> 256 -> 64, SiLU, BF16 MFMA / FP32 accumulate"}
+        sur.attach()
     for name in wanted:
         cfg = forms[name]
         for _ in range(args.warmup):
@@ -135,7 +156,7 @@ def main():
         k_s = float(np.mean(kt))
         if name == "fp64":
             flops64, flops32 = fac.sum() * lu_flops(n), 0.0
-        else:
+        else:  # FP32 factorisations + per refinement step 2n^2 FP64 residual and 2n^2 FP32 solve
             flops32 = fac.sum() * getrf_flops(n) + ref.sum() * 2.0 * n * n
             flops64 = ref.sum() * 2.0 * n * n
         report[name] = dict(
@@ -143,9 +164,12 @@ def main():
             flagged=B - accepted, mean_iterations=float(it.mean()),
             mean_factorizations=float(fac.mean()), mean_refinements=float(ref.mean()),
             newton_kernel_ms=k_s * 1e3, flops_fp64=float(flops64), flops_fp32=float(flops32),
-            tflops=(flops64 + flops32) / k_s / 1e12)
+            tflops=(flops64 + flops32) / k_s / 1e12,
+            fallback_rate=float(np.count_nonzero(fl & (A.SPX_FLAG_GMIN_STEPPING
+                                                     | A.SPX_FLAG_SOURCE_STEPPING)) / B),
+            fp64_escalations=int(np.count_nonzero(fl & A.SPX_FLAG_FP64_ESCALATED)))
 
-    head = "fp32_ir" if "fp32_ir" in report else wanted[0]
+    head = next((f for f in ("fp32_ir_surrogate", "fp32_ir") if f in report), wanted[0])
     h = report[head]
     roofline = {
         "bound": "mfma",
@@ -153,7 +177,7 @@ def main():
                    "the FP64 rate",
         "achieved": h["tflops"], "peak": MI355X_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
         "frac": h["tflops"] / MI355X_FP64_PEAK_TFLOPS, "traffic": None,
-        "kernel": f"k_newton_dense<64,{'true' if head == 'fp32_ir' else 'false'}>",
+        "kernel": f"k_newton_dense<64,{'false' if head == 'fp64' else 'true'}>",
         "kernel_ms": h["newton_kernel_ms"],
     }
 
@@ -179,7 +203,7 @@ def main():
         "metric": "converged DC-OP solves/s, 64k x n=64 sweep: plain FP64 vs FP32+IR+surrogate",
         "value": h["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if head == "fp32_ir" else "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if head == "fp64" else "f32+f64",
         "data": "synthetic (seeded hash-RNG Monte Carlo parameters)",
         "config": {"workload": "inverter_chain_mc", "instances": B, "unknowns": n,
                    "params_per_instance": P, "seed": 42, "form": head,
@@ -187,6 +211,7 @@ def main():
         "forms": report,
         "speedup_vs_fp64": (report[head]["solves_per_s"] / report["fp64"]["solves_per_s"]
                             if "fp64" in report else None),
+        "surrogate_training": training,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -201,7 +201,7 @@ typedef enum spx_limiter {
 typedef enum spx_warm_start {
     SPX_WARM_ZERO = 0,      /* zero initial guess (newton.rs:80-83) */
     SPX_WARM_GIVEN = 1,     /* x0[B][n] supplied by the caller */
-    SPX_WARM_SURROGATE = 2  /* predicted by the handle's trained surrogate */
+    SPX_WARM_SURROGATE = 2  /* predicted by the surrogate attached with spx_set_surrogate */
 } spx_warm_start;
 
 typedef struct spx_nr_config {
@@ -265,6 +265,50 @@ int spx_sweep_dcop_device(spx_handle* h, const spx_circuit* ckt, const spx_sweep
 
 /* Unknown count n of a circuit (num_nodes + number of voltage sources). */
 int32_t spx_circuit_size(const spx_circuit* ckt);
+
+/* ------------------------------------------------------------------------- */
+/* Learned warm start: an MLP  n_in -> n_hidden -> n_hidden -> n_out  with SiLU
+ * hidden activations.  Inputs are chosen parameter slots of the [B][P]
+ * parameter rows, outputs are the first n_out unknowns; both are
+ * standardised with statistics of the training set.  Matmuls run on BF16
+ * MFMA with FP32 accumulation (weights and activations rounded to BF16 as
+ * they enter the matrix cores); FP32 master weights are updated by Adam.
+ * Training uses only the library's own kernels.  No counterpart in the
+ * reference (its sweeps always start Newton from zero, newton.rs:80-83). */
+typedef struct spx_surrogate spx_surrogate;
+
+int spx_surrogate_create(spx_handle* h, int32_t n_in, int32_t n_hidden, int32_t n_out,
+                         uint64_t seed, spx_surrogate** out);
+int spx_surrogate_destroy(spx_surrogate* s);
+/* Parameter slots feeding the inputs (n_in entries, each in [0, P)). */
+int spx_surrogate_set_inputs(spx_surrogate* s, const int32_t* slots, int32_t P);
+/* Number of trainable FP32 values (W1,b1,W2,b2,W3,b3 in that order). */
+int64_t spx_surrogate_num_weights(const spx_surrogate* s);
+int spx_surrogate_get_weights(spx_surrogate* s, float* host, int64_t count);
+int spx_surrogate_set_weights(spx_surrogate* s, const float* host, int64_t count);
+/* Normalisation (host arrays): in_mean/in_std [n_in], out_mean/out_std [n_out]. */
+int spx_surrogate_set_normalization(spx_surrogate* s, const float* in_mean, const float* in_std,
+                                    const float* out_mean, const float* out_std);
+/* Fits the normalisation to the rows whose mask byte is non-zero (mask may be
+ * NULL), then trains with Adam for `epochs` passes in minibatches of `batch`
+ * rows shuffled by `shuffle_seed`.  d_params [B][P], d_x [B][n_x] (n_x >= n_out),
+ * device pointers; loss_history (host, may be NULL) receives each epoch's mean
+ * standardised MSE.  Synchronous. */
+int spx_surrogate_train_device(spx_surrogate* s, const double* d_params, const double* d_x,
+                               int32_t n_x, const uint8_t* d_mask, int64_t B, int32_t epochs,
+                               int32_t batch, float lr, uint64_t shuffle_seed,
+                               float* loss_history);
+/* x0[B][n_x] = prediction for the first n_out unknowns, zero for the rest.
+ * Enqueued on the handle's stream. */
+int spx_surrogate_predict_device(spx_surrogate* s, const double* d_params, int64_t B,
+                                 int32_t n_x, double* d_x0);
+/* Loss and gradient of the standardised MSE for one minibatch given already
+ * standardised FP32 inputs [B][n_in] and targets [B][n_out] (device
+ * pointers); d_grad receives num_weights values.  For gradient checks. */
+int spx_surrogate_gradient_device(spx_surrogate* s, const float* d_in, const float* d_target,
+                                  int64_t B, float* d_grad, float* loss);
+/* Attach (or detach with NULL) the surrogate used by SPX_WARM_SURROGATE. */
+int spx_set_surrogate(spx_handle* h, spx_surrogate* s);
 
 /* ------------------------------------------------------------------------- */
 /* Statistics reduction over a [B][n] buffer (device pointers), skipping

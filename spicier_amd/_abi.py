@@ -118,6 +118,13 @@ def load_library():
         raise RuntimeError(
             f"{LIB_PATH} is missing: build the HIP engine first (python -c "
             "'import __graft_entry__; __graft_entry__.build()')")
+    # torch bundles its own libamdhip64 (same soname, different file name).  If
+    # ours were loaded first, torch would later bring in a second HIP runtime
+    # and see no GPU; loading torch first makes both resolve to one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     vp, i32, i64, u8p = C.c_void_p, C.c_int32, C.c_int64, C.c_void_p
     sigs = {

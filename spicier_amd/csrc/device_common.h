@@ -215,15 +215,24 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // Lane holding the partial-pivot row: the largest |a| over eligible lanes,
 // the smallest `pos` among exact ties (nalgebra's first-maximum rule).  For
 // non-negative IEEE values the bit pattern orders like the value, so the max
-// runs on integer keys.  Returns -1 if every eligible |a| is zero.
+// runs on integer keys: first on the high 32 bits (one v_max_u32_dpp per
+// level), then — only if several lanes share that word — on the low 32 bits,
+// then on the position.  Returns -1 if every eligible |a| is zero.
 __device__ __forceinline__ int pivot_lane(double absval, bool eligible, int pos) {
-  uint64_t key = eligible ? (uint64_t)__double_as_longlong(absval) : 0ull;
-  uint64_t mx = wave_max_u64(key);
-  if (mx == 0) return -1;
-  uint64_t m = __ballot(eligible && key == mx);
-  if (__popcll(m) != 1) {  // exact tie: smallest position wins
-    uint32_t best = wave_max_u32(((m >> __lane_id()) & 1) ? (uint32_t)(64 - pos) : 0u);
-    m = __ballot(pos == (int)(64 - best));
+  const uint64_t bits = eligible ? (uint64_t)__double_as_longlong(absval) : 0ull;
+  const uint32_t hi = (uint32_t)(bits >> 32);
+  const uint32_t mhi = wave_max_u32(hi);
+  uint64_t m = __ballot(eligible && hi == mhi);
+  if (mhi == 0 || __popcll(m) != 1) {
+    const bool cand = (m >> __lane_id()) & 1;
+    const uint32_t lo = cand ? (uint32_t)bits : 0u;
+    const uint32_t mlo = wave_max_u32(lo);
+    if (mhi == 0 && mlo == 0) return -1;
+    m = __ballot(cand && lo == mlo);
+    if (__popcll(m) != 1) {  // exact tie: smallest position wins
+      const uint32_t best = wave_max_u32(((m >> __lane_id()) & 1) ? (uint32_t)(64 - pos) : 0u);
+      m = __ballot(pos == (int)(64 - best));
+    }
   }
   return (int)__ffsll((unsigned long long)m) - 1;
 }

@@ -1,0 +1,75 @@
+// handle.h — internal: the spx_handle definition and host helpers shared by
+// the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "../../include/spicier_hip.h"
+
+struct spx_surrogate;
+
+struct spx_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::map<std::string, std::pair<void*, size_t>> ws;
+  spx_surrogate* surrogate = nullptr;  // used by SPX_WARM_SURROGATE
+};
+
+namespace spx {
+
+// Message of the last failing call on this thread (spx_last_error).
+std::string& last_error();
+
+inline int fail(int code, const std::string& msg) {
+  last_error() = msg;
+  return code;
+}
+
+#define SPX_HIP(call, code)                                                               \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return ::spx::fail(code, std::string(#call) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+// Grow-only named device buffer of a handle (caller holds the handle's mutex).
+inline int workspace(spx_handle* h, const char* name, size_t bytes, void** out) {
+  auto& slot = h->ws[name];
+  if (slot.second < bytes) {
+    if (slot.first) (void)hipFree(slot.first);
+    slot = {nullptr, 0};
+    size_t want = std::max<size_t>(bytes, 256);
+    if (hipMalloc(&slot.first, want) != hipSuccess)
+      return fail(SPX_ERR_MEMORY_ALLOC, std::string("hipMalloc of ") + std::to_string(want) +
+                                            " bytes for " + name + " failed");
+    slot.second = want;
+  }
+  *out = slot.first;
+  return SPX_OK;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    (void)hipGetDevice(&prev);
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    if (prev >= 0 && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+// Enqueues the surrogate's prediction x0[B][n_x] for params [B][P] on the
+// handle's stream (surrogate.hip).
+int surrogate_predict_locked(spx_surrogate* s, const double* d_params, int64_t B, int32_t n_x,
+                             int32_t P, double* d_x0);
+
+}  // namespace spx

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/spicier_hip.h"
+#include "handle.h"
 #include "program.h"
 
 namespace spx {
@@ -31,59 +32,16 @@ hipError_t launch_solve_batch(const double* M, const double* rhs, int n, int64_t
 size_t solve_generic_workspace(int n, int64_t chunk);
 }  // namespace spx
 
+namespace spx {
+std::string& last_error() {
+  static thread_local std::string msg;
+  return msg;
+}
+}  // namespace spx
+
 using namespace spx;
 
-struct spx_handle {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::mutex mu;
-  std::map<std::string, std::pair<void*, size_t>> ws;
-};
-
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-#define SPX_HIP(call, code)                                                               \
-  do {                                                                                    \
-    hipError_t e_ = (call);                                                               \
-    if (e_ != hipSuccess)                                                                 \
-      return fail(code, std::string(#call) + ": " + hipGetErrorString(e_));               \
-  } while (0)
-
-// Grow-only named device buffer.
-int workspace(spx_handle* h, const char* name, size_t bytes, void** out) {
-  auto& slot = h->ws[name];
-  if (slot.second < bytes) {
-    if (slot.first) hipFree(slot.first);
-    slot = {nullptr, 0};
-    size_t want = std::max<size_t>(bytes, 256);
-    if (hipMalloc(&slot.first, want) != hipSuccess)
-      return fail(SPX_ERR_MEMORY_ALLOC, std::string("hipMalloc of ") + std::to_string(want) +
-                                            " bytes for " + name + " failed");
-    slot.second = want;
-  }
-  *out = slot.first;
-  return SPX_OK;
-}
-
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev) {
-    hipGetDevice(&prev);
-    if (prev != dev) hipSetDevice(dev);
-  }
-  ~DeviceGuard() {
-    int cur = -1;
-    hipGetDevice(&cur);
-    if (prev >= 0 && cur != prev) hipSetDevice(prev);
-  }
-};
 
 // ------------------------------------------------------------- circuit compile
 struct HostProgram {
@@ -356,10 +314,8 @@ int check_cfg(const spx_nr_config* c) {
     return fail(SPX_ERR_INVALID_ARGUMENT, "unknown precision");
   if (c->limiter < SPX_LIMIT_REFERENCE || c->limiter > SPX_LIMIT_PNJLIM)
     return fail(SPX_ERR_INVALID_ARGUMENT, "unknown limiter");
-  if (c->warm_start != SPX_WARM_ZERO && c->warm_start != SPX_WARM_GIVEN)
-    return fail(SPX_ERR_INVALID_ARGUMENT,
-                "warm start must be zero or given here (the surrogate feeds x0 through "
-                "spx_sweep_dcop_device)");
+  if (c->warm_start < SPX_WARM_ZERO || c->warm_start > SPX_WARM_SURROGATE)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "unknown warm start");
   if (c->precision == SPX_PREC_F32_IR && (c->ir_max_steps < 1 || c->ir_max_steps > 1000))
     return fail(SPX_ERR_INVALID_ARGUMENT, "ir_max_steps must be 1..1000");
   if (c->gmin_stepping && !(c->gmin_factor > 1.0 && c->gmin_final > 0.0))
@@ -379,6 +335,8 @@ int sweep_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
   if ((rc = check_spec(spec, hp.P))) return rc;
   if (cfg->warm_start == SPX_WARM_GIVEN && !d_x0)
     return fail(SPX_ERR_INVALID_ARGUMENT, "warm start given without x0");
+  if (cfg->warm_start == SPX_WARM_SURROGATE && !h->surrogate)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "surrogate warm start without an attached surrogate");
   const int64_t B = spec->batch;
   if (B == 0) return SPX_OK;
   DevProgram g;
@@ -401,6 +359,14 @@ int sweep_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
            out->refinements};
   DevNrCfg dc = to_dev(cfg);
   const double* x0 = cfg->warm_start == SPX_WARM_GIVEN ? d_x0 : nullptr;
+  if (cfg->warm_start == SPX_WARM_SURROGATE) {
+    void* p = nullptr;
+    if ((rc = workspace(h, "x0_surrogate", sizeof(double) * B * hp.n, &p))) return rc;
+    if ((rc = surrogate_predict_locked(h->surrogate, params, B, hp.n, hp.P,
+                                       static_cast<double*>(p))))
+      return rc;
+    x0 = static_cast<double*>(p);
+  }
   SPX_HIP(launch_newton(g, dc, params, x0, B, o, h->stream), SPX_ERR_BACKEND);
   if (out->node_stats) {
     void* part = nullptr;
@@ -464,7 +430,7 @@ int spx_destroy(spx_handle* h) {
   return SPX_OK;
 }
 
-const char* spx_last_error(void) { return g_err.c_str(); }
+const char* spx_last_error(void) { return last_error().c_str(); }
 
 int spx_synchronize(spx_handle* h) {
   if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");

@@ -22,10 +22,12 @@ __global__ __launch_bounds__(64) void k_solve_rows(const double* __restrict__ M,
                                                    const double* __restrict__ rhs, int n,
                                                    double* __restrict__ x,
                                                    uint8_t* __restrict__ sing) {
+  __shared__ double row[N];
   const int64_t i = blockIdx.x;
   const int lane = threadIdx.x;
   const double* A = M + i * (int64_t)n * n;
   RowLU<N, double> lu;
+  lu.row = row;
   static_for<0, N>([&](auto J) __attribute__((always_inline)) {
     constexpr int j = J;
     lu.a[j] = (j < n && lane < n) ? A[(int64_t)j * n + lane] : (j == lane ? 1.0 : 0.0);

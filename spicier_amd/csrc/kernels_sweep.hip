@@ -152,7 +152,7 @@ SPX_DECLARE_NEWTON(32, true, launch_newton_ir_32)
 SPX_DECLARE_NEWTON(64, true, launch_newton_ir_64)
 
 size_t newton_smem_bytes(const DevProgram& g, int N) {
-  return sizeof(double) * ((size_t)N * (N + 2) + 3 * N + g.num_vals + g.P + g.num_elems);
+  return sizeof(double) * ((size_t)N * (N + 2) + 4 * N + g.num_vals + g.P + g.num_elems);
 }
 
 hipError_t launch_newton(const DevProgram& g, const DevNrCfg& cfg, const double* params,

@@ -8,9 +8,9 @@
 // reference's LU.  The pivot of step k is the largest |A[.][k]| over rows at
 // positions >= k, smallest position on ties — nalgebra's icamax rule (first
 // maximum), which CpuBatchedSolver inherits (spicier-batched-sweep
-// src/solver.rs:370-380).  The pivot row is broadcast to the wave with
-// v_readlane (one SGPR pair per element), multipliers are formed as
-// a * (1/pivot) and updates as fma(-u, l, a).
+// src/solver.rs:370-380).  Multipliers are formed as a * (1/pivot) and
+// updates as fma(-u, l, a).  The caller's workgroup is exactly one wave, so
+// __syncthreads() orders LDS traffic without a real barrier.
 //
 // Lanes >= n carry identity rows (a[j] = [j == lane]) so that padding never
 // wins a pivot search and solves to zero.
@@ -48,11 +48,19 @@ __device__ __forceinline__ float bcast(float v, int lane) { return readlane_f32(
 template <typename T>
 __device__ __forceinline__ T tabs(T v) { return v < T(0) ? -v : v; }
 
-template <int N, typename T>
+// Pivot-row broadcast: kBcastReadlane moves each element through an SGPR
+// (v_readlane, VALU), kBcastLds has the pivot lane store its row to LDS and
+// every lane read it back with same-address loads (LDS pipe).  The first is
+// VALU-bound, the second LDS-bound; which wins depends on n and occupancy.
+enum { kBcastReadlane = 0, kBcastLds = 1 };
+
+template <int N, typename T, int Bcast = kBcastReadlane>
 struct RowLU {
   T a[N];
-  int pos;
+  int pos;     // position of this lane's row after pivoting
+  int orig;    // lane whose row sits at position `lane`
   T dinv;      // 1 / U[pos][pos] of this lane's row
+  T* row;      // N elements of LDS: broadcast buffer (and scatter space)
 
   // Factor in place.  Returns false if a pivot column is zero (or NaN): the
   // reference's LU then reports the system singular.  Every step runs over
@@ -73,39 +81,62 @@ struct RowLU {
       const int bp = __builtin_amdgcn_readlane(pos, p);
       if (pos == k) pos = bp;  // row at position k moves to the pivot's position
       if (lane == p) pos = k;
-      const T pv = bcast(a[k], p);
-      const T inv = T(1) / pv;
-      if (lane == p) dinv = inv;
-      const T l = pos > k ? a[k] * inv : T(0);
-      if (pos > k) a[k] = l;
-      static_for<k + 1, N>([&](auto J) __attribute__((always_inline)) {
-        constexpr int j = J;
-        a[j] = fma(-bcast(a[j], p), l, a[j]);
-      });
+      if constexpr (Bcast == kBcastLds) {
+        if (lane == p) {
+          static_for<k, N>([&](auto J) __attribute__((always_inline)) {
+            constexpr int j = J;
+            row[j] = a[j];
+          });
+        }
+        __syncthreads();
+        const T inv = T(1) / row[k];
+        if (lane == p) dinv = inv;
+        const T l = pos > k ? a[k] * inv : T(0);
+        if (pos > k) a[k] = l;
+        static_for<k + 1, N>([&](auto J) __attribute__((always_inline)) {
+          constexpr int j = J;
+          a[j] = fma(-row[j], l, a[j]);
+        });
+        __syncthreads();
+      } else {
+        const T inv = T(1) / bcast(a[k], p);
+        if (lane == p) dinv = inv;
+        const T l = pos > k ? a[k] * inv : T(0);
+        if (pos > k) a[k] = l;
+        static_for<k + 1, N>([&](auto J) __attribute__((always_inline)) {
+          constexpr int j = J;
+          a[j] = fma(-bcast(a[j], p), l, a[j]);
+        });
+      }
     });
+    if (ok) {  // invert the position map once, so solves need no ballot per step
+      int* slot = reinterpret_cast<int*>(row);
+      slot[pos] = lane;
+      __syncthreads();
+      orig = slot[lane];
+      __syncthreads();
+    }
     return ok;
-  }
-
-  // Lane whose row sits at position k (wave-uniform).
-  __device__ __forceinline__ int lane_at(int k) const {
-    return __builtin_amdgcn_readfirstlane((int)__ffsll((unsigned long long)__ballot(pos == k)) - 1);
   }
 
   // Solve with the stored factors for right-hand side b (b of lane r is
   // b[r]).  Returns x[lane] in each lane.  Forward substitution replays the
   // elimination on b; back substitution divides by the pivot (exact division
-  // when `divide`, multiplication by the stored inverse otherwise).
+  // when `divide`, multiplication by the stored inverse otherwise).  The lane
+  // of step k is read from `orig` of lane k, independent of b, so only the
+  // broadcast of b itself is on the dependency chain.
   template <bool divide>
   __device__ __forceinline__ T solve(T b, int lane) const {
     static_for<0, N>([&](auto K) __attribute__((always_inline)) {
       constexpr int k = K;
-      const T bp = bcast(b, lane_at(k));
-      b = fma(-bp, pos > k ? a[k] : T(0), b);
+      const int p = __builtin_amdgcn_readlane(orig, k);
+      const T l = pos > k ? a[k] : T(0);
+      b = fma(-bcast(b, p), l, b);
     });
     T x = T(0);
     static_for_down<0, N>([&](auto K) __attribute__((always_inline)) {
       constexpr int k = K;
-      const int p = lane_at(k);
+      const int p = __builtin_amdgcn_readlane(orig, k);
       const T bk = bcast(b, p);
       const T xk = divide ? bk / bcast(a[k], p) : bk * bcast(dinv, p);
       if (lane == k) x = xk;

@@ -24,7 +24,7 @@
 namespace spx {
 
 inline size_t newton_smem_doubles(const DevProgram& g, int N) {
-  return (size_t)N * (N + 2) + 3 * N + g.num_vals + g.P + g.num_elems;
+  return (size_t)N * (N + 2) + 4 * N + g.num_vals + g.P + g.num_elems;
 }
 
 template <int N, bool IR>
@@ -39,6 +39,7 @@ struct Instance {
   double* vals;   // num_vals
   double* pv;     // P parameters
   double* vst;    // num_elems pnjlim junction voltages
+  double* prow;   // N: LU pivot-row broadcast buffer
   int nfact = 0;  // LU factorizations performed
   int nref = 0;   // refinement steps performed
   static constexpr int S = N + 2;
@@ -125,6 +126,7 @@ struct Instance {
   __device__ __forceinline__ bool solve_f64(double& xnew) {
     ++nfact;
     RowLU<N, double> lu;
+    lu.row = prow;
     load_rows(lu.a);
     if (!lu.factor(lane)) return false;
     xnew = lu.template solve<true>(lane < g.n ? rhs[lane] : 0.0, lane);
@@ -179,6 +181,7 @@ struct Instance {
     for (int i = lane; i < g.num_elems; i += kWave) vst[i] = 0.0;
     __syncthreads();
     RowLU<N, float> lu32;
+    lu32.row = reinterpret_cast<float*>(prow);
     bool have32 = false;
     for (int it = 0; it < cfg.max_iterations; ++it) {
       const bool lim = assemble(gmin, sf, cfg.limiter);
@@ -277,7 +280,8 @@ __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
   double* vals = xw + N;
   double* pv = vals + g.num_vals;
   double* vst = pv + g.P;
-  Instance<N, IR> I{g, cfg, lane, J, rhs, xs, xw, vals, pv, vst};
+  double* prow = vst + g.num_elems;
+  Instance<N, IR> I{g, cfg, lane, J, rhs, xs, xw, vals, pv, vst, prow};
   const int n = g.n;
 
   for (int i = lane; i < N * S; i += kWave) J[i] = 0.0;

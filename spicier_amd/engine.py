@@ -418,3 +418,148 @@ def sample_params(circuit, batch: int, seed: int = 42, generator: int = A.SPX_GE
     out = np.zeros((batch, circuit.num_params))
     _check(dev.lib.spx_sample_params(dev.handle, C.byref(spec), circuit.num_params, A.ptr(out)))
     return out
+
+
+# ---------------------------------------------------------------- learned warm start
+class Surrogate:
+    """MLP warm start (include/spicier_hip.h spx_surrogate_*): n_in sampled
+    parameters -> H -> H (SiLU) -> n_out node voltages, trained and evaluated
+    by the library's BF16-MFMA kernels.  Device buffers are torch tensors."""
+
+    def __init__(self, n_in: int = 32, hidden: int = 256, n_out: int = 64, seed: int = 1,
+                 device_id: int = 0):
+        self.device = Device.get(device_id)
+        self.lib = self.device.lib
+        self._bind()
+        s = C.c_void_p()
+        _check(self.lib.spx_surrogate_create(self.device.handle, n_in, hidden, n_out, seed,
+                                             C.byref(s)))
+        self.ptr = s
+        self.n_in, self.hidden, self.n_out = n_in, hidden, n_out
+        self.loss_history = []
+
+    def _bind(self):
+        L, vp, i32, i64 = self.lib, C.c_void_p, C.c_int32, C.c_int64
+        sigs = {
+            "spx_surrogate_create": (C.c_int, [vp, i32, i32, i32, C.c_uint64, C.POINTER(vp)]),
+            "spx_surrogate_destroy": (C.c_int, [vp]),
+            "spx_surrogate_set_inputs": (C.c_int, [vp, vp, i32]),
+            "spx_surrogate_num_weights": (C.c_int64, [vp]),
+            "spx_surrogate_get_weights": (C.c_int, [vp, vp, i64]),
+            "spx_surrogate_set_weights": (C.c_int, [vp, vp, i64]),
+            "spx_surrogate_set_normalization": (C.c_int, [vp, vp, vp, vp, vp]),
+            "spx_surrogate_train_device": (C.c_int, [vp, vp, vp, i32, vp, i64, i32, i32,
+                                                     C.c_float, C.c_uint64, vp]),
+            "spx_surrogate_predict_device": (C.c_int, [vp, vp, i64, i32, vp]),
+            "spx_surrogate_gradient_device": (C.c_int, [vp, vp, vp, i64, vp,
+                                                        C.POINTER(C.c_float)]),
+            "spx_set_surrogate": (C.c_int, [vp, vp]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+
+    @property
+    def num_weights(self) -> int:
+        return int(self.lib.spx_surrogate_num_weights(self.ptr))
+
+    def set_inputs(self, slots, num_params: int):
+        arr = np.ascontiguousarray(slots, dtype=np.int32)
+        assert arr.size == self.n_in
+        _check(self.lib.spx_surrogate_set_inputs(self.ptr, A.ptr(arr), num_params))
+
+    def get_weights(self) -> np.ndarray:
+        w = np.zeros(self.num_weights, dtype=np.float32)
+        _check(self.lib.spx_surrogate_get_weights(self.ptr, A.ptr(w), w.size))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(self.lib.spx_surrogate_set_weights(self.ptr, A.ptr(w), w.size))
+
+    def set_normalization(self, in_mean, in_std, out_mean, out_std):
+        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (in_mean, in_std, out_mean,
+                                                               out_std)]
+        _check(self.lib.spx_surrogate_set_normalization(self.ptr, *[A.ptr(v) for v in a]))
+
+    def gradient(self, X, T):
+        """Loss and gradient for standardised inputs/targets (host arrays)."""
+        import torch
+        dev = torch.device(f"cuda:{self.device.device_id}")
+        x = torch.as_tensor(np.ascontiguousarray(X, dtype=np.float32), device=dev)
+        t = torch.as_tensor(np.ascontiguousarray(T, dtype=np.float32), device=dev)
+        g = torch.empty(self.num_weights, dtype=torch.float32, device=dev)
+        loss = C.c_float(0)
+        torch.cuda.synchronize()
+        _check(self.lib.spx_surrogate_gradient_device(self.ptr, A.ptr(x), A.ptr(t), x.shape[0],
+                                                      A.ptr(g), C.byref(loss)))
+        return float(loss.value), g.cpu().numpy()
+
+    def train(self, d_params, d_x, n_x: int, d_mask=None, epochs: int = 30, batch: int = 1024,
+              lr: float = 2e-3, shuffle_seed: int = 7):
+        B = int(d_params.shape[0])
+        hist = np.zeros(max(epochs, 1), dtype=np.float32)
+        _check(self.lib.spx_surrogate_train_device(self.ptr, A.ptr(d_params), A.ptr(d_x), n_x,
+                                                   A.ptr(d_mask), B, epochs, batch, lr,
+                                                   shuffle_seed, A.ptr(hist)))
+        self.loss_history = hist[:epochs].tolist()
+        return self.loss_history
+
+    def predict(self, d_params, n_x: int, d_x0):
+        _check(self.lib.spx_surrogate_predict_device(self.ptr, A.ptr(d_params),
+                                                     int(d_params.shape[0]), n_x, A.ptr(d_x0)))
+
+    def attach(self):
+        """Make this the surrogate used by SPX_WARM_SURROGATE sweeps."""
+        _check(self.lib.spx_set_surrogate(self.device.handle, self.ptr))
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None):
+                self.lib.spx_surrogate_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def sweep_dcop_device(circuit, batch: int, buffers: dict, seed: int = 42,
+                      generator: int = A.SPX_GEN_HASH, sweep_offset: int = 0,
+                      config: A.SpxNrConfig | None = None, device_id: int = 0):
+    """Sweep into caller-owned device buffers (torch tensors keyed x, status,
+    iterations, flags, residual, params, node_stats, factorizations,
+    refinements; any but x may be absent).  Enqueued on the library stream."""
+    dev = Device.get(device_id)
+    cfg = config or nr_config()
+    ckt, keep = circuit.to_c()
+    spec, keep2 = circuit.sweep_spec(batch, seed, generator, sweep_offset)
+    g = buffers.get
+    res = A.SpxSweepResult(A.ptr(g("x")), A.ptr(g("status")), A.ptr(g("iterations")),
+                           A.ptr(g("flags")), A.ptr(g("residual")), A.ptr(g("params")),
+                           A.ptr(g("node_stats")), A.ptr(g("factorizations")),
+                           A.ptr(g("refinements")))
+    _check(dev.lib.spx_sweep_dcop_device(dev.handle, C.byref(ckt), C.byref(spec), C.byref(cfg),
+                                         None, C.byref(res)))
+
+
+def train_surrogate(circuit, input_slots, n_train: int = 262144, train_offset: int = 1 << 20,
+                    seed: int = 42, epochs: int = 30, batch: int = 1024, lr: float = 2e-3,
+                    hidden: int = 256, device_id: int = 0):
+    """Produce FP64 Newton operating points of `circuit` for parameters drawn at
+    sweep indices [train_offset, train_offset + n_train), then train a
+    surrogate on them.  Returns the surrogate (not yet attached)."""
+    import torch
+    dev = torch.device(f"cuda:{device_id}")
+    n, P = circuit.size, circuit.num_params
+    bufs = dict(x=torch.empty(n_train, n, dtype=torch.float64, device=dev),
+                status=torch.empty(n_train, dtype=torch.int32, device=dev),
+                flags=torch.empty(n_train, dtype=torch.int32, device=dev),
+                params=torch.empty(n_train, P, dtype=torch.float64, device=dev))
+    torch.cuda.synchronize()
+    sweep_dcop_device(circuit, n_train, bufs, seed=seed, sweep_offset=train_offset,
+                      config=nr_config(), device_id=device_id)
+    Device.get(device_id).synchronize()
+    mask = ((bufs["status"] == A.SPX_CONVERGED) &
+            ((bufs["flags"] & A.SPX_FLAG_KCL_FAIL) == 0)).to(torch.uint8)
+    s = Surrogate(len(input_slots), hidden, n, seed=seed, device_id=device_id)
+    s.set_inputs(input_slots, P)
+    s.train(bufs["params"], bufs["x"], n, mask, epochs=epochs, batch=batch, lr=lr)
+    return s
