@@ -56,7 +56,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32_ir,fp32_ir_surrogate")
     ap.add_argument("--train", type=int, default=262144)
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the sweep shards by instance range with no "
@@ -167,7 +167,8 @@ def main():
             tflops=(flops64 + flops32) / k_s / 1e12,
             fallback_rate=float(np.count_nonzero(fl & (A.SPX_FLAG_GMIN_STEPPING
                                                      | A.SPX_FLAG_SOURCE_STEPPING)) / B),
-            fp64_escalations=int(np.count_nonzero(fl & A.SPX_FLAG_FP64_ESCALATED)))
+            fp64_escalations=int(np.count_nonzero(fl & A.SPX_FLAG_FP64_ESCALATED)),
+            iteration_histogram=np.bincount(it, minlength=8)[:8].tolist())
 
     head = next((f for f in ("fp32_ir_surrogate", "fp32_ir") if f in report), wanted[0])
     h = report[head]

@@ -292,12 +292,15 @@ int spx_surrogate_set_normalization(spx_surrogate* s, const float* in_mean, cons
 /* Fits the normalisation to the rows whose mask byte is non-zero (mask may be
  * NULL), then trains with Adam for `epochs` passes in minibatches of `batch`
  * rows shuffled by `shuffle_seed`.  d_params [B][P], d_x [B][n_x] (n_x >= n_out),
- * device pointers; loss_history (host, may be NULL) receives each epoch's mean
- * standardised MSE.  Synchronous. */
+ * device pointers.  With tol_rel or tol_abs > 0, output j's squared error is
+ * weighted by (std_j / (tol_rel |mean_j| + tol_abs))^2 (normalised to mean 1),
+ * i.e. measured in units of the Newton tolerance the warm start has to meet.
+ * loss_history (host, may be NULL) receives each epoch's mean loss.
+ * Synchronous. */
 int spx_surrogate_train_device(spx_surrogate* s, const double* d_params, const double* d_x,
                                int32_t n_x, const uint8_t* d_mask, int64_t B, int32_t epochs,
-                               int32_t batch, float lr, uint64_t shuffle_seed,
-                               float* loss_history);
+                               int32_t batch, float lr, float tol_rel, float tol_abs,
+                               uint64_t shuffle_seed, float* loss_history);
 /* x0[B][n_x] = prediction for the first n_out unknowns, zero for the rest.
  * Enqueued on the handle's stream. */
 int spx_surrogate_predict_device(spx_surrogate* s, const double* d_params, int64_t B,

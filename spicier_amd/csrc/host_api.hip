@@ -350,7 +350,7 @@ int sweep_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
   }
   if ((rc = sample_device(h, spec, hp.P, params))) return rc;
   int* status = out->status;
-  if (!status && out->node_stats) {
+  if (!status && (out->node_stats || cfg->precision == SPX_PREC_F32_IR)) {
     void* p = nullptr;
     if ((rc = workspace(h, "status", sizeof(int) * B, &p))) return rc;
     status = static_cast<int*>(p);

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_stats_merge(const double* __restrict__ 
 // ------------------------------------------------------------- launchers
 #define SPX_DECLARE_NEWTON(N, IR, NAME)                                                   \
   hipError_t NAME(const DevProgram&, const DevNrCfg&, const double*, const double*, int64_t, \
-                  const DevOut&, hipStream_t);
+                  const DevOut&, int, hipStream_t);
 SPX_DECLARE_NEWTON(8, false, launch_newton_f64_8)
 SPX_DECLARE_NEWTON(16, false, launch_newton_f64_16)
 SPX_DECLARE_NEWTON(32, false, launch_newton_f64_32)
@@ -155,16 +155,22 @@ size_t newton_smem_bytes(const DevProgram& g, int N) {
   return sizeof(double) * ((size_t)N * (N + 2) + 4 * N + g.num_vals + g.P + g.num_elems);
 }
 
+// FP32+IR runs in two passes: the FP32 kernel leaves instances it cannot
+// resolve at SPX_ACTIVE, and the FP64 kernel, launched over the same grid,
+// solves only those (every other workgroup exits on its first load).
 hipError_t launch_newton(const DevProgram& g, const DevNrCfg& cfg, const double* params,
                          const double* x0, int64_t B, const DevOut& out, hipStream_t st) {
   const bool ir = cfg.precision == SPX_PREC_F32_IR;
-  if (g.n <= 8)
-    return (ir ? launch_newton_ir_8 : launch_newton_f64_8)(g, cfg, params, x0, B, out, st);
-  if (g.n <= 16)
-    return (ir ? launch_newton_ir_16 : launch_newton_f64_16)(g, cfg, params, x0, B, out, st);
-  if (g.n <= 32)
-    return (ir ? launch_newton_ir_32 : launch_newton_f64_32)(g, cfg, params, x0, B, out, st);
-  return (ir ? launch_newton_ir_64 : launch_newton_f64_64)(g, cfg, params, x0, B, out, st);
+  typedef hipError_t (*Fn)(const DevProgram&, const DevNrCfg&, const double*, const double*,
+                           int64_t, const DevOut&, int, hipStream_t);
+  Fn f64 = g.n <= 8 ? launch_newton_f64_8 : g.n <= 16 ? launch_newton_f64_16
+         : g.n <= 32 ? launch_newton_f64_32 : launch_newton_f64_64;
+  Fn f32 = g.n <= 8 ? launch_newton_ir_8 : g.n <= 16 ? launch_newton_ir_16
+         : g.n <= 32 ? launch_newton_ir_32 : launch_newton_ir_64;
+  if (!ir) return f64(g, cfg, params, x0, B, out, 0, st);
+  hipError_t e = f32(g, cfg, params, x0, B, out, 0, st);
+  if (e != hipSuccess) return e;
+  return f64(g, cfg, params, x0, B, out, 1, st);
 }
 
 hipError_t launch_sample(const spx_param_spec* d_specs, int P, int gen, uint64_t seed, int64_t B,

@@ -199,10 +199,9 @@ struct Instance {
           }
           rc = refine(lu32, fresh, xnew);
         }
-        if (rc != 0) {  // FP32 cannot resolve this system: FP64 for this iteration
-          ++fp64_escalations;
-          have32 = false;
-          ok = solve_f64(xnew);
+        if (rc != 0) {  // FP32 cannot resolve this system: hand the instance to FP64
+          iters = it + 1;
+          return -2;
         }
       } else {
         ok = solve_f64(xnew);
@@ -264,14 +263,17 @@ struct Instance {
   }
 };
 
+// escalated_only: run only the instances an FP32+IR pass handed over (status
+// left SPX_ACTIVE), in FP64.
 template <int N, bool IR>
 __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
                                                      const double* __restrict__ params,
                                                      const double* __restrict__ x0, int64_t B,
-                                                     DevOut out) {
+                                                     DevOut out, int escalated_only) {
   extern __shared__ double smem[];
   const int64_t inst = blockIdx.x;
   const int lane = threadIdx.x;
+  if (escalated_only && out.status[inst] != SPX_ACTIVE) return;
   constexpr int S = N + 2;
   double* J = smem;
   double* rhs = J + N * S;
@@ -291,14 +293,25 @@ __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
   __syncthreads();
 
   int total = 0, it = 0, esc = 0, status = SPX_FAILED;
-  int flags = x0 ? SPX_FLAG_WARM_START : 0;
+  int flags = (x0 ? SPX_FLAG_WARM_START : 0) | (escalated_only ? SPX_FLAG_FP64_ESCALATED : 0);
   double final_gmin = 0.0;
+  bool handover = false;  // FP32+IR pass: leave this instance to the FP64 pass
 
   if (!g.has_nonlinear) {
     // linear circuit: one solve (sweep.rs:159-217)
     I.assemble(0.0, 1.0, cfg.limiter);
     double xn = 0.0;
-    bool ok = I.solve_f64(xn);
+    bool ok;
+    if constexpr (IR) {
+      RowLU<N, float> lu32;
+      lu32.row = reinterpret_cast<float*>(prow);
+      I.load_rows(lu32.a);
+      ++I.nfact;
+      ok = lu32.factor(lane) && I.refine(lu32, true, xn) == 0;
+      handover = !ok;  // FP64 decides between singular and an FP32 limitation
+    } else {
+      ok = I.solve_f64(xn);
+    }
     if (lane < n) xs[lane] = ok ? xn : 0.0;
     __syncthreads();
     status = ok ? SPX_CONVERGED : SPX_SINGULAR;
@@ -348,7 +361,10 @@ __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
       }
       const int st = I.newton(lg, lsf, it, esc);
       total += it;
-      if (stage == kPlain) {
+      if (st == -2) {
+        handover = true;
+        stage = kDone;
+      } else if (stage == kPlain) {
         status = st == 1 ? SPX_CONVERGED : (st < 0 ? SPX_SINGULAR : SPX_FAILED);
         stage = status == SPX_CONVERGED ? kDone : after_failure();
       } else if (stage == kGmin) {
@@ -404,7 +420,13 @@ __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
       }
     }
   }
-  if (esc) flags |= SPX_FLAG_FP64_ESCALATED;
+  if (handover) {
+    if (lane == 0) {
+      out.status[inst] = SPX_ACTIVE;
+      if (out.flags) out.flags[inst] = flags | SPX_FLAG_FP64_ESCALATED;
+    }
+    return;
+  }
   double res = I.kcl(final_gmin);
   if (status == SPX_CONVERGED && !(res <= 1.0)) flags |= SPX_FLAG_KCL_FAIL;
   if (lane < n) out.x[inst * n + lane] = status == SPX_SINGULAR ? 0.0 : xs[lane];
@@ -421,13 +443,14 @@ __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
 
 template <int N, bool IR>
 hipError_t launch_newton_impl(const DevProgram& g, const DevNrCfg& cfg, const double* params,
-                              const double* x0, int64_t B, const DevOut& out, hipStream_t st) {
+                              const double* x0, int64_t B, const DevOut& out,
+                              int escalated_only, hipStream_t st) {
   size_t sm = sizeof(double) * newton_smem_doubles(g, N);
   hipError_t e = hipFuncSetAttribute((const void*)k_newton_dense<N, IR>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_newton_dense<N, IR>), dim3((unsigned)B), dim3(kWave), sm, st, g, cfg,
-                     params, x0, B, out);
+                     params, x0, B, out, escalated_only);
   return hipGetLastError();
 }
 

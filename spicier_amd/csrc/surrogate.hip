@@ -33,6 +33,7 @@ struct spx_surrogate {
   float* d_m = nullptr;     // Adam moments
   float* d_v = nullptr;
   float* d_norm = nullptr;  // in_mean[n_in], in_std[n_in], out_mean[n_out], out_std[n_out]
+  float* d_wout = nullptr;  // per-output loss weights (training only)
   int64_t nw = 0;
   int64_t adam_t = 0;
   // activation buffers, sized for `cap` rows
@@ -150,16 +151,19 @@ __global__ void k_gather_targets(const double* __restrict__ x, int n_x, int n_ou
   T[t] = (float)((x[row * n_x + j] - om[j]) / os[j]);
 }
 
-// dY = 2 (Y - T) / count; loss += sum (Y - T)^2 / count
+// dY = 2 w (Y - T) / count; loss += sum w (Y - T)^2 / count, w per output
+// column (1 when wout is null)
 __global__ void k_mse(const float* __restrict__ Y, const float* __restrict__ T, int64_t count,
-                      float* __restrict__ dY, float* __restrict__ loss) {
+                      int n_out, const float* __restrict__ wout, float* __restrict__ dY,
+                      float* __restrict__ loss) {
   __shared__ float red[256];
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float e = 0.f;
   if (t < count) {
+    const float w = wout ? wout[t % n_out] : 1.f;
     float d = Y[t] - T[t];
-    dY[t] = 2.f * d / (float)count;
-    e = d * d / (float)count;
+    dY[t] = 2.f * w * d / (float)count;
+    e = w * d * d / (float)count;
   }
   red[threadIdx.x] = e;
   __syncthreads();
@@ -296,14 +300,15 @@ void forward(spx_surrogate* s, int64_t rows) {
        w + s->off_b3(), nullptr);
 }
 
-// Loss + gradient of the standardised MSE into s->d_grad (targets in s->T).
-void backward(spx_surrogate* s, int64_t rows) {
+// Loss + gradient of the standardised MSE into s->d_grad (targets in s->T);
+// `wout` weights the output columns (null = unweighted).
+void backward(spx_surrogate* s, int64_t rows, const float* wout) {
   hipStream_t st = s->h->stream;
   const int H = s->H, ni = s->n_in, no = s->n_out;
   const float* w = s->d_w;
   float* g = s->d_grad;
   const int64_t cy = rows * no, ch = rows * H;
-  hipLaunchKernelGGL(k_mse, dim3(blocks(cy)), dim3(256), 0, st, s->Y, s->T, cy, s->dY,
+  hipLaunchKernelGGL(k_mse, dim3(blocks(cy)), dim3(256), 0, st, s->Y, s->T, cy, no, wout, s->dY,
                      s->d_loss);
   // layer 3: dW3 = H2^T dY, db3 = colsum dY, dH2 = dY W3^T
   gemm(st, s->H2, 1, H, s->dY, no, 1, g + s->off_W3(), no, H, no, (int)rows, 0.f, nullptr,
@@ -415,7 +420,7 @@ int spx_surrogate_destroy(spx_surrogate* s) {
     (void)hipStreamSynchronize(h->stream);
     if (h->surrogate == s) h->surrogate = nullptr;
     void* ptrs[] = {s->d_w, s->d_grad, s->d_m, s->d_v, s->d_norm, s->d_loss, s->d_slots,
-                    s->d_perm, s->X, s->T, s->Z1, s->H1, s->Z2, s->H2, s->Y, s->dY, s->dH, s->dZ};
+                    s->d_perm, s->d_wout, s->X, s->T, s->Z1, s->H1, s->Z2, s->H2, s->Y, s->dY, s->dH, s->dZ};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
   }
@@ -479,8 +484,8 @@ int spx_surrogate_set_normalization(spx_surrogate* s, const float* in_mean, cons
 
 int spx_surrogate_train_device(spx_surrogate* s, const double* d_params, const double* d_x,
                                int32_t n_x, const uint8_t* d_mask, int64_t B, int32_t epochs,
-                               int32_t batch, float lr, uint64_t shuffle_seed,
-                               float* loss_history) {
+                               int32_t batch, float lr, float tol_rel, float tol_abs,
+                               uint64_t shuffle_seed, float* loss_history) {
   if (!s) return fail(SPX_ERR_INVALID_ARGUMENT, "null surrogate");
   if (s->slots.empty()) return fail(SPX_ERR_INVALID_ARGUMENT, "surrogate inputs not set");
   if (n_x < s->n_out) return fail(SPX_ERR_INVALID_DIMENSION, "n_x smaller than surrogate output");
@@ -504,6 +509,31 @@ int spx_surrogate_train_device(spx_surrogate* s, const double* d_params, const d
                      s->d_slots, B, d_mask, nm, nm + s->n_in);
   hipLaunchKernelGGL(k_colstats, dim3(s->n_out), dim3(256), 0, st, d_x, (int64_t)n_x, d_oidx, B,
                      d_mask, nm + 2 * s->n_in, nm + 2 * s->n_in + s->n_out);
+  // Output weights: a warm start is good when every node lands inside its
+  // Newton tolerance tol_j = tol_rel |mean_j| + tol_abs, so errors are
+  // weighted by (std_j / tol_j)^2, normalised to mean 1 and clipped to 1e-3..1e3.
+  const bool weighted = tol_rel > 0.f || tol_abs > 0.f;
+  if (weighted) {
+    std::vector<float> norm(2 * s->n_in + 2 * s->n_out);
+    SPX_HIP(hipMemcpyAsync(norm.data(), nm, sizeof(float) * norm.size(), hipMemcpyDeviceToHost, st),
+            SPX_ERR_TRANSFER);
+    SPX_HIP(hipStreamSynchronize(st), SPX_ERR_TRANSFER);
+    std::vector<float> w(s->n_out);
+    double sum = 0;
+    for (int j = 0; j < s->n_out; ++j) {
+      const double mu = norm[2 * s->n_in + j], sd = norm[2 * s->n_in + s->n_out + j];
+      const double tol = tol_rel * std::fabs(mu) + tol_abs;
+      w[j] = (float)std::pow(sd / std::max(tol, 1e-30), 2.0);
+      sum += w[j];
+    }
+    for (int j = 0; j < s->n_out; ++j)
+      w[j] = std::min(1e3f, std::max(1e-3f, (float)(w[j] * s->n_out / sum)));
+    if (!s->d_wout) SPX_HIP(hipMalloc(&s->d_wout, sizeof(float) * s->n_out), SPX_ERR_MEMORY_ALLOC);
+    SPX_HIP(hipMemcpyAsync(s->d_wout, w.data(), sizeof(float) * s->n_out, hipMemcpyHostToDevice,
+                           st),
+            SPX_ERR_TRANSFER);
+    SPX_HIP(hipStreamSynchronize(st), SPX_ERR_TRANSFER);
+  }
   // rows used for training: the masked ones
   std::vector<int> rows;
   if (d_mask) {
@@ -540,7 +570,7 @@ int spx_surrogate_train_device(spx_surrogate* s, const double* d_params, const d
       hipLaunchKernelGGL(k_gather_targets, dim3(blocks(nb * s->n_out)), dim3(256), 0, st, d_x,
                          n_x, s->n_out, s->n_in, s->d_norm, s->d_perm, b0, nb, s->T);
       forward(s, nb);
-      backward(s, nb);
+      backward(s, nb, weighted ? s->d_wout : nullptr);
       // cosine decay to 5% of lr, Adam with bias correction
       const double frac = (double)step / (double)std::max<int64_t>(1, total - 1);
       const float lrt = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(M_PI * frac))));
@@ -586,7 +616,7 @@ int spx_surrogate_gradient_device(spx_surrogate* s, const float* d_in, const flo
           SPX_ERR_TRANSFER);
   SPX_HIP(hipMemsetAsync(s->d_loss, 0, sizeof(float), st), SPX_ERR_BACKEND);
   forward(s, B);
-  backward(s, B);
+  backward(s, B, nullptr);
   SPX_HIP(hipMemcpyAsync(d_grad, s->d_grad, sizeof(float) * s->nw, hipMemcpyDeviceToDevice, st),
           SPX_ERR_TRANSFER);
   float l = 0.f;

@@ -449,7 +449,8 @@ class Surrogate:
             "spx_surrogate_set_weights": (C.c_int, [vp, vp, i64]),
             "spx_surrogate_set_normalization": (C.c_int, [vp, vp, vp, vp, vp]),
             "spx_surrogate_train_device": (C.c_int, [vp, vp, vp, i32, vp, i64, i32, i32,
-                                                     C.c_float, C.c_uint64, vp]),
+                                                     C.c_float, C.c_float, C.c_float,
+                                                     C.c_uint64, vp]),
             "spx_surrogate_predict_device": (C.c_int, [vp, vp, i64, i32, vp]),
             "spx_surrogate_gradient_device": (C.c_int, [vp, vp, vp, i64, vp,
                                                         C.POINTER(C.c_float)]),
@@ -496,12 +497,15 @@ class Surrogate:
         return float(loss.value), g.cpu().numpy()
 
     def train(self, d_params, d_x, n_x: int, d_mask=None, epochs: int = 30, batch: int = 1024,
-              lr: float = 2e-3, shuffle_seed: int = 7):
+              lr: float = 2e-3, tol_rel: float = 1e-3, tol_abs: float = 1e-6,
+              shuffle_seed: int = 7):
+        """Train on device rows; errors are weighted in units of the Newton
+        tolerance tol_rel*|x| + tol_abs (0, 0 = plain standardised MSE)."""
         B = int(d_params.shape[0])
         hist = np.zeros(max(epochs, 1), dtype=np.float32)
         _check(self.lib.spx_surrogate_train_device(self.ptr, A.ptr(d_params), A.ptr(d_x), n_x,
-                                                   A.ptr(d_mask), B, epochs, batch, lr,
-                                                   shuffle_seed, A.ptr(hist)))
+                                                   A.ptr(d_mask), B, epochs, batch, lr, tol_rel,
+                                                   tol_abs, shuffle_seed, A.ptr(hist)))
         self.loss_history = hist[:epochs].tolist()
         return self.loss_history
 

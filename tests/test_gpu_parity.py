@@ -215,3 +215,26 @@ def test_full_size_headline_properties():
     assert np.all(close(ir.solutions[same], f64.solutions[same], rtol=1e-9, atol=1e-12))
     np.testing.assert_array_equal(f64.node_stats[:, 0], B)
     assert np.allclose(f64.node_stats[:, 1], f64.solutions.sum(0), rtol=1e-12)
+
+
+def test_fp32_handover_to_fp64_on_ill_conditioned_systems(oracle):
+    # Nodes a and b joined by 1e4 S, each with only ~1e-9 S (plus a diode) to
+    # ground: rounded to FP32, G + eps == G and the Jacobian is exactly
+    # singular, so FP32 factors cannot serve and the instance must go to the
+    # FP64 pass, come back flagged, and equal the FP64 oracle.
+    c = CK.Circuit("fp32_singular")
+    rleak = c.loguniform(0.9e9, 1.1e9, name="Rleak")
+    c.isource("0", "a", 1e-6)
+    c.resistor("a", "b", 1e-4)
+    c.resistor("a", "0", rleak)
+    c.resistor("b", "0", rleak)
+    c.diode("b", "0", 1e-14, 1.0)
+    B = 256
+    got = E.sweep_dcop(c, B, config=E.nr_config(precision=E.Precision.F32_IR))
+    want = oracle.sweep_dcop(c, got.params, A.default_nr_config(), nthreads=8)
+    np.testing.assert_array_equal(got.status, want["status"])
+    # the zero-start Jacobian has condition ~1e13, so two FP64 LUs that round
+    # differently (fused vs separate multiply-add) agree to ~1e13 * 2^-53
+    assert np.all(close(got.solutions, want["x"], rtol=1e-6, atol=1e-9))
+    np.testing.assert_array_equal(got.flags & ~A.SPX_FLAG_FP64_ESCALATED, want["flags"])
+    assert np.count_nonzero(got.flags & A.SPX_FLAG_FP64_ESCALATED) == B
