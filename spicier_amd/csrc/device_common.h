@@ -249,6 +249,28 @@ __device__ __forceinline__ int pivot_lane(float absval, bool eligible, int pos) 
   return (int)__ffsll((unsigned long long)m) - 1;
 }
 
+// Branch-free pivot choice for the register LU: one 32-bit DPP max over
+// key = (high word of |a| with its 6 low bits cleared) | (63 - pos).  The
+// largest |a| wins; candidates equal in sign, exponent and the top 14
+// mantissa bits — exact ties included — go to the smallest position, which is
+// nalgebra's first-maximum rule for exact ties.  Returns the winning position
+// (pos < 64) and whether the column held a non-zero value.
+__device__ __forceinline__ int pivot_pos(double absval, bool eligible, int pos, bool& nonzero) {
+  const uint32_t hi = (uint32_t)((uint64_t)__double_as_longlong(absval) >> 32);
+  const uint32_t key = eligible ? ((hi & ~63u) | (uint32_t)(63 - pos)) : 0u;
+  const uint32_t mx = wave_max_u32(key);
+  nonzero = (mx >> 6) != 0;
+  return 63 - (int)(mx & 63u);
+}
+
+__device__ __forceinline__ int pivot_pos(float absval, bool eligible, int pos, bool& nonzero) {
+  const uint32_t bits = __float_as_uint(absval);
+  const uint32_t key = eligible ? ((bits & ~63u) | (uint32_t)(63 - pos)) : 0u;
+  const uint32_t mx = wave_max_u32(key);
+  nonzero = (mx >> 6) != 0;
+  return 63 - (int)(mx & 63u);
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));

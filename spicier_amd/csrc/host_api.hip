@@ -46,7 +46,7 @@ namespace {
 // ------------------------------------------------------------- circuit compile
 struct HostProgram {
   int n = 0, nn = 0, P = 0, num_elems = 0, num_vals = 2, has_nonlinear = 0;
-  std::vector<int> elems, order, pos_rc, pos_off, contrib, rhs_off, rhs_contrib;
+  std::vector<int> elems, order, pos_rc, pos_off, row_ptr, contrib, rhs_off, rhs_contrib;
   int max_row_nnz = 0;
 };
 
@@ -166,6 +166,8 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n) {
     rownnz[kv.first.first]++;
   }
   hp.max_row_nnz = *std::max_element(rownnz.begin(), rownnz.end());
+  hp.row_ptr.assign(1, 0);
+  for (int r = 0; r < n; ++r) hp.row_ptr.push_back(hp.row_ptr.back() + rownnz[r]);
   hp.rhs_off.push_back(0);
   for (int r = 0; r < n; ++r) {
     hp.rhs_contrib.insert(hp.rhs_contrib.end(), rhs[r].begin(), rhs[r].end());
@@ -199,7 +201,7 @@ int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g) {
   };
   size_t o_el = put(hp.elems), o_or = put(hp.order), o_rc = put(hp.pos_rc),
          o_po = put(hp.pos_off), o_co = put(hp.contrib), o_ro = put(hp.rhs_off),
-         o_rh = put(hp.rhs_contrib);
+         o_rh = put(hp.rhs_contrib), o_rp = put(hp.row_ptr);
   void* d = nullptr;
   int rc = workspace(h, "program", blob.size() * sizeof(int) + 16, &d);
   if (rc) return rc;
@@ -221,6 +223,7 @@ int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g) {
   g.order = base + o_or;
   g.pos_rc = base + o_rc;
   g.pos_off = base + o_po;
+  g.row_ptr = base + o_rp;
   g.contrib = base + o_co;
   g.rhs_off = base + o_ro;
   g.rhs_contrib = base + o_rh;

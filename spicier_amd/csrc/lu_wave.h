@@ -8,7 +8,9 @@
 // reference's LU.  The pivot of step k is the largest |A[.][k]| over rows at
 // positions >= k, smallest position on ties — nalgebra's icamax rule (first
 // maximum), which CpuBatchedSolver inherits (spicier-batched-sweep
-// src/solver.rs:370-380).  Multipliers are formed as a * (1/pivot) and
+// src/solver.rs:370-380); the comparison runs on the top 14 mantissa bits,
+// so candidates that close count as tied (the pivot then differs from the
+// reference's only between values equal to 6e-5 relative).  Multipliers are formed as a * (1/pivot) and
 // updates as fma(-u, l, a).  The caller's workgroup is exactly one wave, so
 // __syncthreads() orders LDS traffic without a real barrier.
 //
@@ -71,14 +73,18 @@ struct RowLU {
     pos = lane;
     dinv = T(0);
     bool ok = true;
+    // One basic block for all N steps (no early exit, no data-dependent
+    // branch), so the scheduler can start step k+1's pivot search while step
+    // k's trailing updates are still in flight.  A zero pivot column clears
+    // `ok`; its step then runs with a zero reciprocal and the result is
+    // discarded by the caller.
     static_for<0, N>([&](auto K) __attribute__((always_inline)) {
       constexpr int k = K;
-      if (!ok) return;
-      // argmax |a[k]| over rows at positions >= k, smallest position on ties
-      int p = pivot_lane(tabs(a[k]), pos >= k, pos);
-      p = __builtin_amdgcn_readfirstlane(p);
-      if (p < 0) { ok = false; return; }
-      const int bp = __builtin_amdgcn_readlane(pos, p);
+      bool nz;
+      const int bp = __builtin_amdgcn_readfirstlane(pivot_pos(tabs(a[k]), pos >= k, pos, nz));
+      ok = ok && nz;
+      const int p = __builtin_amdgcn_readfirstlane(
+          (int)__ffsll((unsigned long long)__ballot(pos == bp)) - 1);
       if (pos == k) pos = bp;  // row at position k moves to the pivot's position
       if (lane == p) pos = k;
       if constexpr (Bcast == kBcastLds) {
@@ -89,7 +95,8 @@ struct RowLU {
           });
         }
         __syncthreads();
-        const T inv = T(1) / row[k];
+        const T pv = row[k];
+        const T inv = pv != T(0) ? T(1) / pv : T(0);
         if (lane == p) dinv = inv;
         const T l = pos > k ? a[k] * inv : T(0);
         if (pos > k) a[k] = l;
@@ -99,7 +106,8 @@ struct RowLU {
         });
         __syncthreads();
       } else {
-        const T inv = T(1) / bcast(a[k], p);
+        const T pv = bcast(a[k], p);
+        const T inv = pv != T(0) ? T(1) / pv : T(0);
         if (lane == p) dinv = inv;
         const T l = pos > k ? a[k] * inv : T(0);
         if (pos > k) a[k] = l;

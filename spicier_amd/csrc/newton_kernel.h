@@ -8,10 +8,12 @@
 //   1. device evaluation: lanes over elements (diode / MOSFET companion
 //      models in FP64, node voltages read from LDS);
 //   2. assembly: lanes over the shared sparsity pattern, each summing its
-//      entry's contributions in the reference's triplet order into a dense
-//      LDS matrix (entries outside the pattern stay zero from the start);
+//      entry's contributions in the reference's triplet order into the
+//      pattern's CSR value array in LDS (the dense matrix is never stored:
+//      a lane expands its row into VGPRs from the row's column bitmask);
 //   3. linear solve: row-per-lane LU in VGPRs (FP64), or FP32 factors reused
-//      across Newton iterations with FP64-residual iterative refinement;
+//      across Newton iterations with FP64-residual iterative refinement
+//      (residuals and the KCL test are sparse row products);
 //   4. Δx convergence test (newton.rs:134-159) as a wave vote.
 #pragma once
 
@@ -23,8 +25,10 @@
 
 namespace spx {
 
+// LDS per instance: pattern values + column indices, rhs, x, scratch x,
+// element values, parameters, junction states, pivot-row buffer.
 inline size_t newton_smem_doubles(const DevProgram& g, int N) {
-  return (size_t)N * (N + 2) + 4 * N + g.num_vals + g.P + g.num_elems;
+  return (size_t)g.num_pos + (g.num_pos + 1) / 2 + 4 * N + g.num_vals + g.P + g.num_elems;
 }
 
 template <int N, bool IR>
@@ -32,7 +36,8 @@ struct Instance {
   const DevProgram& g;
   const DevNrCfg& cfg;
   int lane;
-  double* J;      // N x (N+2) dense, row-major, stride N+2
+  double* J;      // num_pos pattern values, CSR order
+  const int* jc;  // num_pos column indices
   double* rhs;    // N
   double* xs;     // N current x
   double* xw;     // N scratch x (IR iterate / saved level solution)
@@ -40,9 +45,18 @@ struct Instance {
   double* pv;     // P parameters
   double* vst;    // num_elems pnjlim junction voltages
   double* prow;   // N: LU pivot-row broadcast buffer
+  int q0 = 0, q1 = 0;    // this lane's row: pattern entries [q0, q1)
+  uint64_t colmask = 0;  // columns present in this lane's row
   int nfact = 0;  // LU factorizations performed
   int nref = 0;   // refinement steps performed
-  static constexpr int S = N + 2;
+
+  __device__ __forceinline__ void init_row() {
+    if (lane < g.n) {
+      q0 = g.row_ptr[lane];
+      q1 = g.row_ptr[lane + 1];
+      for (int q = q0; q < q1; ++q) colmask |= 1ull << jc[q];
+    }
+  }
 
   __device__ double X(int u) const { return u >= 0 ? xs[u] : 0.0; }
 
@@ -96,8 +110,7 @@ struct Instance {
         const double v = vals[t >> 1];
         sum = (t & 1) ? sum - v : sum + v;
       }
-      const int rc = g.pos_rc[q];
-      J[(rc & 0xffff) * S + (rc >> 16)] = sum;
+      J[q] = sum;
     }
     for (int r = lane; r < g.n; r += kWave) {
       double sum = 0.0;
@@ -113,13 +126,27 @@ struct Instance {
     return __any(lim);
   }
 
+  // Expands this lane's row into registers; rows past n are identity rows.
+  // J[q] at q == q1 is still inside the LDS block (the column indices follow
+  // J), and is discarded when the column bit is clear.
   template <typename T>
   __device__ __forceinline__ void load_rows(T (&a)[N]) const {
-    const int n = g.n;
+    const bool pad = lane >= g.n;
+    const uint32_t mlo = (uint32_t)colmask, mhi = (uint32_t)(colmask >> 32);
+    int q = q0;
     static_for<0, N>([&](auto JJ) __attribute__((always_inline)) {
       constexpr int j = JJ;
-      a[j] = lane < n ? (T)J[lane * S + j] : (j == lane ? T(1) : T(0));
+      const bool b = ((j < 32 ? mlo : mhi) >> (j & 31)) & 1u;
+      const T v = (T)J[q];
+      a[j] = b ? v : ((pad && j == lane) ? T(1) : T(0));
+      q += b ? 1 : 0;
     });
+  }
+
+  // J x for this lane's row (pattern order = column order)
+  __device__ __forceinline__ double row_dot(const double* x, double r) const {
+    for (int q = q0; q < q1; ++q) r = fma(-J[q], x[jc[q]], r);
+    return r;
   }
 
   // x_new = J^{-1} rhs in FP64.  Returns false if singular.
@@ -141,21 +168,17 @@ struct Instance {
   // refactoring (stale factors stopped contracting), 2 failed.
   __device__ __forceinline__ int refine(const RowLU<N, float>& lu, bool fresh, double& xnew) {
     const int n = g.n;
-    const double* jrow = J + (lane < n ? lane : 0) * S;
     const double b = lane < n ? rhs[lane] : 0.0;
     double x = lane < n ? xs[lane] : 0.0;
     double rowsum = 0.0;
-    if (lane < n)
-      for (int j = 0; j < n; ++j) rowsum += fabs(jrow[j]);
+    for (int q = q0; q < q1; ++q) rowsum += fabs(J[q]);
     const double anrm = wave_max(rowsum);
     const double tol = cfg.ir_tol > 0.0 ? cfg.ir_tol : sqrt((double)n) * 1.1102230246251565e-16;
     double prev = INFINITY;
     for (int s = 0; s <= cfg.ir_max_steps; ++s) {
       if (lane < n) xw[lane] = x;
       __syncthreads();
-      double r = b;
-      if (lane < n)
-        for (int j = 0; j < n; ++j) r = fma(-jrow[j], xw[j], r);
+      const double r = row_dot(xw, b);
       __syncthreads();
       const double rn = wave_max(lane < n ? fabs(r) : 0.0);
       const double xn = wave_max(lane < n ? fabs(x) : 0.0);
@@ -250,8 +273,8 @@ struct Instance {
     double q = 0.0;
     if (lane < g.n) {
       double r = -rhs[lane], mag = fabs(rhs[lane]);
-      for (int j = 0; j < g.n; ++j) {
-        double t = J[lane * S + j] * xs[j];
+      for (int q = q0; q < q1; ++q) {
+        double t = J[q] * xs[jc[q]];
         r += t;
         mag = fmax(mag, fabs(t));
       }
@@ -274,23 +297,24 @@ __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
   const int64_t inst = blockIdx.x;
   const int lane = threadIdx.x;
   if (escalated_only && out.status[inst] != SPX_ACTIVE) return;
-  constexpr int S = N + 2;
   double* J = smem;
-  double* rhs = J + N * S;
+  int* jc = reinterpret_cast<int*>(J + g.num_pos);
+  double* rhs = J + g.num_pos + (g.num_pos + 1) / 2;
   double* xs = rhs + N;
   double* xw = xs + N;
   double* vals = xw + N;
   double* pv = vals + g.num_vals;
   double* vst = pv + g.P;
   double* prow = vst + g.num_elems;
-  Instance<N, IR> I{g, cfg, lane, J, rhs, xs, xw, vals, pv, vst, prow};
+  Instance<N, IR> I{g, cfg, lane, J, jc, rhs, xs, xw, vals, pv, vst, prow};
   const int n = g.n;
 
-  for (int i = lane; i < N * S; i += kWave) J[i] = 0.0;
+  for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
   for (int p = lane; p < g.P; p += kWave) pv[p] = params[inst * g.P + p];
   if (lane == 0) { vals[kValOne] = 1.0; vals[kValGmin] = 0.0; }
   if (lane < N) xs[lane] = (x0 && lane < n) ? x0[inst * n + lane] : 0.0;
   __syncthreads();
+  I.init_row();
 
   int total = 0, it = 0, esc = 0, status = SPX_FAILED;
   int flags = (x0 ? SPX_FLAG_WARM_START : 0) | (escalated_only ? SPX_FLAG_FP64_ESCALATED : 0);

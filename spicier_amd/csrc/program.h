@@ -9,6 +9,7 @@
 //             touch, each with its contributions (value slot, sign) in the
 //             reference's triplet push order (mna.rs:62-71), so the sum is
 //             formed in the same order as to_dense_matrix
+//   row_ptr : CSR row starts into the (row, col)-sorted pattern
 //   rhs_*   : the same for the right-hand side
 // Value slots 0 and 1 hold the constant 1.0 (voltage-source couplings) and
 // the current Gmin.  Per-iteration values live in LDS; assembly is one
@@ -34,8 +35,9 @@ struct DevProgram {
   int max_row_nnz;
   const int* elems;
   const int* order;
-  const int* pos_rc;   // row | col << 16
+  const int* pos_rc;   // row | col << 16, sorted by (row, col)
   const int* pos_off;  // num_pos + 1
+  const int* row_ptr;  // n + 1: row r's pattern entries are [row_ptr[r], row_ptr[r+1])
   const int* contrib;  // vid << 1 | negate
   const int* rhs_off;  // n + 1
   const int* rhs_contrib;
