@@ -36,6 +36,15 @@ sys.path.insert(0, ROOT)
 
 MI355X_FP64_PEAK_TFLOPS = 78.6   # vector and matrix FP64; FP32 non-packed issues at the same rate
 
+# HBM bytes per launch of the form's Newton kernel, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the counter's
+# reading for a 1 GiB device copy (FETCH x 2.0, WRITE x 1.0); the runs and
+# figures are in profiles/newton_dense_64_traffic_inverter_chain_65536.txt.
+# Algorithmic bytes per launch are 107.2 MB (FP32+IR warm start) and 73.7 MB
+# (FP64 zero start): parameters and x0 read once, x and per-instance status
+# written once.
+MEASURED_TRAFFIC_BYTES = {"fp64": 94.22e6, "fp32_ir_surrogate": 127.51e6}
+
 
 def lu_flops(n: int) -> float:
     """getrf (2/3 n^3 - 1/2 n^2 + 5/6 n) + getrs (2 n^2), LAPACK Working Note 41."""
@@ -53,6 +62,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32_ir,fp32_ir_surrogate")
     ap.add_argument("--train", type=int, default=262144)
@@ -97,10 +107,10 @@ def main():
                                 A.ptr(buf["stats"]) if with_stats else None,
                                 A.ptr(buf["fac"]), A.ptr(buf["ref"]))
 
-    def step(cfg, sp=spec, res=None):
+    def step(cfg, sp=spec, res=None, x0=None):
         r = res or result_struct()
         E._check(lib.spx_sweep_dcop_device(dev.handle, C.byref(ckt), C.byref(sp), C.byref(cfg),
-                                           None, C.byref(r)))
+                                           x0, C.byref(r)))
 
     forms = {
         "fp64": E.nr_config(precision=E.Precision.F64),
@@ -111,6 +121,7 @@ def main():
     wanted = [f for f in args.forms.split(",") if f in forms]
     report = {}
     training = None
+    sur = None
     if "fp32_ir_surrogate" in wanted:
         t0 = time.perf_counter()
         sur = E.train_surrogate(ckt_py, list(range(32)), n_train=args.train, epochs=args.epochs,
@@ -139,17 +150,28 @@ def main():
         fac = buf["fac"].cpu().numpy().astype(np.float64)
         ref = buf["ref"].cpu().numpy().astype(np.float64)
         accepted = int(np.count_nonzero((st == A.SPX_CONVERGED) & ((fl & A.SPX_FLAG_KCL_FAIL) == 0)))
-        # Newton kernel alone (parameters already resident): events on the
-        # library's stream around a sweep that reuses the sampled parameters
+        # Newton kernel alone: events on the library's stream around a sweep
+        # that reads the sampled parameters in place (explicit generator, no
+        # copy), no statistics, and for the surrogate form the prediction made
+        # beforehand as a given x0 — so the only launches in the interval are
+        # k_newton_dense (the FP32+IR pass and its FP64 hand-over pass).
         exp_spec, keep3 = ckt_py.sweep_spec(B, seed=42, generator=A.SPX_GEN_EXPLICIT)
         exp_spec.explicit_params = A.ptr(buf["params"])
         res_nostats = result_struct(with_stats=False, with_params=False)
+        kcfg, x0 = cfg, None
+        if name == "fp32_ir_surrogate":
+            x0buf = torch.empty(B, n, **f64)
+            torch.cuda.synchronize()
+            sur.predict(buf["params"], n, x0buf)
+            kcfg = E.nr_config(precision=E.Precision.F32_IR)
+            kcfg.warm_start = A.SPX_WARM_GIVEN
+            x0 = A.ptr(x0buf)
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        step(cfg, exp_spec, res_nostats)
+        step(kcfg, exp_spec, res_nostats, x0)
         kt = []
         for _ in range(max(1, args.steps)):
             ev0.record(stream)
-            step(cfg, exp_spec, res_nostats)
+            step(kcfg, exp_spec, res_nostats, x0)
             ev1.record(stream)
             ev1.synchronize()
             kt.append(ev0.elapsed_time(ev1) * 1e-3)
@@ -177,7 +199,8 @@ def main():
         "compute": "FP64/FP32 vector FMA (LU in VGPRs); packed FP32 disabled, so FP32 issues at "
                    "the FP64 rate",
         "achieved": h["tflops"], "peak": MI355X_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-        "frac": h["tflops"] / MI355X_FP64_PEAK_TFLOPS, "traffic": None,
+        "frac": h["tflops"] / MI355X_FP64_PEAK_TFLOPS,
+        "traffic": MEASURED_TRAFFIC_BYTES.get(head) if args.batch == 65536 else None,
         "kernel": f"k_newton_dense<64,{'false' if head == 'fp64' else 'true'}>",
         "kernel_ms": h["newton_kernel_ms"],
     }
@@ -189,16 +212,20 @@ def main():
         S = min(args.cpu_sample, B)
         params = buf["params"][:S].cpu().numpy()
         threads = max(1, min(16, os.cpu_count() or 1))
+        # passes over the same instances until about --cpu-seconds of work
+        acc, passes, dt = 0, 0, 0.0
         t0 = time.perf_counter()
-        out = pyoracle.sweep_dcop(ckt_py, params, A.default_nr_config(), nthreads=threads)
-        dt = time.perf_counter() - t0
-        acc = int(np.count_nonzero((out["status"] == A.SPX_CONVERGED)
-                                   & ((out["flags"] & A.SPX_FLAG_KCL_FAIL) == 0)))
+        while passes < 100 and (passes == 0 or dt < args.cpu_seconds):
+            out = pyoracle.sweep_dcop(ckt_py, params, A.default_nr_config(), nthreads=threads)
+            acc += int(np.count_nonzero((out["status"] == A.SPX_CONVERGED)
+                                        & ((out["flags"] & A.SPX_FLAG_KCL_FAIL) == 0)))
+            passes += 1
+            dt = time.perf_counter() - t0
         cpu = {"value": acc / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-               "sample": f"first {S} of the {B} instances (same seeded parameters), plain FP64 "
-                         f"restated reference path (nalgebra-style LU, NR + Gmin/source "
-                         f"stepping), one task per instance on {threads} host threads; "
-                         f"{dt:.2f} s wall"}
+               "sample": f"{passes} passes over the first {S} of the {B} instances (same seeded "
+                         f"parameters), plain FP64 restated reference path (nalgebra-style LU, "
+                         f"NR + Gmin/source stepping), one task per instance on {threads} host "
+                         f"threads; {dt:.2f} s wall"}
 
     line = {
         "metric": "converged DC-OP solves/s, 64k x n=64 sweep: plain FP64 vs FP32+IR+surrogate",

@@ -8,6 +8,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/spicier_hip.h"
 
@@ -19,6 +20,7 @@ struct spx_handle {
   std::mutex mu;
   std::map<std::string, std::pair<void*, size_t>> ws;
   spx_surrogate* surrogate = nullptr;  // used by SPX_WARM_SURROGATE
+  std::vector<int> program_blob;       // circuit program now in the "program" workspace
 };
 
 namespace spx {

@@ -25,10 +25,11 @@
 
 namespace spx {
 
-// LDS per instance: pattern values + column indices, rhs, x, scratch x,
-// element values, parameters, junction states, pivot-row buffer.
+// LDS per instance: pattern values + column indices, per-row column
+// bitmasks, rhs, x, scratch x, element values, parameters, junction states,
+// pivot-row buffer.
 inline size_t newton_smem_doubles(const DevProgram& g, int N) {
-  return (size_t)g.num_pos + (g.num_pos + 1) / 2 + 4 * N + g.num_vals + g.P + g.num_elems;
+  return (size_t)g.num_pos + (g.num_pos + 1) / 2 + N + 4 * N + g.num_vals + g.P + g.num_elems;
 }
 
 template <int N, bool IR>
@@ -38,6 +39,7 @@ struct Instance {
   int lane;
   double* J;      // num_pos pattern values, CSR order
   const int* jc;  // num_pos column indices
+  uint32_t* rmask;  // 2 words per row: columns present in the row
   double* rhs;    // N
   double* xs;     // N current x
   double* xw;     // N scratch x (IR iterate / saved level solution)
@@ -46,15 +48,20 @@ struct Instance {
   double* vst;    // num_elems pnjlim junction voltages
   double* prow;   // N: LU pivot-row broadcast buffer
   int q0 = 0, q1 = 0;    // this lane's row: pattern entries [q0, q1)
-  uint64_t colmask = 0;  // columns present in this lane's row
   int nfact = 0;  // LU factorizations performed
   int nref = 0;   // refinement steps performed
 
+  // after jc is in LDS; publishes the row masks (a barrier follows)
   __device__ __forceinline__ void init_row() {
+    uint64_t m = 0;
     if (lane < g.n) {
       q0 = g.row_ptr[lane];
       q1 = g.row_ptr[lane + 1];
-      for (int q = q0; q < q1; ++q) colmask |= 1ull << jc[q];
+      for (int q = q0; q < q1; ++q) m |= 1ull << jc[q];
+    }
+    if (lane < N) {
+      rmask[2 * lane] = (uint32_t)m;
+      rmask[2 * lane + 1] = (uint32_t)(m >> 32);
     }
   }
 
@@ -129,16 +136,20 @@ struct Instance {
   // Expands this lane's row into registers; rows past n are identity rows.
   // J[q] at q == q1 is still inside the LDS block (the column indices follow
   // J), and is discarded when the column bit is clear.
+  // The read of J[q] is kept unconditional (the empty asm pins it) so the
+  // expansion is straight-line selects, not a branch per column.
   template <typename T>
   __device__ __forceinline__ void load_rows(T (&a)[N]) const {
     const bool pad = lane >= g.n;
-    const uint32_t mlo = (uint32_t)colmask, mhi = (uint32_t)(colmask >> 32);
+    const int r = pad ? 0 : lane;  // rows past n (and lanes past N) take no pattern entry
+    const uint32_t mlo = pad ? 0u : rmask[2 * r], mhi = pad ? 0u : rmask[2 * r + 1];
     int q = q0;
     static_for<0, N>([&](auto JJ) __attribute__((always_inline)) {
       constexpr int j = JJ;
       const bool b = ((j < 32 ? mlo : mhi) >> (j & 31)) & 1u;
-      const T v = (T)J[q];
-      a[j] = b ? v : ((pad && j == lane) ? T(1) : T(0));
+      double v = J[q];
+      asm volatile("" : "+v"(v));
+      a[j] = b ? (T)v : ((pad && j == lane) ? T(1) : T(0));
       q += b ? 1 : 0;
     });
   }
@@ -299,14 +310,15 @@ __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
   if (escalated_only && out.status[inst] != SPX_ACTIVE) return;
   double* J = smem;
   int* jc = reinterpret_cast<int*>(J + g.num_pos);
-  double* rhs = J + g.num_pos + (g.num_pos + 1) / 2;
+  uint32_t* rmask = reinterpret_cast<uint32_t*>(J + g.num_pos + (g.num_pos + 1) / 2);
+  double* rhs = J + g.num_pos + (g.num_pos + 1) / 2 + N;
   double* xs = rhs + N;
   double* xw = xs + N;
   double* vals = xw + N;
   double* pv = vals + g.num_vals;
   double* vst = pv + g.P;
   double* prow = vst + g.num_elems;
-  Instance<N, IR> I{g, cfg, lane, J, jc, rhs, xs, xw, vals, pv, vst, prow};
+  Instance<N, IR> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
   const int n = g.n;
 
   for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
@@ -315,6 +327,7 @@ __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
   if (lane < N) xs[lane] = (x0 && lane < n) ? x0[inst * n + lane] : 0.0;
   __syncthreads();
   I.init_row();
+  __syncthreads();
 
   int total = 0, it = 0, esc = 0, status = SPX_FAILED;
   int flags = (x0 ? SPX_FLAG_WARM_START : 0) | (escalated_only ? SPX_FLAG_FP64_ESCALATED : 0);
