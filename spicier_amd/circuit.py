@@ -263,6 +263,45 @@ def inverter_chain() -> Circuit:
     return c
 
 
+GRID_N = 32
+GRID_DIODE_EVERY = 8
+
+
+def grid_diodes(N: int = GRID_N, every: int = GRID_DIODE_EVERY) -> Circuit:
+    """N x N resistive grid with diodes: the sparse FP64 sweep workload.
+
+    Node (r, c) is unknown r*N + c.  Every horizontal and vertical edge is a
+    resistor R~U(0.9, 1.1) kOhm (2N(N-1) slots, row edges first); the four
+    corner nodes (0,0), (0,N-1), (N-1,0), (N-1,N-1) are driven by ideal
+    sources of 1.0, 1.7667, 2.5333 and 3.3 V (1.0 + k * 2.3/3, adding four
+    branch unknowns); every `every`-th node, r*N + c = every/2 (mod every),
+    has a diode to ground with IS~logU(1e-15, 1e-13) A, N = 1.0.  The offset
+    keeps diodes off the corner nodes, where a diode straight across an ideal
+    1-3.3 V source would carry hundreds of amperes to exp-overflow currents.
+    At N=32: 1,024 nodes + 4 branches = 1,028 unknowns, 1,984 resistors,
+    128 diodes.
+    """
+    c = Circuit(f"grid_diodes_{N}")
+    for r in range(N):
+        for col in range(N):
+            c.node(f"n{r}_{col}")
+    redges = [((r, col), (r, col + 1)) for r in range(N) for col in range(N - 1)]
+    cedges = [((r, col), (r + 1, col)) for r in range(N - 1) for col in range(N)]
+    rs = [c.uniform(0.9e3, 1.1e3, name=f"R{k}") for k in range(len(redges) + len(cedges))]
+    dn = [k for k in range(N * N) if k % every == every // 2]
+    isl = [c.loguniform(1e-15, 1e-13, name=f"IS{k}") for k in dn]
+    nd = c.const(1.0, "N")
+    name = lambda rc: f"n{rc[0]}_{rc[1]}"
+    for (a, b), r in zip(redges + cedges, rs):
+        c.resistor(name(a), name(b), r)
+    corners = [(0, 0), (0, N - 1), (N - 1, 0), (N - 1, N - 1)]
+    for k, rc in enumerate(corners):
+        c.vsource(name(rc), "0", c.const(1.0 + k * 2.3 / 3, f"V{k}"))
+    for k, s in zip(dn, isl):
+        c.diode(name((k // N, k % N)), "0", s, nd)
+    return c
+
+
 def diode_resistor(v=5.0, r=1e3, is_=1e-14, n=1.0) -> Circuit:
     """V - R - D to ground (the reference's 5 V / 1 kOhm diode check,
     crates/spicier-parser/tests/ngspice_validation.rs:705-768)."""

@@ -1,0 +1,142 @@
+// dcop_driver.h — the per-instance DC-operating-point state machine shared
+// by the dense (register LU) and sparse (shared-symbolic LU) Newton kernels.
+//
+// An instance type provides: newton(gmin, source_factor, iters, escalations)
+// -> 1 converged / 0 not / -1 singular / -2 hand over to FP64;
+// linear_solve() -> 0 / 1 singular / 2 hand over; save_x / restore_x /
+// zero_x; kcl(gmin) -> max_i |KCL_i| / tol_i; write_result; write_handover.
+#pragma once
+
+#include "program.h"
+
+namespace spx {
+
+template <class Inst>
+__device__ __forceinline__ void run_dcop(Inst& I, const DevProgram& g, const DevNrCfg& cfg,
+                                         bool warm, const DevOut& out, int64_t inst,
+                                         int escalated_only) {
+  int total = 0, it = 0, esc = 0, status = SPX_FAILED;
+  int flags = (warm ? SPX_FLAG_WARM_START : 0) | (escalated_only ? SPX_FLAG_FP64_ESCALATED : 0);
+  double final_gmin = 0.0;
+  bool handover = false;  // FP32+IR pass: leave this instance to the FP64 pass
+
+  if (!g.has_nonlinear) {
+    // linear circuit: one solve (sweep.rs:159-217)
+    const int rc = I.linear_solve();
+    handover = rc == 2;
+    status = rc == 0 ? SPX_CONVERGED : SPX_SINGULAR;
+    total = 1;
+  } else {
+    // Plain NR, then Gmin stepping (newton.rs:373-459), then source stepping
+    // (newton.rs:261-356), as one state machine so the Newton loop has a
+    // single, inlined call site.  Retries at one level restart from the same
+    // solution and so replay the first attempt exactly; the first attempt
+    // decides the level.  A stage that fails returns its last converged
+    // level's solution (zeros if none).
+    enum { kPlain, kGmin, kSource, kDone };
+    int stage = kPlain;
+    double gm = 0.0, factor = 0.0, step = 0.0;
+    auto enter_gmin = [&]() -> int {
+      flags |= SPX_FLAG_GMIN_STEPPING;
+      I.zero_x();
+      I.save_x();
+      gm = cfg.gmin_initial;
+      if (gm >= cfg.gmin_final * 0.99) return kGmin;
+      status = SPX_CONVERGED;  // no level to run: the stage returns converged zeros
+      return kDone;
+    };
+    auto enter_source = [&]() -> int {
+      flags |= SPX_FLAG_SOURCE_STEPPING;
+      I.zero_x();
+      I.save_x();
+      factor = cfg.src_initial;
+      step = cfg.src_step;
+      if (factor <= 1.0 + 1e-9) return kSource;
+      status = SPX_CONVERGED;
+      final_gmin = 0.0;
+      return kDone;
+    };
+    auto after_failure = [&]() -> int {
+      if (stage == kPlain && cfg.gmin_stepping) return enter_gmin();
+      if (stage != kSource && cfg.source_stepping) return enter_source();
+      return kDone;
+    };
+    while (stage != kDone) {
+      double lg = 0.0, lsf = 1.0;
+      if (stage == kGmin) {
+        lg = gm;
+      } else if (stage == kSource) {
+        lsf = fmin(factor, 1.0);
+        I.restore_x();
+      }
+      const int st = I.newton(lg, lsf, it, esc);
+      total += it;
+      if (st == -2) {
+        handover = true;
+        stage = kDone;
+      } else if (stage == kPlain) {
+        status = st == 1 ? SPX_CONVERGED : (st < 0 ? SPX_SINGULAR : SPX_FAILED);
+        stage = status == SPX_CONVERGED ? kDone : after_failure();
+      } else if (stage == kGmin) {
+        if (st != 1) {
+          I.restore_x();
+          status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
+          final_gmin = 0.0;
+          stage = after_failure();
+        } else {
+          I.save_x();
+          final_gmin = gm;
+          bool more = gm > cfg.gmin_final;
+          if (more) {
+            gm /= cfg.gmin_factor;
+            if (gm < cfg.gmin_final) gm = cfg.gmin_final;
+            more = gm >= cfg.gmin_final * 0.99;
+          }
+          if (!more) {
+            I.restore_x();
+            status = SPX_CONVERGED;
+            stage = kDone;
+          }
+        }
+      } else {  // kSource
+        const double cur = fmin(factor, 1.0);
+        bool finished = false;
+        if (st != 1) {
+          if (step > cfg.src_min_step) {
+            step /= 2.0;
+            factor -= step;
+            finished = !(factor <= 1.0 + 1e-9);
+          } else {
+            I.restore_x();
+            status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
+            stage = kDone;
+          }
+        } else {
+          I.save_x();
+          if (fabs(cur - 1.0) < 1e-9) {
+            finished = true;
+          } else {
+            factor += step;
+            if (step < cfg.src_step) step = fmin(step * 1.5, cfg.src_step);
+            finished = !(factor <= 1.0 + 1e-9);
+          }
+        }
+        if (finished) {
+          I.restore_x();
+          status = SPX_CONVERGED;
+          final_gmin = 0.0;
+          stage = kDone;
+        }
+      }
+    }
+  }
+  if (handover) {
+    I.write_handover(out, inst, flags);
+    return;
+  }
+  const double res = I.kcl(final_gmin);
+  if (status == SPX_CONVERGED && !(res <= 1.0)) flags |= SPX_FLAG_KCL_FAIL;
+  I.write_result(out, inst, status, total, flags, res);
+}
+
+}  // namespace spx

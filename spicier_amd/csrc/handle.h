@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/spicier_hip.h"
+#include "sparse_plan.h"
 
 struct spx_surrogate;
 
@@ -21,6 +22,8 @@ struct spx_handle {
   std::map<std::string, std::pair<void*, size_t>> ws;
   spx_surrogate* surrogate = nullptr;  // used by SPX_WARM_SURROGATE
   std::vector<int> program_blob;       // circuit program now in the "program" workspace
+  bool has_sparse = false;             // its shared-symbolic LU plan (n > 64) is uploaded
+  spx::HostSparsePlan sparse_plan;
 };
 
 namespace spx {

@@ -17,6 +17,7 @@
 #include "../../include/spicier_hip.h"
 #include "handle.h"
 #include "program.h"
+#include "sparse_plan.h"
 
 namespace spx {
 hipError_t launch_newton(const DevProgram& g, const DevNrCfg& cfg, const double* params,
@@ -26,9 +27,14 @@ hipError_t launch_sample(const spx_param_spec* d_specs, int P, int gen, uint64_t
 hipError_t launch_stats(const double* v, const uint8_t* mask, const int* status, int64_t B, int n,
                         double* part, double* out, hipStream_t st);
 int64_t stats_num_blocks(int64_t B);
-size_t newton_smem_bytes(const DevProgram& g, int N);
 hipError_t launch_solve_batch(const double* M, const double* rhs, int n, int64_t B, double* x,
                               uint8_t* sing, double* work, int64_t chunk, hipStream_t st);
+hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
+                               int64_t* slot, size_t* smem, int* lu_in_lds);
+hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
+                                const double* params, const double* x0, int64_t B,
+                                const DevOut& out, double* ws, int grid, int64_t slot,
+                                size_t smem, int lu_in_lds, hipStream_t st);
 size_t solve_generic_workspace(int n, int64_t chunk);
 }  // namespace spx
 
@@ -42,6 +48,11 @@ std::string& last_error() {
 using namespace spx;
 
 namespace {
+
+// Unknown counts: the register-LU kernels take n <= kDenseMaxN, the
+// shared-symbolic sparse kernel n <= kSparseMaxN.
+constexpr int kDenseMaxN = 64;
+constexpr int kSparseMaxN = 4096;
 
 // ------------------------------------------------------------- circuit compile
 struct HostProgram {
@@ -62,7 +73,7 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n) {
   if (n < 1) return fail(SPX_ERR_INVALID_DIMENSION, "circuit has no unknowns");
   if (n > max_n)
     return fail(SPX_ERR_INVALID_DIMENSION,
-                "circuit has " + std::to_string(n) + " unknowns; the dense sweep path takes <= " +
+                "circuit has " + std::to_string(n) + " unknowns; the sweep path takes <= " +
                     std::to_string(max_n));
   hp.n = n;
   hp.nn = nn;
@@ -190,8 +201,9 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n) {
   return SPX_OK;
 }
 
-// Uploads the program into the handle's "program" workspace.
-int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g) {
+// Uploads the program into the handle's "program" workspace, and for n past
+// the register LU the shared-symbolic LU plan into "sparse_plan".
+int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g, DevSparse* sp) {
   std::vector<int> blob;
   auto put = [&](const std::vector<int>& v) {
     size_t off = blob.size();
@@ -213,12 +225,31 @@ int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g) {
   if (rc) return rc;
   if (!cached) {
     h->program_blob.clear();
+    h->has_sparse = false;
     SPX_HIP(hipMemcpyAsync(d, blob.data(), blob.size() * sizeof(int), hipMemcpyHostToDevice,
                            h->stream),
             SPX_ERR_TRANSFER);
     // the host blob dies at return: wait for the copy
     SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
     h->program_blob = std::move(blob);
+  }
+  if (hp.n > kDenseMaxN) {
+    if (!h->has_sparse) {
+      const char* why = build_sparse_plan(hp.n, hp.nn, hp.pos_rc, h->sparse_plan);
+      if (*why) return fail(SPX_ERR_INVALID_DIMENSION, why);
+      void* ds = nullptr;
+      const auto& pb = h->sparse_plan.blob;
+      if ((rc = workspace(h, "sparse_plan", pb.size() * sizeof(int), &ds))) return rc;
+      SPX_HIP(hipMemcpyAsync(ds, pb.data(), pb.size() * sizeof(int), hipMemcpyHostToDevice,
+                             h->stream),
+              SPX_ERR_TRANSFER);
+      SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+      h->has_sparse = true;
+    }
+    void* ds = nullptr;
+    if ((rc = workspace(h, "sparse_plan", h->sparse_plan.blob.size() * sizeof(int), &ds)))
+      return rc;
+    if (sp) *sp = device_sparse(h->sparse_plan, static_cast<const int*>(ds));
   }
   const int* base = static_cast<const int*>(d);
   g.n = hp.n;
@@ -344,8 +375,12 @@ int sweep_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
   if (!out || !out->x) return fail(SPX_ERR_INVALID_ARGUMENT, "result.x is required");
   if ((rc = check_cfg(cfg))) return rc;
   HostProgram hp;
-  if ((rc = compile_circuit(ckt, hp, 64))) return rc;
+  if ((rc = compile_circuit(ckt, hp, kSparseMaxN))) return rc;
   if ((rc = check_spec(spec, hp.P))) return rc;
+  const bool sparse = hp.n > kDenseMaxN;
+  if (sparse && cfg->precision != SPX_PREC_F64)
+    return fail(SPX_ERR_INVALID_ARGUMENT,
+                "circuits with more than 64 unknowns run in FP64 (precision SPX_PREC_F64)");
   if (cfg->warm_start == SPX_WARM_GIVEN && !d_x0)
     return fail(SPX_ERR_INVALID_ARGUMENT, "warm start given without x0");
   if (cfg->warm_start == SPX_WARM_SURROGATE && !h->surrogate)
@@ -353,7 +388,8 @@ int sweep_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
   const int64_t B = spec->batch;
   if (B == 0) return SPX_OK;
   DevProgram g;
-  if ((rc = upload_program(h, hp, g))) return rc;
+  DevSparse sp{};
+  if ((rc = upload_program(h, hp, g, &sp))) return rc;
   // Parameters the kernels read: the caller's explicit table itself when no
   // copy was asked for, else sampled (or copied) into result.params or a
   // workspace.
@@ -389,7 +425,19 @@ int sweep_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
       return rc;
     x0 = static_cast<double*>(p);
   }
-  SPX_HIP(launch_newton(g, dc, params, x0, B, o, h->stream), SPX_ERR_BACKEND);
+  if (sparse) {
+    int grid = 0, lu_lds = 0;
+    int64_t slot = 0;
+    size_t smem = 0;
+    SPX_HIP(sparse_launch_shape(g, sp, B, &grid, &slot, &smem, &lu_lds), SPX_ERR_BACKEND);
+    void* ws = nullptr;
+    if ((rc = workspace(h, "sparse_ws", sizeof(double) * (size_t)grid * slot, &ws))) return rc;
+    SPX_HIP(launch_newton_sparse(g, sp, dc, params, x0, B, o, static_cast<double*>(ws), grid,
+                                 slot, smem, lu_lds, h->stream),
+            SPX_ERR_BACKEND);
+  } else {
+    SPX_HIP(launch_newton(g, dc, params, x0, B, o, h->stream), SPX_ERR_BACKEND);
+  }
   if (out->node_stats) {
     void* part = nullptr;
     if ((rc = workspace(h, "stats_part", sizeof(double) * 5 * hp.n * stats_num_blocks(B), &part)))
@@ -616,7 +664,7 @@ int spx_sweep_dcop(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* 
   if (!out || !out->x) return fail(SPX_ERR_INVALID_ARGUMENT, "result.x is required");
   int rc;
   HostProgram hp;
-  if ((rc = compile_circuit(ckt, hp, 64))) return rc;
+  if ((rc = compile_circuit(ckt, hp, kSparseMaxN))) return rc;
   if ((rc = check_spec(spec, hp.P))) return rc;
   if ((rc = check_cfg(cfg))) return rc;
   const int64_t B = spec->batch;

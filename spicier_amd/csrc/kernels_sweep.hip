@@ -151,10 +151,6 @@ SPX_DECLARE_NEWTON(16, true, launch_newton_ir_16)
 SPX_DECLARE_NEWTON(32, true, launch_newton_ir_32)
 SPX_DECLARE_NEWTON(64, true, launch_newton_ir_64)
 
-size_t newton_smem_bytes(const DevProgram& g, int N) {
-  return sizeof(double) * ((size_t)N * (N + 2) + 4 * N + g.num_vals + g.P + g.num_elems);
-}
-
 // FP32+IR runs in two passes: the FP32 kernel leaves instances it cannot
 // resolve at SPX_ACTIVE, and the FP64 kernel, launched over the same grid,
 // solves only those (every other workgroup exits on its first load).

@@ -19,6 +19,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "assembly.h"
 #include "device_common.h"
 #include "lu_wave.h"
 #include "program.h"
@@ -65,70 +66,16 @@ struct Instance {
     }
   }
 
-  __device__ double X(int u) const { return u >= 0 ? xs[u] : 0.0; }
-
   // device evaluation + pattern assembly at xs; returns true if a pnjlim
   // limiter acted on any diode
   __device__ __forceinline__ bool assemble(double gmin, double sf, int limiter) {
     bool lim = false;
-    for (int i = lane; i < g.num_elems; i += kWave) {
-      const int e = g.order[i];
-      const int* el = g.elems + e * kElemInts;
-      const int kind = el[0], vid = el[11];
-      switch (kind) {
-        case SPX_EL_RESISTOR:
-          vals[vid] = 1.0 / pv[el[4]];
-          break;
-        case SPX_EL_VSOURCE:
-        case SPX_EL_VPULSE:
-        case SPX_EL_ISOURCE:
-          vals[vid] = pv[el[4]] * sf;
-          break;
-        case SPX_EL_DIODE: {
-          double vd = X(el[1]) - X(el[2]);
-          double gd, ieq;
-          diode_companion(pv[el[4]], pv[el[5]], vd, limiter, vst[e], lim, gd, ieq);
-          vals[vid] = gd;
-          vals[vid + 1] = ieq;
-          break;
-        }
-        case SPX_EL_NMOS:
-        case SPX_EL_PMOS: {
-          double vd = X(el[1]), vg = X(el[2]), vs = X(el[3]);
-          double gds, gm, ieq;
-          mosfet_companion(kind == SPX_EL_PMOS, pv[el[4]], pv[el[5]], pv[el[6]], pv[el[7]],
-                           pv[el[8]], vg - vs, vd - vs, gds, gm, ieq);
-          vals[vid] = gds;
-          vals[vid + 1] = gm;
-          vals[vid + 2] = ieq;
-          break;
-        }
-        default:
-          break;
-      }
-    }
+    for (int i = lane; i < g.num_elems; i += kWave)
+      eval_element(g, g.order[i], xs, pv, vals, vst, sf, limiter, lim);
     if (lane == 0) vals[kValGmin] = gmin;
     __syncthreads();
-    for (int q = lane; q < g.num_pos; q += kWave) {
-      double sum = 0.0;
-      const int c1 = g.pos_off[q + 1];
-      for (int c = g.pos_off[q]; c < c1; ++c) {
-        const int t = g.contrib[c];
-        const double v = vals[t >> 1];
-        sum = (t & 1) ? sum - v : sum + v;
-      }
-      J[q] = sum;
-    }
-    for (int r = lane; r < g.n; r += kWave) {
-      double sum = 0.0;
-      const int c1 = g.rhs_off[r + 1];
-      for (int c = g.rhs_off[r]; c < c1; ++c) {
-        const int t = g.rhs_contrib[c];
-        const double v = vals[t >> 1];
-        sum = (t & 1) ? sum - v : sum + v;
-      }
-      rhs[r] = sum;
-    }
+    for (int q = lane; q < g.num_pos; q += kWave) J[q] = gather_sum(g.pos_off, g.contrib, q, vals);
+    for (int r = lane; r < g.n; r += kWave) rhs[r] = gather_sum(g.rhs_off, g.rhs_contrib, r, vals);
     __syncthreads();
     return __any(lim);
   }
@@ -278,6 +225,49 @@ struct Instance {
     __syncthreads();
   }
 
+  // Linear circuit: one solve (sweep.rs:159-217), x left in xs (zeros if
+  // singular).  Returns 0 solved, 1 singular, 2 FP32 could not resolve the
+  // system (the FP64 pass takes the instance over).
+  __device__ __forceinline__ int linear_solve() {
+    assemble(0.0, 1.0, cfg.limiter);
+    double xn = 0.0;
+    int rc;
+    if constexpr (IR) {
+      RowLU<N, float> lu32;
+      lu32.row = reinterpret_cast<float*>(prow);
+      load_rows(lu32.a);
+      ++nfact;
+      rc = (lu32.factor(lane) && refine(lu32, true, xn) == 0) ? 0 : 2;
+    } else {
+      rc = solve_f64(xn) ? 0 : 1;
+    }
+    if (lane < g.n) xs[lane] = rc == 0 ? xn : 0.0;
+    __syncthreads();
+    return rc;
+  }
+
+  __device__ __forceinline__ void write_result(const DevOut& out, int64_t inst, int status,
+                                               int total, int flags, double res) const {
+    const int n = g.n;
+    if (lane < n) out.x[inst * n + lane] = status == SPX_SINGULAR ? 0.0 : xs[lane];
+    if (lane == 0) {
+      if (out.status) out.status[inst] = status;
+      if (out.iterations) out.iterations[inst] = total;
+      if (out.flags) out.flags[inst] = flags;
+      if (out.residual) out.residual[inst] = res;
+      if (out.factorizations) out.factorizations[inst] = nfact;
+      if (out.refinements) out.refinements[inst] = nref;
+    }
+  }
+
+  __device__ __forceinline__ void write_handover(const DevOut& out, int64_t inst,
+                                                 int flags) const {
+    if (lane == 0) {
+      out.status[inst] = SPX_ACTIVE;
+      if (out.flags) out.flags[inst] = flags | SPX_FLAG_FP64_ESCALATED;
+    }
+  }
+
   // max_i |KCL_i| / tol_i at xs, devices unlimited (include/spicier_hip.h)
   __device__ __forceinline__ double kcl(double gmin) {
     assemble(gmin, 1.0, SPX_LIMIT_NONE);
@@ -296,6 +286,12 @@ struct Instance {
     return wave_max(q);
   }
 };
+
+}  // namespace spx
+
+#include "dcop_driver.h"
+
+namespace spx {
 
 // escalated_only: run only the instances an FP32+IR pass handed over (status
 // left SPX_ACTIVE), in FP64.
@@ -328,155 +324,8 @@ __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
   __syncthreads();
   I.init_row();
   __syncthreads();
-
-  int total = 0, it = 0, esc = 0, status = SPX_FAILED;
-  int flags = (x0 ? SPX_FLAG_WARM_START : 0) | (escalated_only ? SPX_FLAG_FP64_ESCALATED : 0);
-  double final_gmin = 0.0;
-  bool handover = false;  // FP32+IR pass: leave this instance to the FP64 pass
-
-  if (!g.has_nonlinear) {
-    // linear circuit: one solve (sweep.rs:159-217)
-    I.assemble(0.0, 1.0, cfg.limiter);
-    double xn = 0.0;
-    bool ok;
-    if constexpr (IR) {
-      RowLU<N, float> lu32;
-      lu32.row = reinterpret_cast<float*>(prow);
-      I.load_rows(lu32.a);
-      ++I.nfact;
-      ok = lu32.factor(lane) && I.refine(lu32, true, xn) == 0;
-      handover = !ok;  // FP64 decides between singular and an FP32 limitation
-    } else {
-      ok = I.solve_f64(xn);
-    }
-    if (lane < n) xs[lane] = ok ? xn : 0.0;
-    __syncthreads();
-    status = ok ? SPX_CONVERGED : SPX_SINGULAR;
-    total = 1;
-  } else {
-    // Plain NR, then Gmin stepping (newton.rs:373-459), then source stepping
-    // (newton.rs:261-356), as one state machine so the Newton loop has a
-    // single, inlined call site.  Retries at one level restart from the same
-    // solution and so replay the first attempt exactly; the first attempt
-    // decides the level.  A stage that fails returns its last converged
-    // level's solution (zeros if none).
-    enum { kPlain, kGmin, kSource, kDone };
-    int stage = kPlain;
-    double gm = 0.0, factor = 0.0, step = 0.0;
-    auto enter_gmin = [&]() -> int {
-      flags |= SPX_FLAG_GMIN_STEPPING;
-      I.zero_x();
-      I.save_x();
-      gm = cfg.gmin_initial;
-      if (gm >= cfg.gmin_final * 0.99) return kGmin;
-      status = SPX_CONVERGED;  // no level to run: the stage returns converged zeros
-      return kDone;
-    };
-    auto enter_source = [&]() -> int {
-      flags |= SPX_FLAG_SOURCE_STEPPING;
-      I.zero_x();
-      I.save_x();
-      factor = cfg.src_initial;
-      step = cfg.src_step;
-      if (factor <= 1.0 + 1e-9) return kSource;
-      status = SPX_CONVERGED;
-      final_gmin = 0.0;
-      return kDone;
-    };
-    auto after_failure = [&]() -> int {
-      if (stage == kPlain && cfg.gmin_stepping) return enter_gmin();
-      if (stage != kSource && cfg.source_stepping) return enter_source();
-      return kDone;
-    };
-    while (stage != kDone) {
-      double lg = 0.0, lsf = 1.0;
-      if (stage == kGmin) {
-        lg = gm;
-      } else if (stage == kSource) {
-        lsf = fmin(factor, 1.0);
-        I.restore_x();
-      }
-      const int st = I.newton(lg, lsf, it, esc);
-      total += it;
-      if (st == -2) {
-        handover = true;
-        stage = kDone;
-      } else if (stage == kPlain) {
-        status = st == 1 ? SPX_CONVERGED : (st < 0 ? SPX_SINGULAR : SPX_FAILED);
-        stage = status == SPX_CONVERGED ? kDone : after_failure();
-      } else if (stage == kGmin) {
-        if (st != 1) {
-          I.restore_x();
-          status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
-          final_gmin = 0.0;
-          stage = after_failure();
-        } else {
-          I.save_x();
-          final_gmin = gm;
-          bool more = gm > cfg.gmin_final;
-          if (more) {
-            gm /= cfg.gmin_factor;
-            if (gm < cfg.gmin_final) gm = cfg.gmin_final;
-            more = gm >= cfg.gmin_final * 0.99;
-          }
-          if (!more) {
-            I.restore_x();
-            status = SPX_CONVERGED;
-            stage = kDone;
-          }
-        }
-      } else {  // kSource
-        const double cur = fmin(factor, 1.0);
-        bool finished = false;
-        if (st != 1) {
-          if (step > cfg.src_min_step) {
-            step /= 2.0;
-            factor -= step;
-            finished = !(factor <= 1.0 + 1e-9);
-          } else {
-            I.restore_x();
-            status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
-            stage = kDone;
-          }
-        } else {
-          I.save_x();
-          if (fabs(cur - 1.0) < 1e-9) {
-            finished = true;
-          } else {
-            factor += step;
-            if (step < cfg.src_step) step = fmin(step * 1.5, cfg.src_step);
-            finished = !(factor <= 1.0 + 1e-9);
-          }
-        }
-        if (finished) {
-          I.restore_x();
-          status = SPX_CONVERGED;
-          final_gmin = 0.0;
-          stage = kDone;
-        }
-      }
-    }
-  }
-  if (handover) {
-    if (lane == 0) {
-      out.status[inst] = SPX_ACTIVE;
-      if (out.flags) out.flags[inst] = flags | SPX_FLAG_FP64_ESCALATED;
-    }
-    return;
-  }
-  double res = I.kcl(final_gmin);
-  if (status == SPX_CONVERGED && !(res <= 1.0)) flags |= SPX_FLAG_KCL_FAIL;
-  if (lane < n) out.x[inst * n + lane] = status == SPX_SINGULAR ? 0.0 : xs[lane];
-  if (lane == 0) {
-    if (out.status) out.status[inst] = status;
-    if (out.iterations) out.iterations[inst] = total;
-    if (out.flags) out.flags[inst] = flags;
-    if (out.residual) out.residual[inst] = res;
-    if (out.factorizations) out.factorizations[inst] = I.nfact;
-    if (out.refinements) out.refinements[inst] = I.nref;
-  }
+  run_dcop(I, g, cfg, x0 != nullptr, out, inst, escalated_only);
 }
-
 
 template <int N, bool IR>
 hipError_t launch_newton_impl(const DevProgram& g, const DevNrCfg& cfg, const double* params,

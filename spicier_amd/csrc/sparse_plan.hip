@@ -1,0 +1,187 @@
+// sparse_plan.hip — host-side symbolic analysis for the shared-pattern LU
+// (sparse_plan.h).  Runs once per circuit pattern.
+#include <algorithm>
+#include <set>
+#include <unordered_map>
+#include <vector>
+
+#include "sparse_plan.h"
+
+namespace spx {
+
+namespace {
+constexpr int64_t kMaxUpdates = int64_t(1) << 26;  // 256 MB of update targets
+}
+
+const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_rc,
+                              HostSparsePlan& plan) {
+  // symmetrised adjacency without the diagonal
+  std::vector<std::set<int>> adj(n);
+  for (int rc : pos_rc) {
+    const int r = rc & 0xffff, c = rc >> 16;
+    if (r != c) {
+      adj[r].insert(c);
+      adj[c].insert(r);
+    }
+  }
+  // Minimum degree on the elimination graph of the node unknowns; a branch
+  // unknown (index >= num_nodes) follows the first of its nodes.  The
+  // neighbour set of a vertex when it is eliminated is its column structure
+  // S_k.
+  std::vector<int> perm, iperm(n, -1);
+  std::vector<std::vector<int>> S_orig(n);  // by elimination step, original indices
+  perm.reserve(n);
+  std::vector<char> done(n, 0);
+  auto eliminate = [&](int v) {
+    const int k = (int)perm.size();
+    perm.push_back(v);
+    iperm[v] = k;
+    done[v] = 1;
+    std::vector<int> nb(adj[v].begin(), adj[v].end());
+    S_orig[k] = nb;
+    for (int a : nb) {
+      adj[a].erase(v);
+      for (int b : nb)
+        if (a != b) adj[a].insert(b);
+    }
+    adj[v].clear();
+  };
+  // branch unknowns coupled to each node in the original pattern
+  std::vector<std::vector<int>> branches_of(num_nodes);
+  for (int b = num_nodes; b < n; ++b)
+    for (int v : adj[b])
+      if (v < num_nodes) branches_of[v].push_back(b);
+  for (int step = 0; step < num_nodes; ++step) {
+    int best = -1;
+    size_t bdeg = 0;
+    for (int v = 0; v < num_nodes; ++v)
+      if (!done[v] && (best < 0 || adj[v].size() < bdeg)) {
+        best = v;
+        bdeg = adj[v].size();
+      }
+    eliminate(best);
+    for (int b : branches_of[best])
+      if (!done[b]) eliminate(b);
+  }
+  for (int v = num_nodes; v < n; ++v)
+    if (!done[v]) eliminate(v);
+
+  // S_k in elimination indices, sorted
+  std::vector<std::vector<int>> S(n);
+  int64_t nnz_l = 0, num_upd = 0;
+  int max_s = 0;
+  for (int k = 0; k < n; ++k) {
+    for (int v : S_orig[k]) S[k].push_back(iperm[v]);
+    std::sort(S[k].begin(), S[k].end());
+    for (int i : S[k])
+      if (i <= k) return "symbolic factorisation produced an entry above the diagonal";
+    nnz_l += (int64_t)S[k].size();
+    num_upd += (int64_t)S[k].size() * (int64_t)S[k].size();
+    max_s = std::max(max_s, (int)S[k].size());
+  }
+  if (num_upd > kMaxUpdates || 2 * nnz_l + n > (int64_t)0x7fffffff)
+    return "the filled LU pattern is too large for the sparse sweep path";
+  // row structure of L: Lrow[i] = {k < i : i in S_k}, ascending
+  std::vector<std::vector<int>> Lrow(n);
+  for (int k = 0; k < n; ++k)
+    for (int i : S[k]) Lrow[i].push_back(k);
+  // LU storage, row by row: L part, diagonal, U part (= S_i)
+  std::vector<int> row_off(n + 1, 0);
+  for (int i = 0; i < n; ++i) row_off[i + 1] = row_off[i] + (int)Lrow[i].size() + 1 + (int)S[i].size();
+  const int nnz_lu = row_off[n];
+  std::vector<std::unordered_map<int, int>> where(n);
+  for (int i = 0; i < n; ++i) {
+    int o = row_off[i];
+    for (int j : Lrow[i]) where[i][j] = o++;
+    where[i][i] = o++;
+    for (int j : S[i]) where[i][j] = o++;
+  }
+  auto idx = [&](int i, int j) -> int {
+    auto it = where[i].find(j);
+    return it == where[i].end() ? -1 : it->second;
+  };
+
+  std::vector<int> diag(n), col_ptr(n + 1, 0), s_row, s_l, s_u, upd_ptr(n + 1, 0), tgt,
+      ut_ptr(n + 1, 0), ut_row, ut_idx, pos_lu, fill;
+  for (int k = 0; k < n; ++k) {
+    diag[k] = idx(k, k);
+    for (int i : S[k]) {
+      s_row.push_back(i);
+      s_l.push_back(idx(i, k));
+      s_u.push_back(idx(k, i));
+    }
+    col_ptr[k + 1] = (int)s_row.size();
+    for (int a : S[k])
+      for (int b : S[k]) {
+        const int t = idx(a, b);
+        if (t < 0) return "symbolic factorisation is not closed under elimination";
+        tgt.push_back(t);
+      }
+    upd_ptr[k + 1] = (int)tgt.size();
+    for (int i : Lrow[k]) {
+      ut_row.push_back(i);
+      ut_idx.push_back(idx(i, k));
+    }
+    ut_ptr[k + 1] = (int)ut_row.size();
+  }
+  std::vector<char> covered(nnz_lu, 0);
+  for (int rc : pos_rc) {
+    const int t = idx(iperm[rc & 0xffff], iperm[rc >> 16]);
+    if (t < 0) return "pattern entry missing from the filled pattern";
+    pos_lu.push_back(t);
+    covered[t] = 1;
+  }
+  for (int t = 0; t < nnz_lu; ++t)
+    if (!covered[t]) fill.push_back(t);
+
+  plan.nnz_lu = nnz_lu;
+  plan.max_s = max_s;
+  plan.num_fill = (int)fill.size();
+  plan.num_upd = num_upd;
+  plan.blob.clear();
+  auto put = [&](const std::vector<int>& v) {
+    size_t off = plan.blob.size();
+    plan.blob.insert(plan.blob.end(), v.begin(), v.end());
+    while (plan.blob.size() % 4) plan.blob.push_back(0);
+    return off;
+  };
+  plan.o_perm = put(perm);
+  plan.o_diag = put(diag);
+  plan.o_colptr = put(col_ptr);
+  plan.o_srow = put(s_row);
+  plan.o_sl = put(s_l);
+  plan.o_su = put(s_u);
+  plan.o_updptr = put(upd_ptr);
+  plan.o_tgt = put(tgt);
+  plan.o_utptr = put(ut_ptr);
+  plan.o_utrow = put(ut_row);
+  plan.o_utidx = put(ut_idx);
+  plan.o_poslu = put(pos_lu);
+  plan.o_fill = put(fill);
+  plan.blob.push_back(0);  // keeps every offset a valid address
+  return "";
+}
+
+DevSparse device_sparse(const HostSparsePlan& p, const int* base) {
+  DevSparse d;
+  d.nnz_lu = p.nnz_lu;
+  d.max_s = p.max_s;
+  d.num_fill = p.num_fill;
+  d.pad = 0;
+  d.perm = base + p.o_perm;
+  d.diag = base + p.o_diag;
+  d.col_ptr = base + p.o_colptr;
+  d.s_row = base + p.o_srow;
+  d.s_l = base + p.o_sl;
+  d.s_u = base + p.o_su;
+  d.upd_ptr = base + p.o_updptr;
+  d.tgt = base + p.o_tgt;
+  d.ut_ptr = base + p.o_utptr;
+  d.ut_row = base + p.o_utrow;
+  d.ut_idx = base + p.o_utidx;
+  d.pos_lu = base + p.o_poslu;
+  d.fill = base + p.o_fill;
+  return d;
+}
+
+}  // namespace spx
