@@ -1,6 +1,7 @@
 // newton_sparse.hip — Newton DC operating point for systems past the
 // register LU (65 <= n <= 4096): one 256-thread workgroup per instance,
-// the shared-symbolic LU of sparse_plan.h, FP64 throughout.
+// the shared-symbolic, level-scheduled LU of sparse_plan.h, FP64
+// throughout.
 //
 // Per instance the workgroup keeps x, rhs, the saved stepping solution and
 // the triangular-solve vector in LDS; element values, pattern values and the
@@ -35,8 +36,6 @@ struct SparseInstance {
   double* xs;        // LDS n
   double* xw;        // LDS n
   double* y;         // LDS n, elimination order
-  double* lbuf;      // LDS max_s
-  double* ubuf;      // LDS max_s
   int* red;          // LDS T/64 ints
   double* redd;      // LDS T/64 doubles
   int nfact = 0;
@@ -80,68 +79,59 @@ struct SparseInstance {
 
   // Factor lu in place and solve for rhs; the solution is left in y in
   // elimination order.  False if a pivot is exactly zero or the solution is
-  // not finite.
+  // not finite.  Level by level: multipliers of the level's columns, then
+  // its updates target by target (contributions in column order).
   __device__ __forceinline__ bool factor_solve() {
     ++nfact;
     const int n = g.n;
-    bool ok = true;
-    for (int k = 0; k < n; ++k) {
-      const int s0 = sp.col_ptr[k], s = sp.col_ptr[k + 1] - s0;
-      const double piv = lu[sp.diag[k]];
-      ok = ok && piv != 0.0;
-      const double inv = piv != 0.0 ? 1.0 / piv : 0.0;
-      for (int t = tid; t < s; t += T) {
-        const int li = sp.s_l[s0 + t];
-        const double l = lu[li] * inv;
-        lu[li] = l;
-        lbuf[t] = l;
-        ubuf[t] = lu[sp.s_u[s0 + t]];
+    for (int L = 0; L < sp.levels; ++L) {
+      const int a1 = sp.pa_ptr[L + 1];
+      for (int t = sp.pa_ptr[L] + tid; t < a1; t += T) {
+        const int li = sp.pa_l[t];
+        lu[li] = lu[li] * (1.0 / lu[sp.pa_d[t]]);
       }
       __syncthreads();
-      if (s > 0) {
-        const int* tg = sp.tgt + sp.upd_ptr[k];
-        const int ss = s * s;
-        int a = tid / s, b = tid - (tid / s) * s;
-        const int da = T / s, db = T - (T / s) * s;
-        for (int p = tid; p < ss; p += T) {
-          const int ti = tg[p];
-          lu[ti] = fma(-lbuf[a], ubuf[b], lu[ti]);
-          a += da;
-          b += db;
-          if (b >= s) {
-            b -= s;
-            ++a;
-          }
-        }
+      const int b1 = sp.tb_ptr[L + 1];
+      for (int t = sp.tb_ptr[L] + tid; t < b1; t += T) {
+        const int ti = sp.tb_t[t];
+        double v = lu[ti];
+        const int c1 = sp.tb_c[t + 1];
+        for (int c = sp.tb_c[t]; c < c1; ++c) v = fma(-lu[sp.cb_l[c]], lu[sp.cb_u[c]], v);
+        lu[ti] = v;
       }
       __syncthreads();
     }
-    if (!ok) return false;
-    // L y = P b (unit lower, column oriented)
+    bool ok = true;
+    for (int k = tid; k < n; k += T) ok = ok && lu[sp.diag[k]] != 0.0;
+    if (!block_all(ok)) return false;
+    // L y = P b: row-oriented over the forward levels
     for (int k = tid; k < n; k += T) y[k] = rhs[sp.perm[k]];
     __syncthreads();
-    for (int k = 0; k < n; ++k) {
-      const double yk = y[k];
-      const int s0 = sp.col_ptr[k], s1 = sp.col_ptr[k + 1];
-      for (int t = s0 + tid; t < s1; t += T) {
-        const int i = sp.s_row[t];
-        y[i] = fma(-lu[sp.s_l[t]], yk, y[i]);
+    for (int L = 0; L < sp.levels; ++L) {
+      const int r1 = sp.lev_ptr[L + 1];
+      for (int r = sp.lev_ptr[L] + tid; r < r1; r += T) {
+        const int i = sp.lev_rows[r];
+        const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
+        const double* li = lu + sp.diag[i] - nl;
+        double v = y[i];
+        for (int t = 0; t < nl; ++t) v = fma(-li[t], y[sp.ut_row[u0 + t]], v);
+        y[i] = v;
       }
       __syncthreads();
     }
-    // U x = y (column oriented, bottom up); y[k] takes x_k one step later,
-    // after every thread has read y[k]
-    for (int k = n - 1; k >= 0; --k) {
-      const double xk = y[k] * (1.0 / lu[sp.diag[k]]);
-      const int u0 = sp.ut_ptr[k], u1 = sp.ut_ptr[k + 1];
-      for (int t = u0 + tid; t < u1; t += T) {
-        const int i = sp.ut_row[t];
-        y[i] = fma(-lu[sp.ut_idx[t]], xk, y[i]);
+    // U x = y: row-oriented over the levels counted from the root
+    for (int L = 0; L < sp.blevels; ++L) {
+      const int r1 = sp.blev_ptr[L + 1];
+      for (int r = sp.blev_ptr[L] + tid; r < r1; r += T) {
+        const int i = sp.blev_rows[r];
+        const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
+        const double* ui = lu + sp.diag[i];
+        double v = y[i];
+        for (int t = 0; t < ns; ++t) v = fma(-ui[1 + t], y[sp.s_row[s0 + t]], v);
+        y[i] = v * (1.0 / ui[0]);
       }
       __syncthreads();
-      if (tid == 0) y[k] = xk;
     }
-    __syncthreads();
     bool fin = true;
     for (int k = tid; k < n; k += T) fin = fin && isfinite(y[k]);
     return block_all(fin);
@@ -248,7 +238,7 @@ inline int64_t sparse_slot_doubles(const DevProgram& g, const DevSparse& sp, boo
 }
 
 inline size_t sparse_smem_bytes(const DevProgram& g, const DevSparse& sp, bool lu_in_lds) {
-  size_t d = 4 * (size_t)g.n + 2 * (size_t)sp.max_s + kSparseThreads / kWave;
+  size_t d = 4 * (size_t)g.n + kSparseThreads / kWave;
   size_t b = d * sizeof(double) + (kSparseThreads / kWave) * sizeof(int);
   b = (b + 7) & ~size_t(7);
   if (lu_in_lds) b += (size_t)sp.nnz_lu * sizeof(double);
@@ -265,9 +255,7 @@ __global__ __launch_bounds__(kSparseThreads) void k_newton_sparse(
   double* xs = rhs + n;
   double* xw = xs + n;
   double* y = xw + n;
-  double* lbuf = y + n;
-  double* ubuf = lbuf + sp.max_s;
-  double* redd = ubuf + sp.max_s;
+  double* redd = y + n;
   int* red = reinterpret_cast<int*>(redd + kSparseThreads / kWave);
   double* lds_lu = reinterpret_cast<double*>(
       (reinterpret_cast<uintptr_t>(red + kSparseThreads / kWave) + 7) & ~uintptr_t(7));
@@ -284,7 +272,7 @@ __global__ __launch_bounds__(kSparseThreads) void k_newton_sparse(
     for (int i = tid; i < n; i += kSparseThreads) xs[i] = x0 ? x0[inst * n + i] : 0.0;
     __syncthreads();
     SparseInstance I{g, sp, cfg, tid, params + inst * g.P, vals, vst, av, lu, rhs, xs, xw, y,
-                     lbuf, ubuf, red, redd};
+                     red, redd};
     run_dcop(I, g, cfg, x0 != nullptr, out, inst, 0);
     __syncthreads();
   }

@@ -1,6 +1,7 @@
 // sparse_plan.hip — host-side symbolic analysis for the shared-pattern LU
 // (sparse_plan.h).  Runs once per circuit pattern.
 #include <algorithm>
+#include <map>
 #include <set>
 #include <unordered_map>
 #include <vector>
@@ -10,7 +11,7 @@
 namespace spx {
 
 namespace {
-constexpr int64_t kMaxUpdates = int64_t(1) << 26;  // 256 MB of update targets
+constexpr int64_t kMaxUpdates = int64_t(1) << 24;  // 128 MB of update contributions
 }
 
 const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_rc,
@@ -101,28 +102,64 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
     return it == where[i].end() ? -1 : it->second;
   };
 
-  std::vector<int> diag(n), col_ptr(n + 1, 0), s_row, s_l, s_u, upd_ptr(n + 1, 0), tgt,
-      ut_ptr(n + 1, 0), ut_row, ut_idx, pos_lu, fill;
+  std::vector<int> diag(n), col_ptr(n + 1, 0), s_row, ut_ptr(n + 1, 0), ut_row, pos_lu, fill;
   for (int k = 0; k < n; ++k) {
     diag[k] = idx(k, k);
-    for (int i : S[k]) {
-      s_row.push_back(i);
-      s_l.push_back(idx(i, k));
-      s_u.push_back(idx(k, i));
-    }
+    for (int i : S[k]) s_row.push_back(i);
     col_ptr[k + 1] = (int)s_row.size();
-    for (int a : S[k])
-      for (int b : S[k]) {
-        const int t = idx(a, b);
-        if (t < 0) return "symbolic factorisation is not closed under elimination";
-        tgt.push_back(t);
-      }
-    upd_ptr[k + 1] = (int)tgt.size();
-    for (int i : Lrow[k]) {
-      ut_row.push_back(i);
-      ut_idx.push_back(idx(i, k));
-    }
+    for (int i : Lrow[k]) ut_row.push_back(i);
     ut_ptr[k + 1] = (int)ut_row.size();
+  }
+  // forward levels: level(k) = 1 + max level(j) over j in Lrow[k]
+  std::vector<int> lev(n, 0), blev(n, 0);
+  int levels = 0, blevels = 0;
+  for (int k = 0; k < n; ++k) {
+    for (int j : Lrow[k]) lev[k] = std::max(lev[k], lev[j] + 1);
+    levels = std::max(levels, lev[k] + 1);
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    for (int j : S[k]) blev[k] = std::max(blev[k], blev[j] + 1);
+    blevels = std::max(blevels, blev[k] + 1);
+  }
+  std::vector<std::vector<int>> by_lev(levels), by_blev(blevels);
+  for (int k = 0; k < n; ++k) by_lev[lev[k]].push_back(k);
+  for (int k = 0; k < n; ++k) by_blev[blev[k]].push_back(k);
+  std::vector<int> lev_ptr(1, 0), lev_rows, pa_ptr(1, 0), pa_l, pa_d, tb_ptr(1, 0), tb_t,
+      tb_c(1, 0), cb_l, cb_u, blev_ptr(1, 0), blev_rows;
+  int max_level_rows = 0;
+  for (int L = 0; L < levels; ++L) {
+    max_level_rows = std::max(max_level_rows, (int)by_lev[L].size());
+    for (int k : by_lev[L]) {
+      lev_rows.push_back(k);
+      for (int i : S[k]) {
+        pa_l.push_back(idx(i, k));
+        pa_d.push_back(diag[k]);
+      }
+    }
+    lev_ptr.push_back((int)lev_rows.size());
+    pa_ptr.push_back((int)pa_l.size());
+    // phase B: the level's targets, each with its contributions in column order
+    std::map<int, std::vector<std::pair<int, int>>> contrib;
+    for (int k : by_lev[L])
+      for (int a : S[k])
+        for (int b : S[k]) {
+          const int t = idx(a, b);
+          if (t < 0) return "symbolic factorisation is not closed under elimination";
+          contrib[t].push_back({idx(a, k), idx(k, b)});
+        }
+    for (auto& kv : contrib) {
+      tb_t.push_back(kv.first);
+      for (auto& lu : kv.second) {
+        cb_l.push_back(lu.first);
+        cb_u.push_back(lu.second);
+      }
+      tb_c.push_back((int)cb_l.size());
+    }
+    tb_ptr.push_back((int)tb_t.size());
+  }
+  for (int L = 0; L < blevels; ++L) {
+    for (int k : by_blev[L]) blev_rows.push_back(k);
+    blev_ptr.push_back((int)blev_rows.size());
   }
   std::vector<char> covered(nnz_lu, 0);
   for (int rc : pos_rc) {
@@ -138,6 +175,9 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
   plan.max_s = max_s;
   plan.num_fill = (int)fill.size();
   plan.num_upd = num_upd;
+  plan.levels = levels;
+  plan.blevels = blevels;
+  plan.max_level_rows = max_level_rows;
   plan.blob.clear();
   auto put = [&](const std::vector<int>& v) {
     size_t off = plan.blob.size();
@@ -149,13 +189,20 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
   plan.o_diag = put(diag);
   plan.o_colptr = put(col_ptr);
   plan.o_srow = put(s_row);
-  plan.o_sl = put(s_l);
-  plan.o_su = put(s_u);
-  plan.o_updptr = put(upd_ptr);
-  plan.o_tgt = put(tgt);
   plan.o_utptr = put(ut_ptr);
   plan.o_utrow = put(ut_row);
-  plan.o_utidx = put(ut_idx);
+  plan.o_levptr = put(lev_ptr);
+  plan.o_levrows = put(lev_rows);
+  plan.o_paptr = put(pa_ptr);
+  plan.o_pal = put(pa_l);
+  plan.o_pad = put(pa_d);
+  plan.o_tbptr = put(tb_ptr);
+  plan.o_tbt = put(tb_t);
+  plan.o_tbc = put(tb_c);
+  plan.o_cbl = put(cb_l);
+  plan.o_cbu = put(cb_u);
+  plan.o_blevptr = put(blev_ptr);
+  plan.o_blevrows = put(blev_rows);
   plan.o_poslu = put(pos_lu);
   plan.o_fill = put(fill);
   plan.blob.push_back(0);  // keeps every offset a valid address
@@ -165,20 +212,27 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
 DevSparse device_sparse(const HostSparsePlan& p, const int* base) {
   DevSparse d;
   d.nnz_lu = p.nnz_lu;
-  d.max_s = p.max_s;
+  d.levels = p.levels;
+  d.blevels = p.blevels;
   d.num_fill = p.num_fill;
-  d.pad = 0;
   d.perm = base + p.o_perm;
   d.diag = base + p.o_diag;
   d.col_ptr = base + p.o_colptr;
   d.s_row = base + p.o_srow;
-  d.s_l = base + p.o_sl;
-  d.s_u = base + p.o_su;
-  d.upd_ptr = base + p.o_updptr;
-  d.tgt = base + p.o_tgt;
   d.ut_ptr = base + p.o_utptr;
   d.ut_row = base + p.o_utrow;
-  d.ut_idx = base + p.o_utidx;
+  d.lev_ptr = base + p.o_levptr;
+  d.lev_rows = base + p.o_levrows;
+  d.pa_ptr = base + p.o_paptr;
+  d.pa_l = base + p.o_pal;
+  d.pa_d = base + p.o_pad;
+  d.tb_ptr = base + p.o_tbptr;
+  d.tb_t = base + p.o_tbt;
+  d.tb_c = base + p.o_tbc;
+  d.cb_l = base + p.o_cbl;
+  d.cb_u = base + p.o_cbu;
+  d.blev_ptr = base + p.o_blevptr;
+  d.blev_rows = base + p.o_blevrows;
   d.pos_lu = base + p.o_poslu;
   d.fill = base + p.o_fill;
   return d;
