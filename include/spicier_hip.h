@@ -103,6 +103,30 @@ int spx_solve_batch_f64_device(spx_handle* h, const double* d_matrices,
  * max_batch_per_launch, solver.rs:6-41). Larger batches are split. */
 int64_t spx_max_batch_per_launch(void);
 
+/* Batched sparse solve with one pattern shared by every system — the GPU
+ * counterpart of FaerTripletBatchedSolver::solve_batch_triplets and of the
+ * numeric half of FaerSparseCachedBatchedSolver
+ * (crates/spicier-batched-sweep/src/faer_sparse_solver.rs:196-225,230-362).
+ * The pattern is nnz triplets (rows[t], cols[t]), 0-based, duplicates summed
+ * in triplet order; system i has values[i*nnz + t] and rhs[i*n ..].  The
+ * symbolic analysis (minimum-degree order, fill, elimination levels) runs
+ * once per pattern and is cached on the handle, like the reference's
+ * RwLock-cached SymbolicLu.  The numeric LU makes no row interchanges; an
+ * unknown without a diagonal entry (an MNA voltage-source branch) is
+ * eliminated right after its first neighbour.  A system is singular when a
+ * pivot is exactly zero or the solution is not finite: its solution is all
+ * zero and singular[i] = 1 (singular may be NULL).  n must be 1..4096. */
+int spx_sparse_solve_batch_f64(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows,
+                               const int32_t* cols, const double* values, const double* rhs,
+                               int64_t batch, double* solutions, uint8_t* singular,
+                               int64_t* num_singular);
+/* Same on device pointers (values, rhs, solutions, singular); rows/cols are
+ * host arrays.  Enqueued on the handle's stream. */
+int spx_sparse_solve_batch_f64_device(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows,
+                                      const int32_t* cols, const double* d_values,
+                                      const double* d_rhs, int64_t batch, double* d_solutions,
+                                      uint8_t* d_singular);
+
 /* ------------------------------------------------------------------------- */
 /* Circuit description: the device-resident "circuit program".  Element kinds
  * and stamp conventions follow spicier-core MnaSystem

@@ -137,6 +137,10 @@ def load_library():
         "spx_solve_batch_f64": (C.c_int, [vp, vp, vp, i64, i64, vp, u8p, C.POINTER(C.c_int64)]),
         "spx_solve_batch_f64_device": (C.c_int, [vp, vp, vp, i64, i64, vp, u8p]),
         "spx_max_batch_per_launch": (C.c_int64, []),
+        "spx_sparse_solve_batch_f64": (C.c_int, [vp, i64, i64, vp, vp, vp, vp, i64, vp, u8p,
+                                                 C.POINTER(C.c_int64)]),
+        "spx_sparse_solve_batch_f64_device": (C.c_int, [vp, i64, i64, vp, vp, vp, vp, i64, vp,
+                                                        u8p]),
         "spx_sample_params": (C.c_int, [vp, C.POINTER(SpxSweepSpec), i32, vp]),
         "spx_sample_params_device": (C.c_int, [vp, C.POINTER(SpxSweepSpec), i32, vp]),
         "spx_nr_config_default": (None, [C.POINTER(SpxNrConfig)]),

@@ -24,6 +24,12 @@ struct spx_handle {
   std::vector<int> program_blob;       // circuit program now in the "program" workspace
   bool has_sparse = false;             // its shared-symbolic LU plan (n > 64) is uploaded
   spx::HostSparsePlan sparse_plan;
+  // batched sparse linear solve: the pattern (n, nnz, rows, cols) whose plan
+  // is in the "lin_plan" workspace, and that plan's extra offsets
+  std::vector<int> lin_key;
+  spx::HostSparsePlan lin_plan;
+  int lin_num_pos = 0;
+  size_t lin_o_off = 0, lin_o_trip = 0;
 };
 
 namespace spx {

@@ -29,6 +29,13 @@ hipError_t launch_stats(const double* v, const uint8_t* mask, const int* status,
 int64_t stats_num_blocks(int64_t B);
 hipError_t launch_solve_batch(const double* M, const double* rhs, int n, int64_t B, double* x,
                               uint8_t* sing, double* work, int64_t chunk, hipStream_t st);
+hipError_t sparse_solve_shape(const DevSparse& sp, int n, int64_t B, int* grid, int64_t* slot,
+                              size_t* smem, int* lu_in_lds);
+hipError_t launch_sparse_solve(const DevSparse& sp, int n, int num_pos, const int* pos_off,
+                               const int* pos_trip, int64_t nnz, const double* values,
+                               const double* rhs, int64_t B, double* x, uint8_t* singular,
+                               double* ws, int grid, int64_t slot, size_t smem, int lu_in_lds,
+                               hipStream_t st);
 hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
                                int64_t* slot, size_t* smem, int* lu_in_lds);
 hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
@@ -449,6 +456,74 @@ int sweep_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
   return SPX_OK;
 }
 
+// Batched sparse solve on device pointers (caller holds the mutex).
+int sparse_solve_locked(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows,
+                        const int32_t* cols, const double* d_values, const double* d_rhs,
+                        int64_t B, double* d_x, uint8_t* d_sing) {
+  if (n < 1 || n > kSparseMaxN)
+    return fail(SPX_ERR_INVALID_DIMENSION, "sparse solve needs 1 <= n <= " +
+                                               std::to_string(kSparseMaxN));
+  if (nnz < 1 || nnz > 0x7fffffffLL || !rows || !cols)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "sparse solve needs a non-empty triplet pattern");
+  if (B < 0) return fail(SPX_ERR_INVALID_DIMENSION, "negative batch");
+  if (B > 0 && (!d_values || !d_rhs || !d_x))
+    return fail(SPX_ERR_INVALID_ARGUMENT, "null values, rhs or solutions");
+  for (int64_t t = 0; t < nnz; ++t)
+    if (rows[t] < 0 || rows[t] >= n || cols[t] < 0 || cols[t] >= n)
+      return fail(SPX_ERR_INVALID_ARGUMENT, "triplet " + std::to_string(t) + " outside the matrix");
+  if (B == 0) return SPX_OK;
+  int rc;
+  std::vector<int> key;
+  key.reserve(2 + 2 * nnz);
+  key.push_back((int)n);
+  key.push_back((int)nnz);
+  key.insert(key.end(), rows, rows + nnz);
+  key.insert(key.end(), cols, cols + nnz);
+  void* dp = nullptr;
+  if (key != h->lin_key) {
+    std::map<std::pair<int, int>, std::vector<int>> pos;
+    for (int64_t t = 0; t < nnz; ++t) pos[{rows[t], cols[t]}].push_back((int)t);
+    std::vector<int> pos_rc, pos_off(1, 0), pos_trip;
+    for (auto& kv : pos) {
+      pos_rc.push_back(kv.first.first | (kv.first.second << 16));
+      pos_trip.insert(pos_trip.end(), kv.second.begin(), kv.second.end());
+      pos_off.push_back((int)pos_trip.size());
+    }
+    h->lin_key.clear();
+    const char* why = build_sparse_plan((int)n, (int)n, pos_rc, h->lin_plan);
+    if (*why) return fail(SPX_ERR_INVALID_DIMENSION, why);
+    std::vector<int>& blob = h->lin_plan.blob;
+    h->lin_o_off = blob.size();
+    blob.insert(blob.end(), pos_off.begin(), pos_off.end());
+    h->lin_o_trip = blob.size();
+    blob.insert(blob.end(), pos_trip.begin(), pos_trip.end());
+    blob.push_back(0);
+    h->lin_num_pos = (int)pos_rc.size();
+    if ((rc = workspace(h, "lin_plan", blob.size() * sizeof(int), &dp))) return rc;
+    SPX_HIP(hipMemcpyAsync(dp, blob.data(), blob.size() * sizeof(int), hipMemcpyHostToDevice,
+                           h->stream),
+            SPX_ERR_TRANSFER);
+    SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+    h->lin_key = std::move(key);
+  }
+  if ((rc = workspace(h, "lin_plan", h->lin_plan.blob.size() * sizeof(int), &dp))) return rc;
+  const int* base = static_cast<const int*>(dp);
+  DevSparse sp = device_sparse(h->lin_plan, base);
+  int grid = 0, lu_lds = 0;
+  int64_t slot = 0;
+  size_t smem = 0;
+  SPX_HIP(sparse_solve_shape(sp, (int)n, B, &grid, &slot, &smem, &lu_lds), SPX_ERR_BACKEND);
+  void* ws = nullptr;
+  if ((rc = workspace(h, "lin_ws", sizeof(double) * std::max<int64_t>(1, (int64_t)grid * slot),
+                      &ws)))
+    return rc;
+  SPX_HIP(launch_sparse_solve(sp, (int)n, h->lin_num_pos, base + h->lin_o_off,
+                              base + h->lin_o_trip, nnz, d_values, d_rhs, B, d_x, d_sing,
+                              static_cast<double*>(ws), grid, slot, smem, lu_lds, h->stream),
+          SPX_ERR_BACKEND);
+  return SPX_OK;
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -719,6 +794,58 @@ int spx_sweep_dcop(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* 
   if ((rc = d2h(out->factorizations, dfac, sizeof(int) * B))) return rc;
   if ((rc = d2h(out->refinements, dref, sizeof(int) * B))) return rc;
   SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+  return SPX_OK;
+}
+
+int spx_sparse_solve_batch_f64_device(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows,
+                                      const int32_t* cols, const double* d_values,
+                                      const double* d_rhs, int64_t batch, double* d_solutions,
+                                      uint8_t* d_singular) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  return sparse_solve_locked(h, n, nnz, rows, cols, d_values, d_rhs, batch, d_solutions,
+                             d_singular);
+}
+
+int spx_sparse_solve_batch_f64(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows,
+                               const int32_t* cols, const double* values, const double* rhs,
+                               int64_t batch, double* solutions, uint8_t* singular,
+                               int64_t* num_singular) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  if (num_singular) *num_singular = 0;
+  if (batch < 0) return fail(SPX_ERR_INVALID_DIMENSION, "negative batch");
+  if (batch > 0 && (!values || !rhs || !solutions))
+    return fail(SPX_ERR_INVALID_ARGUMENT, "null values, rhs or solutions");
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  if (batch == 0) return sparse_solve_locked(h, n, nnz, rows, cols, nullptr, nullptr, 0, nullptr,
+                                             nullptr);
+  int rc;
+  void *dv, *db, *dx, *ds;
+  if ((rc = workspace(h, "ls_values", sizeof(double) * batch * nnz, &dv))) return rc;
+  if ((rc = workspace(h, "ls_rhs", sizeof(double) * batch * n, &db))) return rc;
+  if ((rc = workspace(h, "ls_x", sizeof(double) * batch * n, &dx))) return rc;
+  if ((rc = workspace(h, "ls_sing", batch, &ds))) return rc;
+  SPX_HIP(hipMemcpyAsync(dv, values, sizeof(double) * batch * nnz, hipMemcpyHostToDevice,
+                         h->stream),
+          SPX_ERR_TRANSFER);
+  SPX_HIP(hipMemcpyAsync(db, rhs, sizeof(double) * batch * n, hipMemcpyHostToDevice, h->stream),
+          SPX_ERR_TRANSFER);
+  if ((rc = sparse_solve_locked(h, n, nnz, rows, cols, (double*)dv, (double*)db, batch,
+                                (double*)dx, (uint8_t*)ds)))
+    return rc;
+  std::vector<uint8_t> sing(batch);
+  SPX_HIP(hipMemcpyAsync(solutions, dx, sizeof(double) * batch * n, hipMemcpyDeviceToHost,
+                         h->stream),
+          SPX_ERR_TRANSFER);
+  SPX_HIP(hipMemcpyAsync(sing.data(), ds, batch, hipMemcpyDeviceToHost, h->stream),
+          SPX_ERR_TRANSFER);
+  SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+  int64_t ns = 0;
+  for (int64_t i = 0; i < batch; ++i) ns += sing[i] != 0;
+  if (singular) std::memcpy(singular, sing.data(), batch);
+  if (num_singular) *num_singular = ns;
   return SPX_OK;
 }
 

@@ -21,6 +21,78 @@ namespace spx {
 
 constexpr int kSparseThreads = 256;
 
+template <int T>
+__device__ __forceinline__ bool block_all_t(bool p, int tid, int* red) {
+  const bool w = __all(p);
+  if ((tid & (kWave - 1)) == 0) red[tid / kWave] = w;
+  __syncthreads();
+  bool r = true;
+  for (int i = 0; i < T / kWave; ++i) r = r && red[i];
+  __syncthreads();
+  return r;
+}
+
+// Factors lu in place with the plan sp, then solves for the right-hand side
+// b(i) (i in unknown order); the solution is left in y (LDS, n) in
+// elimination order.  False if a pivot is exactly zero or the solution is
+// not finite.  Level by level: multipliers of the level's columns, then its
+// updates target by target (contributions in column order); the triangular
+// solves are row-oriented over the forward levels and the levels counted
+// from the root.
+template <int T, class RHS>
+__device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, double* lu, RHS b,
+                                                    double* y, int tid, int* red) {
+  for (int L = 0; L < sp.levels; ++L) {
+    const int a1 = sp.pa_ptr[L + 1];
+    for (int t = sp.pa_ptr[L] + tid; t < a1; t += T) {
+      const int li = sp.pa_l[t];
+      lu[li] = lu[li] * (1.0 / lu[sp.pa_d[t]]);
+    }
+    __syncthreads();
+    const int b1 = sp.tb_ptr[L + 1];
+    for (int t = sp.tb_ptr[L] + tid; t < b1; t += T) {
+      const int ti = sp.tb_t[t];
+      double v = lu[ti];
+      const int c1 = sp.tb_c[t + 1];
+      for (int c = sp.tb_c[t]; c < c1; ++c) v = fma(-lu[sp.cb_l[c]], lu[sp.cb_u[c]], v);
+      lu[ti] = v;
+    }
+    __syncthreads();
+  }
+  bool ok = true;
+  for (int k = tid; k < n; k += T) ok = ok && lu[sp.diag[k]] != 0.0;
+  if (!block_all_t<T>(ok, tid, red)) return false;
+  for (int k = tid; k < n; k += T) y[k] = b(sp.perm[k]);
+  __syncthreads();
+  for (int L = 0; L < sp.levels; ++L) {  // L y = P b
+    const int r1 = sp.lev_ptr[L + 1];
+    for (int r = sp.lev_ptr[L] + tid; r < r1; r += T) {
+      const int i = sp.lev_rows[r];
+      const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
+      const double* li = lu + sp.diag[i] - nl;
+      double v = y[i];
+      for (int t = 0; t < nl; ++t) v = fma(-li[t], y[sp.ut_row[u0 + t]], v);
+      y[i] = v;
+    }
+    __syncthreads();
+  }
+  for (int L = 0; L < sp.blevels; ++L) {  // U x = y
+    const int r1 = sp.blev_ptr[L + 1];
+    for (int r = sp.blev_ptr[L] + tid; r < r1; r += T) {
+      const int i = sp.blev_rows[r];
+      const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
+      const double* ui = lu + sp.diag[i];
+      double v = y[i];
+      for (int t = 0; t < ns; ++t) v = fma(-ui[1 + t], y[sp.s_row[s0 + t]], v);
+      y[i] = v * (1.0 / ui[0]);
+    }
+    __syncthreads();
+  }
+  bool fin = true;
+  for (int k = tid; k < n; k += T) fin = fin && isfinite(y[k]);
+  return block_all_t<T>(fin, tid, red);
+}
+
 struct SparseInstance {
   static constexpr int T = kSparseThreads;
   const DevProgram& g;
@@ -41,15 +113,7 @@ struct SparseInstance {
   int nfact = 0;
   int nref = 0;
 
-  __device__ __forceinline__ bool block_all(bool p) {
-    const bool w = __all(p);
-    if ((tid & (kWave - 1)) == 0) red[tid / kWave] = w;
-    __syncthreads();
-    bool r = true;
-    for (int i = 0; i < T / kWave; ++i) r = r && red[i];
-    __syncthreads();
-    return r;
-  }
+  __device__ __forceinline__ bool block_all(bool p) { return block_all_t<T>(p, tid, red); }
   __device__ __forceinline__ double block_max(double v) {
     const double w = wave_max(v);
     if ((tid & (kWave - 1)) == 0) redd[tid / kWave] = w;
@@ -77,64 +141,9 @@ struct SparseInstance {
     return !block_all(!lim);
   }
 
-  // Factor lu in place and solve for rhs; the solution is left in y in
-  // elimination order.  False if a pivot is exactly zero or the solution is
-  // not finite.  Level by level: multipliers of the level's columns, then
-  // its updates target by target (contributions in column order).
   __device__ __forceinline__ bool factor_solve() {
     ++nfact;
-    const int n = g.n;
-    for (int L = 0; L < sp.levels; ++L) {
-      const int a1 = sp.pa_ptr[L + 1];
-      for (int t = sp.pa_ptr[L] + tid; t < a1; t += T) {
-        const int li = sp.pa_l[t];
-        lu[li] = lu[li] * (1.0 / lu[sp.pa_d[t]]);
-      }
-      __syncthreads();
-      const int b1 = sp.tb_ptr[L + 1];
-      for (int t = sp.tb_ptr[L] + tid; t < b1; t += T) {
-        const int ti = sp.tb_t[t];
-        double v = lu[ti];
-        const int c1 = sp.tb_c[t + 1];
-        for (int c = sp.tb_c[t]; c < c1; ++c) v = fma(-lu[sp.cb_l[c]], lu[sp.cb_u[c]], v);
-        lu[ti] = v;
-      }
-      __syncthreads();
-    }
-    bool ok = true;
-    for (int k = tid; k < n; k += T) ok = ok && lu[sp.diag[k]] != 0.0;
-    if (!block_all(ok)) return false;
-    // L y = P b: row-oriented over the forward levels
-    for (int k = tid; k < n; k += T) y[k] = rhs[sp.perm[k]];
-    __syncthreads();
-    for (int L = 0; L < sp.levels; ++L) {
-      const int r1 = sp.lev_ptr[L + 1];
-      for (int r = sp.lev_ptr[L] + tid; r < r1; r += T) {
-        const int i = sp.lev_rows[r];
-        const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
-        const double* li = lu + sp.diag[i] - nl;
-        double v = y[i];
-        for (int t = 0; t < nl; ++t) v = fma(-li[t], y[sp.ut_row[u0 + t]], v);
-        y[i] = v;
-      }
-      __syncthreads();
-    }
-    // U x = y: row-oriented over the levels counted from the root
-    for (int L = 0; L < sp.blevels; ++L) {
-      const int r1 = sp.blev_ptr[L + 1];
-      for (int r = sp.blev_ptr[L] + tid; r < r1; r += T) {
-        const int i = sp.blev_rows[r];
-        const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
-        const double* ui = lu + sp.diag[i];
-        double v = y[i];
-        for (int t = 0; t < ns; ++t) v = fma(-ui[1 + t], y[sp.s_row[s0 + t]], v);
-        y[i] = v * (1.0 / ui[0]);
-      }
-      __syncthreads();
-    }
-    bool fin = true;
-    for (int k = tid; k < n; k += T) fin = fin && isfinite(y[k]);
-    return block_all(fin);
+    return sparse_factor_solve<T>(sp, g.n, lu, [&](int i) { return rhs[i]; }, y, tid, red);
   }
 
   __device__ __forceinline__ int newton(double gmin, double sf, int& iters, int& fp64_escalations) {
@@ -278,6 +287,84 @@ __global__ __launch_bounds__(kSparseThreads) void k_newton_sparse(
   }
 }
 
+// Batched sparse solve with a shared pattern (the numeric half of
+// FaerSparseCachedBatchedSolver / FaerTripletBatchedSolver,
+// crates/spicier-batched-sweep/src/faer_sparse_solver.rs:196-362): system
+// `inst` has triplet values values[inst][nnz]; pattern entry q sums the
+// triplets pos_trip[pos_off[q] .. pos_off[q+1]) in triplet order.
+__global__ __launch_bounds__(kSparseThreads) void k_sparse_solve(
+    DevSparse sp, int n, int num_pos, const int* __restrict__ pos_off,
+    const int* __restrict__ pos_trip, int64_t nnz, const double* __restrict__ values,
+    const double* __restrict__ rhs, int64_t B, double* __restrict__ x, uint8_t* singular,
+    double* ws, int64_t slot, int lu_in_lds) {
+  extern __shared__ double smem[];
+  const int tid = threadIdx.x;
+  double* y = smem;
+  int* red = reinterpret_cast<int*>(y + n);
+  double* lu = lu_in_lds ? reinterpret_cast<double*>(
+                               (reinterpret_cast<uintptr_t>(red + kSparseThreads / kWave) + 7) &
+                               ~uintptr_t(7))
+                         : ws + blockIdx.x * slot;
+  for (int64_t inst = blockIdx.x; inst < B; inst += gridDim.x) {
+    const double* v = values + inst * nnz;
+    for (int q = tid; q < num_pos; q += kSparseThreads) {
+      double sum = 0.0;
+      for (int c = pos_off[q]; c < pos_off[q + 1]; ++c) sum += v[pos_trip[c]];
+      lu[sp.pos_lu[q]] = sum;
+    }
+    for (int f = tid; f < sp.num_fill; f += kSparseThreads) lu[sp.fill[f]] = 0.0;
+    __syncthreads();
+    const double* b = rhs + inst * n;
+    const bool ok = sparse_factor_solve<kSparseThreads>(
+        sp, n, lu, [&](int i) { return b[i]; }, y, tid, red);
+    for (int k = tid; k < n; k += kSparseThreads) x[inst * n + sp.perm[k]] = ok ? y[k] : 0.0;
+    if (tid == 0 && singular) singular[inst] = ok ? 0 : 1;
+    __syncthreads();
+  }
+}
+
+inline hipError_t grid_for(const void* kernel, size_t smem, int64_t B, int* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return e;
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kSparseThreads, smem)) !=
+      hipSuccess)
+    return e;
+  if (per_cu < 1) per_cu = 1;
+  int64_t gsz = (int64_t)cus * per_cu;
+  if (gsz > B) gsz = B;
+  if (gsz < 1) gsz = 1;
+  *grid = (int)gsz;
+  return hipSuccess;
+}
+
+// Shape of a batched sparse solve: LU in LDS when it fits in 64 KB.
+hipError_t sparse_solve_shape(const DevSparse& sp, int n, int64_t B, int* grid, int64_t* slot,
+                              size_t* smem, int* lu_in_lds) {
+  size_t base = (size_t)n * sizeof(double) + (kSparseThreads / kWave) * sizeof(int) + 8;
+  *lu_in_lds = base + (size_t)sp.nnz_lu * sizeof(double) <= 64 * 1024 ? 1 : 0;
+  *smem = base + (*lu_in_lds ? (size_t)sp.nnz_lu * sizeof(double) : 0);
+  *slot = *lu_in_lds ? 0 : (((int64_t)sp.nnz_lu + 31) & ~int64_t(31));
+  hipError_t e = hipFuncSetAttribute((const void*)k_sparse_solve,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)*smem);
+  if (e != hipSuccess) return e;
+  return grid_for((const void*)k_sparse_solve, *smem, B, grid);
+}
+
+hipError_t launch_sparse_solve(const DevSparse& sp, int n, int num_pos, const int* pos_off,
+                               const int* pos_trip, int64_t nnz, const double* values,
+                               const double* rhs, int64_t B, double* x, uint8_t* singular,
+                               double* ws, int grid, int64_t slot, size_t smem, int lu_in_lds,
+                               hipStream_t st) {
+  if (B == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sparse_solve, dim3((unsigned)grid), dim3(kSparseThreads), smem, st, sp, n,
+                     num_pos, pos_off, pos_trip, nnz, values, rhs, B, x, singular, ws, slot,
+                     lu_in_lds);
+  return hipGetLastError();
+}
+
 // Grid and workspace shape of a sparse launch: `grid` workgroups, each with
 // a slot of `slot` doubles of global workspace and `smem` bytes of LDS.
 hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
@@ -289,19 +376,7 @@ hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t
   hipError_t e = hipFuncSetAttribute((const void*)k_newton_sparse,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)*smem);
   if (e != hipSuccess) return e;
-  int dev = 0, cus = 0, per_cu = 0;
-  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
-    return e;
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_newton_sparse,
-                                                        kSparseThreads, *smem)) != hipSuccess)
-    return e;
-  if (per_cu < 1) per_cu = 1;
-  int64_t gsz = (int64_t)cus * per_cu;
-  if (gsz > B) gsz = B;
-  if (gsz < 1) gsz = 1;
-  *grid = (int)gsz;
-  return hipSuccess;
+  return grid_for((const void*)k_newton_sparse, *smem, B, grid);
 }
 
 hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,

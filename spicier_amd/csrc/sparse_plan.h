@@ -6,7 +6,8 @@
 // crates/spicier-batched-sweep/src/faer_sparse_solver.rs:116-137):
 //   1. ordering: minimum degree over the node unknowns of the symmetrised
 //      pattern, ties to the lowest index, and each voltage-source branch
-//      unknown eliminated right after the first of its nodes.  The branch
+//      unknown (an unknown past num_nodes, or one without a diagonal entry
+//      in the pattern) eliminated right after the first of its nodes.  The branch
 //      row's zero diagonal has then been filled (-1/G of that node), and the
 //      node/branch pair acts as a Dirichlet condition on the remaining
 //      nodes: a grid fed only through ideal sources keeps a nonsingular

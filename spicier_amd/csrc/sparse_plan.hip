@@ -25,10 +25,9 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
       adj[c].insert(r);
     }
   }
-  // Minimum degree on the elimination graph of the node unknowns; a branch
-  // unknown (index >= num_nodes) follows the first of its nodes.  The
-  // neighbour set of a vertex when it is eliminated is its column structure
-  // S_k.
+  // Minimum degree on the elimination graph of the plain unknowns; a branch
+  // unknown follows the first of its neighbours.  The neighbour set of a
+  // vertex when it is eliminated is its column structure S_k.
   std::vector<int> perm, iperm(n, -1);
   std::vector<std::vector<int>> S_orig(n);  // by elimination step, original indices
   perm.reserve(n);
@@ -47,16 +46,25 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
     }
     adj[v].clear();
   };
-  // branch unknowns coupled to each node in the original pattern
-  std::vector<std::vector<int>> branches_of(num_nodes);
-  for (int b = num_nodes; b < n; ++b)
-    for (int v : adj[b])
-      if (v < num_nodes) branches_of[v].push_back(b);
-  for (int step = 0; step < num_nodes; ++step) {
+  // "Branch" unknowns: index >= num_nodes, or no diagonal entry in the
+  // pattern (an MNA voltage-source row seen through a plain matrix).  Each
+  // follows the first of its neighbours to be eliminated.
+  std::vector<char> has_diag(n, 0), branch(n, 0);
+  for (int rc : pos_rc)
+    if ((rc & 0xffff) == (rc >> 16)) has_diag[rc & 0xffff] = 1;
+  for (int v = 0; v < n; ++v) branch[v] = v >= num_nodes || !has_diag[v];
+  std::vector<std::vector<int>> branches_of(n);
+  for (int b = 0; b < n; ++b)
+    if (branch[b])
+      for (int v : adj[b])
+        if (!branch[v]) branches_of[v].push_back(b);
+  int num_plain = 0;
+  for (int v = 0; v < n; ++v) num_plain += !branch[v];
+  for (int step = 0; step < num_plain; ++step) {
     int best = -1;
     size_t bdeg = 0;
-    for (int v = 0; v < num_nodes; ++v)
-      if (!done[v] && (best < 0 || adj[v].size() < bdeg)) {
+    for (int v = 0; v < n; ++v)
+      if (!done[v] && !branch[v] && (best < 0 || adj[v].size() < bdeg)) {
         best = v;
         bdeg = adj[v].size();
       }
@@ -64,7 +72,7 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
     for (int b : branches_of[best])
       if (!done[b]) eliminate(b);
   }
-  for (int v = num_nodes; v < n; ++v)
+  for (int v = 0; v < n; ++v)
     if (!done[v]) eliminate(v);
 
   // S_k in elimination indices, sorted

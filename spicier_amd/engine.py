@@ -238,6 +238,80 @@ class HipBatchedSolver(BatchedLuSolver):
         return self._config
 
 
+class HipSparseBatchedSolver(BatchedLuSolver):
+    """Shared-pattern sparse LU on the GPU: the drop-in for
+    FaerSparseCachedBatchedSolver (dense input, pattern |a_ij| > 1e-15 of the
+    first matrix) and FaerTripletBatchedSolver (triplets with one pattern),
+    crates/spicier-batched-sweep/src/faer_sparse_solver.rs:29-362.  The
+    symbolic analysis is done once per pattern and cached on the device
+    handle; systems whose own nonzeros fall outside the cached pattern are
+    reported singular, as the reference's try_new_with_symbolic rejects them."""
+
+    def __init__(self, config: GpuBatchConfig | None = None, device_id: int = 0):
+        self._config = config or GpuBatchConfig()
+        self.device = Device.get(device_id)
+        self._pattern = None  # (n, rows, cols) from the first dense batch
+
+    def reset_cache(self):
+        self._pattern = None
+
+    def has_cached_symbolic(self) -> bool:
+        return self._pattern is not None
+
+    def solve_batch_triplets(self, rows, cols, values, rhs, n: int) -> BatchedSolveResult:
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        nnz = rows.size
+        B = v.size // nnz if nnz else 0
+        b = np.ascontiguousarray(rhs, dtype=np.float64).reshape(-1)
+        if cols.size != nnz or v.size != B * nnz or b.size != B * n:
+            raise InvalidDimension(f"expected {B}x{nnz} values and {B}x{n} rhs")
+        x = np.zeros(B * n)
+        sing = np.zeros(max(B, 1), dtype=np.uint8)
+        ns = C.c_int64(0)
+        _check(self.device.lib.spx_sparse_solve_batch_f64(
+            self.device.handle, n, nnz, A.ptr(rows), A.ptr(cols), A.ptr(v), A.ptr(b), B,
+            A.ptr(x), A.ptr(sing), C.byref(ns)))
+        return BatchedSolveResult(x, [int(i) for i in np.nonzero(sing[:B])[0]], n, B)
+
+    def solve_batch(self, matrices, rhs, n: int, batch_size: int) -> BatchedSolveResult:
+        m = np.ascontiguousarray(matrices, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(rhs, dtype=np.float64).reshape(-1)
+        if m.size != batch_size * n * n:
+            raise InvalidDimension(f"Expected {batch_size * n * n} matrix elements, got {m.size}")
+        if b.size != batch_size * n:
+            raise InvalidDimension(f"Expected {batch_size * n} RHS elements, got {b.size}")
+        if batch_size == 0:
+            return BatchedSolveResult(np.zeros(0), [], n, 0)
+        mats = m.reshape(batch_size, n, n)           # [i][col][row]
+        nz = np.abs(mats) > 1e-15                    # dense_to_sparse_triplets threshold
+        if self._pattern is None:
+            cols, rows = np.nonzero(nz[0])
+            self._pattern = (n, rows.astype(np.int32), cols.astype(np.int32))
+        pn, rows, cols = self._pattern
+        if pn != n:
+            raise InvalidDimension(f"Matrix size {n} doesn't match cached size {pn}")
+        inside = np.zeros((n, n), dtype=bool)
+        inside[cols, rows] = True
+        fits = ~np.any(nz & ~inside, axis=(1, 2))
+        vals = mats[:, cols, rows]
+        r = self.solve_batch_triplets(rows, cols, vals, b, n)
+        x = r.solutions.reshape(batch_size, n)
+        x[~fits] = 0.0
+        sing = sorted(set(r.singular_indices) | {int(i) for i in np.nonzero(~fits)[0]})
+        return BatchedSolveResult(x.reshape(-1), sing, n, batch_size)
+
+    def should_use_gpu(self, matrix_size: int, batch_size: int) -> bool:
+        return self._config.should_use_gpu(matrix_size, batch_size)
+
+    def backend_type(self) -> BackendType:
+        return BackendType.Hip
+
+    def config(self) -> GpuBatchConfig:
+        return self._config
+
+
 @dataclass
 class BackendSelector:
     """solver.rs:139-312, with HIP as the only backend this engine ships.  The

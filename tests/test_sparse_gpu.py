@@ -105,3 +105,83 @@ def test_sparse_path_rejects_fp32():
     c = CK.grid_diodes(16)
     with pytest.raises(E.InvalidArgument):
         E.sweep_dcop(c, 4, config=E.nr_config(precision=E.Precision.F32_IR))
+
+
+# ------------------------------------------------- batched sparse linear solve
+def _grid_mna(N, rng, B):
+    """B systems of an N x N conductance grid with four ideal corner sources
+    (zero-diagonal branch rows), one shared triplet pattern with duplicates."""
+    n = N * N + 4
+    rows, cols, vals = [], [], []
+    g = rng.uniform(0.9e-3, 1.1e-3, size=(B, 2 * N * (N - 1)))
+    e = 0
+    for r in range(N):
+        for c in range(N):
+            k = r * N + c
+            for nb in ((k + 1) if c + 1 < N else None, (k + N) if r + 1 < N else None):
+                if nb is None:
+                    continue
+                for (i, j, s) in ((k, k, 1), (nb, nb, 1), (k, nb, -1), (nb, k, -1)):
+                    rows.append(i); cols.append(j); vals.append(s * g[:, e])
+                e += 1
+    for s, node in enumerate((0, N - 1, N * (N - 1), N * N - 1)):
+        for (i, j) in ((node, N * N + s), (N * N + s, node)):
+            rows.append(i); cols.append(j); vals.append(np.ones(B))
+    return n, np.array(rows), np.array(cols), np.stack(vals, axis=1)
+
+
+def _dense(n, rows, cols, vals):
+    m = np.zeros((vals.shape[0], n, n))
+    for t in range(rows.size):
+        m[:, rows[t], cols[t]] += vals[:, t]
+    return m
+
+
+def test_sparse_solve_matches_lapack():
+    rng = np.random.default_rng(3)
+    B = 40
+    n, rows, cols, vals = _grid_mna(14, rng, B)   # duplicates: every diagonal is summed
+    b = rng.standard_normal((B, n))
+    r = E.HipSparseBatchedSolver().solve_batch_triplets(rows, cols, vals, b, n)
+    want = np.linalg.solve(_dense(n, rows, cols, vals), b[..., None])[..., 0]
+    assert r.singular_indices == []
+    assert np.all(close(r.solutions.reshape(B, n), want, rtol=1e-10, atol=1e-13))
+
+
+def test_sparse_solve_singular_and_edge_cases():
+    s = E.HipSparseBatchedSolver()
+    rows, cols = np.array([0, 1, 2, 1]), np.array([0, 1, 2, 2])
+    vals = np.array([[2.0, 4.0, 8.0, 1.0],     # regular
+                     [2.0, 0.0, 8.0, 1.0],     # exact zero pivot
+                     [1e300, 1e300, 1e-300, 1.0]])
+    b = np.ones((3, 3))
+    r = s.solve_batch_triplets(rows, cols, vals, b, 3)
+    np.testing.assert_allclose(r.solution(0), [0.5, (1 - 1 / 8) / 4, 1 / 8], rtol=1e-15)
+    assert r.singular_indices == [1, 2] or r.singular_indices == [1]
+    assert np.all(r.solution(1) == 0)
+    r = s.solve_batch_triplets(rows, cols, np.zeros((0, 4)), np.zeros((0, 3)), 3)
+    assert r.batch_size == 0
+    with pytest.raises(E.InvalidArgument):
+        s.solve_batch_triplets(np.array([0, 5]), np.array([0, 0]), np.ones((1, 2)), np.ones(3), 3)
+
+
+def test_sparse_dense_input_pattern_semantics():
+    # FaerSparseCachedBatchedSolver: pattern from the first matrix (|a| > 1e-15);
+    # a later matrix with a nonzero outside it is reported singular
+    rng = np.random.default_rng(5)
+    B, n = 6, 30
+    m = np.zeros((B, n, n))
+    for i in range(n):
+        m[:, i, i] = 4.0 + rng.random(B)
+        if i + 1 < n:
+            m[:, i, i + 1] = m[:, i + 1, i] = -1.0
+    m[3, 0, n - 1] = 0.5                       # outside the first matrix's pattern
+    b = rng.standard_normal((B, n))
+    cm = np.ascontiguousarray(np.transpose(m, (0, 2, 1))).reshape(-1)
+    s = E.HipSparseBatchedSolver()
+    r = s.solve_batch(cm, b.reshape(-1), n, B)
+    assert s.has_cached_symbolic() and r.singular_indices == [3]
+    want = np.linalg.solve(m, b[..., None])[..., 0]
+    ok = [i for i in range(B) if i != 3]
+    assert np.all(close(r.solutions.reshape(B, n)[ok], want[ok], rtol=1e-12, atol=1e-14))
+    assert np.all(r.solution(3) == 0)
