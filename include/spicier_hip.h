@@ -127,6 +127,39 @@ int spx_sparse_solve_batch_f64_device(spx_handle* h, int64_t n, int64_t nnz, con
                                       const double* d_rhs, int64_t batch, double* d_solutions,
                                       uint8_t* d_singular);
 
+/* Batched restarted GMRES with right Jacobi preconditioning, one shared
+ * triplet pattern (duplicates summed) — solve_gmres_real_preconditioned with
+ * JacobiPreconditioner (crates/spicier-solver/src/gmres/real.rs:208-400,
+ * crates/spicier-solver/src/preconditioner.rs:43-100; GmresConfig defaults
+ * gmres/mod.rs:34-48), run for every system on the device in one launch in
+ * place of the Metal sketch's host-driven loop
+ * (crates/spicier-backend-metal/src/batched_gmres.rs:396-657).  Each system
+ * stops when ||b - A x|| / ||b|| < tol or after max_iter Arnoldi steps (the
+ * count that trips the budget is included, as in the reference).  x0 (may be
+ * NULL = zero start) is [B][n]; iterations (int32), residuals (final relative
+ * residual) and converged (uint8) may be NULL.  n must be 1..4096. */
+typedef struct spx_gmres_config {
+    int32_t restart;  /* Krylov dimension before restart (30) */
+    int32_t max_iter; /* total Arnoldi steps (500) */
+    double tol;       /* relative residual (1e-8) */
+    int32_t jacobi;   /* 1: right Jacobi preconditioning (M = diag(A), |d| < 1e-30 -> 1) */
+} spx_gmres_config;
+
+void spx_gmres_config_default(spx_gmres_config* cfg);
+int spx_gmres_solve_batch_f64(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows,
+                              const int32_t* cols, const double* values, const double* rhs,
+                              const double* x0, int64_t batch, const spx_gmres_config* cfg,
+                              double* solutions, int32_t* iterations, double* residuals,
+                              uint8_t* converged);
+/* Same on device pointers (rows/cols stay host arrays); enqueued on the
+ * handle's stream. */
+int spx_gmres_solve_batch_f64_device(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows,
+                                     const int32_t* cols, const double* d_values,
+                                     const double* d_rhs, const double* d_x0, int64_t batch,
+                                     const spx_gmres_config* cfg, double* d_solutions,
+                                     int32_t* d_iterations, double* d_residuals,
+                                     uint8_t* d_converged);
+
 /* ------------------------------------------------------------------------- */
 /* Circuit description: the device-resident "circuit program".  Element kinds
  * and stamp conventions follow spicier-core MnaSystem

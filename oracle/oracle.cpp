@@ -12,8 +12,10 @@
 
 #include <atomic>
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -741,4 +743,140 @@ extern "C" void orc_sweep_statistics(const double* s, int64_t count, double* o) 
   o[1] = std::sqrt(acc / (double)count);
   o[2] = mn;
   o[3] = mx;
+}
+
+// ----------------------------------------------------------- GMRES
+// Restarted GMRES with right Jacobi preconditioning:
+// solve_gmres_real_preconditioned (spicier-solver/src/gmres/real.rs:208-400)
+// with JacobiPreconditioner (preconditioner.rs:43-100: M = diag(A), entries
+// below 1e-30 in magnitude replaced by 1); jacobi == 0 is solve_gmres_real
+// (real.rs:30-205), i.e. M = I.  Givens rotations as helpers.rs:40-48.  The
+// operator sums duplicate triplets per (row, col) in triplet order and
+// applies rows in column order; dot products are sequential (the reference's
+// SIMD dot product sums in a different order, so agreement is to rounding).
+extern "C" int orc_gmres(int32_t n, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                         const double* vals, const double* b, int32_t restart, int32_t max_iter,
+                         double tol, int32_t jacobi, double* x_out, int32_t* iterations,
+                         double* residual) {
+  std::map<std::pair<int, int>, double> m;
+  for (int64_t t = 0; t < nnz; ++t) m[{rows[t], cols[t]}] += vals[t];
+  std::vector<int> rp(n + 1, 0), ci;
+  std::vector<double> av;
+  std::vector<double> diag(n, 0.0);
+  for (auto& kv : m) {
+    rp[kv.first.first + 1]++;
+    ci.push_back(kv.first.second);
+    av.push_back(kv.second);
+    if (kv.first.first == kv.first.second) diag[kv.first.first] = kv.second;
+  }
+  for (int i = 0; i < n; ++i) rp[i + 1] += rp[i];
+  std::vector<double> inv(n, 1.0);
+  if (jacobi)
+    for (int i = 0; i < n; ++i) inv[i] = std::fabs(diag[i]) < 1e-30 ? 1.0 : 1.0 / diag[i];
+  auto apply = [&](const std::vector<double>& v, std::vector<double>& y) {
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int q = rp[i]; q < rp[i + 1]; ++q) s += av[q] * v[ci[q]];
+      y[i] = s;
+    }
+  };
+  auto dot = [&](const std::vector<double>& a, const std::vector<double>& c) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += a[i] * c[i];
+    return s;
+  };
+  std::vector<double> bv(b, b + n), x(n, 0.0);
+  const double bn = std::sqrt(dot(bv, bv));
+  *iterations = 0;
+  if (bn < 1e-30) {
+    for (int i = 0; i < n; ++i) x_out[i] = 0.0;
+    *residual = 0.0;
+    return 1;
+  }
+  int total = 0;
+  std::vector<double> ax(n);
+  for (int cycle = 0; cycle < max_iter; ++cycle) {
+    apply(x, ax);
+    std::vector<double> r(n);
+    for (int i = 0; i < n; ++i) r[i] = bv[i] - ax[i];
+    const double rn = std::sqrt(dot(r, r));
+    if (rn / bn < tol) {
+      for (int i = 0; i < n; ++i) x_out[i] = x[i];
+      *iterations = total;
+      *residual = rn / bn;
+      return 1;
+    }
+    const int mk = std::min(restart, n);
+    std::vector<std::vector<double>> v, z;
+    std::vector<std::vector<double>> h(mk, std::vector<double>(mk + 1, 0.0));
+    for (int i = 0; i < n; ++i) r[i] *= 1.0 / rn;
+    v.push_back(r);
+    std::vector<double> g(mk + 1, 0.0), cs(mk, 0.0), sn(mk, 0.0);
+    g[0] = rn;
+    int k = 0;
+    while (k < mk) {
+      ++total;
+      if (total > max_iter) break;
+      std::vector<double> zk(n);
+      for (int i = 0; i < n; ++i) zk[i] = v[k][i] * inv[i];
+      z.push_back(zk);
+      std::vector<double> w(n);
+      apply(z[k], w);
+      for (int j = 0; j <= k; ++j) {
+        const double hij = dot(v[j], w);
+        h[k][j] = hij;
+        for (int i = 0; i < n; ++i) w[i] -= hij * v[j][i];
+      }
+      const double wn = std::sqrt(dot(w, w));
+      h[k][k + 1] = wn;
+      if (wn < 1e-30) {
+        ++k;
+        break;
+      }
+      for (int i = 0; i < n; ++i) w[i] *= 1.0 / wn;
+      v.push_back(w);
+      for (int j = 0; j < k; ++j) {
+        const double t = cs[j] * h[k][j] + sn[j] * h[k][j + 1];
+        h[k][j + 1] = -sn[j] * h[k][j] + cs[j] * h[k][j + 1];
+        h[k][j] = t;
+      }
+      double c = 1.0, s = 0.0;
+      if (!(std::fabs(h[k][k + 1]) < 1e-30)) {
+        const double rr = std::sqrt(h[k][k] * h[k][k] + h[k][k + 1] * h[k][k + 1]);
+        c = h[k][k] / rr;
+        s = h[k][k + 1] / rr;
+      }
+      cs[k] = c;
+      sn[k] = s;
+      h[k][k] = c * h[k][k] + s * h[k][k + 1];
+      h[k][k + 1] = 0.0;
+      const double tg = c * g[k] + s * g[k + 1];
+      g[k + 1] = -s * g[k] + c * g[k + 1];
+      g[k] = tg;
+      ++k;
+      if (std::fabs(g[k]) / bn < tol) break;
+    }
+    std::vector<double> y(k, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+      double sum = g[i];
+      for (int j = i + 1; j < k; ++j) sum -= h[j][i] * y[j];
+      if (std::fabs(h[i][i]) > 1e-30) y[i] = sum / h[i][i];
+    }
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < n; ++j) x[j] += z[i][j] * y[i];
+    apply(x, ax);
+    double fr = 0.0;
+    for (int i = 0; i < n; ++i) fr += (bv[i] - ax[i]) * (bv[i] - ax[i]);
+    fr = std::sqrt(fr);
+    if (fr / bn < tol || total >= max_iter) {
+      for (int i = 0; i < n; ++i) x_out[i] = x[i];
+      *iterations = total;
+      *residual = fr / bn;
+      return fr / bn < tol ? 1 : 0;
+    }
+  }
+  for (int i = 0; i < n; ++i) x_out[i] = x[i];
+  *iterations = total;
+  *residual = NAN;
+  return 0;
 }

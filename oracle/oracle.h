@@ -97,6 +97,13 @@ void orc_stats_from_moments(int64_t count, double sum, double sum_sq, double* ou
  * out = mean, std_dev, min, max. */
 void orc_sweep_statistics(const double* samples, int64_t count, double* out4);
 
+/* Restarted GMRES(restart) with right Jacobi preconditioning (jacobi != 0)
+ * or none: gmres/real.rs:30-400, preconditioner.rs:43-100.  Triplets with
+ * duplicates summed; x starts at 0.  Returns 1 converged / 0 not. */
+int orc_gmres(int32_t n, int64_t nnz, const int32_t* rows, const int32_t* cols,
+              const double* vals, const double* b, int32_t restart, int32_t max_iter,
+              double tol, int32_t jacobi, double* x, int32_t* iterations, double* residual);
+
 #ifdef __cplusplus
 }
 #endif

@@ -146,3 +146,23 @@ def stats_accumulate(values, mask=None):
     out = np.zeros((n, 5))
     lib().orc_stats_accumulate(_p(v), _p(m), B, n, _p(out))
     return out
+
+
+def gmres(n, rows, cols, vals, b, restart=30, max_iter=500, tol=1e-8, jacobi=True):
+    """orc_gmres: restarted GMRES with right Jacobi preconditioning from x = 0.
+    Returns (x, converged, iterations, residual)."""
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(n)
+    it = C.c_int32(0)
+    res = C.c_double(0)
+    f = lib().orc_gmres
+    f.restype = C.c_int
+    f.argtypes = [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                  C.c_int32, C.c_double, C.c_int32, C.c_void_p, C.POINTER(C.c_int32),
+                  C.POINTER(C.c_double)]
+    conv = f(n, rows.size, _p(rows), _p(cols), _p(vals), _p(b), restart, max_iter, tol,
+             1 if jacobi else 0, _p(x), C.byref(it), C.byref(res))
+    return x, bool(conv), it.value, res.value

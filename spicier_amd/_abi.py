@@ -83,6 +83,11 @@ class SpxNrConfig(C.Structure):
     ]
 
 
+class SpxGmresConfig(C.Structure):
+    _fields_ = [("restart", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
+                ("jacobi", C.c_int32)]
+
+
 class SpxSweepResult(C.Structure):
     _fields_ = [("x", C.c_void_p), ("status", C.c_void_p), ("iterations", C.c_void_p),
                 ("flags", C.c_void_p), ("residual", C.c_void_p), ("params", C.c_void_p),
@@ -141,6 +146,12 @@ def load_library():
                                                  C.POINTER(C.c_int64)]),
         "spx_sparse_solve_batch_f64_device": (C.c_int, [vp, i64, i64, vp, vp, vp, vp, i64, vp,
                                                         u8p]),
+        "spx_gmres_config_default": (None, [C.POINTER(SpxGmresConfig)]),
+        "spx_gmres_solve_batch_f64": (C.c_int, [vp, i64, i64, vp, vp, vp, vp, vp, i64,
+                                                C.POINTER(SpxGmresConfig), vp, vp, vp, u8p]),
+        "spx_gmres_solve_batch_f64_device": (C.c_int, [vp, i64, i64, vp, vp, vp, vp, vp, i64,
+                                                       C.POINTER(SpxGmresConfig), vp, vp, vp,
+                                                       u8p]),
         "spx_sample_params": (C.c_int, [vp, C.POINTER(SpxSweepSpec), i32, vp]),
         "spx_sample_params_device": (C.c_int, [vp, C.POINTER(SpxSweepSpec), i32, vp]),
         "spx_nr_config_default": (None, [C.POINTER(SpxNrConfig)]),

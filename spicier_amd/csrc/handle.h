@@ -30,6 +30,11 @@ struct spx_handle {
   spx::HostSparsePlan lin_plan;
   int lin_num_pos = 0;
   size_t lin_o_off = 0, lin_o_trip = 0;
+  // batched GMRES: the pattern whose CSR tables are in the "gm_pattern" workspace
+  std::vector<int> gm_key;
+  std::vector<int> gm_blob;
+  int gm_num_pos = 0;
+  size_t gm_o_rowptr = 0, gm_o_col = 0, gm_o_off = 0, gm_o_trip = 0, gm_o_diag = 0;
 };
 
 namespace spx {

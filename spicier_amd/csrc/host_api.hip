@@ -17,6 +17,7 @@
 #include "../../include/spicier_hip.h"
 #include "handle.h"
 #include "program.h"
+#include "gmres.h"
 #include "sparse_plan.h"
 
 namespace spx {
@@ -524,6 +525,87 @@ int sparse_solve_locked(spx_handle* h, int64_t n, int64_t nnz, const int32_t* ro
   return SPX_OK;
 }
 
+int gmres_locked(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                 const double* d_values, const double* d_rhs, const double* d_x0, int64_t B,
+                 const spx_gmres_config* cfg, double* d_x, int32_t* d_it, double* d_res,
+                 uint8_t* d_conv) {
+  if (!cfg) return fail(SPX_ERR_INVALID_ARGUMENT, "null GMRES configuration");
+  if (cfg->restart < 1 || cfg->restart > 256 || cfg->max_iter < 0 || !(cfg->tol >= 0.0))
+    return fail(SPX_ERR_INVALID_ARGUMENT, "GMRES needs 1 <= restart <= 256, max_iter >= 0, tol >= 0");
+  if (n < 1 || n > kSparseMaxN)
+    return fail(SPX_ERR_INVALID_DIMENSION, "GMRES needs 1 <= n <= " + std::to_string(kSparseMaxN));
+  if (nnz < 1 || nnz > 0x7fffffffLL || !rows || !cols)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "GMRES needs a non-empty triplet pattern");
+  if (B < 0) return fail(SPX_ERR_INVALID_DIMENSION, "negative batch");
+  if (B > 0 && (!d_values || !d_rhs || !d_x))
+    return fail(SPX_ERR_INVALID_ARGUMENT, "null values, rhs or solutions");
+  for (int64_t t = 0; t < nnz; ++t)
+    if (rows[t] < 0 || rows[t] >= n || cols[t] < 0 || cols[t] >= n)
+      return fail(SPX_ERR_INVALID_ARGUMENT, "triplet " + std::to_string(t) + " outside the matrix");
+  if (B == 0) return SPX_OK;
+  int rc;
+  std::vector<int> key;
+  key.reserve(2 + 2 * nnz);
+  key.push_back((int)n);
+  key.push_back((int)nnz);
+  key.insert(key.end(), rows, rows + nnz);
+  key.insert(key.end(), cols, cols + nnz);
+  void* dp = nullptr;
+  if (key != h->gm_key) {
+    std::map<std::pair<int, int>, std::vector<int>> pos;
+    for (int64_t t = 0; t < nnz; ++t) pos[{rows[t], cols[t]}].push_back((int)t);
+    std::vector<int> row_ptr(n + 1, 0), col, off(1, 0), trip, diag(n, -1);
+    int q = 0;
+    for (auto& kv : pos) {
+      row_ptr[kv.first.first + 1]++;
+      col.push_back(kv.first.second);
+      if (kv.first.first == kv.first.second) diag[kv.first.first] = q;
+      trip.insert(trip.end(), kv.second.begin(), kv.second.end());
+      off.push_back((int)trip.size());
+      ++q;
+    }
+    for (int64_t i = 0; i < n; ++i) row_ptr[i + 1] += row_ptr[i];
+    h->gm_key.clear();
+    std::vector<int>& blob = h->gm_blob;
+    blob.clear();
+    auto put = [&](const std::vector<int>& v) {
+      size_t o = blob.size();
+      blob.insert(blob.end(), v.begin(), v.end());
+      while (blob.size() % 4) blob.push_back(0);
+      return o;
+    };
+    h->gm_o_rowptr = put(row_ptr);
+    h->gm_o_col = put(col);
+    h->gm_o_off = put(off);
+    h->gm_o_trip = put(trip);
+    h->gm_o_diag = put(diag);
+    blob.push_back(0);
+    h->gm_num_pos = q;
+    if ((rc = workspace(h, "gm_pattern", blob.size() * sizeof(int), &dp))) return rc;
+    SPX_HIP(hipMemcpyAsync(dp, blob.data(), blob.size() * sizeof(int), hipMemcpyHostToDevice,
+                           h->stream),
+            SPX_ERR_TRANSFER);
+    SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+    h->gm_key = std::move(key);
+  }
+  if ((rc = workspace(h, "gm_pattern", h->gm_blob.size() * sizeof(int), &dp))) return rc;
+  const int* base = static_cast<const int*>(dp);
+  DevGmres p{(int)n, h->gm_num_pos, base + h->gm_o_rowptr, base + h->gm_o_col,
+             base + h->gm_o_off, base + h->gm_o_trip, base + h->gm_o_diag};
+  if (gmres_smem_bytes((int)n, cfg->restart) > 160 * 1024)
+    return fail(SPX_ERR_INVALID_DIMENSION, "GMRES workgroup state exceeds the LDS");
+  int grid = 0;
+  SPX_HIP(gmres_grid((int)n, cfg->restart, B, &grid), SPX_ERR_BACKEND);
+  void* ws = nullptr;
+  const int64_t slot = gmres_slot_doubles((int)n, h->gm_num_pos, cfg->restart);
+  if ((rc = workspace(h, "gm_ws", sizeof(double) * (size_t)grid * slot, &ws))) return rc;
+  SPX_HIP(launch_gmres(p, nnz, d_values, d_rhs, d_x0, B, cfg->restart, cfg->max_iter, cfg->tol,
+                       cfg->jacobi, d_x, d_it, d_res, d_conv, static_cast<double*>(ws), grid,
+                       h->stream),
+          SPX_ERR_BACKEND);
+  return SPX_OK;
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -846,6 +928,76 @@ int spx_sparse_solve_batch_f64(spx_handle* h, int64_t n, int64_t nnz, const int3
   for (int64_t i = 0; i < batch; ++i) ns += sing[i] != 0;
   if (singular) std::memcpy(singular, sing.data(), batch);
   if (num_singular) *num_singular = ns;
+  return SPX_OK;
+}
+
+void spx_gmres_config_default(spx_gmres_config* cfg) {
+  if (!cfg) return;
+  cfg->restart = 30;  // GmresConfig::default, gmres/mod.rs:34-48
+  cfg->max_iter = 500;
+  cfg->tol = 1e-8;
+  cfg->jacobi = 1;
+}
+
+int spx_gmres_solve_batch_f64_device(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows,
+                                     const int32_t* cols, const double* d_values,
+                                     const double* d_rhs, const double* d_x0, int64_t batch,
+                                     const spx_gmres_config* cfg, double* d_solutions,
+                                     int32_t* d_iterations, double* d_residuals,
+                                     uint8_t* d_converged) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  return gmres_locked(h, n, nnz, rows, cols, d_values, d_rhs, d_x0, batch, cfg, d_solutions,
+                      d_iterations, d_residuals, d_converged);
+}
+
+int spx_gmres_solve_batch_f64(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows,
+                              const int32_t* cols, const double* values, const double* rhs,
+                              const double* x0, int64_t batch, const spx_gmres_config* cfg,
+                              double* solutions, int32_t* iterations, double* residuals,
+                              uint8_t* converged) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  if (batch < 0) return fail(SPX_ERR_INVALID_DIMENSION, "negative batch");
+  if (batch > 0 && (!values || !rhs || !solutions))
+    return fail(SPX_ERR_INVALID_ARGUMENT, "null values, rhs or solutions");
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  if (batch == 0)
+    return gmres_locked(h, n, nnz, rows, cols, nullptr, nullptr, nullptr, 0, cfg, nullptr,
+                        nullptr, nullptr, nullptr);
+  int rc;
+  void *dv, *db, *dx, *dx0 = nullptr, *dit, *dres, *dconv;
+  if ((rc = workspace(h, "gs_values", sizeof(double) * batch * nnz, &dv))) return rc;
+  if ((rc = workspace(h, "gs_rhs", sizeof(double) * batch * n, &db))) return rc;
+  if ((rc = workspace(h, "gs_x", sizeof(double) * batch * n, &dx))) return rc;
+  if ((rc = workspace(h, "gs_it", sizeof(int32_t) * batch, &dit))) return rc;
+  if ((rc = workspace(h, "gs_res", sizeof(double) * batch, &dres))) return rc;
+  if ((rc = workspace(h, "gs_conv", batch, &dconv))) return rc;
+  SPX_HIP(hipMemcpyAsync(dv, values, sizeof(double) * batch * nnz, hipMemcpyHostToDevice,
+                         h->stream),
+          SPX_ERR_TRANSFER);
+  SPX_HIP(hipMemcpyAsync(db, rhs, sizeof(double) * batch * n, hipMemcpyHostToDevice, h->stream),
+          SPX_ERR_TRANSFER);
+  if (x0) {
+    if ((rc = workspace(h, "gs_x0", sizeof(double) * batch * n, &dx0))) return rc;
+    SPX_HIP(hipMemcpyAsync(dx0, x0, sizeof(double) * batch * n, hipMemcpyHostToDevice,
+                           h->stream),
+            SPX_ERR_TRANSFER);
+  }
+  if ((rc = gmres_locked(h, n, nnz, rows, cols, (double*)dv, (double*)db, (double*)dx0, batch,
+                         cfg, (double*)dx, (int32_t*)dit, (double*)dres, (uint8_t*)dconv)))
+    return rc;
+  auto d2h = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (!dst) return SPX_OK;
+    SPX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream), SPX_ERR_TRANSFER);
+    return SPX_OK;
+  };
+  if ((rc = d2h(solutions, dx, sizeof(double) * batch * n))) return rc;
+  if ((rc = d2h(iterations, dit, sizeof(int32_t) * batch))) return rc;
+  if ((rc = d2h(residuals, dres, sizeof(double) * batch))) return rc;
+  if ((rc = d2h(converged, dconv, batch))) return rc;
+  SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
   return SPX_OK;
 }
 

@@ -313,6 +313,69 @@ class HipSparseBatchedSolver(BatchedLuSolver):
 
 
 @dataclass
+class GmresConfig:
+    """GmresConfig (crates/spicier-solver/src/gmres/mod.rs:34-48) plus the
+    preconditioner choice (JacobiPreconditioner, preconditioner.rs:43-100)."""
+    max_iter: int = 500
+    tol: float = 1e-8
+    restart: int = 30
+    jacobi: bool = True
+
+    def to_c(self) -> A.SpxGmresConfig:
+        return A.SpxGmresConfig(restart=self.restart, max_iter=self.max_iter, tol=self.tol,
+                                jacobi=int(self.jacobi))
+
+
+@dataclass
+class GmresBatchResult:
+    """Per system: RealGmresResult (gmres/real.rs:10-21)."""
+    solutions: np.ndarray
+    iterations: np.ndarray
+    residual: np.ndarray
+    converged: np.ndarray
+    n: int
+    batch_size: int
+
+    def solution(self, i: int) -> np.ndarray:
+        return self.solutions.reshape(self.batch_size, self.n)[i]
+
+
+class HipBatchedGmres:
+    """Batched restarted GMRES-Jacobi on the GPU, every system sharing one
+    triplet pattern: solve_gmres_real_preconditioned per system
+    (crates/spicier-solver/src/gmres/real.rs:208-400), one launch for the
+    whole batch (the Metal sketch's GpuBatchedGmres::solve,
+    crates/spicier-backend-metal/src/batched_gmres.rs:396-657)."""
+
+    def __init__(self, config: GmresConfig | None = None, device_id: int = 0):
+        self.config = config or GmresConfig()
+        self.device = Device.get(device_id)
+
+    def solve_batch_triplets(self, rows, cols, values, rhs, n: int, x0=None) -> GmresBatchResult:
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        nnz = rows.size
+        B = v.size // nnz if nnz else 0
+        b = np.ascontiguousarray(rhs, dtype=np.float64).reshape(-1)
+        if cols.size != nnz or v.size != B * nnz or b.size != B * n:
+            raise InvalidDimension(f"expected {B}x{nnz} values and {B}x{n} rhs")
+        if x0 is not None:
+            x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)
+            if x0.size != B * n:
+                raise InvalidDimension(f"expected {B}x{n} initial guesses")
+        x = np.zeros(B * n)
+        it = np.zeros(max(B, 1), dtype=np.int32)
+        res = np.zeros(max(B, 1))
+        conv = np.zeros(max(B, 1), dtype=np.uint8)
+        cfg = self.config.to_c()
+        _check(self.device.lib.spx_gmres_solve_batch_f64(
+            self.device.handle, n, nnz, A.ptr(rows), A.ptr(cols), A.ptr(v), A.ptr(b), A.ptr(x0),
+            B, C.byref(cfg), A.ptr(x), A.ptr(it), A.ptr(res), A.ptr(conv)))
+        return GmresBatchResult(x, it[:B], res[:B], conv[:B].astype(bool), n, B)
+
+
+@dataclass
 class BackendSelector:
     """solver.rs:139-312, with HIP as the only backend this engine ships.  The
     reference falls through to its CPU solver when no GPU is present; this

@@ -414,3 +414,59 @@ def test_statistics_known_answers(oracle):
     np.testing.assert_allclose(full[:, :3], a[:, :3] + b[:, :3], rtol=1e-13)
     np.testing.assert_array_equal(full[:, 3], np.minimum(a[:, 3], b[:, 3]))
     np.testing.assert_array_equal(full[:, 4], np.maximum(a[:, 4], b[:, 4]))
+
+
+# ------------------------------------------------------------------ GMRES
+def _sparse_system(rng, n, extra=3, dom=1.0):
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        js = rng.choice(n, size=extra, replace=False)
+        for j in js:
+            if j != i:
+                rows.append(i); cols.append(j); vals.append(rng.standard_normal())
+        rows.append(i); cols.append(i); vals.append(dom * (extra + 1) + rng.random())
+    rows.append(0); cols.append(0); vals.append(0.25)          # a duplicate, summed
+    return np.array(rows), np.array(cols), np.array(vals)
+
+
+def _dense_of(n, rows, cols, vals):
+    m = np.zeros((n, n))
+    np.add.at(m, (rows, cols), vals)
+    return m
+
+
+def test_gmres_converges_to_lapack_solution(oracle):
+    rng = np.random.default_rng(11)
+    for n, restart in ((60, 30), (200, 30), (200, 5)):
+        rows, cols, vals = _sparse_system(rng, n, dom=0.6)
+        b = rng.standard_normal(n)
+        x, conv, it, res = oracle.gmres(n, rows, cols, vals, b, restart=restart, tol=1e-12)
+        m = _dense_of(n, rows, cols, vals)
+        want = np.linalg.solve(m, b)
+        assert conv and res < 1e-12 and it > 1
+        assert np.linalg.norm(b - m @ x) / np.linalg.norm(b) < 1e-11
+        np.testing.assert_allclose(x, want, rtol=1e-9, atol=1e-11)
+
+
+def test_gmres_jacobi_solves_diagonal_in_one_step_and_edge_cases(oracle):
+    n = 50
+    d = np.linspace(1.0, 1e6, n)
+    idx = np.arange(n)
+    b = np.linspace(-3.0, 5.0, n)
+    x, conv, it, res = oracle.gmres(n, idx, idx, d, b, tol=1e-14)
+    assert conv and it == 1
+    np.testing.assert_allclose(x, b / d, rtol=1e-14)
+    # without Jacobi the Krylov space needs more than one vector
+    x2, conv2, it2, _ = oracle.gmres(n, idx, idx, d, b, tol=1e-14, jacobi=False)
+    assert it2 > 1
+    # zero right-hand side: converged, no iterations, x = 0 (real.rs:38-45)
+    x, conv, it, res = oracle.gmres(n, idx, idx, d, np.zeros(n))
+    assert conv and it == 0 and np.all(x == 0)
+    # iteration budget exhausted: reported not converged; the reference counts
+    # the iteration that trips the budget check (real.rs:289-292), so the
+    # count is max_iter + 1 when the budget runs out inside a cycle
+    rng = np.random.default_rng(2)
+    rows, cols, vals = _sparse_system(rng, 120, dom=0.3)
+    x, conv, it, res = oracle.gmres(120, rows, cols, vals, rng.standard_normal(120),
+                                    restart=4, max_iter=6, tol=1e-14)
+    assert not conv and it == 7 and res > 1e-14
