@@ -320,6 +320,44 @@ int spx_sweep_dcop(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* 
 int spx_sweep_dcop_device(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
                           const spx_nr_config* cfg, const double* d_x0, spx_sweep_result* d_out);
 
+/* ------------------------------------------------------------------------- */
+/* Backward-Euler transient sweep (CapacitorState::stamp_be / update,
+ * crates/spicier-solver/src/transient/companion.rs:41-66; solve_transient,
+ * crates/spicier-solver/src/transient/solver.rs:41-160; PULSE sources by
+ * eval_pulse, crates/spicier-devices/src/waveforms.rs:160-180).  Every
+ * instance: DC operating point with capacitors open and PULSE sources at V1
+ * (the spx_sweep_dcop semantics), then `steps` fixed steps of `tstep` at
+ * t = k*tstep, each capacitor replaced by G = C/h with I = G*v_prev from its
+ * negative to its positive node.  The reference's transient is linear (one
+ * solve per step); circuits with diodes or MOSFETs run plain Newton at every
+ * step from the previous step's solution (no step cutting).  FP64. */
+typedef struct spx_tran_config {
+    double tstep;            /* step h (s), > 0 */
+    int32_t steps;           /* number of steps after t = 0 */
+    int32_t num_probes;      /* unknowns recorded at every step */
+    const int32_t* probes;   /* [num_probes] unknown indices (host array) */
+} spx_tran_config;
+
+typedef struct spx_tran_result {
+    double* x;           /* [B][n] solution at the last step reached */
+    int32_t* status;     /* [B] spx_conv_status of the whole run */
+    int32_t* iterations; /* [B] Newton iterations, operating point and steps */
+    int32_t* flags;      /* [B] SPX_FLAG_* (KCL test of the last step) */
+    double* residual;    /* [B] KCL residual ratio of the last step */
+    double* params;      /* [B][P] sampled parameters (may be NULL) */
+    double* waveforms;   /* [B][steps+1][num_probes] probed unknowns at t = 0, h, ... */
+    int32_t* fail_step;  /* [B] first step that failed (0 = operating point), -1 if none */
+} spx_tran_result;
+
+int spx_transient_sweep(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
+                        const spx_nr_config* cfg, const spx_tran_config* tran,
+                        spx_tran_result* out);
+/* Device-pointer variant (result buffers and explicit parameters on the
+ * device); returns once the work is enqueued. */
+int spx_transient_sweep_device(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
+                               const spx_nr_config* cfg, const spx_tran_config* tran,
+                               spx_tran_result* d_out);
+
 /* Unknown count n of a circuit (num_nodes + number of voltage sources). */
 int32_t spx_circuit_size(const spx_circuit* ckt);
 

@@ -374,9 +374,30 @@ double pnjlim(double vnew, double vold, double vt, double vcrit, int* limited) {
   return vnew;
 }
 
+// Backward-Euler transient context (transient/companion.rs:41-66): mode 1
+// is a step at time t of size h with capacitor voltages vprev[e] of the
+// previous step; without it (or mode 0) capacitors are open and PULSE
+// sources sit at V1 (waveforms.rs:150-160).
+struct OTran {
+  int mode;
+  double t, h;
+  const double* vprev;
+};
+
+// eval_pulse (waveforms.rs:160-180)
+double eval_pulse(double v1, double v2, double td, double tr, double tf, double pw, double per,
+                  double t) {
+  if (t < td) return v1;
+  const double tr_ = per > 0.0 ? std::fmod(t - td, per) : t - td;
+  if (tr_ < tr) return v1 + (v2 - v1) * tr_ / tr;
+  if (tr_ < tr + pw) return v2;
+  if (tr_ < tr + pw + tf) return v2 - (v2 - v1) * (tr_ - tr - pw) / tf;
+  return v1;
+}
+
 void stamp_impl(const spx_circuit* c, const double* params, const double* x, double gmin,
                 double sf, int32_t limiter, double* J, double* rhs, double* vstate,
-                int* limited) {
+                int* limited, const OTran* tc = nullptr) {
   const int n = orc_circuit_size(c);
   std::memset(J, 0, sizeof(double) * (size_t)n * n);
   std::memset(rhs, 0, sizeof(double) * (size_t)n);
@@ -389,12 +410,26 @@ void stamp_impl(const spx_circuit* c, const double* params, const double* x, dou
       case SPX_EL_RESISTOR:
         m.conductance(a, b, 1.0 / P(params, el.param[0]));
         break;
-      case SPX_EL_CAPACITOR:
-        break;  // open at DC (passive.rs:233-245)
+      case SPX_EL_CAPACITOR:  // open at DC (passive.rs:233-245); BE companion in a step
+        if (tc && tc->mode == 1) {  // stamp_be, companion.rs:41-48
+          const double geq = P(params, el.param[0]) / tc->h;
+          const double ieq = geq * tc->vprev[e];
+          m.conductance(a, b, geq);
+          m.current(b, a, ieq);
+        }
+        break;
       case SPX_EL_VSOURCE:
-      case SPX_EL_VPULSE:
         m.vsource(a, b, k++, P(params, el.param[0]) * sf);
         break;
+      case SPX_EL_VPULSE: {
+        double v = P(params, el.param[0]);
+        if (tc && tc->mode == 1)
+          v = eval_pulse(P(params, el.param[0]), P(params, el.param[1]), P(params, el.param[2]),
+                         P(params, el.param[3]), P(params, el.param[4]), P(params, el.param[5]),
+                         P(params, el.param[6]), tc->t);
+        m.vsource(a, b, k++, v * sf);
+        break;
+      }
       case SPX_EL_ISOURCE:
         m.current(a, b, P(params, el.param[0]) * sf);
         break;
@@ -477,9 +512,22 @@ bool converged_dx(const double* old, const double* nw, int nn, int n, const spx_
 
 }  // namespace
 
+namespace {
+int newton_impl(const spx_circuit* c, const double* params, const spx_nr_config* cfg, double gmin,
+                double sf, const double* x0, double* x, int32_t* iterations, const OTran* tc);
+double kcl_impl(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
+                const double* x, double gmin, const OTran* tc);
+}  // namespace
+
 extern "C" int orc_newton(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
                           double gmin, double sf, const double* x0, double* x,
                           int32_t* iterations) {
+  return newton_impl(c, params, cfg, gmin, sf, x0, x, iterations, nullptr);
+}
+
+namespace {
+int newton_impl(const spx_circuit* c, const double* params, const spx_nr_config* cfg, double gmin,
+                double sf, const double* x0, double* x, int32_t* iterations, const OTran* tc) {
   const int n = orc_circuit_size(c);
   std::vector<double> J((size_t)n * n), rhs(n), xn(n), sol(n, 0.0);
   std::vector<double> vstate(c->num_elements, 0.0);  // pnjlim junction voltages
@@ -487,7 +535,7 @@ extern "C" int orc_newton(const spx_circuit* c, const double* params, const spx_
   for (int it = 0; it < cfg->max_iterations; ++it) {
     int limited = 0;
     stamp_impl(c, params, sol.data(), gmin, sf, cfg->limiter, J.data(), rhs.data(),
-               vstate.data(), &limited);
+               vstate.data(), &limited, tc);
     if (orc_lu_solve(n, J.data(), rhs.data(), xn.data())) {
       std::memcpy(x, sol.data(), sizeof(double) * n);
       *iterations = it + 1;
@@ -505,16 +553,23 @@ extern "C" int orc_newton(const spx_circuit* c, const double* params, const spx_
   *iterations = cfg->max_iterations;
   return 0;
 }
+}  // namespace
 
 extern "C" double orc_kcl_residual(const spx_circuit* c, const double* params,
                                    const spx_nr_config* cfg, const double* x, double gmin) {
+  return kcl_impl(c, params, cfg, x, gmin, nullptr);
+}
+
+namespace {
+double kcl_impl(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
+                const double* x, double gmin, const OTran* tc) {
   // KCL at x with devices linearised, unlimited, at x: r = J(x) x - b(x)
   // equals the sum of element currents leaving each node.  For each row,
   // tol = reltol * max(|J_ij x_j|, |b_i|) + abstol (node rows, amperes) or
   // + vntol (branch rows, volts).  Returns max |r_i| / tol_i.
   const int n = orc_circuit_size(c);
   std::vector<double> J((size_t)n * n), rhs(n);
-  stamp_impl(c, params, x, gmin, 1.0, SPX_LIMIT_NONE, J.data(), rhs.data(), nullptr, nullptr);
+  stamp_impl(c, params, x, gmin, 1.0, SPX_LIMIT_NONE, J.data(), rhs.data(), nullptr, nullptr, tc);
   double worst = 0.0;
   for (int i = 0; i < n; ++i) {
     double r = -rhs[i], mag = std::fabs(rhs[i]);
@@ -529,6 +584,7 @@ extern "C" double orc_kcl_residual(const spx_circuit* c, const double* params,
   }
   return worst;
 }
+}  // namespace
 
 namespace {
 
@@ -879,4 +935,86 @@ extern "C" int orc_gmres(int32_t n, int64_t nnz, const int32_t* rows, const int3
   *iterations = total;
   *residual = NAN;
   return 0;
+}
+
+// ----------------------------------------------------------- transient
+// Backward-Euler transient (transient/solver.rs:41-160, companion.rs:41-66):
+// DC operating point (orc_dcop semantics, capacitors open, PULSE at V1),
+// then `steps` fixed steps at t = k*h.  Linear circuits take one solve per
+// step as the reference does; nonlinear ones plain Newton from the previous
+// step.  waves[(k)*np + p] = x[probes[p]] at step k (k = 0 the operating
+// point).  Returns the spx_conv_status of the run.
+extern "C" int orc_transient(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
+                             double h, int32_t steps, int32_t np, const int32_t* probes,
+                             double* x, double* waves, int32_t* iterations, int32_t* flags,
+                             double* residual, int32_t* fail_step) {
+  const int n = orc_circuit_size(c);
+  int32_t it = 0, fl = 0;
+  double res = 0.0;
+  int status = orc_dcop(c, params, cfg, nullptr, x, &it, &fl, &res);
+  fl &= ~SPX_FLAG_KCL_FAIL;
+  int32_t total = it, fail = -1;
+  if (status != SPX_CONVERGED) {
+    fail = 0;
+  } else {
+    std::vector<double> vprev(c->num_elements, 0.0), xn(n), J((size_t)n * n), rhs(n);
+    auto cap_update = [&]() {
+      for (int e = 0; e < c->num_elements; ++e)
+        if (c->elements[e].kind == SPX_EL_CAPACITOR)
+          vprev[e] = volt(x, c->elements[e].node[0]) - volt(x, c->elements[e].node[1]);
+    };
+    auto probe = [&](int k) {
+      if (waves)
+        for (int p = 0; p < np; ++p) waves[(size_t)k * np + p] = x[probes[p]];
+    };
+    cap_update();
+    probe(0);
+    const bool nonlin = has_nonlinear(c);
+    for (int k = 1; k <= steps; ++k) {
+      OTran tc{1, k * h, h, vprev.data()};
+      int st;
+      if (!nonlin) {
+        std::vector<double> zero(n, 0.0);
+        stamp_impl(c, params, zero.data(), 0.0, 1.0, cfg->limiter, J.data(), rhs.data(), nullptr,
+                   nullptr, &tc);
+        st = orc_lu_solve(n, J.data(), rhs.data(), x) ? -1 : 1;
+        it = 1;
+      } else {
+        st = newton_impl(c, params, cfg, 0.0, 1.0, x, xn.data(), &it, &tc);
+        std::memcpy(x, xn.data(), sizeof(double) * n);
+      }
+      total += it;
+      if (st != 1) {
+        status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
+        fail = k;
+        break;
+      }
+      if (k == steps) res = kcl_impl(c, params, cfg, x, 0.0, &tc);
+      cap_update();
+      probe(k);
+    }
+  }
+  if (status == SPX_CONVERGED && !(res <= 1.0)) fl |= SPX_FLAG_KCL_FAIL;
+  if (status == SPX_SINGULAR)
+    for (int i = 0; i < n; ++i) x[i] = 0.0;
+  if (iterations) *iterations = total;
+  if (flags) *flags = fl;
+  if (residual) *residual = res;
+  if (fail_step) *fail_step = fail;
+  return status;
+}
+
+extern "C" int orc_transient_sweep(const spx_circuit* c, const double* params, int64_t B,
+                                   const spx_nr_config* cfg, double h, int32_t steps, int32_t np,
+                                   const int32_t* probes, double* x, int32_t* status,
+                                   double* waves, int32_t* iterations, int32_t* flags,
+                                   double* residual, int32_t* fail_step, int32_t nthreads) {
+  const int n = orc_circuit_size(c);
+  const int P = c->num_params;
+  run_pool(B, nthreads, [&](int64_t i) {
+    status[i] = orc_transient(c, params + i * P, cfg, h, steps, np, probes, x + i * n,
+                              waves ? waves + i * (int64_t)(steps + 1) * np : nullptr,
+                              iterations + i, flags + i, residual + i, fail_step + i);
+  });
+  return SPX_OK;
 }

@@ -97,6 +97,17 @@ void orc_stats_from_moments(int64_t count, double sum, double sum_sq, double* ou
  * out = mean, std_dev, min, max. */
 void orc_sweep_statistics(const double* samples, int64_t count, double* out4);
 
+/* Backward-Euler transient of one instance (transient/solver.rs:41-160);
+ * waves is [(steps+1)][np].  Returns the spx_conv_status of the run. */
+int orc_transient(const spx_circuit* c, const double* params, const spx_nr_config* cfg, double h,
+                  int32_t steps, int32_t np, const int32_t* probes, double* x, double* waves,
+                  int32_t* iterations, int32_t* flags, double* residual, int32_t* fail_step);
+int orc_transient_sweep(const spx_circuit* c, const double* params, int64_t B,
+                        const spx_nr_config* cfg, double h, int32_t steps, int32_t np,
+                        const int32_t* probes, double* x, int32_t* status, double* waves,
+                        int32_t* iterations, int32_t* flags, double* residual,
+                        int32_t* fail_step, int32_t nthreads);
+
 /* Restarted GMRES(restart) with right Jacobi preconditioning (jacobi != 0)
  * or none: gmres/real.rs:30-400, preconditioner.rs:43-100.  Triplets with
  * duplicates summed; x starts at 0.  Returns 1 converged / 0 not. */

@@ -166,3 +166,24 @@ def gmres(n, rows, cols, vals, b, restart=30, max_iter=500, tol=1e-8, jacobi=Tru
     conv = f(n, rows.size, _p(rows), _p(cols), _p(vals), _p(b), restart, max_iter, tol,
              1 if jacobi else 0, _p(x), C.byref(it), C.byref(res))
     return x, bool(conv), it.value, res.value
+
+
+def transient(circuit, params, cfg, tstep, steps, probes=None, nthreads=8):
+    """orc_transient_sweep over params [B][P]: Backward-Euler transient per
+    instance.  Returns a dict like the engine's TransientSweepResult."""
+    ckt, keep = circuit.to_c()
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    B, n = params.shape[0], circuit.size
+    pr = np.ascontiguousarray(np.arange(n) if probes is None else probes, dtype=np.int32)
+    out = dict(x=np.zeros((B, n)), status=np.zeros(B, np.int32),
+               waveforms=np.zeros((B, steps + 1, pr.size)), iterations=np.zeros(B, np.int32),
+               flags=np.zeros(B, np.int32), residual=np.zeros(B), fail_step=np.zeros(B, np.int32))
+    f = lib().orc_transient_sweep
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(A.SpxCircuit), C.c_void_p, C.c_int64, C.POINTER(A.SpxNrConfig),
+                  C.c_double, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+    f(C.byref(ckt), _p(params), B, C.byref(cfg), tstep, steps, pr.size, _p(pr), _p(out["x"]),
+      _p(out["status"]), _p(out["waveforms"]), _p(out["iterations"]), _p(out["flags"]),
+      _p(out["residual"]), _p(out["fail_step"]), nthreads)
+    return out

@@ -95,6 +95,17 @@ class SpxSweepResult(C.Structure):
                 ("refinements", C.c_void_p)]
 
 
+class SpxTranConfig(C.Structure):
+    _fields_ = [("tstep", C.c_double), ("steps", C.c_int32), ("num_probes", C.c_int32),
+                ("probes", C.c_void_p)]
+
+
+class SpxTranResult(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("status", C.c_void_p), ("iterations", C.c_void_p),
+                ("flags", C.c_void_p), ("residual", C.c_void_p), ("params", C.c_void_p),
+                ("waveforms", C.c_void_p), ("fail_step", C.c_void_p)]
+
+
 def default_nr_config(**overrides) -> SpxNrConfig:
     """Defaults of ConvergenceCriteria / GminSteppingParams / SourceSteppingParams
     (crates/spicier-solver/src/newton.rs:24-34,185-194,222-231)."""
@@ -161,6 +172,13 @@ def load_library():
                                             C.POINTER(SpxNrConfig), vp,
                                             C.POINTER(SpxSweepResult)]),
         "spx_circuit_size": (C.c_int32, [C.POINTER(SpxCircuit)]),
+        "spx_transient_sweep": (C.c_int, [vp, C.POINTER(SpxCircuit), C.POINTER(SpxSweepSpec),
+                                          C.POINTER(SpxNrConfig), C.POINTER(SpxTranConfig),
+                                          C.POINTER(SpxTranResult)]),
+        "spx_transient_sweep_device": (C.c_int, [vp, C.POINTER(SpxCircuit),
+                                                 C.POINTER(SpxSweepSpec), C.POINTER(SpxNrConfig),
+                                                 C.POINTER(SpxTranConfig),
+                                                 C.POINTER(SpxTranResult)]),
         "spx_stats_reduce_device": (C.c_int, [vp, vp, vp, i64, i32, vp]),
     }
     for name, (res, args) in sigs.items():

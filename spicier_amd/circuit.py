@@ -302,6 +302,58 @@ def grid_diodes(N: int = GRID_N, every: int = GRID_DIODE_EVERY) -> Circuit:
     return c
 
 
+TRAN_PULSE = dict(v1=0.0, v2=1.8, td=0.0, tr=10e-12, tf=10e-12, pw=1e-9, per=2e-9)
+
+
+def ladder(n: int, seed: int = 0) -> Circuit:
+    """Random R/C/diode/MOSFET ladder with exactly n unknowns, driven by a
+    PULSE source (0 -> 1.8 V, 10 ps rise, 1 ns wide): the mixed-size
+    transient workload.
+
+    Nodes in, l1 .. l(n-2) plus the source branch.  Rung k joins l(k-1) (or
+    in) to l(k) through R~U(1,10) kOhm and loads l(k) with C~U(1,10) fF to
+    ground; a structure seed picks for each rung (from k = 2 on) nothing, a
+    diode l(k) -> 0 (IS~logU(1e-15,1e-13) A, N = 1), or an NMOS pull-down
+    d=l(k), g=l(k-1), s=0 (VTO~N(0.7,0.03) V, KP~N(110e-6, 5%), LAMBDA
+    0.02, W/L = 10), with probabilities 0.6 / 0.2 / 0.2.  The structure is
+    fixed by `seed`; instances differ only in parameter values.
+    """
+    if n < 3:
+        raise ValueError("a ladder needs n >= 3 unknowns")
+    rng = np.random.default_rng(seed)
+    c = Circuit(f"ladder_{n}_{seed}")
+    c.node("in")
+    for k in range(1, n - 1):
+        c.node(f"l{k}")
+    pulse = [c.const(TRAN_PULSE[k], k.upper()) for k in ("v1", "v2", "td", "tr", "tf", "pw", "per")]
+    c.vpulse("in", "0", *pulse)
+    lam, w, l, nd = c.const(0.02, "LAMBDA"), c.const(10e-6, "W"), c.const(1e-6, "L"), c.const(1.0, "N")
+    prev = "in"
+    for k in range(1, n - 1):
+        node = f"l{k}"
+        c.resistor(prev, node, c.uniform(1e3, 10e3, name=f"R{k}"))
+        c.capacitor(node, "0", c.uniform(1e-15, 10e-15, name=f"C{k}"))
+        pick = rng.random() if k >= 2 else 0.0
+        if 0.6 <= pick < 0.8:
+            c.diode(node, "0", c.loguniform(1e-15, 1e-13, name=f"IS{k}"), nd)
+        elif pick >= 0.8:
+            c.nmos(node, f"l{k - 1}", "0", vto=c.gauss(0.7, 0.03, name=f"VTO{k}"),
+                   kp=c.gauss(110e-6, 0.05 * 110e-6, name=f"KP{k}"), lam=lam, w=w, l=l)
+        prev = node
+    assert c.size == n
+    return c
+
+
+def rc_step(r=1e3, cap=1e-12, v=1.0) -> Circuit:
+    """PULSE 0 -> v (no rise) through R into C: the Backward-Euler step
+    response v_k = v (1 - (1 + h/RC)^-k) (unknowns in, out, source branch)."""
+    c = Circuit("rc_step")
+    c.vpulse("in", "0", 0.0, v, 0.0, 0.0, 0.0, 1.0, 0.0)
+    c.resistor("in", "out", r)
+    c.capacitor("out", "0", cap)
+    return c
+
+
 def diode_resistor(v=5.0, r=1e3, is_=1e-14, n=1.0) -> Circuit:
     """V - R - D to ground (the reference's 5 V / 1 kOhm diode check,
     crates/spicier-parser/tests/ngspice_validation.rs:705-768)."""

@@ -9,13 +9,27 @@
 
 namespace spx {
 
+// PULSE source value at time t (waveforms.rs:160-180).
+__device__ __forceinline__ double eval_pulse(double v1, double v2, double td, double tr,
+                                             double tf, double pw, double per, double t) {
+  if (t < td) return v1;
+  const double tr_ = per > 0.0 ? fmod(t - td, per) : t - td;
+  if (tr_ < tr) return v1 + (v2 - v1) * tr_ / tr;
+  if (tr_ < tr + pw) return v2;
+  if (tr_ < tr + pw + tf) return v2 - (v2 - v1) * (tr_ - tr - pw) / tf;
+  return v1;
+}
+
 // Companion values of element e at node voltages xs (unknown u -> xs[u],
 // ground -> 0): conductances, currents and couplings into vals[vid..].
 // sf scales independent sources (source stepping); pnjlim junction state
-// lives in vst[e]; `lim` is set when a pnjlim limiter acted.
+// lives in vst[e]; `lim` is set when a pnjlim limiter acted.  tc is the
+// transient context (null for a DC sweep, whose program has no capacitor
+// slots).
 __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const double* xs,
                                              const double* pv, double* vals, double* vst,
-                                             double sf, int limiter, bool& lim) {
+                                             double sf, int limiter, bool& lim,
+                                             const TranCtx* tc = nullptr) {
   const int* el = g.elems + e * kElemInts;
   const int kind = el[0], vid = el[11];
   auto X = [&](int u) { return u >= 0 ? xs[u] : 0.0; };
@@ -23,8 +37,21 @@ __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const d
     case SPX_EL_RESISTOR:
       vals[vid] = 1.0 / pv[el[4]];
       break;
-    case SPX_EL_VSOURCE:
+    case SPX_EL_CAPACITOR:  // companion.rs:41-48
+      if (vid >= 0) {
+        const double geq = (tc && tc->mode == 1) ? pv[el[4]] / tc->h : 0.0;
+        vals[vid] = geq;
+        vals[vid + 1] = (tc && tc->mode == 1) ? geq * tc->vprev[e] : 0.0;
+      }
+      break;
     case SPX_EL_VPULSE:
+      vals[vid] = ((tc && tc->mode == 1)
+                       ? eval_pulse(pv[el[4]], pv[el[5]], pv[el[6]], pv[el[7]], pv[el[8]],
+                                    pv[el[9]], pv[el[10]], tc->t)
+                       : pv[el[4]]) *
+                  sf;
+      break;
+    case SPX_EL_VSOURCE:
     case SPX_EL_ISOURCE:
       vals[vid] = pv[el[4]] * sf;
       break;

@@ -11,10 +11,17 @@
 
 namespace spx {
 
+// Outcome of one DC operating point: x is left in the instance.
+struct DcopOutcome {
+  int status, total, flags;
+  double final_gmin;
+  bool handover;  // FP32+IR could not resolve the system: the FP64 pass takes it
+};
+
 template <class Inst>
-__device__ __forceinline__ void run_dcop(Inst& I, const DevProgram& g, const DevNrCfg& cfg,
-                                         bool warm, const DevOut& out, int64_t inst,
-                                         int escalated_only) {
+__device__ __forceinline__ DcopOutcome solve_dcop(Inst& I, const DevProgram& g,
+                                                  const DevNrCfg& cfg, bool warm,
+                                                  int escalated_only) {
   int total = 0, it = 0, esc = 0, status = SPX_FAILED;
   int flags = (warm ? SPX_FLAG_WARM_START : 0) | (escalated_only ? SPX_FLAG_FP64_ESCALATED : 0);
   double final_gmin = 0.0;
@@ -130,13 +137,77 @@ __device__ __forceinline__ void run_dcop(Inst& I, const DevProgram& g, const Dev
       }
     }
   }
-  if (handover) {
-    I.write_handover(out, inst, flags);
+  return DcopOutcome{status, total, flags, final_gmin, handover};
+}
+
+template <class Inst>
+__device__ __forceinline__ void run_dcop(Inst& I, const DevProgram& g, const DevNrCfg& cfg,
+                                         bool warm, const DevOut& out, int64_t inst,
+                                         int escalated_only) {
+  DcopOutcome o = solve_dcop(I, g, cfg, warm, escalated_only);
+  if (o.handover) {
+    I.write_handover(out, inst, o.flags);
     return;
   }
-  const double res = I.kcl(final_gmin);
+  const double res = I.kcl(o.final_gmin);
+  if (o.status == SPX_CONVERGED && !(res <= 1.0)) o.flags |= SPX_FLAG_KCL_FAIL;
+  I.write_result(out, inst, o.status, o.total, o.flags, res);
+}
+
+// Backward-Euler transient of one instance (transient/solver.rs:41-160):
+// the operating point (capacitors open, PULSE sources at V1), then `steps`
+// fixed steps of h, each solved at t = step * h with every capacitor's
+// companion built from the previous step.  The reference's transient is
+// linear (one solve per step); a circuit with diodes or MOSFETs runs plain
+// Newton at each step from the previous step's solution.  Probes record
+// t = 0 and every step; the result row carries the final step's solution,
+// the total Newton iterations (operating point included), and its FP64 KCL
+// residual.  An instance whose operating point or a step does not converge
+// stops there with status FAILED (or SINGULAR) and fail_step set.
+// The instance type also provides cap_update() (vprev from x) and
+// write_probes(row).
+template <class Inst>
+__device__ __forceinline__ void run_transient(Inst& I, const DevProgram& g, const DevNrCfg& cfg,
+                                              const DevTran& tr, TranCtx& tc,
+                                              const DevOut& out, const DevTranOut& tout,
+                                              int64_t inst) {
+  tc.mode = 0;
+  tc.t = 0.0;
+  DcopOutcome o = solve_dcop(I, g, cfg, false, 0);
+  int status = o.status, total = o.total, flags = o.flags, fail = -1;
+  double res = 0.0;
+  const int64_t row0 = inst * (int64_t)(tr.steps + 1);
+  if (status == SPX_CONVERGED) {
+    I.cap_update();
+    if (tout.waves) I.write_probes(tr, tout.waves + row0 * tr.num_probes);
+    tc.mode = 1;
+    for (int step = 1; step <= tr.steps; ++step) {
+      tc.t = step * tr.h;
+      int it = 0, esc = 0, st;
+      if (!g.has_nonlinear) {
+        st = I.linear_solve() == 0 ? 1 : -1;
+        it = 1;
+      } else {
+        st = I.newton(0.0, 1.0, it, esc);
+      }
+      total += it;
+      if (st != 1) {
+        status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
+        fail = step;
+        break;
+      }
+      if (step == tr.steps) res = I.kcl(0.0);  // before the capacitors move on
+      I.cap_update();
+      if (tout.waves) I.write_probes(tr, tout.waves + (row0 + step) * tr.num_probes);
+    }
+    if (tr.steps == 0) res = I.kcl(o.final_gmin);
+  } else {
+    fail = 0;
+    res = I.kcl(o.final_gmin);
+  }
   if (status == SPX_CONVERGED && !(res <= 1.0)) flags |= SPX_FLAG_KCL_FAIL;
   I.write_result(out, inst, status, total, flags, res);
+  if (tout.fail_step) I.write_scalar(tout.fail_step + inst, fail);
 }
 
 }  // namespace spx

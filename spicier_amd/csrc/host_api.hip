@@ -37,6 +37,21 @@ hipError_t launch_sparse_solve(const DevSparse& sp, int n, int num_pos, const in
                                const double* rhs, int64_t B, double* x, uint8_t* singular,
                                double* ws, int grid, int64_t slot, size_t smem, int lu_in_lds,
                                hipStream_t st);
+hipError_t launch_transient_8(const DevProgram&, const DevNrCfg&, const DevTran&, const double*,
+                              int64_t, const DevOut&, const DevTranOut&, hipStream_t);
+hipError_t launch_transient_16(const DevProgram&, const DevNrCfg&, const DevTran&, const double*,
+                               int64_t, const DevOut&, const DevTranOut&, hipStream_t);
+hipError_t launch_transient_32(const DevProgram&, const DevNrCfg&, const DevTran&, const double*,
+                               int64_t, const DevOut&, const DevTranOut&, hipStream_t);
+hipError_t launch_transient_64(const DevProgram&, const DevNrCfg&, const DevTran&, const double*,
+                               int64_t, const DevOut&, const DevTranOut&, hipStream_t);
+hipError_t sparse_transient_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
+                                  int64_t* slot, size_t* smem, int* lu_in_lds);
+hipError_t launch_transient_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
+                                   const DevTran& tr, const double* params, int64_t B,
+                                   const DevOut& out, const DevTranOut& tout, double* ws,
+                                   int grid, int64_t slot, size_t smem, int lu_in_lds,
+                                   hipStream_t st);
 hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
                                int64_t* slot, size_t* smem, int* lu_in_lds);
 hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
@@ -69,7 +84,9 @@ struct HostProgram {
   int max_row_nnz = 0;
 };
 
-int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n) {
+// transient: capacitors get companion slots (G = C/h, I = G v_prev) instead
+// of being left open.
+int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool transient = false) {
   if (!c || c->num_nodes < 0 || c->num_elements < 0 || c->num_params < 0 ||
       (c->num_elements > 0 && !c->elements))
     return fail(SPX_ERR_INVALID_ARGUMENT, "invalid circuit description");
@@ -129,8 +146,14 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n) {
         vid = hp.num_vals++;
         cond(u[0], u[1], vid);
         break;
-      case SPX_EL_CAPACITOR:
-        break;  // open at DC
+      case SPX_EL_CAPACITOR:  // open at DC; BE companion in a transient (companion.rs:41-48)
+        if (transient) {
+          vid = hp.num_vals;
+          hp.num_vals += 2;
+          cond(u[0], u[1], vid);
+          cur(u[1], u[0], vid + 1);
+        }
+        break;
       case SPX_EL_VSOURCE:
       case SPX_EL_VPULSE: {  // mna.rs:124-145
         vid = hp.num_vals++;
@@ -606,6 +629,80 @@ int gmres_locked(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows, con
   return SPX_OK;
 }
 
+int check_tran(const spx_tran_config* t, int n) {
+  if (!t) return fail(SPX_ERR_INVALID_ARGUMENT, "null transient configuration");
+  if (!(t->tstep > 0.0) || t->steps < 0 || t->steps > 1000000)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "transient needs tstep > 0 and 0 <= steps <= 1e6");
+  if (t->num_probes < 0 || (t->num_probes > 0 && !t->probes))
+    return fail(SPX_ERR_INVALID_ARGUMENT, "bad probe list");
+  for (int p = 0; p < t->num_probes; ++p)
+    if (t->probes[p] < 0 || t->probes[p] >= n)
+      return fail(SPX_ERR_INVALID_ARGUMENT, "probe " + std::to_string(p) + " outside the unknowns");
+  return SPX_OK;
+}
+
+int transient_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
+                            const spx_nr_config* cfg, const spx_tran_config* tran,
+                            spx_tran_result* out) {
+  int rc;
+  if (!out || !out->x) return fail(SPX_ERR_INVALID_ARGUMENT, "result.x is required");
+  if ((rc = check_cfg(cfg))) return rc;
+  if (cfg->precision != SPX_PREC_F64)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "the transient sweep runs in FP64");
+  HostProgram hp;
+  if ((rc = compile_circuit(ckt, hp, kSparseMaxN, true))) return rc;
+  if ((rc = check_spec(spec, hp.P))) return rc;
+  if ((rc = check_tran(tran, hp.n))) return rc;
+  const int64_t B = spec->batch;
+  if (B == 0) return SPX_OK;
+  DevProgram g;
+  DevSparse sp{};
+  if ((rc = upload_program(h, hp, g, &sp))) return rc;
+  const double* params = out->params;
+  if (!out->params && spec->generator == SPX_GEN_EXPLICIT) {
+    params = spec->explicit_params;
+  } else {
+    double* dst = out->params;
+    if (!dst) {
+      void* p = nullptr;
+      if ((rc = workspace(h, "params", sizeof(double) * std::max<int64_t>(1, B * hp.P), &p)))
+        return rc;
+      dst = static_cast<double*>(p);
+    }
+    if ((rc = sample_device(h, spec, hp.P, dst))) return rc;
+    params = dst;
+  }
+  void* dprobe = nullptr;
+  if ((rc = workspace(h, "tran_probes", sizeof(int) * std::max(1, tran->num_probes), &dprobe)))
+    return rc;
+  if (tran->num_probes > 0) {
+    SPX_HIP(hipMemcpyAsync(dprobe, tran->probes, sizeof(int) * tran->num_probes,
+                           hipMemcpyHostToDevice, h->stream),
+            SPX_ERR_TRANSFER);
+    SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+  }
+  DevTran tr{tran->tstep, tran->steps, tran->num_probes, static_cast<const int*>(dprobe)};
+  DevOut o{out->x, out->status, out->iterations, out->flags, out->residual, nullptr, nullptr};
+  DevTranOut to{out->waveforms, out->fail_step};
+  DevNrCfg dc = to_dev(cfg);
+  if (hp.n > kDenseMaxN) {
+    int grid = 0, lu_lds = 0;
+    int64_t slot = 0;
+    size_t smem = 0;
+    SPX_HIP(sparse_transient_shape(g, sp, B, &grid, &slot, &smem, &lu_lds), SPX_ERR_BACKEND);
+    void* ws = nullptr;
+    if ((rc = workspace(h, "sparse_ws", sizeof(double) * (size_t)grid * slot, &ws))) return rc;
+    SPX_HIP(launch_transient_sparse(g, sp, dc, tr, params, B, o, to, static_cast<double*>(ws),
+                                    grid, slot, smem, lu_lds, h->stream),
+            SPX_ERR_BACKEND);
+  } else {
+    auto f = hp.n <= 8 ? launch_transient_8 : hp.n <= 16 ? launch_transient_16
+           : hp.n <= 32 ? launch_transient_32 : launch_transient_64;
+    SPX_HIP(f(g, dc, tr, params, B, o, to, h->stream), SPX_ERR_BACKEND);
+  }
+  return SPX_OK;
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -997,6 +1094,72 @@ int spx_gmres_solve_batch_f64(spx_handle* h, int64_t n, int64_t nnz, const int32
   if ((rc = d2h(iterations, dit, sizeof(int32_t) * batch))) return rc;
   if ((rc = d2h(residuals, dres, sizeof(double) * batch))) return rc;
   if ((rc = d2h(converged, dconv, batch))) return rc;
+  SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+  return SPX_OK;
+}
+
+int spx_transient_sweep_device(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
+                               const spx_nr_config* cfg, const spx_tran_config* tran,
+                               spx_tran_result* out) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  return transient_device_locked(h, ckt, spec, cfg, tran, out);
+}
+
+int spx_transient_sweep(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
+                        const spx_nr_config* cfg, const spx_tran_config* tran,
+                        spx_tran_result* out) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  if (!out || !out->x) return fail(SPX_ERR_INVALID_ARGUMENT, "result.x is required");
+  int rc;
+  HostProgram hp;
+  if ((rc = compile_circuit(ckt, hp, kSparseMaxN, true))) return rc;
+  if ((rc = check_spec(spec, hp.P))) return rc;
+  if ((rc = check_tran(tran, hp.n))) return rc;
+  const int64_t B = spec->batch;
+  if (B == 0) return SPX_OK;
+  const int n = hp.n, P = hp.P;
+  const int64_t W = B * (int64_t)(tran->steps + 1) * tran->num_probes;
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  void *dx, *dst, *dit, *dfl, *dres, *dpar, *dwav = nullptr, *dfs, *dexp = nullptr;
+  if ((rc = workspace(h, "tr_x", sizeof(double) * B * n, &dx))) return rc;
+  if ((rc = workspace(h, "tr_status", sizeof(int) * B, &dst))) return rc;
+  if ((rc = workspace(h, "tr_iter", sizeof(int) * B, &dit))) return rc;
+  if ((rc = workspace(h, "tr_flags", sizeof(int) * B, &dfl))) return rc;
+  if ((rc = workspace(h, "tr_res", sizeof(double) * B, &dres))) return rc;
+  if ((rc = workspace(h, "tr_params", sizeof(double) * std::max<int64_t>(1, B * P), &dpar)))
+    return rc;
+  if ((rc = workspace(h, "tr_fail", sizeof(int) * B, &dfs))) return rc;
+  if (out->waveforms && W > 0 &&
+      (rc = workspace(h, "tr_waves", sizeof(double) * W, &dwav)))
+    return rc;
+  spx_sweep_spec local = *spec;
+  if (spec->generator == SPX_GEN_EXPLICIT && P > 0) {
+    if ((rc = workspace(h, "tr_explicit", sizeof(double) * B * P, &dexp))) return rc;
+    SPX_HIP(hipMemcpyAsync(dexp, spec->explicit_params, sizeof(double) * B * P,
+                           hipMemcpyHostToDevice, h->stream),
+            SPX_ERR_TRANSFER);
+    local.explicit_params = static_cast<double*>(dexp);
+  }
+  spx_tran_result d{(double*)dx,   (int*)dst,      (int*)dit, (int*)dfl, (double*)dres,
+                    (double*)dpar, (double*)dwav,  (int*)dfs};
+  if ((rc = transient_device_locked(h, ckt, &local, cfg, tran, &d))) return rc;
+  auto d2h = [&](void* dst_h, const void* src_d, size_t bytes) -> int {
+    if (!dst_h || !src_d || bytes == 0) return SPX_OK;
+    SPX_HIP(hipMemcpyAsync(dst_h, src_d, bytes, hipMemcpyDeviceToHost, h->stream),
+            SPX_ERR_TRANSFER);
+    return SPX_OK;
+  };
+  if ((rc = d2h(out->x, dx, sizeof(double) * B * n))) return rc;
+  if ((rc = d2h(out->status, dst, sizeof(int) * B))) return rc;
+  if ((rc = d2h(out->iterations, dit, sizeof(int) * B))) return rc;
+  if ((rc = d2h(out->flags, dfl, sizeof(int) * B))) return rc;
+  if ((rc = d2h(out->residual, dres, sizeof(double) * B))) return rc;
+  if ((rc = d2h(out->params, dpar, sizeof(double) * B * P))) return rc;
+  if ((rc = d2h(out->waveforms, dwav, sizeof(double) * W))) return rc;
+  if ((rc = d2h(out->fail_step, dfs, sizeof(int) * B))) return rc;
   SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
   return SPX_OK;
 }

@@ -22,7 +22,7 @@ __global__ __launch_bounds__(64) void k_solve_rows(const double* __restrict__ M,
                                                    const double* __restrict__ rhs, int n,
                                                    double* __restrict__ x,
                                                    uint8_t* __restrict__ sing) {
-  __shared__ double row[N];
+  __shared__ double row[row_buffer_doubles(N)];
   const int64_t i = blockIdx.x;
   const int lane = threadIdx.x;
   const double* A = M + i * (int64_t)n * n;

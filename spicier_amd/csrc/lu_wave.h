@@ -56,6 +56,10 @@ __device__ __forceinline__ T tabs(T v) { return v < T(0) ? -v : v; }
 // VALU-bound, the second LDS-bound; which wins depends on n and occupancy.
 enum { kBcastReadlane = 0, kBcastLds = 1 };
 
+// Doubles of the LDS buffer a RowLU<N, *> needs behind `row`: N for the
+// pivot-row broadcast, and at least kWave ints for the position-map scatter.
+__host__ __device__ constexpr int row_buffer_doubles(int N) { return N < 32 ? 32 : N; }
+
 template <int N, typename T, int Bcast = kBcastReadlane>
 struct RowLU {
   T a[N];
@@ -118,6 +122,7 @@ struct RowLU {
       }
     });
     if (ok) {  // invert the position map once, so solves need no ballot per step
+      // all 64 lanes scatter: `row` must hold kWave ints (row_buffer_doubles)
       int* slot = reinterpret_cast<int*>(row);
       slot[pos] = lane;
       __syncthreads();

@@ -30,7 +30,8 @@ namespace spx {
 // bitmasks, rhs, x, scratch x, element values, parameters, junction states,
 // pivot-row buffer.
 inline size_t newton_smem_doubles(const DevProgram& g, int N) {
-  return (size_t)g.num_pos + (g.num_pos + 1) / 2 + N + 4 * N + g.num_vals + g.P + g.num_elems;
+  return (size_t)g.num_pos + (g.num_pos + 1) / 2 + N + 3 * N + g.num_vals + g.P + g.num_elems +
+         row_buffer_doubles(N);
 }
 
 template <int N, bool IR>
@@ -47,10 +48,11 @@ struct Instance {
   double* vals;   // num_vals
   double* pv;     // P parameters
   double* vst;    // num_elems pnjlim junction voltages
-  double* prow;   // N: LU pivot-row broadcast buffer
+  double* prow;   // row_buffer_doubles(N): LU pivot-row broadcast / position-map scratch
   int q0 = 0, q1 = 0;    // this lane's row: pattern entries [q0, q1)
   int nfact = 0;  // LU factorizations performed
   int nref = 0;   // refinement steps performed
+  const TranCtx* tc = nullptr;  // transient context (null: DC sweep)
 
   // after jc is in LDS; publishes the row masks (a barrier follows)
   __device__ __forceinline__ void init_row() {
@@ -71,7 +73,7 @@ struct Instance {
   __device__ __forceinline__ bool assemble(double gmin, double sf, int limiter) {
     bool lim = false;
     for (int i = lane; i < g.num_elems; i += kWave)
-      eval_element(g, g.order[i], xs, pv, vals, vst, sf, limiter, lim);
+      eval_element(g, g.order[i], xs, pv, vals, vst, sf, limiter, lim, tc);
     if (lane == 0) vals[kValGmin] = gmin;
     __syncthreads();
     for (int q = lane; q < g.num_pos; q += kWave) J[q] = gather_sum(g.pos_off, g.contrib, q, vals);
@@ -260,6 +262,23 @@ struct Instance {
     }
   }
 
+  // vprev of every capacitor from xs (transient)
+  __device__ __forceinline__ void cap_update() {
+    double* vp = const_cast<double*>(tc->vprev);
+    for (int e = lane; e < g.num_elems; e += kWave) {
+      const int* el = g.elems + e * kElemInts;
+      if (el[0] == SPX_EL_CAPACITOR)
+        vp[e] = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+    }
+    __syncthreads();
+  }
+  __device__ __forceinline__ void write_probes(const DevTran& tr, double* row) const {
+    for (int p = lane; p < tr.num_probes; p += kWave) row[p] = xs[tr.probes[p]];
+  }
+  __device__ __forceinline__ void write_scalar(int* dst, int v) const {
+    if (lane == 0) *dst = v;
+  }
+
   __device__ __forceinline__ void write_handover(const DevOut& out, int64_t inst,
                                                  int flags) const {
     if (lane == 0) {
@@ -325,6 +344,54 @@ __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
   I.init_row();
   __syncthreads();
   run_dcop(I, g, cfg, x0 != nullptr, out, inst, escalated_only);
+}
+
+// Batched Backward-Euler transient, FP64, one wave per instance: LDS as
+// k_newton_dense plus the capacitor states.
+template <int N>
+__global__ __launch_bounds__(64) void k_transient_dense(DevProgram g, DevNrCfg cfg, DevTran tr,
+                                                        const double* __restrict__ params,
+                                                        int64_t B, DevOut out, DevTranOut tout) {
+  extern __shared__ double smem[];
+  const int64_t inst = blockIdx.x;
+  const int lane = threadIdx.x;
+  double* J = smem;
+  int* jc = reinterpret_cast<int*>(J + g.num_pos);
+  uint32_t* rmask = reinterpret_cast<uint32_t*>(J + g.num_pos + (g.num_pos + 1) / 2);
+  double* rhs = J + g.num_pos + (g.num_pos + 1) / 2 + N;
+  double* xs = rhs + N;
+  double* xw = xs + N;
+  double* vals = xw + N;
+  double* pv = vals + g.num_vals;
+  double* vst = pv + g.P;
+  double* prow = vst + g.num_elems;
+  double* vcap = prow + row_buffer_doubles(N);
+  Instance<N, false> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
+  TranCtx tc{0, 0.0, tr.h, vcap};
+  I.tc = &tc;
+  for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
+  for (int p = lane; p < g.P; p += kWave) pv[p] = params[inst * g.P + p];
+  for (int e = lane; e < g.num_elems; e += kWave) vcap[e] = 0.0;
+  if (lane == 0) { vals[kValOne] = 1.0; vals[kValGmin] = 0.0; }
+  if (lane < N) xs[lane] = 0.0;
+  __syncthreads();
+  I.init_row();
+  __syncthreads();
+  run_transient(I, g, cfg, tr, tc, out, tout, inst);
+}
+
+template <int N>
+hipError_t launch_transient_impl(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
+                                 const double* params, int64_t B, const DevOut& out,
+                                 const DevTranOut& tout, hipStream_t st) {
+  size_t sm = sizeof(double) * (newton_smem_doubles(g, N) + g.num_elems);
+  hipError_t e = hipFuncSetAttribute((const void*)k_transient_dense<N>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+  if (e != hipSuccess) return e;
+  if (B == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_transient_dense<N>), dim3((unsigned)B), dim3(kWave), sm, st, g, cfg, tr,
+                     params, B, out, tout);
+  return hipGetLastError();
 }
 
 template <int N, bool IR>

@@ -112,6 +112,7 @@ struct SparseInstance {
   double* redd;      // LDS T/64 doubles
   int nfact = 0;
   int nref = 0;
+  const TranCtx* tc = nullptr;  // transient context (null: DC sweep)
 
   __device__ __forceinline__ bool block_all(bool p) { return block_all_t<T>(p, tid, red); }
   __device__ __forceinline__ double block_max(double v) {
@@ -127,7 +128,7 @@ struct SparseInstance {
   __device__ __forceinline__ bool assemble(double gmin, double sf, int limiter) {
     bool lim = false;
     for (int i = tid; i < g.num_elems; i += T)
-      eval_element(g, g.order[i], xs, pv, vals, vst, sf, limiter, lim);
+      eval_element(g, g.order[i], xs, pv, vals, vst, sf, limiter, lim, tc);
     if (tid == 0) vals[kValGmin] = gmin;
     __syncthreads();
     for (int q = tid; q < g.num_pos; q += T) {
@@ -231,6 +232,22 @@ struct SparseInstance {
       if (out.refinements) out.refinements[inst] = nref;
     }
   }
+  __device__ __forceinline__ void cap_update() {
+    double* vp = const_cast<double*>(tc->vprev);
+    for (int e = tid; e < g.num_elems; e += T) {
+      const int* el = g.elems + e * kElemInts;
+      if (el[0] == SPX_EL_CAPACITOR)
+        vp[e] = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+    }
+    __syncthreads();
+  }
+  __device__ __forceinline__ void write_probes(const DevTran& tr, double* row) const {
+    for (int p = tid; p < tr.num_probes; p += T) row[p] = xs[tr.probes[p]];
+  }
+  __device__ __forceinline__ void write_scalar(int* dst, int v) const {
+    if (tid == 0) *dst = v;
+  }
+
   __device__ __forceinline__ void write_handover(const DevOut& out, int64_t inst,
                                                  int flags) const {
     if (tid == 0) {
@@ -287,6 +304,84 @@ __global__ __launch_bounds__(kSparseThreads) void k_newton_sparse(
   }
 }
 
+inline hipError_t grid_for(const void* kernel, size_t smem, int64_t B, int* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return e;
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kSparseThreads, smem)) !=
+      hipSuccess)
+    return e;
+  if (per_cu < 1) per_cu = 1;
+  int64_t gsz = (int64_t)cus * per_cu;
+  if (gsz > B) gsz = B;
+  if (gsz < 1) gsz = 1;
+  *grid = (int)gsz;
+  return hipSuccess;
+}
+
+// Batched Backward-Euler transient on the sparse path; the workspace slot
+// adds the capacitor states after the sparse Newton slot.
+__global__ __launch_bounds__(kSparseThreads) void k_transient_sparse(
+    DevProgram g, DevSparse sp, DevNrCfg cfg, DevTran tr, const double* __restrict__ params,
+    int64_t B, DevOut out, DevTranOut tout, double* ws, int64_t slot, int lu_in_lds) {
+  extern __shared__ double smem[];
+  const int n = g.n, tid = threadIdx.x;
+  double* rhs = smem;
+  double* xs = rhs + n;
+  double* xw = xs + n;
+  double* y = xw + n;
+  double* redd = y + n;
+  int* red = reinterpret_cast<int*>(redd + kSparseThreads / kWave);
+  double* lds_lu = reinterpret_cast<double*>(
+      (reinterpret_cast<uintptr_t>(red + kSparseThreads / kWave) + 7) & ~uintptr_t(7));
+  double* w = ws + blockIdx.x * slot;
+  double* vals = w;
+  double* vst = vals + g.num_vals;
+  double* av = vst + g.num_elems;
+  double* vcap = av + g.num_pos;
+  double* lu = lu_in_lds ? lds_lu : vcap + g.num_elems;
+  for (int64_t inst = blockIdx.x; inst < B; inst += gridDim.x) {
+    if (tid == 0) {
+      vals[kValOne] = 1.0;
+      vals[kValGmin] = 0.0;
+    }
+    for (int i = tid; i < n; i += kSparseThreads) xs[i] = 0.0;
+    for (int e = tid; e < g.num_elems; e += kSparseThreads) vcap[e] = 0.0;
+    __syncthreads();
+    TranCtx tc{0, 0.0, tr.h, vcap};
+    SparseInstance I{g, sp, cfg, tid, params + inst * g.P, vals, vst, av, lu, rhs, xs, xw, y,
+                     red, redd};
+    I.tc = &tc;
+    run_transient(I, g, cfg, tr, tc, out, tout, inst);
+    __syncthreads();
+  }
+}
+
+hipError_t sparse_transient_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
+                                  int64_t* slot, size_t* smem, int* lu_in_lds) {
+  const size_t lds_budget = 64 * 1024;
+  *lu_in_lds = sparse_smem_bytes(g, sp, true) <= lds_budget ? 1 : 0;
+  *smem = sparse_smem_bytes(g, sp, *lu_in_lds);
+  *slot = sparse_slot_doubles(g, sp, *lu_in_lds) + ((g.num_elems + 31) & ~31);
+  hipError_t e = hipFuncSetAttribute((const void*)k_transient_sparse,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)*smem);
+  if (e != hipSuccess) return e;
+  return grid_for((const void*)k_transient_sparse, *smem, B, grid);
+}
+
+hipError_t launch_transient_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
+                                   const DevTran& tr, const double* params, int64_t B,
+                                   const DevOut& out, const DevTranOut& tout, double* ws,
+                                   int grid, int64_t slot, size_t smem, int lu_in_lds,
+                                   hipStream_t st) {
+  if (B == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_transient_sparse, dim3((unsigned)grid), dim3(kSparseThreads), smem, st, g,
+                     sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds);
+  return hipGetLastError();
+}
+
 // Batched sparse solve with a shared pattern (the numeric half of
 // FaerSparseCachedBatchedSolver / FaerTripletBatchedSolver,
 // crates/spicier-batched-sweep/src/faer_sparse_solver.rs:196-362): system
@@ -321,23 +416,6 @@ __global__ __launch_bounds__(kSparseThreads) void k_sparse_solve(
     if (tid == 0 && singular) singular[inst] = ok ? 0 : 1;
     __syncthreads();
   }
-}
-
-inline hipError_t grid_for(const void* kernel, size_t smem, int64_t B, int* grid) {
-  int dev = 0, cus = 0, per_cu = 0;
-  hipError_t e;
-  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
-    return e;
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kSparseThreads, smem)) !=
-      hipSuccess)
-    return e;
-  if (per_cu < 1) per_cu = 1;
-  int64_t gsz = (int64_t)cus * per_cu;
-  if (gsz > B) gsz = B;
-  if (gsz < 1) gsz = 1;
-  *grid = (int)gsz;
-  return hipSuccess;
 }
 
 // Shape of a batched sparse solve: LU in LDS when it fits in 64 KB.

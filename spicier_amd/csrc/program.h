@@ -57,6 +57,31 @@ struct DevNrCfg {
   double kcl_reltol, kcl_abstol, kcl_vntol;
 };
 
+// Backward-Euler transient context (transient/companion.rs:41-66): mode 0
+// is the operating point the transient starts from (capacitors open, PULSE
+// sources at V1, waveforms.rs:150-160); mode 1 a time step at t with step h
+// (capacitor C -> G = C/h with I = G * v_prev from its negative to its
+// positive node, PULSE sources at eval_pulse(t)).  vprev[e] holds the
+// previous step's voltage of capacitor element e.
+struct TranCtx {
+  int mode;
+  double t, h;
+  const double* vprev;
+};
+
+// Transient run: h, step count and the probed unknowns.
+struct DevTran {
+  double h;
+  int steps;
+  int num_probes;
+  const int* probes;
+};
+
+struct DevTranOut {
+  double* waves;   // [B][steps + 1][num_probes]
+  int* fail_step;  // [B] first step that did not converge, -1 if none
+};
+
 struct DevOut {
   double* x;
   int* status;

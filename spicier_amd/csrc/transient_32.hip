@@ -1,0 +1,11 @@
+// transient_32.hip — instantiation of the Backward-Euler transient kernel for
+// systems of up to 32 unknowns.
+#include "newton_kernel.h"
+
+namespace spx {
+hipError_t launch_transient_32(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
+                              const double* params, int64_t B, const DevOut& out,
+                              const DevTranOut& tout, hipStream_t st) {
+  return launch_transient_impl<32>(g, cfg, tr, params, B, out, tout, st);
+}
+}  // namespace spx

@@ -1,0 +1,11 @@
+// transient_64.hip — instantiation of the Backward-Euler transient kernel for
+// systems of up to 64 unknowns.
+#include "newton_kernel.h"
+
+namespace spx {
+hipError_t launch_transient_64(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
+                              const double* params, int64_t B, const DevOut& out,
+                              const DevTranOut& tout, hipStream_t st) {
+  return launch_transient_impl<64>(g, cfg, tr, params, B, out, tout, st);
+}
+}  // namespace spx

@@ -1,0 +1,11 @@
+// transient_8.hip — instantiation of the Backward-Euler transient kernel for
+// systems of up to 8 unknowns.
+#include "newton_kernel.h"
+
+namespace spx {
+hipError_t launch_transient_8(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
+                              const double* params, int64_t B, const DevOut& out,
+                              const DevTranOut& tout, hipStream_t st) {
+  return launch_transient_impl<8>(g, cfg, tr, params, B, out, tout, st);
+}
+}  // namespace spx

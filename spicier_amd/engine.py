@@ -548,6 +548,57 @@ def sweep_dcop(circuit, batch: int, seed: int = 42, generator: int = A.SPX_GEN_H
     return out
 
 
+@dataclass
+class TransientSweepResult:
+    """Backward-Euler transient of every instance (TransientResult,
+    crates/spicier-solver/src/transient/result.rs, restricted to the probed
+    unknowns): waveforms[i, k, p] = x[probes[p]] at t = k * tstep."""
+    solutions: np.ndarray
+    status: np.ndarray
+    iterations: np.ndarray
+    flags: np.ndarray
+    residual: np.ndarray
+    params: np.ndarray
+    waveforms: np.ndarray
+    fail_step: np.ndarray
+    tstep: float
+
+    @property
+    def times(self) -> np.ndarray:
+        return np.arange(self.waveforms.shape[1]) * self.tstep
+
+    def accepted_mask(self) -> np.ndarray:
+        return (self.status == A.SPX_CONVERGED) & ((self.flags & A.SPX_FLAG_KCL_FAIL) == 0)
+
+
+def transient_sweep(circuit, batch: int, tstep: float, steps: int, probes=None, seed: int = 42,
+                    generator: int = A.SPX_GEN_HASH, sweep_offset: int = 0,
+                    config: A.SpxNrConfig | None = None, explicit_params=None,
+                    device_id: int = 0) -> TransientSweepResult:
+    """Monte Carlo Backward-Euler transient on the GPU (host buffers)."""
+    dev = Device.get(device_id)
+    cfg = config or nr_config()
+    ckt, keep = circuit.to_c()
+    if explicit_params is not None:
+        explicit_params = np.ascontiguousarray(explicit_params, dtype=np.float64)
+        generator = A.SPX_GEN_EXPLICIT
+    spec, keep2 = circuit.sweep_spec(batch, seed, generator, sweep_offset, explicit_params)
+    n, P = circuit.size, circuit.num_params
+    pr = np.ascontiguousarray(np.arange(n) if probes is None else probes, dtype=np.int32)
+    tr = A.SpxTranConfig(tstep=tstep, steps=steps, num_probes=pr.size, probes=A.ptr(pr))
+    out = TransientSweepResult(np.zeros((batch, n)), np.zeros(batch, np.int32),
+                               np.zeros(batch, np.int32), np.zeros(batch, np.int32),
+                               np.zeros(batch), np.zeros((batch, P)),
+                               np.zeros((batch, steps + 1, pr.size)), np.zeros(batch, np.int32),
+                               tstep)
+    res = A.SpxTranResult(A.ptr(out.solutions), A.ptr(out.status), A.ptr(out.iterations),
+                          A.ptr(out.flags), A.ptr(out.residual), A.ptr(out.params),
+                          A.ptr(out.waveforms), A.ptr(out.fail_step))
+    _check(dev.lib.spx_transient_sweep(dev.handle, C.byref(ckt), C.byref(spec), C.byref(cfg),
+                                       C.byref(tr), C.byref(res)))
+    return out
+
+
 def sample_params(circuit, batch: int, seed: int = 42, generator: int = A.SPX_GEN_HASH,
                   sweep_offset: int = 0, device_id: int = 0) -> np.ndarray:
     dev = Device.get(device_id)

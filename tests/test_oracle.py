@@ -470,3 +470,35 @@ def test_gmres_jacobi_solves_diagonal_in_one_step_and_edge_cases(oracle):
     x, conv, it, res = oracle.gmres(120, rows, cols, vals, rng.standard_normal(120),
                                     restart=4, max_iter=6, tol=1e-14)
     assert not conv and it == 7 and res > 1e-14
+
+
+# ------------------------------------------------------------- BE transient
+def test_transient_rc_step_is_the_backward_euler_recurrence(oracle):
+    # v_k = V (1 - (1 + h/tau)^-k): the exact Backward-Euler recurrence of an
+    # RC charged from a step, with the source branch current (V - v_k)/R
+    R, Cap, V = 2e3, 1.5e-12, 1.0
+    c = CK.rc_step(R, Cap, V)
+    h, steps = 1e-12, 200
+    out = oracle.transient(c, CK.instance_params(c, 1), A.default_nr_config(), h, steps)
+    k = np.arange(steps + 1)
+    want = V * (1.0 - (1.0 + h / (R * Cap)) ** (-k.astype(float)))
+    # unknowns: in, out, branch
+    np.testing.assert_allclose(out["waveforms"][0, :, 1], want, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(out["waveforms"][0, 1:, 0], V, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(out["waveforms"][0, 1:, 2], -(V - want[1:]) / R, rtol=1e-10)
+    assert out["status"][0] == A.SPX_CONVERGED and out["fail_step"][0] == -1
+    assert out["iterations"][0] == steps + 1 and out["residual"][0] <= 1.0
+
+
+def test_transient_pulse_waveform_and_ladder(oracle):
+    # the driven node follows eval_pulse exactly (waveforms.rs:160-180)
+    c = CK.ladder(16, seed=3)
+    h, steps = 1e-12, 40
+    out = oracle.transient(c, CK.instance_params(c, 2), A.default_nr_config(), h, steps)
+    t = np.arange(steps + 1) * h
+    ramp = np.clip(t / 10e-12, 0, 1) * 1.8
+    np.testing.assert_allclose(out["waveforms"][0, :, 0], ramp, rtol=1e-12, atol=1e-15)
+    assert np.all(out["status"] == A.SPX_CONVERGED)
+    # the ladder charges monotonically from the source side for the first steps
+    w = out["waveforms"][0]
+    assert np.all(w[5:, 1] > 0) and w[20, 1] > w[20, 5]
