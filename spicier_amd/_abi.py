@@ -106,6 +106,15 @@ class SpxTranResult(C.Structure):
                 ("waveforms", C.c_void_p), ("fail_step", C.c_void_p)]
 
 
+class SpxTranClass(C.Structure):
+    _fields_ = [("circuit", C.POINTER(SpxCircuit)), ("spec", SpxSweepSpec),
+                ("tran", SpxTranConfig), ("result", SpxTranResult)]
+
+
+SPX_MIXED_GRAPH, SPX_MIXED_AUTOTUNE = 1, 2
+SPX_KERNEL_AUTO, SPX_KERNEL_DENSE, SPX_KERNEL_SPARSE64, SPX_KERNEL_SPARSE256 = 0, 1, 2, 3
+
+
 def default_nr_config(**overrides) -> SpxNrConfig:
     """Defaults of ConvergenceCriteria / GminSteppingParams / SourceSteppingParams
     (crates/spicier-solver/src/newton.rs:24-34,185-194,222-231)."""
@@ -172,6 +181,8 @@ def load_library():
                                             C.POINTER(SpxNrConfig), vp,
                                             C.POINTER(SpxSweepResult)]),
         "spx_circuit_size": (C.c_int32, [C.POINTER(SpxCircuit)]),
+        "spx_transient_mixed_device": (C.c_int, [vp, C.POINTER(SpxTranClass), i32,
+                                                 C.POINTER(SpxNrConfig), C.c_uint32, vp]),
         "spx_transient_sweep": (C.c_int, [vp, C.POINTER(SpxCircuit), C.POINTER(SpxSweepSpec),
                                           C.POINTER(SpxNrConfig), C.POINTER(SpxTranConfig),
                                           C.POINTER(SpxTranResult)]),

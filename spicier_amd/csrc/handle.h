@@ -21,9 +21,21 @@ struct spx_handle {
   std::mutex mu;
   std::map<std::string, std::pair<void*, size_t>> ws;
   spx_surrogate* surrogate = nullptr;  // used by SPX_WARM_SURROGATE
-  std::vector<int> program_blob;       // circuit program now in the "program" workspace
-  bool has_sparse = false;             // its shared-symbolic LU plan (n > 64) is uploaded
-  spx::HostSparsePlan sparse_plan;
+  // circuit programs by slot ("" for single sweeps, one per class of a mixed
+  // batch): the program now in workspace "program<slot>" and, for n past the
+  // register LU, its shared-symbolic LU plan in "sparse_plan<slot>"
+  struct ProgramSlot {
+    std::vector<int> blob;
+    bool has_sparse = false;
+    spx::HostSparsePlan plan;
+  };
+  std::map<std::string, ProgramSlot> programs;
+  // mixed-batch hipGraph: the executable, the description it was captured
+  // for, and the device addresses it baked in
+  hipGraphExec_t mixed_exec = nullptr;
+  std::vector<uint64_t> mixed_key;
+  // autotuned kernel kind per circuit program (transient program blob)
+  std::map<std::vector<int>, int> tuned;
   // batched sparse linear solve: the pattern (n, nnz, rows, cols) whose plan
   // is in the "lin_plan" workspace, and that plan's extra offsets
   std::vector<int> lin_key;

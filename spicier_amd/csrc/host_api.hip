@@ -38,20 +38,21 @@ hipError_t launch_sparse_solve(const DevSparse& sp, int n, int num_pos, const in
                                double* ws, int grid, int64_t slot, size_t smem, int lu_in_lds,
                                hipStream_t st);
 hipError_t launch_transient_8(const DevProgram&, const DevNrCfg&, const DevTran&, const double*,
-                              int64_t, const DevOut&, const DevTranOut&, hipStream_t);
+                              int64_t, const DevOut&, const DevTranOut&, hipStream_t, int);
 hipError_t launch_transient_16(const DevProgram&, const DevNrCfg&, const DevTran&, const double*,
-                               int64_t, const DevOut&, const DevTranOut&, hipStream_t);
+                               int64_t, const DevOut&, const DevTranOut&, hipStream_t, int);
 hipError_t launch_transient_32(const DevProgram&, const DevNrCfg&, const DevTran&, const double*,
-                               int64_t, const DevOut&, const DevTranOut&, hipStream_t);
+                               int64_t, const DevOut&, const DevTranOut&, hipStream_t, int);
 hipError_t launch_transient_64(const DevProgram&, const DevNrCfg&, const DevTran&, const double*,
-                               int64_t, const DevOut&, const DevTranOut&, hipStream_t);
+                               int64_t, const DevOut&, const DevTranOut&, hipStream_t, int);
 hipError_t sparse_transient_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
-                                  int64_t* slot, size_t* smem, int* lu_in_lds);
+                                  int64_t* slot, size_t* smem, int* lu_in_lds, int threads,
+                                  int force_lu);
 hipError_t launch_transient_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
                                    const DevTran& tr, const double* params, int64_t B,
                                    const DevOut& out, const DevTranOut& tout, double* ws,
                                    int grid, int64_t slot, size_t smem, int lu_in_lds,
-                                   hipStream_t st);
+                                   int threads, hipStream_t st);
 hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
                                int64_t* slot, size_t* smem, int* lu_in_lds);
 hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
@@ -80,6 +81,7 @@ constexpr int kSparseMaxN = 4096;
 // ------------------------------------------------------------- circuit compile
 struct HostProgram {
   int n = 0, nn = 0, P = 0, num_elems = 0, num_vals = 2, has_nonlinear = 0;
+  bool want_sparse = false;  // build the sparse plan even when n fits the register LU
   std::vector<int> elems, order, pos_rc, pos_off, row_ptr, contrib, rhs_off, rhs_contrib;
   int max_row_nnz = 0;
 };
@@ -232,9 +234,14 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
   return SPX_OK;
 }
 
-// Uploads the program into the handle's "program" workspace, and for n past
-// the register LU the shared-symbolic LU plan into "sparse_plan".
-int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g, DevSparse* sp) {
+inline bool sp_always(const HostProgram& hp) { return hp.want_sparse; }
+
+// Uploads the program into the handle's "program<slot>" workspace, and for n
+// past the register LU the shared-symbolic LU plan into "sparse_plan<slot>".
+int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g, DevSparse* sp,
+                   const std::string& slot = "") {
+  auto& ps = h->programs[slot];
+  const std::string wprog = "program" + slot, wplan = "sparse_plan" + slot;
   std::vector<int> blob;
   auto put = [&](const std::vector<int>& v) {
     size_t off = blob.size();
@@ -249,38 +256,37 @@ int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g, DevSpars
   // the stream wait its pageable source needs) is skipped while the program
   // in the workspace is byte-identical.
   const size_t bytes = blob.size() * sizeof(int) + 16;
-  auto it = h->ws.find("program");
-  const bool cached = it != h->ws.end() && it->second.second >= bytes && blob == h->program_blob;
+  auto it = h->ws.find(wprog);
+  const bool cached = it != h->ws.end() && it->second.second >= bytes && blob == ps.blob;
   void* d = nullptr;
-  int rc = workspace(h, "program", bytes, &d);
+  int rc = workspace(h, wprog.c_str(), bytes, &d);
   if (rc) return rc;
   if (!cached) {
-    h->program_blob.clear();
-    h->has_sparse = false;
+    ps.blob.clear();
+    ps.has_sparse = false;
     SPX_HIP(hipMemcpyAsync(d, blob.data(), blob.size() * sizeof(int), hipMemcpyHostToDevice,
                            h->stream),
             SPX_ERR_TRANSFER);
     // the host blob dies at return: wait for the copy
     SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
-    h->program_blob = std::move(blob);
+    ps.blob = std::move(blob);
   }
-  if (hp.n > kDenseMaxN) {
-    if (!h->has_sparse) {
-      const char* why = build_sparse_plan(hp.n, hp.nn, hp.pos_rc, h->sparse_plan);
+  if (hp.n > kDenseMaxN || sp_always(hp)) {
+    if (!ps.has_sparse) {
+      const char* why = build_sparse_plan(hp.n, hp.nn, hp.pos_rc, ps.plan);
       if (*why) return fail(SPX_ERR_INVALID_DIMENSION, why);
       void* ds = nullptr;
-      const auto& pb = h->sparse_plan.blob;
-      if ((rc = workspace(h, "sparse_plan", pb.size() * sizeof(int), &ds))) return rc;
+      const auto& pb = ps.plan.blob;
+      if ((rc = workspace(h, wplan.c_str(), pb.size() * sizeof(int), &ds))) return rc;
       SPX_HIP(hipMemcpyAsync(ds, pb.data(), pb.size() * sizeof(int), hipMemcpyHostToDevice,
                              h->stream),
               SPX_ERR_TRANSFER);
       SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
-      h->has_sparse = true;
+      ps.has_sparse = true;
     }
     void* ds = nullptr;
-    if ((rc = workspace(h, "sparse_plan", h->sparse_plan.blob.size() * sizeof(int), &ds)))
-      return rc;
-    if (sp) *sp = device_sparse(h->sparse_plan, static_cast<const int*>(ds));
+    if ((rc = workspace(h, wplan.c_str(), ps.plan.blob.size() * sizeof(int), &ds))) return rc;
+    if (sp) *sp = device_sparse(ps.plan, static_cast<const int*>(ds));
   }
   const int* base = static_cast<const int*>(d);
   g.n = hp.n;
@@ -689,17 +695,277 @@ int transient_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_swe
     int grid = 0, lu_lds = 0;
     int64_t slot = 0;
     size_t smem = 0;
-    SPX_HIP(sparse_transient_shape(g, sp, B, &grid, &slot, &smem, &lu_lds), SPX_ERR_BACKEND);
+    SPX_HIP(sparse_transient_shape(g, sp, B, &grid, &slot, &smem, &lu_lds, 256, -1),
+            SPX_ERR_BACKEND);
     void* ws = nullptr;
     if ((rc = workspace(h, "sparse_ws", sizeof(double) * (size_t)grid * slot, &ws))) return rc;
     SPX_HIP(launch_transient_sparse(g, sp, dc, tr, params, B, o, to, static_cast<double*>(ws),
-                                    grid, slot, smem, lu_lds, h->stream),
+                                    grid, slot, smem, lu_lds, 256, h->stream),
             SPX_ERR_BACKEND);
   } else {
     auto f = hp.n <= 8 ? launch_transient_8 : hp.n <= 16 ? launch_transient_16
            : hp.n <= 32 ? launch_transient_32 : launch_transient_64;
-    SPX_HIP(f(g, dc, tr, params, B, o, to, h->stream), SPX_ERR_BACKEND);
+    SPX_HIP(f(g, dc, tr, params, B, o, to, h->stream, 3), SPX_ERR_BACKEND);
   }
+  return SPX_OK;
+}
+
+// ------------------------------------------------------------ mixed batches
+struct MixedClass {
+  HostProgram hp;
+  DevProgram g;
+  DevSparse sp{};
+  DevTran tr;
+  DevOut o;
+  DevTranOut to;
+  const double* params = nullptr;
+  double* sample_dst = nullptr;  // sampled into here (null: explicit parameters read in place)
+  void* d_specs = nullptr;
+  int gen = 0, nvar = 0;
+  uint64_t seed = 0;
+  int64_t B = 0, offset = 0;
+  int kernel = SPX_KERNEL_AUTO, grid = 0, lu_lds = 0;
+  int64_t slot = 0;
+  size_t smem = 0;
+  double* ws = nullptr;
+  const void* prog = nullptr;
+};
+
+using TranFn = hipError_t (*)(const DevProgram&, const DevNrCfg&, const DevTran&, const double*,
+                              int64_t, const DevOut&, const DevTranOut&, hipStream_t, int);
+TranFn dense_tran(int n) {
+  return n <= 8 ? launch_transient_8 : n <= 16 ? launch_transient_16
+       : n <= 32 ? launch_transient_32 : launch_transient_64;
+}
+
+// Shapes class `mc` for `kernel` over B instances: LDS attributes, grid and
+// the "mx_ws<i>" workspace (name given).
+int shape_class(spx_handle* h, MixedClass& mc, int kernel, int64_t B, const DevNrCfg& dc,
+                const std::string& wsname) {
+  int rc;
+  mc.kernel = kernel;
+  if (kernel == SPX_KERNEL_DENSE) {
+    SPX_HIP(dense_tran(mc.hp.n)(mc.g, dc, mc.tr, nullptr, B, mc.o, mc.to, h->stream, 1),
+            SPX_ERR_BACKEND);
+    mc.grid = (int)B;
+    mc.ws = nullptr;
+    return SPX_OK;
+  }
+  const int threads = kernel == SPX_KERNEL_SPARSE64 ? 64 : 256;
+  SPX_HIP(sparse_transient_shape(mc.g, mc.sp, B, &mc.grid, &mc.slot, &mc.smem, &mc.lu_lds,
+                                 threads, -1),
+          SPX_ERR_BACKEND);
+  void* ws = nullptr;
+  if ((rc = workspace(h, wsname.c_str(), sizeof(double) * (size_t)mc.grid * mc.slot, &ws)))
+    return rc;
+  mc.ws = static_cast<double*>(ws);
+  return SPX_OK;
+}
+
+hipError_t enqueue_class(const MixedClass& mc, const DevNrCfg& dc, const DevTran& tr,
+                         const DevOut& o, const DevTranOut& to, int64_t B, hipStream_t st) {
+  if (mc.kernel == SPX_KERNEL_DENSE) return dense_tran(mc.hp.n)(mc.g, dc, tr, mc.params, B, o, to, st, 2);
+  const int threads = mc.kernel == SPX_KERNEL_SPARSE64 ? 64 : 256;
+  return launch_transient_sparse(mc.g, mc.sp, dc, tr, mc.params, B, o, to, mc.ws, mc.grid,
+                                 mc.slot, mc.smem, mc.lu_lds, threads, st);
+}
+
+// Times each candidate kernel on a sample of the class; returns the fastest.
+int autotune_class(spx_handle* h, MixedClass& mc, const DevNrCfg& dc, int* best) {
+  const int64_t Bs = std::min<int64_t>(mc.B, 1024);
+  DevTran trs = mc.tr;
+  trs.steps = std::min(mc.tr.steps, 20);
+  trs.num_probes = 0;
+  void *dx, *dst, *dit, *dfl, *dres;
+  int rc;
+  if ((rc = workspace(h, "tune_x", sizeof(double) * Bs * mc.hp.n, &dx))) return rc;
+  if ((rc = workspace(h, "tune_status", sizeof(int) * Bs, &dst))) return rc;
+  if ((rc = workspace(h, "tune_iter", sizeof(int) * Bs, &dit))) return rc;
+  if ((rc = workspace(h, "tune_flags", sizeof(int) * Bs, &dfl))) return rc;
+  if ((rc = workspace(h, "tune_res", sizeof(double) * Bs, &dres))) return rc;
+  DevOut o{(double*)dx, (int*)dst, (int*)dit, (int*)dfl, (double*)dres, nullptr, nullptr};
+  DevTranOut to{nullptr, nullptr};
+  std::vector<int> cands;
+  if (mc.hp.n <= kDenseMaxN) cands.push_back(SPX_KERNEL_DENSE);
+  cands.push_back(SPX_KERNEL_SPARSE64);
+  cands.push_back(SPX_KERNEL_SPARSE256);
+  hipEvent_t e0, e1;
+  SPX_HIP(hipEventCreate(&e0), SPX_ERR_BACKEND);
+  SPX_HIP(hipEventCreate(&e1), SPX_ERR_BACKEND);
+  float best_ms = 1e30f;
+  *best = cands[0];
+  for (int k : cands) {
+    MixedClass t = mc;
+    if ((rc = shape_class(h, t, k, Bs, dc, "tune_ws"))) break;
+    float ms = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {  // first run warms caches and code
+      if (hipEventRecord(e0, h->stream) != hipSuccess ||
+          enqueue_class(t, dc, trs, o, to, Bs, h->stream) != hipSuccess ||
+          hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        rc = fail(SPX_ERR_BACKEND, "autotune launch failed");
+        break;
+      }
+      float m = 0.0f;
+      (void)hipEventElapsedTime(&m, e0, e1);
+      if (rep > 0) ms = m;
+    }
+    if (rc) break;
+    if (ms < best_ms) {
+      best_ms = ms;
+      *best = k;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_config* cfg,
+                 uint32_t options, int32_t* kernels) {
+  int rc;
+  if (nc < 0 || (nc > 0 && !cls)) return fail(SPX_ERR_INVALID_ARGUMENT, "bad class list");
+  if ((rc = check_cfg(cfg))) return rc;
+  if (cfg->precision != SPX_PREC_F64)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "the transient sweep runs in FP64");
+  const DevNrCfg dc = to_dev(cfg);
+  std::vector<MixedClass> mcs(nc);
+  std::vector<uint64_t> key;
+  auto kput = [&](const void* p, size_t bytes) {
+    std::vector<uint64_t> w((bytes + 7) / 8, 0);
+    std::memcpy(w.data(), p, bytes);
+    key.insert(key.end(), w.begin(), w.end());
+  };
+  kput(&dc, sizeof(dc));
+  for (int i = 0; i < nc; ++i) {
+    const spx_tran_class& c = cls[i];
+    MixedClass& mc = mcs[i];
+    const std::string tag = "_m" + std::to_string(i);
+    if (!c.result.x) return fail(SPX_ERR_INVALID_ARGUMENT, "class result.x is required");
+    if ((rc = compile_circuit(c.circuit, mc.hp, kSparseMaxN, true))) return rc;
+    if ((rc = check_spec(&c.spec, mc.hp.P))) return rc;
+    if ((rc = check_tran(&c.tran, mc.hp.n))) return rc;
+    int req = kernels ? kernels[i] : SPX_KERNEL_AUTO;
+    if (req < SPX_KERNEL_AUTO || req > SPX_KERNEL_SPARSE256 ||
+        (req == SPX_KERNEL_DENSE && mc.hp.n > kDenseMaxN))
+      return fail(SPX_ERR_INVALID_ARGUMENT, "class " + std::to_string(i) + ": bad kernel kind");
+    mc.hp.want_sparse = true;
+    if ((rc = upload_program(h, mc.hp, mc.g, &mc.sp, tag))) return rc;
+    mc.prog = mc.g.elems;
+    mc.B = c.spec.batch;
+    mc.gen = c.spec.generator;
+    mc.seed = c.spec.seed;
+    mc.offset = c.spec.sweep_offset;
+    const int P = mc.hp.P;
+    if (c.spec.generator == SPX_GEN_EXPLICIT) {
+      mc.params = c.spec.explicit_params;
+      if (c.result.params && mc.B * P > 0)  // copy out for the caller, outside any graph
+        SPX_HIP(hipMemcpyAsync(c.result.params, c.spec.explicit_params,
+                               sizeof(double) * mc.B * P, hipMemcpyDeviceToDevice, h->stream),
+                SPX_ERR_TRANSFER);
+    } else {
+      double* dst = c.result.params;
+      if (!dst) {
+        void* p = nullptr;
+        if ((rc = workspace(h, ("mx_params" + tag).c_str(),
+                            sizeof(double) * std::max<int64_t>(1, mc.B * P), &p)))
+          return rc;
+        dst = static_cast<double*>(p);
+      }
+      mc.sample_dst = dst;
+      mc.params = dst;
+      if (P > 0) {
+        if ((rc = workspace(h, ("mx_specs" + tag).c_str(), sizeof(spx_param_spec) * P,
+                            &mc.d_specs)))
+          return rc;
+        SPX_HIP(hipMemcpyAsync(mc.d_specs, c.spec.params, sizeof(spx_param_spec) * P,
+                               hipMemcpyHostToDevice, h->stream),
+                SPX_ERR_TRANSFER);
+        mc.nvar = count_varied(c.spec.params, P);
+      }
+    }
+    void* dprobe = nullptr;
+    if ((rc = workspace(h, ("mx_probes" + tag).c_str(), sizeof(int) * std::max(1, c.tran.num_probes),
+                        &dprobe)))
+      return rc;
+    if (c.tran.num_probes > 0)
+      SPX_HIP(hipMemcpyAsync(dprobe, c.tran.probes, sizeof(int) * c.tran.num_probes,
+                             hipMemcpyHostToDevice, h->stream),
+              SPX_ERR_TRANSFER);
+    SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+    mc.tr = DevTran{c.tran.tstep, c.tran.steps, c.tran.num_probes, static_cast<const int*>(dprobe)};
+    mc.o = DevOut{c.result.x, c.result.status, c.result.iterations, c.result.flags,
+                  c.result.residual, nullptr, nullptr};
+    mc.to = DevTranOut{c.result.waveforms, c.result.fail_step};
+    int kind = req;
+    if (kind == SPX_KERNEL_AUTO) {
+      auto it = h->tuned.find(h->programs[tag].blob);
+      if (it != h->tuned.end()) {
+        kind = it->second;
+      } else if ((options & SPX_MIXED_AUTOTUNE) && mc.B > 0) {
+        if (mc.sample_dst && mc.hp.P > 0)  // the sample needs parameters now
+          SPX_HIP(launch_sample(static_cast<spx_param_spec*>(mc.d_specs), mc.hp.P, mc.gen,
+                                mc.seed, mc.B, mc.offset, mc.nvar, mc.sample_dst, h->stream),
+                  SPX_ERR_BACKEND);
+        if ((rc = autotune_class(h, mc, dc, &kind))) return rc;
+        h->tuned[h->programs[tag].blob] = kind;
+      } else {
+        kind = mc.hp.n <= kDenseMaxN ? SPX_KERNEL_DENSE : SPX_KERNEL_SPARSE256;
+      }
+    }
+    if ((rc = shape_class(h, mc, kind, mc.B, dc, "mx_ws" + tag))) return rc;
+    if (kernels) kernels[i] = kind;
+    const uint64_t words[] = {(uint64_t)kind, (uint64_t)mc.B, (uint64_t)mc.hp.n,
+                              (uint64_t)mc.tr.steps, (uint64_t)mc.tr.num_probes,
+                              (uint64_t)mc.gen, mc.seed, (uint64_t)mc.offset, (uint64_t)mc.grid,
+                              (uint64_t)mc.slot, (uint64_t)mc.smem, (uint64_t)mc.lu_lds};
+    kput(words, sizeof(words));
+    kput(&mc.tr.h, sizeof(double));
+    const void* ptrs[] = {mc.params, mc.sample_dst, mc.d_specs, dprobe, mc.ws, mc.prog,
+                          mc.sp.perm, c.result.x, c.result.status, c.result.iterations,
+                          c.result.flags, c.result.residual, c.result.waveforms,
+                          c.result.fail_step};
+    kput(ptrs, sizeof(ptrs));
+  }
+  auto enqueue_all = [&](hipStream_t st) -> hipError_t {
+    for (int i = 0; i < nc; ++i) {
+      const MixedClass& mc = mcs[i];
+      hipError_t e;
+      if (mc.sample_dst && mc.hp.P > 0 &&
+          (e = launch_sample(static_cast<spx_param_spec*>(mc.d_specs), mc.hp.P, mc.gen, mc.seed,
+                             mc.B, mc.offset, mc.nvar, mc.sample_dst, st)) != hipSuccess)
+        return e;
+      if ((e = enqueue_class(mc, dc, mc.tr, mc.o, mc.to, mc.B, st)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+  };
+  if (!(options & SPX_MIXED_GRAPH)) {
+    SPX_HIP(enqueue_all(h->stream), SPX_ERR_BACKEND);
+    return SPX_OK;
+  }
+  if (!h->mixed_exec || key != h->mixed_key) {
+    if (h->mixed_exec) {
+      (void)hipGraphExecDestroy(h->mixed_exec);
+      h->mixed_exec = nullptr;
+    }
+    h->mixed_key.clear();
+    hipGraph_t graph = nullptr;
+    SPX_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed), SPX_ERR_BACKEND);
+    hipError_t e = enqueue_all(h->stream);
+    hipError_t e2 = hipStreamEndCapture(h->stream, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(SPX_ERR_BACKEND, std::string("graph capture failed: ") +
+                                       hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess)
+      return fail(SPX_ERR_BACKEND, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    h->mixed_exec = exec;
+    h->mixed_key = std::move(key);
+  }
+  SPX_HIP(hipGraphLaunch(h->mixed_exec, h->stream), SPX_ERR_BACKEND);
   return SPX_OK;
 }
 
@@ -746,6 +1012,7 @@ int spx_destroy(spx_handle* h) {
     std::lock_guard<std::mutex> lk(h->mu);
     DeviceGuard dg(h->device);
     hipStreamSynchronize(h->stream);
+    if (h->mixed_exec) hipGraphExecDestroy(h->mixed_exec);
     for (auto& kv : h->ws)
       if (kv.second.first) hipFree(kv.second.first);
     hipStreamDestroy(h->stream);
@@ -1162,6 +1429,14 @@ int spx_transient_sweep(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
   if ((rc = d2h(out->fail_step, dfs, sizeof(int) * B))) return rc;
   SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
   return SPX_OK;
+}
+
+int spx_transient_mixed_device(spx_handle* h, const spx_tran_class* classes, int32_t num_classes,
+                               const spx_nr_config* cfg, uint32_t options, int32_t* kernels) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  return mixed_locked(h, classes, num_classes, cfg, options, kernels);
 }
 
 int spx_stats_reduce_device(spx_handle* h, const double* d_values, const uint8_t* d_mask,

@@ -380,15 +380,19 @@ __global__ __launch_bounds__(64) void k_transient_dense(DevProgram g, DevNrCfg c
   run_transient(I, g, cfg, tr, tc, out, tout, inst);
 }
 
+// phase bit 1: set the kernel's LDS attribute (not allowed while a stream is
+// being captured into a graph); bit 2: launch.
 template <int N>
 hipError_t launch_transient_impl(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
                                  const double* params, int64_t B, const DevOut& out,
-                                 const DevTranOut& tout, hipStream_t st) {
+                                 const DevTranOut& tout, hipStream_t st, int phase) {
   size_t sm = sizeof(double) * (newton_smem_doubles(g, N) + g.num_elems);
-  hipError_t e = hipFuncSetAttribute((const void*)k_transient_dense<N>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-  if (e != hipSuccess) return e;
-  if (B == 0) return hipSuccess;
+  if (phase & 1) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_transient_dense<N>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+  }
+  if (!(phase & 2) || B == 0) return hipSuccess;
   hipLaunchKernelGGL((k_transient_dense<N>), dim3((unsigned)B), dim3(kWave), sm, st, g, cfg, tr,
                      params, B, out, tout);
   return hipGetLastError();

@@ -93,8 +93,9 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
   return block_all_t<T>(fin, tid, red);
 }
 
+template <int T_>
 struct SparseInstance {
-  static constexpr int T = kSparseThreads;
+  static constexpr int T = T_;
   const DevProgram& g;
   const DevSparse& sp;
   const DevNrCfg& cfg;
@@ -263,15 +264,17 @@ inline int64_t sparse_slot_doubles(const DevProgram& g, const DevSparse& sp, boo
   return (d + 31) & ~int64_t(31);  // 256-byte slots
 }
 
-inline size_t sparse_smem_bytes(const DevProgram& g, const DevSparse& sp, bool lu_in_lds) {
-  size_t d = 4 * (size_t)g.n + kSparseThreads / kWave;
-  size_t b = d * sizeof(double) + (kSparseThreads / kWave) * sizeof(int);
+inline size_t sparse_smem_bytes(const DevProgram& g, const DevSparse& sp, bool lu_in_lds,
+                                int threads = kSparseThreads) {
+  size_t d = 4 * (size_t)g.n + threads / kWave;
+  size_t b = d * sizeof(double) + (threads / kWave) * sizeof(int);
   b = (b + 7) & ~size_t(7);
   if (lu_in_lds) b += (size_t)sp.nnz_lu * sizeof(double);
   return b;
 }
 
-__global__ __launch_bounds__(kSparseThreads) void k_newton_sparse(
+template <int T>
+__global__ __launch_bounds__(T) void k_newton_sparse(
     DevProgram g, DevSparse sp, DevNrCfg cfg, const double* __restrict__ params,
     const double* __restrict__ x0, int64_t B, DevOut out, double* ws, int64_t slot,
     int lu_in_lds) {
@@ -282,9 +285,9 @@ __global__ __launch_bounds__(kSparseThreads) void k_newton_sparse(
   double* xw = xs + n;
   double* y = xw + n;
   double* redd = y + n;
-  int* red = reinterpret_cast<int*>(redd + kSparseThreads / kWave);
+  int* red = reinterpret_cast<int*>(redd + T / kWave);
   double* lds_lu = reinterpret_cast<double*>(
-      (reinterpret_cast<uintptr_t>(red + kSparseThreads / kWave) + 7) & ~uintptr_t(7));
+      (reinterpret_cast<uintptr_t>(red + T / kWave) + 7) & ~uintptr_t(7));
   double* w = ws + blockIdx.x * slot;
   double* vals = w;
   double* vst = vals + g.num_vals;
@@ -295,22 +298,23 @@ __global__ __launch_bounds__(kSparseThreads) void k_newton_sparse(
       vals[kValOne] = 1.0;
       vals[kValGmin] = 0.0;
     }
-    for (int i = tid; i < n; i += kSparseThreads) xs[i] = x0 ? x0[inst * n + i] : 0.0;
+    for (int i = tid; i < n; i += T) xs[i] = x0 ? x0[inst * n + i] : 0.0;
     __syncthreads();
-    SparseInstance I{g, sp, cfg, tid, params + inst * g.P, vals, vst, av, lu, rhs, xs, xw, y,
+    SparseInstance<T> I{g, sp, cfg, tid, params + inst * g.P, vals, vst, av, lu, rhs, xs, xw, y,
                      red, redd};
     run_dcop(I, g, cfg, x0 != nullptr, out, inst, 0);
     __syncthreads();
   }
 }
 
-inline hipError_t grid_for(const void* kernel, size_t smem, int64_t B, int* grid) {
+inline hipError_t grid_for(const void* kernel, size_t smem, int64_t B, int* grid,
+                           int threads = kSparseThreads) {
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
     return e;
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kSparseThreads, smem)) !=
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, smem)) !=
       hipSuccess)
     return e;
   if (per_cu < 1) per_cu = 1;
@@ -323,7 +327,8 @@ inline hipError_t grid_for(const void* kernel, size_t smem, int64_t B, int* grid
 
 // Batched Backward-Euler transient on the sparse path; the workspace slot
 // adds the capacitor states after the sparse Newton slot.
-__global__ __launch_bounds__(kSparseThreads) void k_transient_sparse(
+template <int T>
+__global__ __launch_bounds__(T) void k_transient_sparse(
     DevProgram g, DevSparse sp, DevNrCfg cfg, DevTran tr, const double* __restrict__ params,
     int64_t B, DevOut out, DevTranOut tout, double* ws, int64_t slot, int lu_in_lds) {
   extern __shared__ double smem[];
@@ -333,9 +338,9 @@ __global__ __launch_bounds__(kSparseThreads) void k_transient_sparse(
   double* xw = xs + n;
   double* y = xw + n;
   double* redd = y + n;
-  int* red = reinterpret_cast<int*>(redd + kSparseThreads / kWave);
+  int* red = reinterpret_cast<int*>(redd + T / kWave);
   double* lds_lu = reinterpret_cast<double*>(
-      (reinterpret_cast<uintptr_t>(red + kSparseThreads / kWave) + 7) & ~uintptr_t(7));
+      (reinterpret_cast<uintptr_t>(red + T / kWave) + 7) & ~uintptr_t(7));
   double* w = ws + blockIdx.x * slot;
   double* vals = w;
   double* vst = vals + g.num_vals;
@@ -347,11 +352,11 @@ __global__ __launch_bounds__(kSparseThreads) void k_transient_sparse(
       vals[kValOne] = 1.0;
       vals[kValGmin] = 0.0;
     }
-    for (int i = tid; i < n; i += kSparseThreads) xs[i] = 0.0;
-    for (int e = tid; e < g.num_elems; e += kSparseThreads) vcap[e] = 0.0;
+    for (int i = tid; i < n; i += T) xs[i] = 0.0;
+    for (int e = tid; e < g.num_elems; e += T) vcap[e] = 0.0;
     __syncthreads();
     TranCtx tc{0, 0.0, tr.h, vcap};
-    SparseInstance I{g, sp, cfg, tid, params + inst * g.P, vals, vst, av, lu, rhs, xs, xw, y,
+    SparseInstance<T> I{g, sp, cfg, tid, params + inst * g.P, vals, vst, av, lu, rhs, xs, xw, y,
                      red, redd};
     I.tc = &tc;
     run_transient(I, g, cfg, tr, tc, out, tout, inst);
@@ -359,26 +364,44 @@ __global__ __launch_bounds__(kSparseThreads) void k_transient_sparse(
   }
 }
 
-hipError_t sparse_transient_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
-                                  int64_t* slot, size_t* smem, int* lu_in_lds) {
+template <int T>
+hipError_t sparse_transient_shape_t(const DevProgram& g, const DevSparse& sp, int64_t B,
+                                    int* grid, int64_t* slot, size_t* smem, int* lu_in_lds,
+                                    int force_lu) {
   const size_t lds_budget = 64 * 1024;
-  *lu_in_lds = sparse_smem_bytes(g, sp, true) <= lds_budget ? 1 : 0;
-  *smem = sparse_smem_bytes(g, sp, *lu_in_lds);
+  *lu_in_lds = force_lu >= 0 ? force_lu : (sparse_smem_bytes(g, sp, true, T) <= lds_budget ? 1 : 0);
+  if (*lu_in_lds && sparse_smem_bytes(g, sp, true, T) > 160 * 1024) *lu_in_lds = 0;
+  *smem = sparse_smem_bytes(g, sp, *lu_in_lds, T);
   *slot = sparse_slot_doubles(g, sp, *lu_in_lds) + ((g.num_elems + 31) & ~31);
-  hipError_t e = hipFuncSetAttribute((const void*)k_transient_sparse,
+  hipError_t e = hipFuncSetAttribute((const void*)k_transient_sparse<T>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)*smem);
   if (e != hipSuccess) return e;
-  return grid_for((const void*)k_transient_sparse, *smem, B, grid);
+  return grid_for((const void*)k_transient_sparse<T>, *smem, B, grid, T);
+}
+
+// Shape of a sparse transient launch with `threads` (64 or 256) per
+// workgroup; force_lu: -1 = LU values in LDS when they fit in 64 KB,
+// 0 / 1 = global / LDS.
+hipError_t sparse_transient_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
+                                  int64_t* slot, size_t* smem, int* lu_in_lds, int threads,
+                                  int force_lu) {
+  return threads == 64
+             ? sparse_transient_shape_t<64>(g, sp, B, grid, slot, smem, lu_in_lds, force_lu)
+             : sparse_transient_shape_t<256>(g, sp, B, grid, slot, smem, lu_in_lds, force_lu);
 }
 
 hipError_t launch_transient_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
                                    const DevTran& tr, const double* params, int64_t B,
                                    const DevOut& out, const DevTranOut& tout, double* ws,
                                    int grid, int64_t slot, size_t smem, int lu_in_lds,
-                                   hipStream_t st) {
+                                   int threads, hipStream_t st) {
   if (B == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_transient_sparse, dim3((unsigned)grid), dim3(kSparseThreads), smem, st, g,
-                     sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds);
+  if (threads == 64)
+    hipLaunchKernelGGL(k_transient_sparse<64>, dim3((unsigned)grid), dim3(64), smem, st, g, sp,
+                       cfg, tr, params, B, out, tout, ws, slot, lu_in_lds);
+  else
+    hipLaunchKernelGGL(k_transient_sparse<256>, dim3((unsigned)grid), dim3(256), smem, st, g, sp,
+                       cfg, tr, params, B, out, tout, ws, slot, lu_in_lds);
   return hipGetLastError();
 }
 
@@ -451,10 +474,10 @@ hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t
   *lu_in_lds = sparse_smem_bytes(g, sp, true) <= lds_budget ? 1 : 0;
   *smem = sparse_smem_bytes(g, sp, *lu_in_lds);
   *slot = sparse_slot_doubles(g, sp, *lu_in_lds);
-  hipError_t e = hipFuncSetAttribute((const void*)k_newton_sparse,
+  hipError_t e = hipFuncSetAttribute((const void*)k_newton_sparse<kSparseThreads>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)*smem);
   if (e != hipSuccess) return e;
-  return grid_for((const void*)k_newton_sparse, *smem, B, grid);
+  return grid_for((const void*)k_newton_sparse<kSparseThreads>, *smem, B, grid);
 }
 
 hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
@@ -462,7 +485,8 @@ hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const 
                                 const DevOut& out, double* ws, int grid, int64_t slot,
                                 size_t smem, int lu_in_lds, hipStream_t st) {
   if (B == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_newton_sparse, dim3((unsigned)grid), dim3(kSparseThreads), smem, st, g,
+  hipLaunchKernelGGL(k_newton_sparse<kSparseThreads>, dim3((unsigned)grid), dim3(kSparseThreads),
+                     smem, st, g,
                      sp, cfg, params, x0, B, out, ws, slot, lu_in_lds);
   return hipGetLastError();
 }

@@ -5,7 +5,7 @@
 namespace spx {
 hipError_t launch_transient_64(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
                               const double* params, int64_t B, const DevOut& out,
-                              const DevTranOut& tout, hipStream_t st) {
-  return launch_transient_impl<64>(g, cfg, tr, params, B, out, tout, st);
+                              const DevTranOut& tout, hipStream_t st, int phase) {
+  return launch_transient_impl<64>(g, cfg, tr, params, B, out, tout, st, phase);
 }
 }  // namespace spx

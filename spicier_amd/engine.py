@@ -599,6 +599,83 @@ def transient_sweep(circuit, batch: int, tstep: float, steps: int, probes=None, 
     return out
 
 
+@dataclass
+class TranClass:
+    """One size class of a mixed transient batch: a circuit, its instance
+    count and sampling, its transient settings, and (filled by
+    transient_mixed_device) torch device buffers of the results."""
+    circuit: object
+    batch: int
+    tstep: float
+    steps: int
+    probes: object = None
+    seed: int = 42
+    sweep_offset: int = 0
+    buffers: dict = field(default_factory=dict)
+
+    def allocate(self, device_id: int = 0):
+        import torch
+        dev = torch.device(f"cuda:{device_id}")
+        n, P, B = self.circuit.size, self.circuit.num_params, self.batch
+        pr = np.arange(n) if self.probes is None else np.asarray(self.probes)
+        f64 = dict(dtype=torch.float64, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.buffers = dict(x=torch.empty(B, n, **f64), status=torch.empty(B, **i32),
+                            iterations=torch.empty(B, **i32), flags=torch.empty(B, **i32),
+                            residual=torch.empty(B, **f64), params=torch.empty(B, P, **f64),
+                            waveforms=torch.empty(B, self.steps + 1, pr.size, **f64),
+                            fail_step=torch.empty(B, **i32))
+        return self
+
+    def host_result(self) -> "TransientSweepResult":
+        b = {k: v.cpu().numpy() for k, v in self.buffers.items()}
+        return TransientSweepResult(b["x"], b["status"], b["iterations"], b["flags"],
+                                    b["residual"], b["params"], b["waveforms"], b["fail_step"],
+                                    self.tstep)
+
+
+class KernelKind(enum.IntEnum):
+    Auto = A.SPX_KERNEL_AUTO
+    Dense = A.SPX_KERNEL_DENSE
+    Sparse64 = A.SPX_KERNEL_SPARSE64
+    Sparse256 = A.SPX_KERNEL_SPARSE256
+
+
+def transient_mixed_device(classes, config: A.SpxNrConfig | None = None, graph: bool = True,
+                           autotune: bool = False, kernels=None, device_id: int = 0):
+    """One launch over a mixed-size transient batch (include/spicier_hip.h,
+    spx_transient_mixed_device), enqueued on the library stream.  Classes
+    without buffers get them allocated.  Returns the kernel kind each class
+    ran with."""
+    dev = Device.get(device_id)
+    cfg = config or nr_config()
+    keep = []
+    arr = (A.SpxTranClass * max(1, len(classes)))()
+    for i, c in enumerate(classes):
+        if not c.buffers:
+            c.allocate(device_id)
+        ckt, k1 = c.circuit.to_c()
+        spec, k2 = c.circuit.sweep_spec(c.batch, c.seed, A.SPX_GEN_HASH, c.sweep_offset)
+        pr = np.ascontiguousarray(np.arange(c.circuit.size) if c.probes is None else c.probes,
+                                  dtype=np.int32)
+        keep += [ckt, k1, k2, pr]
+        b = c.buffers
+        arr[i].circuit = C.pointer(ckt)
+        arr[i].spec = spec
+        arr[i].tran = A.SpxTranConfig(tstep=c.tstep, steps=c.steps, num_probes=pr.size,
+                                      probes=A.ptr(pr))
+        arr[i].result = A.SpxTranResult(A.ptr(b["x"]), A.ptr(b["status"]),
+                                        A.ptr(b["iterations"]), A.ptr(b["flags"]),
+                                        A.ptr(b["residual"]), A.ptr(b["params"]),
+                                        A.ptr(b["waveforms"]), A.ptr(b["fail_step"]))
+    kinds = np.array([int(k) for k in (kernels or [KernelKind.Auto] * len(classes))],
+                     dtype=np.int32)
+    opts = (A.SPX_MIXED_GRAPH if graph else 0) | (A.SPX_MIXED_AUTOTUNE if autotune else 0)
+    _check(dev.lib.spx_transient_mixed_device(dev.handle, arr, len(classes), C.byref(cfg), opts,
+                                              A.ptr(kinds)))
+    return [KernelKind(int(k)) for k in kinds]
+
+
 def sample_params(circuit, batch: int, seed: int = 42, generator: int = A.SPX_GEN_HASH,
                   sweep_offset: int = 0, device_id: int = 0) -> np.ndarray:
     dev = Device.get(device_id)
