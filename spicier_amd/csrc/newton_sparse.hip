@@ -20,6 +20,9 @@
 namespace spx {
 
 constexpr int kSparseThreads = 256;
+// Set in a shape's LDS byte count when the plan's tables are staged in LDS
+// (LDS sizes stay far below this bit).
+constexpr size_t kPlanInLdsBit = size_t(1) << 40;
 
 template <int T>
 __device__ __forceinline__ bool block_all_t(bool p, int tid, int* red) {
@@ -42,31 +45,27 @@ __device__ __forceinline__ bool block_all_t(bool p, int tid, int* red) {
 template <int T, class RHS>
 __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, double* lu, RHS b,
                                                     double* y, int tid, int* red) {
-  for (int L = 0; L < sp.levels; ++L) {
+  // one level's factor work, threads [t0, t0 + stride)
+  auto factor_level = [&](int L, int t0, int stride, bool sync) {
     const int a1 = sp.pa_ptr[L + 1];
-    for (int t = sp.pa_ptr[L] + tid; t < a1; t += T) {
+    for (int t = sp.pa_ptr[L] + t0; t < a1; t += stride) {
       const int li = sp.pa_l[t];
       lu[li] = lu[li] * (1.0 / lu[sp.pa_d[t]]);
     }
-    __syncthreads();
+    if (sync) __syncthreads();
     const int b1 = sp.tb_ptr[L + 1];
-    for (int t = sp.tb_ptr[L] + tid; t < b1; t += T) {
+    for (int t = sp.tb_ptr[L] + t0; t < b1; t += stride) {
       const int ti = sp.tb_t[t];
       double v = lu[ti];
       const int c1 = sp.tb_c[t + 1];
       for (int c = sp.tb_c[t]; c < c1; ++c) v = fma(-lu[sp.cb_l[c]], lu[sp.cb_u[c]], v);
       lu[ti] = v;
     }
-    __syncthreads();
-  }
-  bool ok = true;
-  for (int k = tid; k < n; k += T) ok = ok && lu[sp.diag[k]] != 0.0;
-  if (!block_all_t<T>(ok, tid, red)) return false;
-  for (int k = tid; k < n; k += T) y[k] = b(sp.perm[k]);
-  __syncthreads();
-  for (int L = 0; L < sp.levels; ++L) {  // L y = P b
+    if (sync) __syncthreads();
+  };
+  auto forward_level = [&](int L, int t0, int stride) {
     const int r1 = sp.lev_ptr[L + 1];
-    for (int r = sp.lev_ptr[L] + tid; r < r1; r += T) {
+    for (int r = sp.lev_ptr[L] + t0; r < r1; r += stride) {
       const int i = sp.lev_rows[r];
       const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
       const double* li = lu + sp.diag[i] - nl;
@@ -74,11 +73,10 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
       for (int t = 0; t < nl; ++t) v = fma(-li[t], y[sp.ut_row[u0 + t]], v);
       y[i] = v;
     }
-    __syncthreads();
-  }
-  for (int L = 0; L < sp.blevels; ++L) {  // U x = y
+  };
+  auto backward_level = [&](int L, int t0, int stride) {
     const int r1 = sp.blev_ptr[L + 1];
-    for (int r = sp.blev_ptr[L] + tid; r < r1; r += T) {
+    for (int r = sp.blev_ptr[L] + t0; r < r1; r += stride) {
       const int i = sp.blev_rows[r];
       const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
       const double* ui = lu + sp.diag[i];
@@ -86,8 +84,42 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
       for (int t = 0; t < ns; ++t) v = fma(-ui[1 + t], y[sp.s_row[s0 + t]], v);
       y[i] = v * (1.0 / ui[0]);
     }
-    __syncthreads();
-  }
+  };
+  // Runs levels [0, count) of one pass: a run of levels flagged narrow goes
+  // to thread 0 in order (one barrier after the run), any other level to
+  // the whole workgroup (barriers inside `wide`).
+  auto pass = [&](int count, const int* flags, int bit, auto narrow_fn, auto wide_fn) {
+    for (int L = 0; L < count;) {
+      if (flags[L] & bit) {
+        int L1 = L + 1;
+        while (L1 < count && (flags[L1] & bit)) ++L1;
+        if (tid == 0)
+          for (int l = L; l < L1; ++l) narrow_fn(l);
+        __syncthreads();
+        L = L1;
+      } else {
+        wide_fn(L);
+        ++L;
+      }
+    }
+  };
+  pass(sp.levels, sp.lev_flags, 1, [&](int l) { factor_level(l, 0, 1, false); },
+       [&](int l) { factor_level(l, tid, T, true); });
+  bool ok = true;
+  for (int k = tid; k < n; k += T) ok = ok && lu[sp.diag[k]] != 0.0;
+  if (!block_all_t<T>(ok, tid, red)) return false;
+  for (int k = tid; k < n; k += T) y[k] = b(sp.perm[k]);
+  __syncthreads();
+  pass(sp.levels, sp.lev_flags, 2, [&](int l) { forward_level(l, 0, 1); },
+       [&](int l) {
+         forward_level(l, tid, T);
+         __syncthreads();
+       });
+  pass(sp.blevels, sp.blev_flags, 1, [&](int l) { backward_level(l, 0, 1); },
+       [&](int l) {
+         backward_level(l, tid, T);
+         __syncthreads();
+       });
   bool fin = true;
   for (int k = tid; k < n; k += T) fin = fin && isfinite(y[k]);
   return block_all_t<T>(fin, tid, red);
@@ -330,7 +362,8 @@ inline hipError_t grid_for(const void* kernel, size_t smem, int64_t B, int* grid
 template <int T>
 __global__ __launch_bounds__(T) void k_transient_sparse(
     DevProgram g, DevSparse sp, DevNrCfg cfg, DevTran tr, const double* __restrict__ params,
-    int64_t B, DevOut out, DevTranOut tout, double* ws, int64_t slot, int lu_in_lds) {
+    int64_t B, DevOut out, DevTranOut tout, double* ws, int64_t slot, int lu_in_lds,
+    int plan_in_lds) {
   extern __shared__ double smem[];
   const int n = g.n, tid = threadIdx.x;
   double* rhs = smem;
@@ -347,6 +380,14 @@ __global__ __launch_bounds__(T) void k_transient_sparse(
   double* av = vst + g.num_elems;
   double* vcap = av + g.num_pos;
   double* lu = lu_in_lds ? lds_lu : vcap + g.num_elems;
+  // small plans are read from LDS: every elimination level reads its index
+  // tables, and from global memory each level would cost L2 round trips
+  if (plan_in_lds) {
+    int* lp = reinterpret_cast<int*>(lu_in_lds ? lds_lu + sp.nnz_lu : lds_lu);
+    for (int i = tid; i < sp.blob_ints; i += T) lp[i] = sp.blob[i];
+    sp = rebase_sparse(sp, lp);
+    __syncthreads();
+  }
   for (int64_t inst = blockIdx.x; inst < B; inst += gridDim.x) {
     if (tid == 0) {
       vals[kValOne] = 1.0;
@@ -372,11 +413,15 @@ hipError_t sparse_transient_shape_t(const DevProgram& g, const DevSparse& sp, in
   *lu_in_lds = force_lu >= 0 ? force_lu : (sparse_smem_bytes(g, sp, true, T) <= lds_budget ? 1 : 0);
   if (*lu_in_lds && sparse_smem_bytes(g, sp, true, T) > 160 * 1024) *lu_in_lds = 0;
   *smem = sparse_smem_bytes(g, sp, *lu_in_lds, T);
+  // the plan's tables follow in LDS while the whole stays within the budget
+  const size_t plan_bytes = (size_t)sp.blob_ints * sizeof(int);
+  if (*smem + plan_bytes <= lds_budget) *smem = (*smem + plan_bytes) | kPlanInLdsBit;
   *slot = sparse_slot_doubles(g, sp, *lu_in_lds) + ((g.num_elems + 31) & ~31);
+  const size_t bytes = *smem & ~kPlanInLdsBit;
   hipError_t e = hipFuncSetAttribute((const void*)k_transient_sparse<T>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)*smem);
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return e;
-  return grid_for((const void*)k_transient_sparse<T>, *smem, B, grid, T);
+  return grid_for((const void*)k_transient_sparse<T>, bytes, B, grid, T);
 }
 
 // Shape of a sparse transient launch with `threads` (64 or 256) per
@@ -396,12 +441,14 @@ hipError_t launch_transient_sparse(const DevProgram& g, const DevSparse& sp, con
                                    int grid, int64_t slot, size_t smem, int lu_in_lds,
                                    int threads, hipStream_t st) {
   if (B == 0) return hipSuccess;
+  const int plan = (smem & kPlanInLdsBit) ? 1 : 0;
+  smem &= ~kPlanInLdsBit;
   if (threads == 64)
     hipLaunchKernelGGL(k_transient_sparse<64>, dim3((unsigned)grid), dim3(64), smem, st, g, sp,
-                       cfg, tr, params, B, out, tout, ws, slot, lu_in_lds);
+                       cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
   else
     hipLaunchKernelGGL(k_transient_sparse<256>, dim3((unsigned)grid), dim3(256), smem, st, g, sp,
-                       cfg, tr, params, B, out, tout, ws, slot, lu_in_lds);
+                       cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
   return hipGetLastError();
 }
 

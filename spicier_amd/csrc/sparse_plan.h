@@ -25,8 +25,10 @@
 //      target, each target summing its contributions in column order — no
 //      atomics, and the result is deterministic.  The triangular solves are
 //      row-oriented over the same levels (forward) and over levels counted
-//      from the root (backward).  A 32x32 grid has 1,028 columns in 113
-//      levels.
+//      from the root (backward).  A 32x32 grid has 1,028 columns in 118
+//      levels.  A level whose work is a handful of operations (a chain in
+//      the tree: a ladder's is all chain) is marked narrow; a run of narrow
+//      levels is done by one thread in order, with no barrier inside.
 // The numeric factorisation is Gaussian elimination without pivoting in
 // this order; multipliers are formed as a * (1 / pivot) and updates as
 // fma(-l, u, a).
@@ -61,9 +63,44 @@ struct DevSparse {
   const int* cb_u;     // contribution U(k, b)
   const int* blev_ptr; // blevels + 1 into blev_rows
   const int* blev_rows;
+  const int* lev_flags;   // levels: bit 0 the level's factor work is narrow, bit 1 its forward rows
+  const int* blev_flags;  // blevels: bit 0 the level's backward rows are narrow
   const int* pos_lu;   // num_pos: LU index of pattern entry q (circuit program order)
   const int* fill;     // num_fill LU indices to zero before scattering
+  const int* blob;     // the plan's one int block: every pointer above points into it
+  int blob_ints;
 };
+
+// The plan with every table moved from `blob` to `to` (a copy of the block,
+// e.g. in LDS).
+__host__ __device__ inline DevSparse rebase_sparse(const DevSparse& s, const int* to) {
+  DevSparse d = s;
+  auto mv = [&](const int* p) { return to + (p - s.blob); };
+  d.perm = mv(s.perm);
+  d.diag = mv(s.diag);
+  d.col_ptr = mv(s.col_ptr);
+  d.s_row = mv(s.s_row);
+  d.ut_ptr = mv(s.ut_ptr);
+  d.ut_row = mv(s.ut_row);
+  d.lev_ptr = mv(s.lev_ptr);
+  d.lev_rows = mv(s.lev_rows);
+  d.pa_ptr = mv(s.pa_ptr);
+  d.pa_l = mv(s.pa_l);
+  d.pa_d = mv(s.pa_d);
+  d.tb_ptr = mv(s.tb_ptr);
+  d.tb_t = mv(s.tb_t);
+  d.tb_c = mv(s.tb_c);
+  d.cb_l = mv(s.cb_l);
+  d.cb_u = mv(s.cb_u);
+  d.blev_ptr = mv(s.blev_ptr);
+  d.blev_rows = mv(s.blev_rows);
+  d.lev_flags = mv(s.lev_flags);
+  d.blev_flags = mv(s.blev_flags);
+  d.pos_lu = mv(s.pos_lu);
+  d.fill = mv(s.fill);
+  d.blob = to;
+  return d;
+}
 
 // Host form of the plan: one int blob plus offsets; `build_sparse_plan`
 // fills it from the circuit pattern.  Returns an empty string on success,
@@ -75,7 +112,7 @@ struct HostSparsePlan {
   size_t o_perm = 0, o_diag = 0, o_colptr = 0, o_srow = 0, o_utptr = 0, o_utrow = 0,
          o_levptr = 0, o_levrows = 0, o_paptr = 0, o_pal = 0, o_pad = 0, o_tbptr = 0,
          o_tbt = 0, o_tbc = 0, o_cbl = 0, o_cbu = 0, o_blevptr = 0, o_blevrows = 0,
-         o_poslu = 0, o_fill = 0;
+         o_poslu = 0, o_fill = 0, o_levflags = 0, o_blevflags = 0;
 };
 
 // n unknowns, the first num_nodes of them node voltages; pattern entries

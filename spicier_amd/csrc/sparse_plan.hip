@@ -169,6 +169,21 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
     for (int k : by_blev[L]) blev_rows.push_back(k);
     blev_ptr.push_back((int)blev_rows.size());
   }
+  // narrow levels: serial cost (operations one thread would do) small enough
+  // that doing them in order beats two workgroup barriers
+  constexpr int kNarrow = 12;
+  std::vector<int> lev_flags(levels, 0), blev_flags(blevels, 0);
+  for (int L = 0; L < levels; ++L) {
+    const int fa = (pa_ptr[L + 1] - pa_ptr[L]) + (tb_c[tb_ptr[L + 1]] - tb_c[tb_ptr[L]]);
+    int fw = 0;
+    for (int r = lev_ptr[L]; r < lev_ptr[L + 1]; ++r) fw += 1 + (int)Lrow[lev_rows[r]].size();
+    lev_flags[L] = (fa <= kNarrow ? 1 : 0) | (fw <= kNarrow ? 2 : 0);
+  }
+  for (int L = 0; L < blevels; ++L) {
+    int bw = 0;
+    for (int r = blev_ptr[L]; r < blev_ptr[L + 1]; ++r) bw += 1 + (int)S[blev_rows[r]].size();
+    blev_flags[L] = bw <= kNarrow ? 1 : 0;
+  }
   std::vector<char> covered(nnz_lu, 0);
   for (int rc : pos_rc) {
     const int t = idx(iperm[rc & 0xffff], iperm[rc >> 16]);
@@ -213,6 +228,8 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
   plan.o_blevrows = put(blev_rows);
   plan.o_poslu = put(pos_lu);
   plan.o_fill = put(fill);
+  plan.o_levflags = put(lev_flags);
+  plan.o_blevflags = put(blev_flags);
   plan.blob.push_back(0);  // keeps every offset a valid address
   return "";
 }
@@ -243,6 +260,10 @@ DevSparse device_sparse(const HostSparsePlan& p, const int* base) {
   d.blev_rows = base + p.o_blevrows;
   d.pos_lu = base + p.o_poslu;
   d.fill = base + p.o_fill;
+  d.lev_flags = base + p.o_levflags;
+  d.blev_flags = base + p.o_blevflags;
+  d.blob = base;
+  d.blob_ints = (int)p.blob.size();
   return d;
 }
 
