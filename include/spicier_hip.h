@@ -361,14 +361,16 @@ int spx_transient_sweep_device(spx_handle* h, const spx_circuit* ckt, const spx_
 /* Mixed-size transient batch: several circuits (size classes), each with its
  * own sweep spec, transient settings and device result buffers, run as one
  * launch.  Each class runs on a persistent kernel shaped for it: the
- * register-LU kernel (n <= 64) or the shared-symbolic sparse kernel with 64
- * or 256 threads per instance.  With SPX_MIXED_GRAPH every class's sampling
+ * register-LU kernel (n <= 64), the shared-symbolic sparse kernel with 64
+ * or 256 threads per instance, or the same plan with one lane per instance
+ * (chain-like elimination trees).  With SPX_MIXED_GRAPH every class's sampling
  * and transient kernel is captured into one hipGraph on the first call for a
  * batch description and replayed by later calls with the same description
  * (buffers, sizes, seeds, settings).  With SPX_MIXED_AUTOTUNE a class whose
  * kernel is SPX_KERNEL_AUTO is timed once on a sample (up to 1,024
- * instances, 20 steps) with every candidate kernel, and the fastest is
- * kept for that circuit on this handle.  `kernels` (may be NULL): per class,
+ * instances, 20 steps) with every candidate kernel (register LU when n <= 64,
+ * sparse 64, sparse 256, lanes), and the fastest is kept for that circuit on
+ * this handle.  `kernels` (may be NULL): per class,
  * in = the requested kind, out = the kind that ran.  No counterpart in the
  * reference, whose transient runs one circuit at a time on the host. */
 typedef struct spx_tran_class {
@@ -383,7 +385,8 @@ enum {
     SPX_KERNEL_AUTO = 0,      /* register LU if n <= 64, else sparse 256 (or the tuned choice) */
     SPX_KERNEL_DENSE = 1,     /* register LU, one wave per instance (n <= 64) */
     SPX_KERNEL_SPARSE64 = 2,  /* shared-symbolic LU, 64 threads per instance */
-    SPX_KERNEL_SPARSE256 = 3  /* shared-symbolic LU, 256 threads per instance */
+    SPX_KERNEL_SPARSE256 = 3, /* shared-symbolic LU, 256 threads per instance */
+    SPX_KERNEL_LANES = 4      /* shared-symbolic LU, one lane per instance, 64 per wave */
 };
 
 int spx_transient_mixed_device(spx_handle* h, const spx_tran_class* classes, int32_t num_classes,

@@ -113,6 +113,7 @@ class SpxTranClass(C.Structure):
 
 SPX_MIXED_GRAPH, SPX_MIXED_AUTOTUNE = 1, 2
 SPX_KERNEL_AUTO, SPX_KERNEL_DENSE, SPX_KERNEL_SPARSE64, SPX_KERNEL_SPARSE256 = 0, 1, 2, 3
+SPX_KERNEL_LANES = 4
 
 
 def default_nr_config(**overrides) -> SpxNrConfig:

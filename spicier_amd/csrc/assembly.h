@@ -26,52 +26,58 @@ __device__ __forceinline__ double eval_pulse(double v1, double v2, double td, do
 // lives in vst[e]; `lim` is set when a pnjlim limiter acted.  tc is the
 // transient context (null for a DC sweep, whose program has no capacitor
 // slots).
+//
+// ST is the stride of every per-instance array (xs, pv, vals, vst and the
+// transient's vprev): 1 when one workgroup owns an instance, 64 when the 64
+// lanes of a wave are 64 instances with their values interleaved.
+template <int ST = 1>
 __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const double* xs,
                                              const double* pv, double* vals, double* vst,
                                              double sf, int limiter, bool& lim,
                                              const TranCtx* tc = nullptr) {
   const int* el = g.elems + e * kElemInts;
   const int kind = el[0], vid = el[11];
-  auto X = [&](int u) { return u >= 0 ? xs[u] : 0.0; };
+  auto X = [&](int u) { return u >= 0 ? xs[u * ST] : 0.0; };
+  auto PV = [&](int k) { return pv[el[k] * ST]; };
   switch (kind) {
     case SPX_EL_RESISTOR:
-      vals[vid] = 1.0 / pv[el[4]];
+      vals[vid * ST] = 1.0 / PV(4);
       break;
     case SPX_EL_CAPACITOR:  // companion.rs:41-48
       if (vid >= 0) {
-        const double geq = (tc && tc->mode == 1) ? pv[el[4]] / tc->h : 0.0;
-        vals[vid] = geq;
-        vals[vid + 1] = (tc && tc->mode == 1) ? geq * tc->vprev[e] : 0.0;
+        const double geq = (tc && tc->mode == 1) ? PV(4) / tc->h : 0.0;
+        vals[vid * ST] = geq;
+        vals[(vid + 1) * ST] = (tc && tc->mode == 1) ? geq * tc->vprev[e * ST] : 0.0;
       }
       break;
     case SPX_EL_VPULSE:
-      vals[vid] = ((tc && tc->mode == 1)
-                       ? eval_pulse(pv[el[4]], pv[el[5]], pv[el[6]], pv[el[7]], pv[el[8]],
-                                    pv[el[9]], pv[el[10]], tc->t)
-                       : pv[el[4]]) *
-                  sf;
+      vals[vid * ST] =
+          ((tc && tc->mode == 1) ? eval_pulse(PV(4), PV(5), PV(6), PV(7), PV(8), PV(9), PV(10),
+                                              tc->t)
+                                 : PV(4)) *
+          sf;
       break;
     case SPX_EL_VSOURCE:
     case SPX_EL_ISOURCE:
-      vals[vid] = pv[el[4]] * sf;
+      vals[vid * ST] = PV(4) * sf;
       break;
     case SPX_EL_DIODE: {
       double vd = X(el[1]) - X(el[2]);
       double gd, ieq;
-      diode_companion(pv[el[4]], pv[el[5]], vd, limiter, vst[e], lim, gd, ieq);
-      vals[vid] = gd;
-      vals[vid + 1] = ieq;
+      diode_companion(PV(4), PV(5), vd, limiter, vst[e * ST], lim, gd, ieq);
+      vals[vid * ST] = gd;
+      vals[(vid + 1) * ST] = ieq;
       break;
     }
     case SPX_EL_NMOS:
     case SPX_EL_PMOS: {
       double vd = X(el[1]), vg = X(el[2]), vs = X(el[3]);
       double gds, gm, ieq;
-      mosfet_companion(kind == SPX_EL_PMOS, pv[el[4]], pv[el[5]], pv[el[6]], pv[el[7]],
-                       pv[el[8]], vg - vs, vd - vs, gds, gm, ieq);
-      vals[vid] = gds;
-      vals[vid + 1] = gm;
-      vals[vid + 2] = ieq;
+      mosfet_companion(kind == SPX_EL_PMOS, PV(4), PV(5), PV(6), PV(7), PV(8), vg - vs, vd - vs,
+                       gds, gm, ieq);
+      vals[vid * ST] = gds;
+      vals[(vid + 1) * ST] = gm;
+      vals[(vid + 2) * ST] = ieq;
       break;
     }
     default:
@@ -80,13 +86,14 @@ __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const d
 }
 
 // Sum of entry q's contributions (value slot << 1 | negate), in push order.
+template <int ST = 1>
 __device__ __forceinline__ double gather_sum(const int* off, const int* contrib, int q,
                                              const double* vals) {
   double sum = 0.0;
   const int c1 = off[q + 1];
   for (int c = off[q]; c < c1; ++c) {
     const int t = contrib[c];
-    const double v = vals[t >> 1];
+    const double v = vals[(t >> 1) * ST];
     sum = (t & 1) ? sum - v : sum + v;
   }
   return sum;

@@ -53,6 +53,12 @@ hipError_t launch_transient_sparse(const DevProgram& g, const DevSparse& sp, con
                                    const DevOut& out, const DevTranOut& tout, double* ws,
                                    int grid, int64_t slot, size_t smem, int lu_in_lds,
                                    int threads, hipStream_t st);
+hipError_t lanes_transient_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
+                                 int64_t* slot);
+hipError_t launch_transient_lanes(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
+                                  const DevTran& tr, const double* params, int64_t B,
+                                  const DevOut& out, const DevTranOut& tout, double* ws, int grid,
+                                  int64_t slot, hipStream_t st);
 hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
                                int64_t* slot, size_t* smem, int* lu_in_lds);
 hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
@@ -751,10 +757,16 @@ int shape_class(spx_handle* h, MixedClass& mc, int kernel, int64_t B, const DevN
     mc.ws = nullptr;
     return SPX_OK;
   }
-  const int threads = kernel == SPX_KERNEL_SPARSE64 ? 64 : 256;
-  SPX_HIP(sparse_transient_shape(mc.g, mc.sp, B, &mc.grid, &mc.slot, &mc.smem, &mc.lu_lds,
-                                 threads, -1),
-          SPX_ERR_BACKEND);
+  if (kernel == SPX_KERNEL_LANES) {
+    SPX_HIP(lanes_transient_shape(mc.g, mc.sp, B, &mc.grid, &mc.slot), SPX_ERR_BACKEND);
+    mc.smem = 0;
+    mc.lu_lds = 0;
+  } else {
+    const int threads = kernel == SPX_KERNEL_SPARSE64 ? 64 : 256;
+    SPX_HIP(sparse_transient_shape(mc.g, mc.sp, B, &mc.grid, &mc.slot, &mc.smem, &mc.lu_lds,
+                                   threads, -1),
+            SPX_ERR_BACKEND);
+  }
   void* ws = nullptr;
   if ((rc = workspace(h, wsname.c_str(), sizeof(double) * (size_t)mc.grid * mc.slot, &ws)))
     return rc;
@@ -765,6 +777,9 @@ int shape_class(spx_handle* h, MixedClass& mc, int kernel, int64_t B, const DevN
 hipError_t enqueue_class(const MixedClass& mc, const DevNrCfg& dc, const DevTran& tr,
                          const DevOut& o, const DevTranOut& to, int64_t B, hipStream_t st) {
   if (mc.kernel == SPX_KERNEL_DENSE) return dense_tran(mc.hp.n)(mc.g, dc, tr, mc.params, B, o, to, st, 2);
+  if (mc.kernel == SPX_KERNEL_LANES)
+    return launch_transient_lanes(mc.g, mc.sp, dc, tr, mc.params, B, o, to, mc.ws, mc.grid,
+                                  mc.slot, st);
   const int threads = mc.kernel == SPX_KERNEL_SPARSE64 ? 64 : 256;
   return launch_transient_sparse(mc.g, mc.sp, dc, tr, mc.params, B, o, to, mc.ws, mc.grid,
                                  mc.slot, mc.smem, mc.lu_lds, threads, st);
@@ -789,6 +804,7 @@ int autotune_class(spx_handle* h, MixedClass& mc, const DevNrCfg& dc, int* best)
   if (mc.hp.n <= kDenseMaxN) cands.push_back(SPX_KERNEL_DENSE);
   cands.push_back(SPX_KERNEL_SPARSE64);
   cands.push_back(SPX_KERNEL_SPARSE256);
+  cands.push_back(SPX_KERNEL_LANES);
   hipEvent_t e0, e1;
   SPX_HIP(hipEventCreate(&e0), SPX_ERR_BACKEND);
   SPX_HIP(hipEventCreate(&e1), SPX_ERR_BACKEND);
@@ -845,7 +861,7 @@ int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_
     if ((rc = check_spec(&c.spec, mc.hp.P))) return rc;
     if ((rc = check_tran(&c.tran, mc.hp.n))) return rc;
     int req = kernels ? kernels[i] : SPX_KERNEL_AUTO;
-    if (req < SPX_KERNEL_AUTO || req > SPX_KERNEL_SPARSE256 ||
+    if (req < SPX_KERNEL_AUTO || req > SPX_KERNEL_LANES ||
         (req == SPX_KERNEL_DENSE && mc.hp.n > kDenseMaxN))
       return fail(SPX_ERR_INVALID_ARGUMENT, "class " + std::to_string(i) + ": bad kernel kind");
     mc.hp.want_sparse = true;

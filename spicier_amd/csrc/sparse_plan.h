@@ -4,10 +4,14 @@
 // Every instance of a sweep has the same sparsity pattern, so the host
 // analyses it once (FaerSparseCachedBatchedSolver's symbolic-once design,
 // crates/spicier-batched-sweep/src/faer_sparse_solver.rs:116-137):
-//   1. ordering: minimum degree over the node unknowns of the symmetrised
-//      pattern, ties to the lowest index, and each voltage-source branch
-//      unknown (an unknown past num_nodes, or one without a diagonal entry
-//      in the pattern) eliminated right after the first of its nodes.  The branch
+//   1. ordering: minimum degree or nested dissection (recursive
+//      breadth-first bisection) over the node unknowns of the symmetrised
+//      pattern, whichever a cost model of critical path (elimination levels)
+//      and update work rates cheaper — a grid keeps minimum degree, a
+//      ladder takes nested dissection (16 levels instead of a 256-long
+//      chain) — and each voltage-source branch unknown (an unknown past
+//      num_nodes, or one without a diagonal entry in the pattern)
+//      eliminated right after the first of its nodes.  The branch
 //      row's zero diagonal has then been filled (-1/G of that node), and the
 //      node/branch pair acts as a Dirichlet condition on the remaining
 //      nodes: a grid fed only through ideal sources keeps a nonsingular
@@ -107,6 +111,7 @@ __host__ __device__ inline DevSparse rebase_sparse(const DevSparse& s, const int
 // else the reason the pattern cannot use the sparse path.
 struct HostSparsePlan {
   int nnz_lu = 0, max_s = 0, num_fill = 0, levels = 0, blevels = 0, max_level_rows = 0;
+  int ordering = 0;  // 0 minimum degree, 1 nested dissection
   int64_t num_upd = 0;
   std::vector<int> blob;
   size_t o_perm = 0, o_diag = 0, o_colptr = 0, o_srow = 0, o_utptr = 0, o_utrow = 0,

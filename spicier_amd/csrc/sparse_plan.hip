@@ -1,6 +1,7 @@
 // sparse_plan.hip — host-side symbolic analysis for the shared-pattern LU
 // (sparse_plan.h).  Runs once per circuit pattern.
 #include <algorithm>
+#include <functional>
 #include <map>
 #include <set>
 #include <unordered_map>
@@ -14,31 +15,45 @@ namespace {
 constexpr int64_t kMaxUpdates = int64_t(1) << 24;  // 128 MB of update contributions
 }
 
-const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_rc,
-                              HostSparsePlan& plan) {
-  // symmetrised adjacency without the diagonal
-  std::vector<std::set<int>> adj(n);
-  for (int rc : pos_rc) {
-    const int r = rc & 0xffff, c = rc >> 16;
-    if (r != c) {
-      adj[r].insert(c);
-      adj[c].insert(r);
+namespace {
+
+using Adj = std::vector<std::set<int>>;
+
+// Elimination in `order`: perm/iperm and S_k (elimination indices, sorted).
+void eliminate_in(const Adj& adj0, const std::vector<int>& order, std::vector<int>& perm,
+                  std::vector<int>& iperm, std::vector<std::vector<int>>& S) {
+  const int n = (int)adj0.size();
+  Adj adj = adj0;
+  perm = order;
+  iperm.assign(n, -1);
+  for (int k = 0; k < n; ++k) iperm[order[k]] = k;
+  S.assign(n, {});
+  for (int k = 0; k < n; ++k) {
+    const int v = order[k];
+    std::vector<int> nb(adj[v].begin(), adj[v].end());
+    for (int u : nb) S[k].push_back(iperm[u]);
+    std::sort(S[k].begin(), S[k].end());
+    for (int a : nb) {
+      adj[a].erase(v);
+      for (int b : nb)
+        if (a != b) adj[a].insert(b);
     }
+    adj[v].clear();
   }
-  // Minimum degree on the elimination graph of the plain unknowns; a branch
-  // unknown follows the first of its neighbours.  The neighbour set of a
-  // vertex when it is eliminated is its column structure S_k.
-  std::vector<int> perm, iperm(n, -1);
-  std::vector<std::vector<int>> S_orig(n);  // by elimination step, original indices
-  perm.reserve(n);
+}
+
+// Minimum degree over the plain unknowns, ties to the lowest index, each
+// branch unknown right after its first neighbour.
+std::vector<int> md_order(const Adj& adj0, const std::vector<char>& branch,
+                          const std::vector<std::vector<int>>& branches_of) {
+  const int n = (int)adj0.size();
+  Adj adj = adj0;
   std::vector<char> done(n, 0);
-  auto eliminate = [&](int v) {
-    const int k = (int)perm.size();
-    perm.push_back(v);
-    iperm[v] = k;
+  std::vector<int> order;
+  auto elim = [&](int v) {
+    order.push_back(v);
     done[v] = 1;
     std::vector<int> nb(adj[v].begin(), adj[v].end());
-    S_orig[k] = nb;
     for (int a : nb) {
       adj[a].erase(v);
       for (int b : nb)
@@ -46,6 +61,146 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
     }
     adj[v].clear();
   };
+  int num_plain = 0;
+  for (int v = 0; v < n; ++v) num_plain += !branch[v];
+  for (int step = 0; step < num_plain; ++step) {
+    int best = -1;
+    size_t bdeg = 0;
+    for (int v = 0; v < n; ++v)
+      if (!done[v] && !branch[v] && (best < 0 || adj[v].size() < bdeg)) {
+        best = v;
+        bdeg = adj[v].size();
+      }
+    elim(best);
+    for (int b : branches_of[best])
+      if (!done[b]) elim(b);
+  }
+  for (int v = 0; v < n; ++v)
+    if (!done[v]) elim(v);
+  return order;
+}
+
+// Nested dissection over the plain unknowns (recursive bisection at the
+// middle breadth-first level from a pseudo-peripheral vertex), then each
+// branch unknown right after its first neighbour.  A path or ladder becomes
+// a cyclic-reduction order: an elimination tree of depth ~2 log2 n instead
+// of the chain of length n that minimum degree yields.
+std::vector<int> nd_order(const Adj& adj0, const std::vector<char>& branch,
+                          const std::vector<std::vector<int>>& branches_of) {
+  const int n = (int)adj0.size();
+  std::vector<int> plain_order;
+  std::vector<int> mark(n, -1), level(n, -1);
+  int stamp = 0;
+  auto bfs = [&](int src, int tag, std::vector<int>& seen) {
+    seen.clear();
+    seen.push_back(src);
+    level[src] = 0;
+    mark[src] = tag + 1;  // visited in this BFS
+    for (size_t h = 0; h < seen.size(); ++h) {
+      const int v = seen[h];
+      for (int u : adj0[v])
+        if (mark[u] == tag) {
+          mark[u] = tag + 1;
+          level[u] = level[v] + 1;
+          seen.push_back(u);
+        }
+    }
+  };
+  std::function<void(std::vector<int>&)> rec = [&](std::vector<int>& V) {
+    if (V.empty()) return;
+    if (V.size() <= 8) {
+      plain_order.insert(plain_order.end(), V.begin(), V.end());
+      return;
+    }
+    // BFS twice from V[0] restricted to V: pseudo-peripheral start
+    const int t0 = stamp;
+    stamp += 4;
+    for (int v : V) mark[v] = t0;
+    std::vector<int> seen;
+    bfs(V[0], t0, seen);
+    int far = seen.back();
+    std::vector<int> rest;  // other components of V
+    for (int v : V)
+      if (mark[v] == t0) rest.push_back(v);
+    for (int v : seen) mark[v] = t0 + 2;
+    bfs(far, t0 + 2, seen);
+    const int nlev = level[seen.back()] + 1;
+    std::vector<int> A, B, Sep;
+    if (nlev >= 3) {
+      std::vector<int> cnt(nlev, 0);
+      for (int v : seen) cnt[level[v]]++;
+      int acc = 0, sl = 0;
+      for (; sl < nlev; ++sl) {
+        acc += cnt[sl];
+        if (2 * acc >= (int)seen.size()) break;
+      }
+      if (sl == 0) sl = 1;
+      if (sl >= nlev - 1) sl = nlev - 2;
+      for (int v : seen) (level[v] < sl ? A : level[v] > sl ? B : Sep).push_back(v);
+      std::sort(A.begin(), A.end());
+      std::sort(B.begin(), B.end());
+      std::sort(Sep.begin(), Sep.end());
+      rec(A);
+      rec(B);
+      plain_order.insert(plain_order.end(), Sep.begin(), Sep.end());
+    } else {
+      std::sort(seen.begin(), seen.end());
+      plain_order.insert(plain_order.end(), seen.begin(), seen.end());
+    }
+    rec(rest);
+  };
+  std::vector<int> plain;
+  for (int v = 0; v < n; ++v)
+    if (!branch[v]) plain.push_back(v);
+  rec(plain);
+  std::vector<char> placed(n, 0);
+  std::vector<int> order;
+  for (int v : plain_order) {
+    order.push_back(v);
+    placed[v] = 1;
+    for (int b : branches_of[v])
+      if (!placed[b]) {
+        order.push_back(b);
+        placed[b] = 1;
+      }
+  }
+  for (int v = 0; v < n; ++v)
+    if (!placed[v]) order.push_back(v);
+  return order;
+}
+
+// Critical path and work of an ordering, in units of one dependent update:
+// every elimination level costs a fixed latency (barriers, index loads) in
+// each of the factorisation and the two solves, plus its updates, which
+// stream through L2 at about 16 per level-latency unit (fitted on MI355X to
+// the 32x32 grid, where minimum degree wins, and the ladders, where nested
+// dissection does).
+double plan_cost(const std::vector<std::vector<int>>& S) {
+  const int n = (int)S.size();
+  std::vector<int> lev(n, 0);
+  int levels = 0;
+  double work = 0.0;
+  for (int k = 0; k < n; ++k) {
+    levels = std::max(levels, lev[k] + 1);
+    for (int i : S[k]) lev[i] = std::max(lev[i], lev[k] + 1);
+    work += (double)S[k].size() * (double)S[k].size() + 2.0 * S[k].size();
+  }
+  return 3.0 * 24.0 * levels + work / 16.0;
+}
+
+}  // namespace
+
+const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_rc,
+                              HostSparsePlan& plan) {
+  // symmetrised adjacency without the diagonal
+  Adj adj(n);
+  for (int rc : pos_rc) {
+    const int r = rc & 0xffff, c = rc >> 16;
+    if (r != c) {
+      adj[r].insert(c);
+      adj[c].insert(r);
+    }
+  }
   // "Branch" unknowns: index >= num_nodes, or no diagonal entry in the
   // pattern (an MNA voltage-source row seen through a plain matrix).  Each
   // follows the first of its neighbours to be eliminated.
@@ -58,30 +213,26 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
     if (branch[b])
       for (int v : adj[b])
         if (!branch[v]) branches_of[v].push_back(b);
-  int num_plain = 0;
-  for (int v = 0; v < n; ++v) num_plain += !branch[v];
-  for (int step = 0; step < num_plain; ++step) {
-    int best = -1;
-    size_t bdeg = 0;
-    for (int v = 0; v < n; ++v)
-      if (!done[v] && !branch[v] && (best < 0 || adj[v].size() < bdeg)) {
-        best = v;
-        bdeg = adj[v].size();
-      }
-    eliminate(best);
-    for (int b : branches_of[best])
-      if (!done[b]) eliminate(b);
+  // two orderings, the cheaper kept
+  std::vector<int> perm, iperm;
+  std::vector<std::vector<int>> S;
+  eliminate_in(adj, md_order(adj, branch, branches_of), perm, iperm, S);
+  {
+    std::vector<int> perm2, iperm2;
+    std::vector<std::vector<int>> S2;
+    eliminate_in(adj, nd_order(adj, branch, branches_of), perm2, iperm2, S2);
+    if (plan_cost(S2) < plan_cost(S)) {
+      perm.swap(perm2);
+      iperm.swap(iperm2);
+      S.swap(S2);
+      plan.ordering = 1;
+    } else {
+      plan.ordering = 0;
+    }
   }
-  for (int v = 0; v < n; ++v)
-    if (!done[v]) eliminate(v);
-
-  // S_k in elimination indices, sorted
-  std::vector<std::vector<int>> S(n);
   int64_t nnz_l = 0, num_upd = 0;
   int max_s = 0;
   for (int k = 0; k < n; ++k) {
-    for (int v : S_orig[k]) S[k].push_back(iperm[v]);
-    std::sort(S[k].begin(), S[k].end());
     for (int i : S[k])
       if (i <= k) return "symbolic factorisation produced an entry above the diagonal";
     nnz_l += (int64_t)S[k].size();

@@ -639,6 +639,7 @@ class KernelKind(enum.IntEnum):
     Dense = A.SPX_KERNEL_DENSE
     Sparse64 = A.SPX_KERNEL_SPARSE64
     Sparse256 = A.SPX_KERNEL_SPARSE256
+    Lanes = A.SPX_KERNEL_LANES
 
 
 def transient_mixed_device(classes, config: A.SpxNrConfig | None = None, graph: bool = True,

@@ -63,9 +63,9 @@ def test_graph_replays_equal_eager():
 
 
 @pytest.mark.parametrize("kind", [E.KernelKind.Dense, E.KernelKind.Sparse64,
-                                  E.KernelKind.Sparse256])
+                                  E.KernelKind.Sparse256, E.KernelKind.Lanes])
 def test_every_kernel_kind_matches_oracle(oracle, kind):
-    c = E.TranClass(CK.ladder(48, seed=4), 40, 1e-12, 50, probes=[0, 10, 46], seed=3)
+    c = E.TranClass(CK.ladder(48, seed=4), 100, 1e-12, 50, probes=[0, 10, 46], seed=3)
     got_kinds = E.transient_mixed_device([c], _cfg(), graph=False, kernels=[kind])
     E.Device.get(0).synchronize()
     assert got_kinds == [kind]

@@ -143,9 +143,16 @@ def test_sparse_solve_matches_lapack():
     n, rows, cols, vals = _grid_mna(14, rng, B)   # duplicates: every diagonal is summed
     b = rng.standard_normal((B, n))
     r = E.HipSparseBatchedSolver().solve_batch_triplets(rows, cols, vals, b, n)
-    want = np.linalg.solve(_dense(n, rows, cols, vals), b[..., None])[..., 0]
+    m = _dense(n, rows, cols, vals)
+    want = np.linalg.solve(m, b[..., None])[..., 0]
     assert r.singular_indices == []
-    assert np.all(close(r.solutions.reshape(B, n), want, rtol=1e-10, atol=1e-13))
+    x = r.solutions.reshape(B, n)
+    # two backward-stable eliminations in different orders agree normwise to
+    # about cond * eps; the residual pins the engine's own solution
+    err = np.abs(x - want).max(axis=1) / np.abs(want).max(axis=1)
+    assert np.all(err < 1e-10), err.max()
+    resid = np.abs(b - np.einsum("bij,bj->bi", m, x)).max(axis=1)
+    assert np.all(resid < 1e-12 * (np.abs(m).sum(axis=2).max(axis=1) * np.abs(x).max(axis=1))), resid.max()
 
 
 def test_sparse_solve_singular_and_edge_cases():
