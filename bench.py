@@ -55,6 +55,102 @@ def getrf_flops(n: int) -> float:
     return 2.0 / 3.0 * n ** 3 - 0.5 * n ** 2 + 5.0 / 6.0 * n
 
 
+def _event_ms(stream, torch, fn, reps=2):
+    """Mean HIP-event time of fn() on the library stream after one warm-up."""
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(sum(ts) / len(ts))
+
+
+def other_workloads(E, A, CK, torch, np, dev, stream):
+    """The sweep path's other configurations, measured beside the headline:
+    the 4,096-instance 32x32 grid + diodes DC sweep (sparse LU Newton), the
+    same grid's Jacobian family through GMRES-Jacobi(30) and through the
+    batched sparse LU, and the 32,768-instance mixed-size Backward-Euler
+    transient batch (one graph launch, autotuned kernel per size class)."""
+    out = {}
+    lib = dev.lib
+    cuda = torch.device("cuda:0")
+    # grid + diodes DC-OP sweep, FP64, SPICE junction limiter
+    g = CK.grid_diodes(32)
+    B = 4096
+    bufs = dict(x=torch.empty(B, g.size, dtype=torch.float64, device=cuda),
+                status=torch.empty(B, dtype=torch.int32, device=cuda),
+                iterations=torch.empty(B, dtype=torch.int32, device=cuda),
+                flags=torch.empty(B, dtype=torch.int32, device=cuda),
+                params=torch.empty(B, g.num_params, dtype=torch.float64, device=cuda))
+    cfg = E.nr_config(limiter=E.Limiter.Pnjlim)
+    ms = _event_ms(stream, torch, lambda: E.sweep_dcop_device(g, B, bufs, seed=42, config=cfg))
+    st, fl = bufs["status"].cpu().numpy(), bufs["flags"].cpu().numpy()
+    acc = int(np.count_nonzero((st == A.SPX_CONVERGED) & ((fl & A.SPX_FLAG_KCL_FAIL) == 0)))
+    out["grid_diodes_dcop"] = dict(
+        instances=B, unknowns=g.size, ms_per_sweep=ms, solves_per_s=acc / ms * 1e3,
+        accepted=acc, mean_iterations=float(bufs["iterations"].float().mean().item()),
+        linear_solver="shared-symbolic sparse LU, one 256-thread workgroup per instance",
+        limiter="SPICE pnjlim (the memoryless reference limiter lands diodes on its "
+                "spurious fixed point above vcrit here; the KCL test flags those)")
+    # the grid's Jacobian family: GMRES-Jacobi(30) vs batched sparse LU
+    n, rows, cols, vals, rhs = CK.grid_linear_batch(32, B, seed=1)
+    dv = torch.as_tensor(vals, device=cuda)
+    db = torch.as_tensor(rhs, device=cuda)
+    dx = torch.empty(B, n, dtype=torch.float64, device=cuda)
+    dit = torch.empty(B, dtype=torch.int32, device=cuda)
+    dres = torch.empty(B, dtype=torch.float64, device=cuda)
+    dconv = torch.empty(B, dtype=torch.uint8, device=cuda)
+    rows_c, cols_c = np.ascontiguousarray(rows), np.ascontiguousarray(cols)
+    gcfg = E.GmresConfig(restart=30, max_iter=5000, tol=1e-8).to_c()
+    torch.cuda.synchronize()
+
+    def gm():
+        E._check(lib.spx_gmres_solve_batch_f64_device(
+            dev.handle, n, rows_c.size, A.ptr(rows_c), A.ptr(cols_c), A.ptr(dv), A.ptr(db), None,
+            B, C.byref(gcfg), A.ptr(dx), A.ptr(dit), A.ptr(dres), A.ptr(dconv)))
+
+    gm_ms = _event_ms(stream, torch, gm, reps=1)
+    conv = int(dconv.sum().item())
+    it = float(dit.float().mean().item())
+
+    def lu():
+        E._check(lib.spx_sparse_solve_batch_f64_device(
+            dev.handle, n, rows_c.size, A.ptr(rows_c), A.ptr(cols_c), A.ptr(dv), A.ptr(db), B,
+            A.ptr(dx), None))
+
+    lu_ms = _event_ms(stream, torch, lu)
+    out["grid_linear_systems"] = dict(
+        systems=B, unknowns=n, gmres_jacobi30_ms=gm_ms, gmres_systems_per_s=B / gm_ms * 1e3,
+        gmres_converged=conv, gmres_mean_iterations=it, gmres_tol=1e-8,
+        sparse_lu_ms=lu_ms, sparse_lu_systems_per_s=B / lu_ms * 1e3)
+    # mixed-size transient batch, one graph launch, autotuned
+    sizes = (8, 16, 32, 64, 128, 256)
+    total = 32768
+    per = [total // 6 + (1 if i < total % 6 else 0) for i in range(6)]
+    cls = [E.TranClass(CK.ladder(m, seed=1000 + m), b, 1e-12, 200, probes=[0, m // 2, m - 2],
+                       seed=42 + m) for m, b in zip(sizes, per)]
+    tcfg = E.nr_config(limiter=E.Limiter.Pnjlim)
+    kinds = E.transient_mixed_device(cls, tcfg, graph=False, autotune=True)
+    eager_ms = _event_ms(stream, torch, lambda: E.transient_mixed_device(cls, tcfg, graph=False))
+    graph_ms = _event_ms(stream, torch, lambda: E.transient_mixed_device(cls, tcfg, graph=True))
+    acc = 0
+    for c in cls:
+        s_ = c.buffers["status"].cpu().numpy()
+        f_ = c.buffers["flags"].cpu().numpy()
+        acc += int(np.count_nonzero((s_ == A.SPX_CONVERGED) & ((f_ & A.SPX_FLAG_KCL_FAIL) == 0)))
+    out["mixed_transient"] = dict(
+        instances=total, sizes=list(sizes), per_class=per, steps=200, tstep=1e-12,
+        kernels=[k.name for k in kinds], graph_ms_per_batch=graph_ms, eager_ms_per_batch=eager_ms,
+        transient_runs_per_s=acc / graph_ms * 1e3,
+        instance_steps_per_s=acc * 200 / graph_ms * 1e3, accepted=acc)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +163,8 @@ def main():
     ap.add_argument("--forms", default="fp64,fp32_ir,fp32_ir_surrogate")
     ap.add_argument("--train", type=int, default=262144)
     ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the other workloads (grid sweep, GMRES, mixed transient)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the sweep shards by instance range with no "
@@ -241,6 +339,8 @@ def main():
                          f"NR + Gmin/source stepping), one task per instance on {threads} host "
                          f"threads; {dt:.2f} s wall"}
 
+    workloads = None if args.no_workloads else other_workloads(E, A, CK, torch, np, dev, stream)
+
     line = {
         "metric": "converged DC-OP solves/s, 64k x n=64 sweep: plain FP64 vs FP32+IR+surrogate",
         "value": h["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -257,6 +357,7 @@ def main():
         "pcie_inclusive": pcie,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "workloads": workloads,
     }
     print(json.dumps(line))
 

@@ -302,6 +302,45 @@ def grid_diodes(N: int = GRID_N, every: int = GRID_DIODE_EVERY) -> Circuit:
     return c
 
 
+def grid_linear_batch(N: int = GRID_N, batch: int = 4096, seed: int = 0, g_diode: float = 0.04):
+    """Triplets of `batch` linear systems of the N x N grid workload's
+    Jacobian family: edge conductances 1/R with R~U(0.9, 1.1) kOhm, the four
+    corner sources as MNA branch rows (zero diagonal), and at every diode
+    node the junction conductance g_diode (about a conducting diode's
+    1 mA / 26 mV).  Right-hand sides are the corner voltages.  Returns
+    (n, rows, cols, values[batch][nnz], rhs[batch][n]); duplicates are
+    stamped as separate triplets (summed by the solvers)."""
+    rng = np.random.default_rng(seed)
+    nn = N * N
+    n = nn + 4
+    rows, cols, sign, which = [], [], [], []
+    e = 0
+    for r in range(N):
+        for c in range(N):
+            k = r * N + c
+            for nb in ((k + 1) if c + 1 < N else None, (k + N) if r + 1 < N else None):
+                if nb is None:
+                    continue
+                for (i, j, sg) in ((k, k, 1.0), (nb, nb, 1.0), (k, nb, -1.0), (nb, k, -1.0)):
+                    rows.append(i); cols.append(j); sign.append(sg); which.append(e)
+                e += 1
+    g = 1.0 / rng.uniform(0.9e3, 1.1e3, size=(batch, e))
+    vals = g[:, which] * np.array(sign)
+    extra_r, extra_c, extra_v = [], [], []
+    for k in range(nn):
+        if k % GRID_DIODE_EVERY == GRID_DIODE_EVERY // 2:
+            extra_r.append(k); extra_c.append(k); extra_v.append(g_diode)
+    corners = (0, N - 1, N * (N - 1), N * N - 1)
+    for s_, node in enumerate(corners):
+        extra_r += [node, nn + s_]; extra_c += [nn + s_, node]; extra_v += [1.0, 1.0]
+    rows = np.array(rows + extra_r, dtype=np.int32)
+    cols = np.array(cols + extra_c, dtype=np.int32)
+    vals = np.concatenate([vals, np.tile(np.array(extra_v), (batch, 1))], axis=1)
+    rhs = np.zeros((batch, n))
+    rhs[:, nn:] = [1.0 + k * 2.3 / 3 for k in range(4)]
+    return n, rows, cols, vals, rhs
+
+
 TRAN_PULSE = dict(v1=0.0, v2=1.8, td=0.0, tr=10e-12, tf=10e-12, pw=1e-9, per=2e-9)
 
 
