@@ -271,10 +271,28 @@ __device__ __forceinline__ int pivot_pos(float absval, bool eligible, int pos, b
   return 63 - (int)(mx & 63u);
 }
 
+// One DPP max step on doubles that are >= 0 (or NaN): lanes whose source is
+// outside the pattern read +0.0, which leaves the max unchanged.
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ double dpp_max_nonneg(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, Ctrl, RowMask,
+                                                            0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), Ctrl,
+                                                            RowMask, 0xf, false);
+  return fmax(v, __longlong_as_double((long long)(((uint64_t)hi << 32) | lo)));
+}
+
+// Wave-wide max of values >= 0 (NaN lanes are skipped, as fmax skips them),
+// on the DPP ladder of wave_max_u32: no LDS round trips.
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
-  return v;
+  v = dpp_max_nonneg<0xb1, 0xf>(v);
+  v = dpp_max_nonneg<0x4e, 0xf>(v);
+  v = dpp_max_nonneg<0x114, 0xf>(v);
+  v = dpp_max_nonneg<0x118, 0xf>(v);
+  v = dpp_max_nonneg<0x142, 0xa>(v);
+  v = dpp_max_nonneg<0x143, 0xc>(v);
+  return readlane_f64(v, 63);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

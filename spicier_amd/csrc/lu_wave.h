@@ -2,10 +2,13 @@
 // 64-wide wavefront, row `lane` in lane `lane`'s registers.
 //
 // Layout: a[j] of lane r is A[r][j]; j is a compile-time index so the row
-// stays in VGPRs (N doubles = 2N VGPRs).  Rows are never moved: each lane
-// keeps `pos`, its row's current position in the partially pivoted matrix,
-// and a swap of positions k <-> p reproduces the physical row swap of the
-// reference's LU.  The pivot of step k is the largest |A[.][k]| over rows at
+// stays in VGPRs (N doubles = 2N VGPRs).  Rows do not move during the
+// elimination: each lane keeps `pos`, its row's current position in the
+// partially pivoted matrix, and a swap of positions k <-> p reproduces the
+// physical row swap of the reference's LU.  After the last step every row is
+// pushed to the lane of its final position (one cross-lane permute per
+// register), so lane r holds row r of P A = L U and the solves address the
+// row of step k as lane k, a compile-time constant.  The pivot of step k is the largest |A[.][k]| over rows at
 // positions >= k, smallest position on ties — nalgebra's icamax rule (first
 // maximum), which CpuBatchedSolver inherits (spicier-batched-sweep
 // src/solver.rs:370-380); the comparison runs on the top 14 mantissa bits,
@@ -47,6 +50,32 @@ __device__ __forceinline__ void static_for_down(F&& f) {
 __device__ __forceinline__ double bcast(double v, int lane) { return readlane_f64(v, lane); }
 __device__ __forceinline__ float bcast(float v, int lane) { return readlane_f32(v, lane); }
 
+// Cross-lane moves through the LDS crossbar (no LDS storage): push_to sends
+// this lane's v to lane `dst` (dst must be a permutation of the lanes),
+// pull_from returns v of lane `src`.
+__device__ __forceinline__ int push_to(int v, int dst) { return __builtin_amdgcn_ds_permute(dst << 2, v); }
+__device__ __forceinline__ float push_to(float v, int dst) {
+  return __int_as_float(push_to(__float_as_int(v), dst));
+}
+__device__ __forceinline__ double push_to(double v, int dst) {
+  int2 p = *reinterpret_cast<int2*>(&v);
+  p.x = push_to(p.x, dst);
+  p.y = push_to(p.y, dst);
+  return *reinterpret_cast<double*>(&p);
+}
+__device__ __forceinline__ int pull_from(int v, int src) {
+  return __builtin_amdgcn_ds_bpermute(src << 2, v);
+}
+__device__ __forceinline__ float pull_from(float v, int src) {
+  return __int_as_float(pull_from(__float_as_int(v), src));
+}
+__device__ __forceinline__ double pull_from(double v, int src) {
+  int2 p = *reinterpret_cast<int2*>(&v);
+  p.x = pull_from(p.x, src);
+  p.y = pull_from(p.y, src);
+  return *reinterpret_cast<double*>(&p);
+}
+
 template <typename T>
 __device__ __forceinline__ T tabs(T v) { return v < T(0) ? -v : v; }
 
@@ -57,16 +86,17 @@ __device__ __forceinline__ T tabs(T v) { return v < T(0) ? -v : v; }
 enum { kBcastReadlane = 0, kBcastLds = 1 };
 
 // Doubles of the LDS buffer a RowLU<N, *> needs behind `row`: N for the
-// pivot-row broadcast, and at least kWave ints for the position-map scatter.
+// pivot-row broadcast of the LDS variant (the callers reserve at least 32).
 __host__ __device__ constexpr int row_buffer_doubles(int N) { return N < 32 ? 32 : N; }
 
 template <int N, typename T, int Bcast = kBcastReadlane>
 struct RowLU {
   T a[N];
-  int pos;     // position of this lane's row after pivoting
-  int orig;    // lane whose row sits at position `lane`
-  T dinv;      // 1 / U[pos][pos] of this lane's row
-  T* row;      // N elements of LDS: broadcast buffer (and scatter space)
+  int pos;     // position of this lane's row (== lane once factored)
+  int orig;    // row of the input whose factor row lane `lane` holds
+  T piv;       // U[lane][lane] once factored
+  T dinv;      // 1 / U[lane][lane]
+  T* row;      // N elements of LDS: broadcast buffer of the LDS variant
 
   // Factor in place.  Returns false if a pivot column is zero (or NaN): the
   // reference's LU then reports the system singular.  Every step runs over
@@ -76,6 +106,7 @@ struct RowLU {
   __device__ __forceinline__ bool factor(int lane) {
     pos = lane;
     dinv = T(0);
+    piv = T(0);
     bool ok = true;
     // One basic block for all N steps (no early exit, no data-dependent
     // branch), so the scheduler can start step k+1's pivot search while step
@@ -101,7 +132,10 @@ struct RowLU {
         __syncthreads();
         const T pv = row[k];
         const T inv = pv != T(0) ? T(1) / pv : T(0);
-        if (lane == p) dinv = inv;
+        if (lane == p) {
+          dinv = inv;
+          piv = pv;
+        }
         const T l = pos > k ? a[k] * inv : T(0);
         if (pos > k) a[k] = l;
         static_for<k + 1, N>([&](auto J) __attribute__((always_inline)) {
@@ -112,7 +146,10 @@ struct RowLU {
       } else {
         const T pv = bcast(a[k], p);
         const T inv = pv != T(0) ? T(1) / pv : T(0);
-        if (lane == p) dinv = inv;
+        if (lane == p) {
+          dinv = inv;
+          piv = pv;
+        }
         const T l = pos > k ? a[k] * inv : T(0);
         if (pos > k) a[k] = l;
         static_for<k + 1, N>([&](auto J) __attribute__((always_inline)) {
@@ -121,41 +158,53 @@ struct RowLU {
         });
       }
     });
-    if (ok) {  // invert the position map once, so solves need no ballot per step
-      // all 64 lanes scatter: `row` must hold kWave ints (row_buffer_doubles)
-      int* slot = reinterpret_cast<int*>(row);
-      slot[pos] = lane;
-      __syncthreads();
-      orig = slot[lane];
-      __syncthreads();
+    if (ok) {  // pos is a permutation of the 64 lanes: push every row home
+      orig = push_to(lane, pos);
+      static_for<0, N>([&](auto J) __attribute__((always_inline)) {
+        constexpr int j = J;
+        a[j] = push_to(a[j], pos);
+      });
+      dinv = push_to(dinv, pos);
+      piv = push_to(piv, pos);
+      pos = lane;
     }
     return ok;
   }
 
   // Solve with the stored factors for right-hand side b (b of lane r is
-  // b[r]).  Returns x[lane] in each lane.  Forward substitution replays the
-  // elimination on b; back substitution divides by the pivot (exact division
-  // when `divide`, multiplication by the stored inverse otherwise).  The lane
-  // of step k is read from `orig` of lane k, independent of b, so only the
-  // broadcast of b itself is on the dependency chain.
+  // b[r]).  Returns x[lane] in each lane.  b is first pulled into pivoted
+  // order; forward substitution then runs the elimination's updates on it
+  // (lanes below the diagonal, L[r][k] = a[k]); back substitution divides by
+  // the pivot (exact division when `divide`, multiplication by the stored
+  // inverse otherwise) and updates the rows above it.  Step k reads lane k,
+  // so the only cross-lane read on the dependency chain is that broadcast.
+  // x of lane k is the value step k broadcast: the last update of lane k's b
+  // happens before step k, so one vector divide (or multiply) at the end
+  // reproduces it bit for bit.
   template <bool divide>
   __device__ __forceinline__ T solve(T b, int lane) const {
-    static_for<0, N>([&](auto K) __attribute__((always_inline)) {
+    b = pull_from(b, orig);
+    // a fresh value per call, so the per-step lane tests are not hoisted out
+    // of a caller's loop as 2N live lane masks
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    static_for<0, N - 1>([&](auto K) __attribute__((always_inline)) {
       constexpr int k = K;
-      const int p = __builtin_amdgcn_readlane(orig, k);
-      const T l = pos > k ? a[k] : T(0);
-      b = fma(-bcast(b, p), l, b);
+      const T yk = bcast(b, k);
+      if (ln > k) b = fma(-yk, a[k], b);
     });
-    T x = T(0);
-    static_for_down<0, N>([&](auto K) __attribute__((always_inline)) {
+    static_for_down<1, N>([&](auto K) __attribute__((always_inline)) {
       constexpr int k = K;
-      const int p = __builtin_amdgcn_readlane(orig, k);
-      const T bk = bcast(b, p);
-      const T xk = divide ? bk / bcast(a[k], p) : bk * bcast(dinv, p);
-      if (lane == k) x = xk;
-      b = fma(-(pos < k ? xk : T(0)), a[k], b);
+      T xk;
+      if constexpr (divide) {
+        xk = bcast(b, k) / bcast(piv, k);
+      } else {
+        xk = bcast(b * dinv, k);
+      }
+      if (ln < k) b = fma(-xk, a[k], b);
     });
-    return x;
+    const T x = divide ? b / piv : b * dinv;
+    return lane < N ? x : T(0);  // lanes past N hold no row (piv = 0)
   }
 };
 

@@ -91,7 +91,10 @@ struct Instance {
   __device__ __forceinline__ void load_rows(T (&a)[N]) const {
     const bool pad = lane >= g.n;
     const int r = pad ? 0 : lane;  // rows past n (and lanes past N) take no pattern entry
-    const uint32_t mlo = pad ? 0u : rmask[2 * r], mhi = pad ? 0u : rmask[2 * r + 1];
+    uint32_t mlo = pad ? 0u : rmask[2 * r], mhi = pad ? 0u : rmask[2 * r + 1];
+    // opaque per call: otherwise the 64 column tests are hoisted out of the
+    // Newton loop as 64 live lane masks, which spill
+    asm volatile("" : "+v"(mlo), "+v"(mhi));
     int q = q0;
     static_for<0, N>([&](auto JJ) __attribute__((always_inline)) {
       constexpr int j = JJ;
@@ -100,6 +103,9 @@ struct Instance {
       asm volatile("" : "+v"(v));
       a[j] = b ? (T)v : ((pad && j == lane) ? T(1) : T(0));
       q += b ? 1 : 0;
+      // at most 8 loads in flight: the scheduler would otherwise issue all N
+      // first and hold 2N VGPRs of loaded values beside the row
+      if constexpr ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     });
   }
 
@@ -315,7 +321,7 @@ namespace spx {
 // escalated_only: run only the instances an FP32+IR pass handed over (status
 // left SPX_ACTIVE), in FP64.
 template <int N, bool IR>
-__global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
+__global__ __launch_bounds__(64, 2) void k_newton_dense(DevProgram g, DevNrCfg cfg,
                                                      const double* __restrict__ params,
                                                      const double* __restrict__ x0, int64_t B,
                                                      DevOut out, int escalated_only) {
@@ -349,7 +355,7 @@ __global__ __launch_bounds__(64) void k_newton_dense(DevProgram g, DevNrCfg cfg,
 // Batched Backward-Euler transient, FP64, one wave per instance: LDS as
 // k_newton_dense plus the capacitor states.
 template <int N>
-__global__ __launch_bounds__(64) void k_transient_dense(DevProgram g, DevNrCfg cfg, DevTran tr,
+__global__ __launch_bounds__(64, 2) void k_transient_dense(DevProgram g, DevNrCfg cfg, DevTran tr,
                                                         const double* __restrict__ params,
                                                         int64_t B, DevOut out, DevTranOut tout) {
   extern __shared__ double smem[];
