@@ -450,6 +450,135 @@ int spx_set_surrogate(spx_handle* h, spx_surrogate* s);
 int spx_stats_reduce_device(spx_handle* h, const double* d_values, const uint8_t* d_mask,
                             int64_t batch, int32_t n, double* d_out);
 
+/* ------------------------------------------------------------------------- */
+/* Convergence tracking over device arrays (ConvergenceTracker,
+ * crates/spicier-batched-sweep/src/convergence.rs:76-387): status[B] holds
+ * spx_conv_status codes, iterations[B] the per-instance counts.  Counts of
+ * newly converged instances are written to device memory (may be NULL), so no
+ * call below waits for the device except the summary. */
+
+/* check_convergence (convergence.rs:274-314): an ACTIVE instance converges when
+ * |change_j| <= abstol + reltol * |solution_j| for every j (a NaN change
+ * passes, as in the reference's `delta > tol` test).  Buffers are [B][n]. */
+int spx_tracker_check_convergence_device(spx_handle* h, int32_t* d_status, int64_t batch,
+                                         int32_t n, const double* d_changes,
+                                         const double* d_solutions, double abstol, double reltol,
+                                         int64_t* d_newly_converged);
+/* check_residual_convergence (convergence.rs:327-357): ACTIVE instances whose
+ * residual 2-norm (summed in index order) is below tolerance converge. */
+int spx_tracker_check_residual_device(spx_handle* h, int32_t* d_status, int64_t batch, int32_t n,
+                                      const double* d_residuals, double tolerance,
+                                      int64_t* d_newly_converged);
+/* increment_all_active (convergence.rs:197-205) with indices NULL: ACTIVE
+ * instances count one more iteration and fail on reaching max_iterations.
+ * With host indices: increment_iteration (convergence.rs:184-190) on each
+ * listed instance, whatever its status. */
+int spx_tracker_increment_device(spx_handle* h, int32_t* d_status, int32_t* d_iterations,
+                                 int64_t batch, int32_t max_iterations, const int64_t* indices,
+                                 int64_t count);
+/* mark_converged / mark_failed / mark_singular (convergence.rs:150-182) on the
+ * listed instances (host indices); only ACTIVE ones change.  *marked (host,
+ * may be NULL) = how many changed.  Waits for the handle's stream. */
+int spx_tracker_mark_device(spx_handle* h, int32_t* d_status, int64_t batch,
+                            const int64_t* indices, int64_t count, int32_t new_status,
+                            int64_t* marked);
+
+/* ConvergenceSummary (convergence.rs:360-400); average and maximum over the
+ * converged instances. */
+typedef struct spx_tracker_summary {
+    int64_t total_points;
+    int64_t converged_count;
+    int64_t failed_count;      /* FAILED + SINGULAR */
+    int64_t active_count;
+    int64_t iteration_sum;     /* over converged instances */
+    int32_t max_iterations;    /* over converged instances */
+    double average_iterations;
+} spx_tracker_summary;
+/* Host result; waits for the handle's stream.  d_iterations may be NULL. */
+int spx_tracker_summary_device(spx_handle* h, const int32_t* d_status, const int32_t* d_iterations,
+                               int64_t batch, spx_tracker_summary* out);
+
+/* active_indices / converged_indices / failed_indices (convergence.rs:209-238):
+ * ascending indices of the selected instances into d_indices (room for
+ * `batch`), their number into *d_count (device). */
+enum { SPX_SELECT_ACTIVE = 0, SPX_SELECT_CONVERGED = 1, SPX_SELECT_FAILED = 2 };
+int spx_select_indices_device(spx_handle* h, const int32_t* d_status, int64_t batch,
+                              int32_t which, int64_t* d_indices, int64_t* d_count);
+/* compact_active (convergence.rs:432-441): out[k] = data[indices[k]], rows of
+ * item_size doubles. */
+int spx_compact_rows_device(spx_handle* h, const double* d_data, int64_t item_size,
+                            const int64_t* d_indices, int64_t count, double* d_out);
+/* expand_active (convergence.rs:453-471): out = fill everywhere, then
+ * out[indices[k]] = compacted[k].  Indices must be distinct. */
+int spx_expand_rows_device(spx_handle* h, const double* d_compacted, int64_t item_size,
+                           const int64_t* d_indices, int64_t count, int64_t batch,
+                           double fill_value, double* d_out);
+
+/* ------------------------------------------------------------------------- */
+/* Histogram of one node over a [B][n] solution buffer (Histogram::new and
+ * Histogram::from_sweep, crates/spicier-batched-sweep/src/statistics.rs:244-293).
+ * Bin = floor((v - min) / width), NaN to bin 0, clamped to the bins.  With
+ * from_sweep the range is the column's min/max (NaN skipped), widened by 0.5
+ * each way if they are within f64::EPSILON; otherwise max must exceed min.
+ * d_counts: num_bins int64 (device); d_range (device, may be NULL): the
+ * range used. */
+int spx_histogram_device(spx_handle* h, const double* d_solutions, int64_t batch, int32_t n,
+                         int32_t node, int32_t num_bins, double min, double max,
+                         int32_t from_sweep, int64_t* d_counts, double* d_range);
+
+/* YieldSpec (statistics.rs:331-376): lower <= v <= upper passes. */
+typedef struct spx_yield_spec {
+    int32_t node;
+    double lower_limit;
+    double upper_limit;
+} spx_yield_spec;
+/* YieldAnalysis::compute_yield / individual_yields (statistics.rs:379-425):
+ * d_counts[0] = instances passing every spec, d_counts[1 + s] = instances
+ * passing spec s (device, num_specs + 1 int64). */
+int spx_yield_device(spx_handle* h, const double* d_solutions, int64_t batch, int32_t n,
+                     const spx_yield_spec* specs, int32_t num_specs, int64_t* d_counts);
+
+/* StreamingStatistics::process_chunk (statistics.rs:470-587): reduces a chunk
+ * [B][n] (mask as spx_stats_reduce_device) and merges it into the running
+ * accumulators d_acc[n][5] (count,sum,sumsq,min,max; start from
+ * 0,0,0,+inf,-inf). */
+int spx_stats_accumulate_device(spx_handle* h, const double* d_values, const uint8_t* d_mask,
+                                int64_t batch, int32_t n, double* d_acc);
+
+/* StreamingStatistics::merge (statistics.rs:545-556): d_acc[n][5] merged
+ * with d_other[n][5] (StatisticsAccumulator::merge, :148-154). */
+int spx_stats_merge_device(spx_handle* h, double* d_acc, const double* d_other, int32_t n);
+
+/* ------------------------------------------------------------------------- */
+/* BatchLayout packing (crates/spicier-batched-sweep/src/batch_layout.rs:43-199):
+ * column-major FP64 systems to row-major FP32 rows of row_stride floats (0:
+ * n rounded up to a multiple of 32, BatchLayout::new), padding zero. */
+int spx_pack_matrices_f32_device(spx_handle* h, const double* d_matrices, int64_t n,
+                                 int64_t batch, int64_t row_stride, float* d_packed);
+/* pack_rhs_f32 / unpack_solutions_f64 (batch_layout.rs:190-199). */
+int spx_pack_rhs_f32_device(spx_handle* h, const double* d_rhs, int64_t count, float* d_packed);
+int spx_unpack_solutions_f64_device(spx_handle* h, const float* d_packed, int64_t count,
+                                    double* d_solutions);
+
+/* ------------------------------------------------------------------------- */
+/* Pipelined batched solve from host buffers (PipelinedSweep,
+ * crates/spicier-batched-sweep/src/pipeline.rs:130-361; PipelineConfig
+ * :7-87).  Same inputs, outputs and semantics as spx_solve_batch_f64; the
+ * batch runs in chunks over num_buffers pinned staging slots, each with its
+ * own stream, so the host copies, both transfer directions and the solves of
+ * different chunks overlap. */
+typedef struct spx_pipeline_config {
+    int64_t chunk_size;
+    int32_t num_buffers;
+} spx_pipeline_config;
+/* PipelineConfig::auto (pipeline.rs:68-87). */
+void spx_pipeline_config_auto(int64_t n, int64_t total_points, spx_pipeline_config* cfg);
+/* cfg NULL: spx_pipeline_config_auto. */
+int spx_solve_batch_f64_pipelined(spx_handle* h, const double* matrices, const double* rhs,
+                                  int64_t n, int64_t batch, const spx_pipeline_config* cfg,
+                                  double* solutions, uint8_t* singular, int64_t* num_singular,
+                                  int64_t* chunks_processed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,6 +111,23 @@ class SpxTranClass(C.Structure):
                 ("tran", SpxTranConfig), ("result", SpxTranResult)]
 
 
+class SpxTrackerSummary(C.Structure):
+    _fields_ = [("total_points", C.c_int64), ("converged_count", C.c_int64),
+                ("failed_count", C.c_int64), ("active_count", C.c_int64),
+                ("iteration_sum", C.c_int64), ("max_iterations", C.c_int32),
+                ("average_iterations", C.c_double)]
+
+
+class SpxYieldSpec(C.Structure):
+    _fields_ = [("node", C.c_int32), ("lower_limit", C.c_double), ("upper_limit", C.c_double)]
+
+
+class SpxPipelineConfig(C.Structure):
+    _fields_ = [("chunk_size", C.c_int64), ("num_buffers", C.c_int32)]
+
+
+SPX_SELECT_ACTIVE, SPX_SELECT_CONVERGED, SPX_SELECT_FAILED = 0, 1, 2
+
 SPX_MIXED_GRAPH, SPX_MIXED_AUTOTUNE = 1, 2
 SPX_KERNEL_AUTO, SPX_KERNEL_DENSE, SPX_KERNEL_SPARSE64, SPX_KERNEL_SPARSE256 = 0, 1, 2, 3
 SPX_KERNEL_LANES = 4
@@ -192,6 +209,28 @@ def load_library():
                                                  C.POINTER(SpxTranConfig),
                                                  C.POINTER(SpxTranResult)]),
         "spx_stats_reduce_device": (C.c_int, [vp, vp, vp, i64, i32, vp]),
+        "spx_tracker_check_convergence_device": (C.c_int, [vp, vp, i64, i32, vp, vp, C.c_double,
+                                                           C.c_double, vp]),
+        "spx_tracker_check_residual_device": (C.c_int, [vp, vp, i64, i32, vp, C.c_double, vp]),
+        "spx_tracker_increment_device": (C.c_int, [vp, vp, vp, i64, i32, vp, i64]),
+        "spx_tracker_mark_device": (C.c_int, [vp, vp, i64, vp, i64, i32, C.POINTER(C.c_int64)]),
+        "spx_tracker_summary_device": (C.c_int, [vp, vp, vp, i64, C.POINTER(SpxTrackerSummary)]),
+        "spx_select_indices_device": (C.c_int, [vp, vp, i64, i32, vp, vp]),
+        "spx_compact_rows_device": (C.c_int, [vp, vp, i64, vp, i64, vp]),
+        "spx_expand_rows_device": (C.c_int, [vp, vp, i64, vp, i64, i64, C.c_double, vp]),
+        "spx_histogram_device": (C.c_int, [vp, vp, i64, i32, i32, i32, C.c_double, C.c_double,
+                                           i32, vp, vp]),
+        "spx_yield_device": (C.c_int, [vp, vp, i64, i32, C.POINTER(SpxYieldSpec), i32, vp]),
+        "spx_stats_accumulate_device": (C.c_int, [vp, vp, vp, i64, i32, vp]),
+        "spx_stats_merge_device": (C.c_int, [vp, vp, vp, i32]),
+        "spx_pack_matrices_f32_device": (C.c_int, [vp, vp, i64, i64, i64, vp]),
+        "spx_pack_rhs_f32_device": (C.c_int, [vp, vp, i64, vp]),
+        "spx_unpack_solutions_f64_device": (C.c_int, [vp, vp, i64, vp]),
+        "spx_pipeline_config_auto": (None, [i64, i64, C.POINTER(SpxPipelineConfig)]),
+        "spx_solve_batch_f64_pipelined": (C.c_int, [vp, vp, vp, i64, i64,
+                                                    C.POINTER(SpxPipelineConfig), vp, u8p,
+                                                    C.POINTER(C.c_int64),
+                                                    C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

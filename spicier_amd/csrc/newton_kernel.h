@@ -92,16 +92,18 @@ struct Instance {
     const bool pad = lane >= g.n;
     const int r = pad ? 0 : lane;  // rows past n (and lanes past N) take no pattern entry
     uint32_t mlo = pad ? 0u : rmask[2 * r], mhi = pad ? 0u : rmask[2 * r + 1];
-    // opaque per call: otherwise the 64 column tests are hoisted out of the
-    // Newton loop as 64 live lane masks, which spill
-    asm volatile("" : "+v"(mlo), "+v"(mhi));
+    // opaque per call: otherwise the 64 column tests and the identity-row
+    // constants are hoisted out of the Newton loop as 64 live lane masks and
+    // 64 live values, which spill
+    int diag = pad ? lane : -1;
+    asm volatile("" : "+v"(mlo), "+v"(mhi), "+v"(diag));
     int q = q0;
     static_for<0, N>([&](auto JJ) __attribute__((always_inline)) {
       constexpr int j = JJ;
       const bool b = ((j < 32 ? mlo : mhi) >> (j & 31)) & 1u;
       double v = J[q];
       asm volatile("" : "+v"(v));
-      a[j] = b ? (T)v : ((pad && j == lane) ? T(1) : T(0));
+      a[j] = b ? (T)v : (j == diag ? T(1) : T(0));
       q += b ? 1 : 0;
       // at most 8 loads in flight: the scheduler would otherwise issue all N
       // first and hold 2N VGPRs of loaded values beside the row
