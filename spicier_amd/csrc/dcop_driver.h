@@ -174,6 +174,10 @@ __device__ __forceinline__ void run_transient(Inst& I, const DevProgram& g, cons
   tc.mode = 0;
   tc.t = 0.0;
   DcopOutcome o = solve_dcop(I, g, cfg, false, 0);
+  if (o.handover) {  // FP32 factors cannot resolve this instance: the FP64 pass reruns it
+    I.write_handover(out, inst, o.flags);
+    return;
+  }
   int status = o.status, total = o.total, flags = o.flags, fail = -1;
   double res = 0.0;
   const int64_t row0 = inst * (int64_t)(tr.steps + 1);
@@ -181,16 +185,22 @@ __device__ __forceinline__ void run_transient(Inst& I, const DevProgram& g, cons
     I.cap_update();
     if (tout.waves) I.write_probes(tr, tout.waves + row0 * tr.num_probes);
     tc.mode = 1;
+    I.set_reuse(true);
     for (int step = 1; step <= tr.steps; ++step) {
       tc.t = step * tr.h;
       int it = 0, esc = 0, st;
       if (!g.has_nonlinear) {
-        st = I.linear_solve() == 0 ? 1 : -1;
+        const int r = I.linear_solve();
+        st = r == 0 ? 1 : (r == 2 ? -2 : -1);
         it = 1;
       } else {
         st = I.newton(0.0, 1.0, it, esc);
       }
       total += it;
+      if (st == -2) {
+        I.write_handover(out, inst, flags);
+        return;
+      }
       if (st != 1) {
         status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
         fail = step;

@@ -84,6 +84,10 @@ namespace {
 constexpr int kDenseMaxN = 64;
 constexpr int kSparseMaxN = 4096;
 
+// FP32+IR form of the dense transient kernels (phase bit 4); the sparse and
+// lane kernels factor in FP64 whatever the precision.
+int ir_phase(const DevNrCfg& dc) { return dc.precision == SPX_PREC_F32_IR ? 4 : 0; }
+
 // ------------------------------------------------------------- circuit compile
 struct HostProgram {
   int n = 0, nn = 0, P = 0, num_elems = 0, num_vals = 2, has_nonlinear = 0;
@@ -659,8 +663,6 @@ int transient_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_swe
   int rc;
   if (!out || !out->x) return fail(SPX_ERR_INVALID_ARGUMENT, "result.x is required");
   if ((rc = check_cfg(cfg))) return rc;
-  if (cfg->precision != SPX_PREC_F64)
-    return fail(SPX_ERR_INVALID_ARGUMENT, "the transient sweep runs in FP64");
   HostProgram hp;
   if ((rc = compile_circuit(ckt, hp, kSparseMaxN, true))) return rc;
   if ((rc = check_spec(spec, hp.P))) return rc;
@@ -697,6 +699,11 @@ int transient_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_swe
   DevOut o{out->x, out->status, out->iterations, out->flags, out->residual, nullptr, nullptr};
   DevTranOut to{out->waveforms, out->fail_step};
   DevNrCfg dc = to_dev(cfg);
+  if (!o.status && hp.n <= kDenseMaxN && dc.precision == SPX_PREC_F32_IR) {
+    void* st = nullptr;  // the FP64 hand-over pass reads the statuses
+    if ((rc = workspace(h, "tran_status_ir", sizeof(int) * B, &st))) return rc;
+    o.status = static_cast<int*>(st);
+  }
   if (hp.n > kDenseMaxN) {
     int grid = 0, lu_lds = 0;
     int64_t slot = 0;
@@ -711,7 +718,7 @@ int transient_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_swe
   } else {
     auto f = hp.n <= 8 ? launch_transient_8 : hp.n <= 16 ? launch_transient_16
            : hp.n <= 32 ? launch_transient_32 : launch_transient_64;
-    SPX_HIP(f(g, dc, tr, params, B, o, to, h->stream, 3), SPX_ERR_BACKEND);
+    SPX_HIP(f(g, dc, tr, params, B, o, to, h->stream, 3 | ir_phase(dc)), SPX_ERR_BACKEND);
   }
   return SPX_OK;
 }
@@ -751,7 +758,8 @@ int shape_class(spx_handle* h, MixedClass& mc, int kernel, int64_t B, const DevN
   int rc;
   mc.kernel = kernel;
   if (kernel == SPX_KERNEL_DENSE) {
-    SPX_HIP(dense_tran(mc.hp.n)(mc.g, dc, mc.tr, nullptr, B, mc.o, mc.to, h->stream, 1),
+    SPX_HIP(dense_tran(mc.hp.n)(mc.g, dc, mc.tr, nullptr, B, mc.o, mc.to, h->stream,
+                                1 | ir_phase(dc)),
             SPX_ERR_BACKEND);
     mc.grid = (int)B;
     mc.ws = nullptr;
@@ -776,7 +784,8 @@ int shape_class(spx_handle* h, MixedClass& mc, int kernel, int64_t B, const DevN
 
 hipError_t enqueue_class(const MixedClass& mc, const DevNrCfg& dc, const DevTran& tr,
                          const DevOut& o, const DevTranOut& to, int64_t B, hipStream_t st) {
-  if (mc.kernel == SPX_KERNEL_DENSE) return dense_tran(mc.hp.n)(mc.g, dc, tr, mc.params, B, o, to, st, 2);
+  if (mc.kernel == SPX_KERNEL_DENSE)
+    return dense_tran(mc.hp.n)(mc.g, dc, tr, mc.params, B, o, to, st, 2 | ir_phase(dc));
   if (mc.kernel == SPX_KERNEL_LANES)
     return launch_transient_lanes(mc.g, mc.sp, dc, tr, mc.params, B, o, to, mc.ws, mc.grid,
                                   mc.slot, st);
@@ -841,8 +850,6 @@ int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_
   int rc;
   if (nc < 0 || (nc > 0 && !cls)) return fail(SPX_ERR_INVALID_ARGUMENT, "bad class list");
   if ((rc = check_cfg(cfg))) return rc;
-  if (cfg->precision != SPX_PREC_F64)
-    return fail(SPX_ERR_INVALID_ARGUMENT, "the transient sweep runs in FP64");
   const DevNrCfg dc = to_dev(cfg);
   std::vector<MixedClass> mcs(nc);
   std::vector<uint64_t> key;
@@ -911,10 +918,20 @@ int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_
     mc.tr = DevTran{c.tran.tstep, c.tran.steps, c.tran.num_probes, static_cast<const int*>(dprobe)};
     mc.o = DevOut{c.result.x, c.result.status, c.result.iterations, c.result.flags,
                   c.result.residual, nullptr, nullptr};
+    if (!mc.o.status && dc.precision == SPX_PREC_F32_IR) {
+      void* st = nullptr;  // the FP64 hand-over pass reads the statuses
+      if ((rc = workspace(h, ("mx_status" + tag).c_str(), sizeof(int) * std::max<int64_t>(1, mc.B),
+                          &st)))
+        return rc;
+      mc.o.status = static_cast<int*>(st);
+    }
     mc.to = DevTranOut{c.result.waveforms, c.result.fail_step};
+    // the fastest kernel depends on the precision form too
+    std::vector<int> tkey = h->programs[tag].blob;
+    tkey.push_back(dc.precision);
     int kind = req;
     if (kind == SPX_KERNEL_AUTO) {
-      auto it = h->tuned.find(h->programs[tag].blob);
+      auto it = h->tuned.find(tkey);
       if (it != h->tuned.end()) {
         kind = it->second;
       } else if ((options & SPX_MIXED_AUTOTUNE) && mc.B > 0) {
@@ -923,7 +940,7 @@ int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_
                                 mc.seed, mc.B, mc.offset, mc.nvar, mc.sample_dst, h->stream),
                   SPX_ERR_BACKEND);
         if ((rc = autotune_class(h, mc, dc, &kind))) return rc;
-        h->tuned[h->programs[tag].blob] = kind;
+        h->tuned[tkey] = kind;
       } else {
         kind = mc.hp.n <= kDenseMaxN ? SPX_KERNEL_DENSE : SPX_KERNEL_SPARSE256;
       }
@@ -937,7 +954,7 @@ int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_
     kput(words, sizeof(words));
     kput(&mc.tr.h, sizeof(double));
     const void* ptrs[] = {mc.params, mc.sample_dst, mc.d_specs, dprobe, mc.ws, mc.prog,
-                          mc.sp.perm, c.result.x, c.result.status, c.result.iterations,
+                          mc.sp.perm, c.result.x, mc.o.status, c.result.iterations,
                           c.result.flags, c.result.residual, c.result.waveforms,
                           c.result.fail_step};
     kput(ptrs, sizeof(ptrs));

@@ -162,6 +162,7 @@ struct LaneInstance {
     return q;
   }
 
+  __device__ __forceinline__ void set_reuse(bool) {}  // FP64 factors: nothing carried over
   __device__ __forceinline__ void cap_update() {
     double* vp = const_cast<double*>(tc->vprev);
     for (int e = 0; e < g.num_elems; ++e) {

@@ -53,9 +53,19 @@ struct Instance {
   int nfact = 0;  // LU factorizations performed
   int nref = 0;   // refinement steps performed
   const TranCtx* tc = nullptr;  // transient context (null: DC sweep)
+  // FP32+IR form: the FP32 factors, whether they are valid, and whether they
+  // may be carried from one Newton solve to the next (transient steps: the
+  // Jacobian moves little from step to step, and refinement against the
+  // current one corrects the difference or asks for a refactorization)
+  RowLU<N, float> lu32;
+  bool have32 = false;
+  bool reuse32 = false;
+
+  __device__ __forceinline__ void set_reuse(bool v) { reuse32 = v; }
 
   // after jc is in LDS; publishes the row masks (a barrier follows)
   __device__ __forceinline__ void init_row() {
+    lu32.row = reinterpret_cast<float*>(prow);
     uint64_t m = 0;
     if (lane < g.n) {
       q0 = g.row_ptr[lane];
@@ -171,9 +181,7 @@ struct Instance {
     const int n = g.n, nn = g.num_nodes;
     for (int i = lane; i < g.num_elems; i += kWave) vst[i] = 0.0;
     __syncthreads();
-    RowLU<N, float> lu32;
-    lu32.row = reinterpret_cast<float*>(prow);
-    bool have32 = false;
+    if (!reuse32) have32 = false;
     for (int it = 0; it < cfg.max_iterations; ++it) {
       const bool lim = assemble(gmin, sf, cfg.limiter);
       double xnew = 0.0;
@@ -243,11 +251,21 @@ struct Instance {
     double xn = 0.0;
     int rc;
     if constexpr (IR) {
-      RowLU<N, float> lu32;
-      lu32.row = reinterpret_cast<float*>(prow);
-      load_rows(lu32.a);
-      ++nfact;
-      rc = (lu32.factor(lane) && refine(lu32, true, xn) == 0) ? 0 : 2;
+      int r = 1;
+      for (int attempt = 0; attempt < 2 && r == 1; ++attempt) {
+        const bool fresh = !(reuse32 && have32) || attempt > 0;
+        if (fresh) {
+          load_rows(lu32.a);
+          ++nfact;
+          have32 = lu32.factor(lane);
+          if (!have32) {
+            r = 2;
+            break;
+          }
+        }
+        r = refine(lu32, fresh, xn);
+      }
+      rc = r == 0 ? 0 : 2;
     } else {
       rc = solve_f64(xn) ? 0 : 1;
     }
@@ -354,15 +372,19 @@ __global__ __launch_bounds__(64, 2) void k_newton_dense(DevProgram g, DevNrCfg c
   run_dcop(I, g, cfg, x0 != nullptr, out, inst, escalated_only);
 }
 
-// Batched Backward-Euler transient, FP64, one wave per instance: LDS as
-// k_newton_dense plus the capacitor states.
-template <int N>
+// Batched Backward-Euler transient, one wave per instance: LDS as
+// k_newton_dense plus the capacitor states.  IR: FP32 factors carried across
+// the time steps with FP64 refinement; an instance FP32 cannot resolve is left
+// SPX_ACTIVE for the FP64 pass (escalated_only), which reruns it from t = 0.
+template <int N, bool IR>
 __global__ __launch_bounds__(64, 2) void k_transient_dense(DevProgram g, DevNrCfg cfg, DevTran tr,
                                                         const double* __restrict__ params,
-                                                        int64_t B, DevOut out, DevTranOut tout) {
+                                                        int64_t B, DevOut out, DevTranOut tout,
+                                                        int escalated_only) {
   extern __shared__ double smem[];
   const int64_t inst = blockIdx.x;
   const int lane = threadIdx.x;
+  if (escalated_only && out.status[inst] != SPX_ACTIVE) return;
   double* J = smem;
   int* jc = reinterpret_cast<int*>(J + g.num_pos);
   uint32_t* rmask = reinterpret_cast<uint32_t*>(J + g.num_pos + (g.num_pos + 1) / 2);
@@ -374,7 +396,7 @@ __global__ __launch_bounds__(64, 2) void k_transient_dense(DevProgram g, DevNrCf
   double* vst = pv + g.P;
   double* prow = vst + g.num_elems;
   double* vcap = prow + row_buffer_doubles(N);
-  Instance<N, false> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
+  Instance<N, IR> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
   TranCtx tc{0, 0.0, tr.h, vcap};
   I.tc = &tc;
   for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
@@ -388,21 +410,34 @@ __global__ __launch_bounds__(64, 2) void k_transient_dense(DevProgram g, DevNrCf
   run_transient(I, g, cfg, tr, tc, out, tout, inst);
 }
 
-// phase bit 1: set the kernel's LDS attribute (not allowed while a stream is
-// being captured into a graph); bit 2: launch.
+// phase bit 1: set the kernels' LDS attribute (not allowed while a stream is
+// being captured into a graph); bit 2: launch; bit 4: the FP32+IR form (the
+// FP32 kernel, then the FP64 pass over the instances it handed over; needs
+// out.status).
 template <int N>
 hipError_t launch_transient_impl(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
                                  const double* params, int64_t B, const DevOut& out,
                                  const DevTranOut& tout, hipStream_t st, int phase) {
   size_t sm = sizeof(double) * (newton_smem_doubles(g, N) + g.num_elems);
+  const bool ir = (phase & 4) != 0;
   if (phase & 1) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_transient_dense<N>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_transient_dense<N, false>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e == hipSuccess && ir)
+      e = hipFuncSetAttribute((const void*)k_transient_dense<N, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return e;
   }
   if (!(phase & 2) || B == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_transient_dense<N>), dim3((unsigned)B), dim3(kWave), sm, st, g, cfg, tr,
-                     params, B, out, tout);
+  if (ir) {
+    if (!out.status) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_transient_dense<N, true>), dim3((unsigned)B), dim3(kWave), sm, st, g,
+                       cfg, tr, params, B, out, tout, 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((k_transient_dense<N, false>), dim3((unsigned)B), dim3(kWave), sm, st, g, cfg,
+                     tr, params, B, out, tout, ir ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -265,6 +265,7 @@ struct SparseInstance {
       if (out.refinements) out.refinements[inst] = nref;
     }
   }
+  __device__ __forceinline__ void set_reuse(bool) {}  // FP64 factors: nothing carried over
   __device__ __forceinline__ void cap_update() {
     double* vp = const_cast<double*>(tc->vprev);
     for (int e = tid; e < g.num_elems; e += T) {

@@ -88,3 +88,35 @@ def test_autotune_picks_a_valid_kernel_and_keeps_results():
     for c, r in zip(cls, ref):
         w = c.buffers["waveforms"]
         assert torch.allclose(w, r, rtol=1e-9, atol=1e-12)
+
+
+def test_mixed_ir_matches_oracle(oracle):
+    # FP32+IR on the register-LU classes (n <= 64); the sparse classes factor
+    # in FP64.  Both against the FP64 oracle; graph replay equals eager.
+    cls = _classes(B=24, steps=60)
+    cfg = E.nr_config(limiter=E.Limiter.Pnjlim, precision=E.Precision.F32_IR)
+    kinds = E.transient_mixed_device(cls, cfg, graph=False)
+    E.Device.get(0).synchronize()
+    assert kinds == [E.KernelKind.Dense] * 4 + [E.KernelKind.Sparse256] * 2
+    ref = [{k: v.clone() for k, v in c.buffers.items()} for c in cls]
+    for c in cls:
+        got = c.host_result()
+        want = oracle.transient(c.circuit, got.params,
+                                A.default_nr_config(limiter=A.SPX_LIMIT_PNJLIM), c.tstep,
+                                c.steps, probes=c.probes, nthreads=16)
+        np.testing.assert_array_equal(got.status, want["status"])
+        same = got.iterations == want["iterations"]
+        assert same.mean() >= 0.9
+        assert _close(got.waveforms[same], want["waveforms"][same])
+        assert np.all(got.accepted_mask())
+    for rep in range(2):
+        for c in cls:
+            for v in c.buffers.values():
+                v.fill_(-7)
+        torch.cuda.synchronize()
+        E.transient_mixed_device(cls, cfg, graph=True)
+        E.Device.get(0).synchronize()
+        _ = torch.ones(1000, device="cuda").sum().item()
+        for c, r in zip(cls, ref):
+            for k in r:
+                assert torch.equal(c.buffers[k], r[k]), (rep, c.circuit.name, k)

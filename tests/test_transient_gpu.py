@@ -67,3 +67,43 @@ def test_transient_probes_and_no_steps(oracle):
         E.transient_sweep(c, 2, 1e-12, 5, probes=[24])
     with pytest.raises(E.InvalidArgument):
         E.transient_sweep(c, 2, 0.0, 5)
+
+
+# --------------------------------------------------------- FP32 LU + FP64 IR
+def _ir_cfg():
+    return E.nr_config(limiter=E.Limiter.Pnjlim, precision=E.Precision.F32_IR)
+
+
+def test_rc_step_response_ir_reuses_factors():
+    # linear: the matrix is the same at every step, so the FP32 factors of the
+    # first step serve all 200; refinement keeps the FP64 recurrence
+    R, Cap, V = 2e3, 1.5e-12, 1.0
+    c = CK.rc_step(R, Cap, V)
+    h, steps = 1e-12, 200
+    got = E.transient_sweep(c, 4, h, steps, seed=1, config=_ir_cfg())
+    k = np.arange(steps + 1, dtype=float)
+    want = V * (1.0 - (1.0 + h / (R * Cap)) ** (-k))
+    for i in range(4):
+        np.testing.assert_allclose(got.waveforms[i, :, 1], want, rtol=1e-12, atol=1e-15)
+    assert np.all(got.status == A.SPX_CONVERGED) and np.all(got.fail_step == -1)
+    assert np.all((got.flags & A.SPX_FLAG_FP64_ESCALATED) == 0)
+
+
+@pytest.mark.parametrize("n", [8, 16, 32, 64])
+def test_ladder_transient_ir_matches_fp64_oracle(oracle, n):
+    # FP32 factors carried across steps, FP64 residuals: each Newton step is
+    # solved to FP64 backward error, so the iterates follow the FP64 oracle's
+    c = CK.ladder(n, seed=n)
+    B, h, steps = 48, 1e-12, 200
+    got = E.transient_sweep(c, B, h, steps, seed=7, config=_ir_cfg())
+    want = oracle.transient(c, got.params, A.default_nr_config(limiter=A.SPX_LIMIT_PNJLIM), h,
+                            steps, nthreads=16)
+    np.testing.assert_array_equal(got.status, want["status"])
+    np.testing.assert_array_equal(got.fail_step, want["fail_step"])
+    np.testing.assert_array_equal(got.flags & ~A.SPX_FLAG_FP64_ESCALATED, want["flags"])
+    same = got.iterations == want["iterations"]
+    assert same.mean() >= 0.95, (n, same.mean())
+    ok = close(got.waveforms[same], want["waveforms"][same])
+    assert np.all(ok), (n, np.abs(got.waveforms[same] - want["waveforms"][same]).max())
+    assert np.all(close(got.solutions[same], want["x"][same]))
+    assert np.all(got.accepted_mask())
