@@ -128,26 +128,31 @@ def other_workloads(E, A, CK, torch, np, dev, stream):
         systems=B, unknowns=n, gmres_jacobi30_ms=gm_ms, gmres_systems_per_s=B / gm_ms * 1e3,
         gmres_converged=conv, gmres_mean_iterations=it, gmres_tol=1e-8,
         sparse_lu_ms=lu_ms, sparse_lu_systems_per_s=B / lu_ms * 1e3)
-    # mixed-size transient batch, one graph launch, autotuned
+    # mixed-size transient batch, one graph launch, autotuned, in both forms
     sizes = (8, 16, 32, 64, 128, 256)
     total = 32768
     per = [total // 6 + (1 if i < total % 6 else 0) for i in range(6)]
-    cls = [E.TranClass(CK.ladder(m, seed=1000 + m), b, 1e-12, 200, probes=[0, m // 2, m - 2],
-                       seed=42 + m) for m, b in zip(sizes, per)]
-    tcfg = E.nr_config(limiter=E.Limiter.Pnjlim)
-    kinds = E.transient_mixed_device(cls, tcfg, graph=False, autotune=True)
-    eager_ms = _event_ms(stream, torch, lambda: E.transient_mixed_device(cls, tcfg, graph=False))
-    graph_ms = _event_ms(stream, torch, lambda: E.transient_mixed_device(cls, tcfg, graph=True))
-    acc = 0
-    for c in cls:
-        s_ = c.buffers["status"].cpu().numpy()
-        f_ = c.buffers["flags"].cpu().numpy()
-        acc += int(np.count_nonzero((s_ == A.SPX_CONVERGED) & ((f_ & A.SPX_FLAG_KCL_FAIL) == 0)))
-    out["mixed_transient"] = dict(
-        instances=total, sizes=list(sizes), per_class=per, steps=200, tstep=1e-12,
-        kernels=[k.name for k in kinds], graph_ms_per_batch=graph_ms, eager_ms_per_batch=eager_ms,
-        transient_runs_per_s=acc / graph_ms * 1e3,
-        instance_steps_per_s=acc * 200 / graph_ms * 1e3, accepted=acc)
+    for key, prec in (("mixed_transient", E.Precision.F64),
+                      ("mixed_transient_fp32_ir", E.Precision.F32_IR)):
+        cls = [E.TranClass(CK.ladder(m, seed=1000 + m), b, 1e-12, 200,
+                           probes=[0, m // 2, m - 2], seed=42 + m) for m, b in zip(sizes, per)]
+        tcfg = E.nr_config(limiter=E.Limiter.Pnjlim, precision=prec)
+        kinds = E.transient_mixed_device(cls, tcfg, graph=False, autotune=True)
+        eager_ms = _event_ms(stream, torch, lambda: E.transient_mixed_device(cls, tcfg, graph=False))
+        graph_ms = _event_ms(stream, torch, lambda: E.transient_mixed_device(cls, tcfg, graph=True))
+        acc = esc = 0
+        for c in cls:
+            s_ = c.buffers["status"].cpu().numpy()
+            f_ = c.buffers["flags"].cpu().numpy()
+            acc += int(np.count_nonzero((s_ == A.SPX_CONVERGED) & ((f_ & A.SPX_FLAG_KCL_FAIL) == 0)))
+            esc += int(np.count_nonzero(f_ & A.SPX_FLAG_FP64_ESCALATED))
+        out[key] = dict(
+            instances=total, sizes=list(sizes), per_class=per, steps=200, tstep=1e-12,
+            precision=prec.name, kernels=[k.name for k in kinds], graph_ms_per_batch=graph_ms,
+            eager_ms_per_batch=eager_ms, transient_runs_per_s=acc / graph_ms * 1e3,
+            instance_steps_per_s=acc * 200 / graph_ms * 1e3, accepted=acc, fp64_escalated=esc,
+            note="FP32+IR applies to the register-LU classes; sparse classes factor in FP64"
+            if prec == E.Precision.F32_IR else "")
     return out
 
 
@@ -162,7 +167,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32_ir,fp32_ir_surrogate")
     ap.add_argument("--train", type=int, default=262144)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the other workloads (grid sweep, GMRES, mixed transient)")
     args = ap.parse_args()

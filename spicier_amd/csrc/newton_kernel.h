@@ -340,8 +340,12 @@ namespace spx {
 
 // escalated_only: run only the instances an FP32+IR pass handed over (status
 // left SPX_ACTIVE), in FP64.
+// Occupancy: the FP64 form at three waves per SIMD and the FP32+IR form at
+// four, measured best on the 64-unknown sweep (two and five were slower).  The
+// register budget this sets spills a few hundred bytes per wave outside the
+// LU's inner steps; latency hiding across the extra waves is worth more.
 template <int N, bool IR>
-__global__ __launch_bounds__(64, 2) void k_newton_dense(DevProgram g, DevNrCfg cfg,
+__global__ __launch_bounds__(64, IR ? 4 : 3) void k_newton_dense(DevProgram g, DevNrCfg cfg,
                                                      const double* __restrict__ params,
                                                      const double* __restrict__ x0, int64_t B,
                                                      DevOut out, int escalated_only) {
@@ -377,7 +381,7 @@ __global__ __launch_bounds__(64, 2) void k_newton_dense(DevProgram g, DevNrCfg c
 // the time steps with FP64 refinement; an instance FP32 cannot resolve is left
 // SPX_ACTIVE for the FP64 pass (escalated_only), which reruns it from t = 0.
 template <int N, bool IR>
-__global__ __launch_bounds__(64, 2) void k_transient_dense(DevProgram g, DevNrCfg cfg, DevTran tr,
+__global__ __launch_bounds__(64, IR ? 4 : 3) void k_transient_dense(DevProgram g, DevNrCfg cfg, DevTran tr,
                                                         const double* __restrict__ params,
                                                         int64_t B, DevOut out, DevTranOut tout,
                                                         int escalated_only) {
