@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "handle.h"
@@ -42,6 +43,10 @@ struct spx_surrogate {
         *Y = nullptr, *dY = nullptr, *dH = nullptr, *dZ = nullptr, *d_loss = nullptr;
   int* d_perm = nullptr;
   int64_t perm_cap = 0;
+  // BF16 copies of W1, W2, W3 for the fused inference kernel, transposed to
+  // [out][in] with the input dimension padded to a multiple of 16
+  unsigned short* d_wt = nullptr;
+  int64_t wt_cap = 0;
 
   int64_t off_W1() const { return 0; }
   int64_t off_b1() const { return (int64_t)n_in * H; }
@@ -225,6 +230,126 @@ __global__ void k_write_prediction(const float* __restrict__ Y, int n_out, int n
   x0[(base + r) * n_x + j] = j < n_out ? (double)Y[r * n_out + j] * os[j] + om[j] : 0.0;
 }
 
+// ------------------------------------------------------- fused inference
+// The forward pass of one block of 64 rows in one workgroup: inputs gathered
+// and standardised straight from the sweep parameters, the three layers on
+// v_mfma_f32_16x16x16_bf16 with the hidden activations kept in LDS as BF16,
+// and the prediction written de-standardised as FP64.  Every rounding step is
+// the layered path's (inputs rounded to BF16 as the GEMM stages them,
+// accumulation over K in steps of 16 in order, bias then SiLU in FP32, BF16
+// rounding of the activation as the next GEMM stages it), so both paths give
+// the same x0.  Weights are read as BF16 [out][in] straight from L2: each
+// workgroup streams them once.
+constexpr int kInfRows = 64;
+
+// Wt[n][k] = bf16(W[k][n]) for n < N, k < K; zero up to np rows of kp
+// columns (the padding), so no separate clear is needed
+__global__ void k_weights_to_bf16t(const float* __restrict__ W, int K, int N, int kp, int np,
+                                   unsigned short* __restrict__ Wt) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)np * kp) return;
+  const int n = (int)(t / kp), k = (int)(t - (int64_t)n * kp);
+  Wt[t] = (k < K && n < N) ? to_bf16(W[(int64_t)k * N + n]) : (unsigned short)0;
+}
+
+// out[r][c] for c in this wave's columns: act(in[r][:] . Wt[c][:] + bias[c])
+// over K (multiple of 16); `in` is LDS BF16 with row stride ldi.  Each wave
+// takes ncol/4 columns; cols per wave must be a multiple of 16 (NT tiles).
+template <int NT, int K, class Epi>
+__device__ __forceinline__ void infer_layer(const unsigned short* in, int ldi,
+                                            const unsigned short* __restrict__ Wt, int col0,
+                                            int lane, Epi epi) {
+  floatx4 acc[4][NT];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int r16 = lane & 15, kq = (lane >> 4) * 4;
+#pragma unroll 4
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    bf16x4 b[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const unsigned short* wp = Wt + (int64_t)(col0 + j * 16 + r16) * K + k0 + kq;
+      const uint2 w = *reinterpret_cast<const uint2*>(wp);
+      b[j][0] = (short)(w.x & 0xffffu);
+      b[j][1] = (short)(w.x >> 16);
+      b[j][2] = (short)(w.y & 0xffffu);
+      b[j][3] = (short)(w.y >> 16);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint2 av = *reinterpret_cast<const uint2*>(in + (i * 16 + r16) * ldi + k0 + kq);
+      bf16x4 a;
+      a[0] = (short)(av.x & 0xffffu);
+      a[1] = (short)(av.x >> 16);
+      a[2] = (short)(av.y & 0xffffu);
+      a[3] = (short)(av.y >> 16);
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  const int c16 = lane & 15, rq = (lane >> 4) * 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) epi(i * 16 + rq + t, col0 + j * 16 + c16, acc[i][j][t]);
+}
+
+// hidden width = 64 * HT (each wave owns 16 * HT columns), inputs padded to KP
+template <int HT, int KP>
+__global__ __launch_bounds__(256, 2) void k_mlp_infer(
+    const double* __restrict__ params, int P, const int* __restrict__ slots, int n_in,
+    const float* __restrict__ norm, const unsigned short* __restrict__ W1t,
+    const float* __restrict__ b1, const unsigned short* __restrict__ W2t,
+    const float* __restrict__ b2, const unsigned short* __restrict__ W3t,
+    const float* __restrict__ b3, int n_out, int n_x, int64_t B, double* __restrict__ x0) {
+  constexpr int H = 64 * HT;
+  constexpr int LDH = H + 8;  // BF16 row stride (pads LDS banks)
+  constexpr int kp = KP, ldx = KP + 8;
+  extern __shared__ unsigned short lds[];
+  unsigned short* Xs = lds;                  // [64][ldx]
+  unsigned short* H1 = Xs + kInfRows * ldx;  // [64][LDH]
+  unsigned short* H2 = H1 + kInfRows * LDH;  // [64][LDH]
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const float* om = norm + 2 * n_in;
+  const float* os = om + n_out;
+  for (int64_t r0 = (int64_t)blockIdx.x * kInfRows; r0 < B; r0 += (int64_t)gridDim.x * kInfRows) {
+    // inputs (k_gather_inputs' arithmetic), padded columns zero
+    for (int e = tid; e < kInfRows * kp; e += 256) {
+      const int r = e / kp, i = e - r * kp;
+      const int64_t row = r0 + r;
+      float v = 0.f;
+      if (i < n_in && row < B) v = (float)((params[row * P + slots[i]] - norm[i]) / norm[n_in + i]);
+      Xs[r * ldx + i] = to_bf16(v);
+    }
+    __syncthreads();
+    infer_layer<HT, KP>(Xs, ldx, W1t, wave * 16 * HT, lane, [&](int r, int c, float a) {
+      const float v = a + b1[c];
+      H1[r * LDH + c] = to_bf16(v * sigmoidf(v));
+    });
+    __syncthreads();
+    infer_layer<HT, H>(H1, LDH, W2t, wave * 16 * HT, lane, [&](int r, int c, float a) {
+      const float v = a + b2[c];
+      H2[r * LDH + c] = to_bf16(v * sigmoidf(v));
+    });
+    __syncthreads();
+    // output layer: n_out <= 64, one 16-column tile per wave
+    infer_layer<1, H>(H2, LDH, W3t, wave * 16, lane, [&](int r, int c, float a) {
+      const int64_t row = r0 + r;
+      if (row < B && c < n_out) x0[row * n_x + c] = (double)(a + b3[c]) * os[c] + om[c];
+    });
+    for (int e = tid; e < kInfRows * (n_x - n_out); e += 256) {
+      const int r = e / (n_x - n_out), j = n_out + e % (n_x - n_out);
+      if (r0 + r < B) x0[(r0 + r) * n_x + j] = 0.0;
+    }
+    __syncthreads();
+  }
+}
+
 // Column means / standard deviations over the masked rows (FP64 sums).
 __global__ void k_colstats(const double* __restrict__ src, int64_t ld, const int* __restrict__ cols,
                            int64_t B, const uint8_t* __restrict__ mask, float* __restrict__ mean,
@@ -334,6 +459,50 @@ int surrogate_predict_locked(spx_surrogate* s, const double* d_params, int64_t B
   if (P != s->P) return fail(SPX_ERR_INVALID_ARGUMENT, "surrogate was set up for another P");
   if (n_x < s->n_out) return fail(SPX_ERR_INVALID_DIMENSION, "n_x smaller than surrogate output");
   hipStream_t st = s->h->stream;
+  // fused path: hidden width 64..256 in steps of 64, inputs and outputs <= 64
+  const int H = s->H, kp = (s->n_in + 15) / 16 * 16;
+  if (H % 64 == 0 && H <= 256 && s->n_in <= 64 && s->n_out <= 64) {
+    const int64_t n1 = (int64_t)H * kp, n2 = (int64_t)H * H, n3 = (int64_t)64 * H;
+    if (s->wt_cap < n1 + n2 + n3) {
+      if (s->d_wt) (void)hipFree(s->d_wt);
+      s->d_wt = nullptr;
+      s->wt_cap = 0;
+      if (hipMalloc(&s->d_wt, sizeof(unsigned short) * (n1 + n2 + n3)) != hipSuccess)
+        return fail(SPX_ERR_MEMORY_ALLOC, "surrogate BF16 weights");
+      s->wt_cap = n1 + n2 + n3;
+    }
+    unsigned short *w1t = s->d_wt, *w2t = w1t + n1, *w3t = w2t + n2;
+    const float* w = s->d_w;
+    // the output layer's columns past n_out are zero rows of W3t
+    hipLaunchKernelGGL(k_weights_to_bf16t, dim3(blocks(n1)), dim3(256), 0, st, w + s->off_W1(),
+                       s->n_in, H, kp, H, w1t);
+    hipLaunchKernelGGL(k_weights_to_bf16t, dim3(blocks(n2)), dim3(256), 0, st, w + s->off_W2(), H,
+                       H, H, H, w2t);
+    hipLaunchKernelGGL(k_weights_to_bf16t, dim3(blocks(n3)), dim3(256), 0, st, w + s->off_W3(), H,
+                       s->n_out, H, 64, w3t);
+    const size_t sm = sizeof(unsigned short) * kInfRows * ((kp + 8) + 2 * (H + 8));
+    const int64_t nb = std::min<int64_t>((B + kInfRows - 1) / kInfRows, 1 << 20);
+    auto launch = [&](auto kern) -> hipError_t {
+      hipError_t e = hipFuncSetAttribute((const void*)kern,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), sm, st, d_params, P, s->d_slots,
+                         s->n_in, s->d_norm, w1t, w + s->off_b1(), w2t, w + s->off_b2(), w3t,
+                         w + s->off_b3(), s->n_out, n_x, B, d_x0);
+      return hipGetLastError();
+    };
+    auto by_k = [&](auto ht) -> hipError_t {
+      constexpr int HT = decltype(ht)::value;
+      return kp == 16 ? launch(k_mlp_infer<HT, 16>) : kp == 32 ? launch(k_mlp_infer<HT, 32>)
+           : kp == 48 ? launch(k_mlp_infer<HT, 48>) : launch(k_mlp_infer<HT, 64>);
+    };
+    hipError_t e = H == 64 ? by_k(std::integral_constant<int, 1>{})
+                 : H == 128 ? by_k(std::integral_constant<int, 2>{})
+                 : H == 192 ? by_k(std::integral_constant<int, 3>{})
+                 : by_k(std::integral_constant<int, 4>{});
+    SPX_HIP(e, SPX_ERR_BACKEND);
+    return SPX_OK;
+  }
   const int64_t chunk = 65536;
   int rc = ensure_capacity(s, std::min<int64_t>(B, chunk));
   if (rc) return rc;
@@ -420,7 +589,8 @@ int spx_surrogate_destroy(spx_surrogate* s) {
     (void)hipStreamSynchronize(h->stream);
     if (h->surrogate == s) h->surrogate = nullptr;
     void* ptrs[] = {s->d_w, s->d_grad, s->d_m, s->d_v, s->d_norm, s->d_loss, s->d_slots,
-                    s->d_perm, s->d_wout, s->X, s->T, s->Z1, s->H1, s->Z2, s->H2, s->Y, s->dY, s->dH, s->dZ};
+                    s->d_perm, s->d_wout, s->X, s->T, s->Z1, s->H1, s->Z2, s->H2, s->Y, s->dY, s->dH, s->dZ,
+                    s->d_wt};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
   }

@@ -116,3 +116,30 @@ def test_trained_surrogate_saves_newton_iterations(oracle):
     # and the warm-started result agrees with the zero-start one at SPICE tolerance
     assert np.all(np.abs(warm.solutions - base.solutions)
                   <= 2e-3 * np.abs(base.solutions) + 2e-6)
+
+
+@pytest.mark.parametrize("n_in,h,n_out", [(20, 128, 48), (7, 64, 5), (32, 320, 64)])
+def test_predict_other_shapes_match_oracle(n_in, h, n_out):
+    # widths the fused inference kernel takes (hidden a multiple of 64 up to
+    # 256, padded inputs / outputs) and one it does not (layered GEMMs)
+    import torch
+    rng = np.random.default_rng(n_in + h)
+    s = E.Surrogate(n_in, h, n_out, seed=5)
+    w = (rng.standard_normal(s.num_weights) * 0.2).astype(np.float32)
+    s.set_weights(w)
+    P, B, nx = 40, 333, n_out + 3
+    params = rng.standard_normal((B, P))
+    slots = rng.choice(P, n_in, replace=False).astype(np.int32)
+    s.set_inputs(slots, P)
+    im, isd = rng.standard_normal(n_in), 0.5 + rng.random(n_in)
+    om, osd = rng.standard_normal(n_out), 0.5 + rng.random(n_out)
+    s.set_normalization(im, isd, om, osd)
+    x0 = torch.full((B, nx), -1.0, dtype=torch.float64, device="cuda")
+    s.predict(torch.as_tensor(params, device="cuda"), nx, x0)
+    E.Device.get(0).synchronize()
+    got = x0.cpu().numpy()
+    Xn = ((params[:, slots] - im.astype(np.float32)) / isd.astype(np.float32)).astype(np.float32)
+    Y, _ = M.forward(w, Xn, n_in, h, n_out, bf=True)
+    want = Y * osd.astype(np.float32) + om.astype(np.float32)
+    np.testing.assert_allclose(got[:, :n_out], want, rtol=2e-3, atol=2e-3 * np.abs(want).max())
+    assert np.all(got[:, n_out:] == 0)
