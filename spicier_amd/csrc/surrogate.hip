@@ -47,6 +47,9 @@ struct spx_surrogate {
   // [out][in] with the input dimension padded to a multiple of 16
   unsigned short* d_wt = nullptr;
   int64_t wt_cap = 0;
+  // the FP32 weights change with every version bump (creation, set_weights,
+  // each training step); d_wt holds the BF16 copy of version wt_version
+  uint64_t w_version = 1, wt_version = 0;
 
   int64_t off_W1() const { return 0; }
   int64_t off_b1() const { return (int64_t)n_in * H; }
@@ -473,13 +476,17 @@ int surrogate_predict_locked(spx_surrogate* s, const double* d_params, int64_t B
     }
     unsigned short *w1t = s->d_wt, *w2t = w1t + n1, *w3t = w2t + n2;
     const float* w = s->d_w;
-    // the output layer's columns past n_out are zero rows of W3t
-    hipLaunchKernelGGL(k_weights_to_bf16t, dim3(blocks(n1)), dim3(256), 0, st, w + s->off_W1(),
-                       s->n_in, H, kp, H, w1t);
-    hipLaunchKernelGGL(k_weights_to_bf16t, dim3(blocks(n2)), dim3(256), 0, st, w + s->off_W2(), H,
-                       H, H, H, w2t);
-    hipLaunchKernelGGL(k_weights_to_bf16t, dim3(blocks(n3)), dim3(256), 0, st, w + s->off_W3(), H,
-                       s->n_out, H, 64, w3t);
+    if (s->wt_version != s->w_version) {
+      // the output layer's columns past n_out are zero rows of W3t
+      hipLaunchKernelGGL(k_weights_to_bf16t, dim3(blocks(n1)), dim3(256), 0, st, w + s->off_W1(),
+                         s->n_in, H, kp, H, w1t);
+      hipLaunchKernelGGL(k_weights_to_bf16t, dim3(blocks(n2)), dim3(256), 0, st, w + s->off_W2(),
+                         H, H, H, H, w2t);
+      hipLaunchKernelGGL(k_weights_to_bf16t, dim3(blocks(n3)), dim3(256), 0, st, w + s->off_W3(),
+                         H, s->n_out, H, 64, w3t);
+      SPX_HIP(hipGetLastError(), SPX_ERR_BACKEND);
+      s->wt_version = s->w_version;
+    }
     const size_t sm = sizeof(unsigned short) * kInfRows * ((kp + 8) + 2 * (H + 8));
     const int64_t nb = std::min<int64_t>((B + kInfRows - 1) / kInfRows, 1 << 20);
     auto launch = [&](auto kern) -> hipError_t {
@@ -631,6 +638,7 @@ int spx_surrogate_set_weights(spx_surrogate* s, const float* host, int64_t count
   DeviceGuard dg(s->h->device);
   SPX_HIP(hipStreamSynchronize(s->h->stream), SPX_ERR_BACKEND);
   SPX_HIP(hipMemcpy(s->d_w, host, sizeof(float) * count, hipMemcpyHostToDevice), SPX_ERR_TRANSFER);
+  ++s->w_version;
   return SPX_OK;
 }
 
@@ -747,6 +755,7 @@ int spx_surrogate_train_device(spx_surrogate* s, const double* d_params, const d
       ++s->adam_t;
       const float b1 = 0.9f, b2 = 0.999f;
       const float c1 = 1.f - std::pow(b1, (float)s->adam_t), c2 = 1.f - std::pow(b2, (float)s->adam_t);
+      ++s->w_version;
       hipLaunchKernelGGL(k_adam, dim3(blocks(s->nw)), dim3(256), 0, st, s->d_w, s->d_grad, s->d_m,
                          s->d_v, s->nw, lrt, b1, b2, 1e-8f, c1, c2);
     }
