@@ -117,6 +117,12 @@ def other_workloads(E, A, CK, torch, np, dev, stream):
     gm_ms = _event_ms(stream, torch, gm, reps=1)
     conv = int(dconv.sum().item())
     it = float(dit.float().mean().item())
+    gcfg = E.GmresConfig(restart=30, max_iter=5000, tol=1e-8,
+                         precision=E.Precision.F32_IR).to_c()
+    gm32_ms = _event_ms(stream, torch, gm, reps=1)
+    conv32 = int(dconv.sum().item())
+    it32 = float(dit.float().mean().item())
+    res32 = float(dres.max().item())
 
     def lu():
         E._check(lib.spx_sparse_solve_batch_f64_device(
@@ -127,6 +133,9 @@ def other_workloads(E, A, CK, torch, np, dev, stream):
     out["grid_linear_systems"] = dict(
         systems=B, unknowns=n, gmres_jacobi30_ms=gm_ms, gmres_systems_per_s=B / gm_ms * 1e3,
         gmres_converged=conv, gmres_mean_iterations=it, gmres_tol=1e-8,
+        gmres_fp32_ir_ms=gm32_ms, gmres_fp32_ir_systems_per_s=B / gm32_ms * 1e3,
+        gmres_fp32_ir_converged=conv32, gmres_fp32_ir_mean_inner_iterations=it32,
+        gmres_fp32_ir_max_fp64_residual=res32,
         sparse_lu_ms=lu_ms, sparse_lu_systems_per_s=B / lu_ms * 1e3)
     # mixed-size transient batch, one graph launch, autotuned, in both forms
     sizes = (8, 16, 32, 64, 128, 256)

@@ -143,6 +143,10 @@ typedef struct spx_gmres_config {
     int32_t max_iter; /* total Arnoldi steps (500) */
     double tol;       /* relative residual (1e-8) */
     int32_t jacobi;   /* 1: right Jacobi preconditioning (M = diag(A), |d| < 1e-30 -> 1) */
+    int32_t precision; /* SPX_PREC_F64, or SPX_PREC_F32_IR: FP32 Krylov vectors, inner
+                          products and rotations inside FP64 iterative refinement (the
+                          residual and x in FP64, the same tol on the FP64 residual;
+                          iterations count the FP32 Arnoldi steps) */
 } spx_gmres_config;
 
 void spx_gmres_config_default(spx_gmres_config* cfg);

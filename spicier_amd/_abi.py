@@ -85,7 +85,7 @@ class SpxNrConfig(C.Structure):
 
 class SpxGmresConfig(C.Structure):
     _fields_ = [("restart", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
-                ("jacobi", C.c_int32)]
+                ("jacobi", C.c_int32), ("precision", C.c_int32)]
 
 
 class SpxSweepResult(C.Structure):

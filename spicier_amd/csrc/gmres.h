@@ -18,12 +18,13 @@ struct DevGmres {
   const int* diag_pos;  // n: position of (i, i), -1 if absent
 };
 
-size_t gmres_smem_bytes(int n, int restart);
-int64_t gmres_slot_doubles(int n, int num_pos, int restart);
-hipError_t gmres_grid(int n, int restart, int64_t B, int* grid);
+// precision: 0 FP64 throughout, 1 FP32 Krylov solver inside FP64 refinement
+size_t gmres_smem_bytes(int n, int restart, int precision);
+int64_t gmres_slot_doubles(int n, int num_pos, int restart, int precision);
+hipError_t gmres_grid(int n, int restart, int precision, int64_t B, int* grid);
 hipError_t launch_gmres(const DevGmres& p, int64_t nnz, const double* values, const double* rhs,
                         const double* x0, int64_t B, int restart, int max_iter, double tol,
-                        int jacobi, double* x, int* iters, double* res, uint8_t* conv,
-                        double* ws, int grid, hipStream_t st);
+                        int jacobi, int precision, double* x, int* iters, double* res,
+                        uint8_t* conv, double* ws, int grid, hipStream_t st);
 
 }  // namespace spx

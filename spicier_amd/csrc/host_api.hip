@@ -571,6 +571,9 @@ int gmres_locked(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows, con
   if (!cfg) return fail(SPX_ERR_INVALID_ARGUMENT, "null GMRES configuration");
   if (cfg->restart < 1 || cfg->restart > 256 || cfg->max_iter < 0 || !(cfg->tol >= 0.0))
     return fail(SPX_ERR_INVALID_ARGUMENT, "GMRES needs 1 <= restart <= 256, max_iter >= 0, tol >= 0");
+  if (cfg->precision != SPX_PREC_F64 && cfg->precision != SPX_PREC_F32_IR)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "unknown GMRES precision");
+  const int prec = cfg->precision == SPX_PREC_F32_IR ? 1 : 0;
   if (n < 1 || n > kSparseMaxN)
     return fail(SPX_ERR_INVALID_DIMENSION, "GMRES needs 1 <= n <= " + std::to_string(kSparseMaxN));
   if (nnz < 1 || nnz > 0x7fffffffLL || !rows || !cols)
@@ -631,15 +634,15 @@ int gmres_locked(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows, con
   const int* base = static_cast<const int*>(dp);
   DevGmres p{(int)n, h->gm_num_pos, base + h->gm_o_rowptr, base + h->gm_o_col,
              base + h->gm_o_off, base + h->gm_o_trip, base + h->gm_o_diag};
-  if (gmres_smem_bytes((int)n, cfg->restart) > 160 * 1024)
+  if (gmres_smem_bytes((int)n, cfg->restart, prec) > 160 * 1024)
     return fail(SPX_ERR_INVALID_DIMENSION, "GMRES workgroup state exceeds the LDS");
   int grid = 0;
-  SPX_HIP(gmres_grid((int)n, cfg->restart, B, &grid), SPX_ERR_BACKEND);
+  SPX_HIP(gmres_grid((int)n, cfg->restart, prec, B, &grid), SPX_ERR_BACKEND);
   void* ws = nullptr;
-  const int64_t slot = gmres_slot_doubles((int)n, h->gm_num_pos, cfg->restart);
+  const int64_t slot = gmres_slot_doubles((int)n, h->gm_num_pos, cfg->restart, prec);
   if ((rc = workspace(h, "gm_ws", sizeof(double) * (size_t)grid * slot, &ws))) return rc;
   SPX_HIP(launch_gmres(p, nnz, d_values, d_rhs, d_x0, B, cfg->restart, cfg->max_iter, cfg->tol,
-                       cfg->jacobi, d_x, d_it, d_res, d_conv, static_cast<double*>(ws), grid,
+                       cfg->jacobi, prec, d_x, d_it, d_res, d_conv, static_cast<double*>(ws), grid,
                        h->stream),
           SPX_ERR_BACKEND);
   return SPX_OK;
@@ -1334,6 +1337,7 @@ void spx_gmres_config_default(spx_gmres_config* cfg) {
   cfg->max_iter = 500;
   cfg->tol = 1e-8;
   cfg->jacobi = 1;
+  cfg->precision = SPX_PREC_F64;
 }
 
 int spx_gmres_solve_batch_f64_device(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows,

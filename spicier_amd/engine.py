@@ -320,10 +320,12 @@ class GmresConfig:
     tol: float = 1e-8
     restart: int = 30
     jacobi: bool = True
+    # Precision.F32_IR: FP32 Krylov arithmetic inside FP64 iterative refinement
+    precision: int = A.SPX_PREC_F64
 
     def to_c(self) -> A.SpxGmresConfig:
         return A.SpxGmresConfig(restart=self.restart, max_iter=self.max_iter, tol=self.tol,
-                                jacobi=int(self.jacobi))
+                                jacobi=int(self.jacobi), precision=int(self.precision))
 
 
 @dataclass

@@ -103,3 +103,43 @@ def test_gmres_mna_grid_with_warm_start():
     # starting from the solution converges at once
     r2 = g.solve_batch_triplets(rows, cols, vals, b, n, x0=want)
     assert np.all(r2.iterations <= 1) and np.all(r2.converged)
+
+
+# ------------------------------------------- FP32 Krylov inside FP64 refinement
+@pytest.mark.parametrize("tol", [1e-8, 1e-12])
+def test_gmres_fp32_ir_reaches_fp64_residual(tol):
+    # same stopping test on the FP64 true residual as the FP64 kernel; the
+    # Krylov vectors, inner products and rotations are FP32
+    from test_sparse_gpu import _dense as dense3, _grid_mna
+    rng = np.random.default_rng(6)
+    B = 12
+    n, rows, cols, vals = _grid_mna(16, rng, B)
+    b = np.zeros((B, n))
+    b[:, 256:] = [1.0, 1.5, 2.0, 3.3]
+    b[:, :256] += 1e-4 * rng.standard_normal((B, 256))
+    cfg = E.GmresConfig(tol=tol, max_iter=20000, precision=E.Precision.F32_IR)
+    r = E.HipBatchedGmres(cfg).solve_batch_triplets(rows, cols, vals, b, n)
+    m = dense3(n, rows, cols, vals)
+    x = r.solutions.reshape(B, n)
+    resid = np.linalg.norm(b - np.einsum("bij,bj->bi", m, x), axis=1) / np.linalg.norm(b, axis=1)
+    assert np.all(r.converged) and np.all(resid < tol), resid.max()
+    np.testing.assert_allclose(r.residual, resid, rtol=1e-6, atol=1e-3 * tol)
+    want = np.linalg.solve(m, b[..., None])[..., 0]
+    err = np.abs(x - want).max(axis=1) / np.abs(want).max(axis=1)
+    assert np.all(err < 1e5 * tol), err.max()   # condition number ~1e4
+
+
+def test_gmres_fp32_ir_random_family_and_zero_rhs():
+    rng = np.random.default_rng(8)
+    n, B = 150, 30
+    rows, cols, vals = _family(rng, n, B)
+    b = rng.standard_normal((B, n))
+    b[3] = 0.0
+    cfg = E.GmresConfig(tol=1e-10, precision=E.Precision.F32_IR)
+    r = E.HipBatchedGmres(cfg).solve_batch_triplets(rows, cols, vals, b, n)
+    assert np.all(r.converged)
+    assert np.all(r.solutions.reshape(B, n)[3] == 0) and r.iterations[3] == 0
+    for i in range(B):
+        m = _dense(n, rows, cols, vals[i])
+        x = r.solution(i)
+        assert np.linalg.norm(b[i] - m @ x) <= 1e-10 * max(np.linalg.norm(b[i]), 1e-300) or i == 3
