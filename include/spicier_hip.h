@@ -583,6 +583,127 @@ int spx_solve_batch_f64_pipelined(spx_handle* h, const double* matrices, const d
                                   double* solutions, uint8_t* singular, int64_t* num_singular,
                                   int64_t* chunks_processed);
 
+/* ------------------------------------------------------------------------- */
+/* Batch spectral analysis of sweep waveforms: windowed FFT spectra
+ * (compute_fft / compute_fft_from_samples,
+ * crates/spicier-solver/src/spectral/fft.rs:98-367), THD with harmonic peak
+ * search (compute_thd / compute_thd_from_samples, spectral/thd.rs:60-312),
+ * and their batch forms over a sweep (compute_batch_thd,
+ * compute_batch_thd_with_stats, compute_thd_statistics,
+ * compute_batch_fft_stats, crates/spicier-batched-sweep/src/spectral.rs:46-330).
+ * One workgroup runs one waveform: resampling, window, DFT in LDS (a
+ * power-of-two Stockham transform; other lengths by chirp-z), spectrum,
+ * harmonic search and the per-bin ensemble statistics are one kernel.
+ * Transform lengths: powers of two up to 8192, other lengths up to 4096. */
+enum {
+    SPX_WINDOW_RECTANGULAR = 0,
+    SPX_WINDOW_HANNING = 1, /* the reference's default */
+    SPX_WINDOW_HAMMING = 2,
+    SPX_WINDOW_BLACKMAN = 3
+};
+
+/* SpectralConfig (fft.rs:10-29); fft_size 0 = None (next power of two). */
+typedef struct spx_spectral_config {
+    int32_t window;
+    int32_t zero_padding;
+    int64_t fft_size;
+} spx_spectral_config;
+void spx_spectral_config_default(spx_spectral_config* cfg);
+
+/* One node's waveform for every instance of a sweep.  Point t of waveform b
+ * is data[b*batch_stride + t*point_stride] (device memory), so the probe
+ * column of spx_tran_result.waveforms is read in place.  times (host,
+ * num_points, strictly increasing and finite): the waveform is resampled to
+ * uniform steps first, as compute_fft does (resample_uniform with its
+ * automatic rate, fft.rs:98-147, interpolate_at, transient/result.rs:41-68).
+ * times NULL: the points are already uniform at sample_rate
+ * (compute_fft_from_samples). */
+typedef struct spx_waveform_batch {
+    const double* data;
+    int64_t batch;
+    int64_t batch_stride;
+    int32_t num_points;
+    int32_t point_stride;
+    const double* times;
+    double sample_rate;
+} spx_waveform_batch;
+
+/* Sizes of the spectrum a batch produces.  fft_size 0: the reference's empty
+ * SpectralResult (no samples, or sample_rate <= 0). */
+typedef struct spx_spectrum_info {
+    int64_t num_samples;     /* samples after resampling */
+    int64_t fft_size;        /* DFT length */
+    int64_t num_bins;        /* fft_size/2 + 1 (DC to Nyquist) */
+    double sample_rate;
+    double freq_resolution;  /* frequencies[i] = i * freq_resolution */
+} spx_spectrum_info;
+int spx_spectrum_info_query(spx_handle* h, const spx_waveform_batch* wf,
+                            const spx_spectral_config* cfg, spx_spectrum_info* info);
+
+/* Device outputs of spx_batch_fft_device; any pointer may be NULL.
+ * Per waveform [batch][num_bins]: SpectralResult's magnitude, magnitude_db,
+ * phase and power_spectral_density.  Per bin [num_bins]:
+ * BatchSpectralResult's mean/std/min/max magnitude over the batch
+ * (population standard deviation, spectral.rs:262-330). */
+typedef struct spx_spectrum_out {
+    double* magnitude;
+    double* magnitude_db;
+    double* phase;
+    double* psd;
+    double* mean_magnitude;
+    double* std_magnitude;
+    double* min_magnitude;
+    double* max_magnitude;
+} spx_spectrum_out;
+/* info (may be NULL) receives the sizes.  Returns once the work is enqueued. */
+int spx_batch_fft_device(spx_handle* h, const spx_waveform_batch* wf,
+                         const spx_spectral_config* cfg, const spx_spectrum_out* out,
+                         spx_spectrum_info* info);
+
+/* Device outputs of spx_batch_thd_device; any pointer may be NULL.  Per
+ * waveform [batch]: ThdResult's thd_percent, thd_db, fundamental_freq (the
+ * bin found), fundamental_magnitude and harmonic_rms.  harmonics
+ * [batch][num_harmonics+1][5]: per HarmonicInfo (harmonic h+1) frequency,
+ * magnitude, magnitude_db, phase and relative_percent; rows past
+ * found_harmonics are left untouched. */
+typedef struct spx_thd_out {
+    double* thd_percent;
+    double* thd_db;
+    double* fundamental_freq;
+    double* fundamental_magnitude;
+    double* harmonic_rms;
+    double* harmonics;
+} spx_thd_out;
+/* compute_batch_thd (spectral.rs:46-71): Hanning window, zero padding to the
+ * next power of two.  found_harmonics (host, may be NULL) receives
+ * ThdResult::num_harmonics, the harmonics at or below Nyquist (the same for
+ * every waveform of the batch). */
+int spx_batch_thd_device(spx_handle* h, const spx_waveform_batch* wf, double fundamental_freq,
+                         int32_t num_harmonics, const spx_thd_out* out, int32_t* found_harmonics);
+
+/* ThdStatistics (spectral.rs:12-31). */
+typedef struct spx_thd_statistics {
+    double mean_thd;
+    double std_thd;
+    double min_thd;
+    double max_thd;
+    double median_thd;
+    double yield_percent;
+    int64_t num_samples;
+    double thd_spec;
+} spx_thd_statistics;
+/* compute_thd_statistics (spectral.rs:103-160) over count device values:
+ * population std, median of the sorted values, yield = share <= thd_spec.
+ * A NaN value is an SPX_ERR_INVALID_ARGUMENT error (the reference's sort
+ * panics on it).  Blocks until the statistics are on the host. */
+int spx_thd_statistics_device(spx_handle* h, const double* d_thd_percent, int64_t count,
+                              double thd_spec, spx_thd_statistics* stats);
+
+/* WindowFunction::apply's weights and coherent_gain (window.rs:24-108);
+ * host arithmetic, the same expressions the spectra use. */
+int spx_window_coefficients(int32_t window, int64_t n, double* out);
+double spx_window_coherent_gain(int32_t window, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

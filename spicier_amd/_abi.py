@@ -128,6 +128,41 @@ class SpxPipelineConfig(C.Structure):
 
 SPX_SELECT_ACTIVE, SPX_SELECT_CONVERGED, SPX_SELECT_FAILED = 0, 1, 2
 
+SPX_WINDOW_RECTANGULAR, SPX_WINDOW_HANNING, SPX_WINDOW_HAMMING, SPX_WINDOW_BLACKMAN = 0, 1, 2, 3
+
+
+class SpxSpectralConfig(C.Structure):
+    _fields_ = [("window", C.c_int32), ("zero_padding", C.c_int32), ("fft_size", C.c_int64)]
+
+
+class SpxWaveformBatch(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("batch", C.c_int64), ("batch_stride", C.c_int64),
+                ("num_points", C.c_int32), ("point_stride", C.c_int32),
+                ("times", C.POINTER(C.c_double)), ("sample_rate", C.c_double)]
+
+
+class SpxSpectrumInfo(C.Structure):
+    _fields_ = [("num_samples", C.c_int64), ("fft_size", C.c_int64), ("num_bins", C.c_int64),
+                ("sample_rate", C.c_double), ("freq_resolution", C.c_double)]
+
+
+class SpxSpectrumOut(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in ("magnitude", "magnitude_db", "phase", "psd",
+                                          "mean_magnitude", "std_magnitude", "min_magnitude",
+                                          "max_magnitude")]
+
+
+class SpxThdOut(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in ("thd_percent", "thd_db", "fundamental_freq",
+                                          "fundamental_magnitude", "harmonic_rms", "harmonics")]
+
+
+class SpxThdStatistics(C.Structure):
+    _fields_ = [("mean_thd", C.c_double), ("std_thd", C.c_double), ("min_thd", C.c_double),
+                ("max_thd", C.c_double), ("median_thd", C.c_double),
+                ("yield_percent", C.c_double), ("num_samples", C.c_int64),
+                ("thd_spec", C.c_double)]
+
 SPX_MIXED_GRAPH, SPX_MIXED_AUTOTUNE = 1, 2
 SPX_KERNEL_AUTO, SPX_KERNEL_DENSE, SPX_KERNEL_SPARSE64, SPX_KERNEL_SPARSE256 = 0, 1, 2, 3
 SPX_KERNEL_LANES = 4
@@ -231,6 +266,20 @@ def load_library():
                                                     C.POINTER(SpxPipelineConfig), vp, u8p,
                                                     C.POINTER(C.c_int64),
                                                     C.POINTER(C.c_int64)]),
+        "spx_spectral_config_default": (None, [C.POINTER(SpxSpectralConfig)]),
+        "spx_spectrum_info_query": (C.c_int, [vp, C.POINTER(SpxWaveformBatch),
+                                              C.POINTER(SpxSpectralConfig),
+                                              C.POINTER(SpxSpectrumInfo)]),
+        "spx_batch_fft_device": (C.c_int, [vp, C.POINTER(SpxWaveformBatch),
+                                           C.POINTER(SpxSpectralConfig),
+                                           C.POINTER(SpxSpectrumOut),
+                                           C.POINTER(SpxSpectrumInfo)]),
+        "spx_batch_thd_device": (C.c_int, [vp, C.POINTER(SpxWaveformBatch), C.c_double, i32,
+                                           C.POINTER(SpxThdOut), C.POINTER(C.c_int32)]),
+        "spx_thd_statistics_device": (C.c_int, [vp, vp, i64, C.c_double,
+                                                C.POINTER(SpxThdStatistics)]),
+        "spx_window_coefficients": (C.c_int, [i32, i64, vp]),
+        "spx_window_coherent_gain": (C.c_double, [i32, i64]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

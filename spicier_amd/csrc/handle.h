@@ -12,6 +12,7 @@
 
 #include "../../include/spicier_hip.h"
 #include "sparse_plan.h"
+#include "spectral.h"
 
 struct spx_surrogate;
 
@@ -47,6 +48,8 @@ struct spx_handle {
   std::vector<int> gm_blob;
   int gm_num_pos = 0;
   size_t gm_o_rowptr = 0, gm_o_col = 0, gm_o_off = 0, gm_o_trip = 0, gm_o_diag = 0;
+  // batch spectral analysis: the plan whose tables are in the "sp_tables" workspace
+  spx::SpectralPlan spectral;
 };
 
 namespace spx {
