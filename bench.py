@@ -74,8 +74,9 @@ def other_workloads(E, A, CK, torch, np, dev, stream):
     """The sweep path's other configurations, measured beside the headline:
     the 4,096-instance 32x32 grid + diodes DC sweep (sparse LU Newton), the
     same grid's Jacobian family through GMRES-Jacobi(30) and through the
-    batched sparse LU, and the 32,768-instance mixed-size Backward-Euler
-    transient batch (one graph launch, autotuned kernel per size class)."""
+    batched sparse LU, the 32,768-instance mixed-size Backward-Euler
+    transient batch (one graph launch, autotuned kernel per size class), and
+    batch THD / spectrum statistics of 32,768 waveforms."""
     out = {}
     lib = dev.lib
     cuda = torch.device("cuda:0")
@@ -162,6 +163,39 @@ def other_workloads(E, A, CK, torch, np, dev, stream):
             instance_steps_per_s=acc * 200 / graph_ms * 1e3, accepted=acc, fp64_escalated=esc,
             note="FP32+IR applies to the register-LU classes; sparse classes factor in FP64"
             if prec == E.Precision.F32_IR else "")
+    # batch spectral analysis: THD and ensemble spectrum statistics of 32,768
+    # waveforms of 1,000 points (10 periods of a 1 kHz fundamental at 100 kHz,
+    # harmonics at 2-5 kHz), Hanning window, zero padding to 1,024
+    from spicier_amd import spectral as SP
+    Bw, T, H = 32768, 1000, 5
+    gen = torch.Generator(device=cuda).manual_seed(5)
+    t = torch.arange(T, dtype=torch.float64, device=cuda) / 1e5
+    a = torch.rand(Bw, 5, dtype=torch.float64, device=cuda, generator=gen) * 0.1
+    a[:, 0] += 0.9
+    wav = sum(a[:, h:h + 1] * torch.sin(2 * np.pi * 1e3 * (h + 1) * t) for h in range(5))
+    wav = wav.contiguous()
+    torch.cuda.synchronize()
+    wc = SP.WaveformBatch(wav, sample_rate=1e5)._prepare()
+    cols = [torch.empty(Bw, dtype=torch.float64, device=cuda) for _ in range(5)]
+    harm = torch.empty(Bw, H + 1, 5, dtype=torch.float64, device=cuda)
+    tout = A.SpxThdOut(*[A.ptr(v) for v in cols + [harm]])
+    found = C.c_int32()
+    thd_ms = _event_ms(stream, torch, lambda: E._check(lib.spx_batch_thd_device(
+        dev.handle, C.byref(wc), 1e3, H, C.byref(tout), C.byref(found))))
+    scfg = SP.SpectralConfig().to_c()
+    info = A.SpxSpectrumInfo()
+    E._check(lib.spx_spectrum_info_query(dev.handle, C.byref(wc), C.byref(scfg), C.byref(info)))
+    agg = [torch.empty(info.num_bins, dtype=torch.float64, device=cuda) for _ in range(4)]
+    sout = A.SpxSpectrumOut(None, None, None, None, *[A.ptr(v) for v in agg])
+    stats_ms = _event_ms(stream, torch, lambda: E._check(lib.spx_batch_fft_device(
+        dev.handle, C.byref(wc), C.byref(scfg), C.byref(sout), None)))
+    st = SP.compute_thd_statistics(cols[0], 10.0)
+    out["batch_thd"] = dict(
+        waveforms=Bw, points=T, fft_size=int(info.fft_size), harmonics=int(found.value),
+        thd_ms=thd_ms, thd_waveforms_per_s=Bw / thd_ms * 1e3,
+        thd_input_GBps=Bw * T * 8 / thd_ms / 1e6, mean_thd_percent=st.mean_thd,
+        fft_stats_ms=stats_ms, fft_stats_waveforms_per_s=Bw / stats_ms * 1e3,
+        note="one workgroup per waveform: window, packed real transform in LDS, harmonic search")
     return out
 
 

@@ -592,9 +592,12 @@ int spx_solve_batch_f64_pipelined(spx_handle* h, const double* matrices, const d
  * compute_batch_thd_with_stats, compute_thd_statistics,
  * compute_batch_fft_stats, crates/spicier-batched-sweep/src/spectral.rs:46-330).
  * One workgroup runs one waveform: resampling, window, DFT in LDS (a
- * power-of-two Stockham transform; other lengths by chirp-z), spectrum,
- * harmonic search and the per-bin ensemble statistics are one kernel.
- * Transform lengths: powers of two up to 8192, other lengths up to 4096. */
+ * power-of-two Stockham transform, two real samples per complex point; other
+ * lengths by chirp-z), spectrum, harmonic search and the per-bin ensemble
+ * statistics are one kernel, for lengths up to 16384 (powers of two) or 4096
+ * (others).  Longer transforms, up to 2^22 points (2^21 for lengths that are
+ * not powers of two), run as four-step transforms through HBM in chunks of
+ * the batch. */
 enum {
     SPX_WINDOW_RECTANGULAR = 0,
     SPX_WINDOW_HANNING = 1, /* the reference's default */

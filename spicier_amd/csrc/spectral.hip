@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "device_common.h"
@@ -33,13 +34,15 @@
 namespace spx {
 namespace {
 
-// Power-of-two transforms up to 8192 points run in one workgroup's LDS
-// (128 KB for the complex vector, within gfx950's 160 KB per workgroup);
-// longer ones (up to 2^22) take the four-step path through HBM.
+// Transforms of up to 8192 complex points run in one workgroup's LDS (128 KB
+// for the vector, within gfx950's 160 KB per workgroup): real waveforms up
+// to 16384 points, packed two samples per complex point, and chirp-z
+// transforms up to 8192; longer ones (up to 2^22) take the four-step path
+// through HBM.
 constexpr int kMaxLog2 = 13;
 constexpr int kMaxLog2Global = 22;
 constexpr int kMaxHarmonics = 255;    // harmonic rows per waveform in LDS
-constexpr int kStatsGrid = 1024;      // workgroups when per-bin statistics are accumulated
+constexpr int kStatsGrid = 4096;      // workgroups when per-bin statistics are accumulated
 constexpr int kPlainGrid = 8192;      // workgroups otherwise (grid-stride over the batch)
 constexpr size_t kAbsent = SIZE_MAX;
 
@@ -61,8 +64,9 @@ struct SpecArgs {
   const double2* chirp;  // [N] exp(-i pi m^2 / N)
   const double2* bfft;   // [M] transform of the conjugate chirp, wrapped
   double norm;           // samples * coherent gain
+  double inv_norm;       // 1 / norm: bins are scaled by it (within an ulp of dividing)
   double *mag, *db, *phase, *psd;
-  double* part;          // [gridDim.x][num_bins][4] Welford partials, NULL = none
+  double* part;          // [gridDim.x][num_bins][5] Welford partials (count, mean, M2, min, max)
   int32_t thd, found, hstride;
   double f0, fr, search;  // fr: bin spacing; search: 2 x (frequencies[1] - frequencies[0])
   double *thd_pct, *thd_db, *fund_freq, *fund_mag, *hrms, *harm;
@@ -171,15 +175,46 @@ __global__ __launch_bounds__(threads_for(LOG2M)) void k_fft_inplace(double2* v,
   for (int s = threadIdx.x; s < M; s += NT) v[s] = make_double2(re[s], im[s]);
 }
 
-// The spectrum of bin k of the transform in LDS, as fft.rs:228-258 forms it.
+// Bin k of a waveform's DFT, X_k, read from a transform in LDS or HBM.
+// Direct: the transform is X itself.  Packed: a real waveform of even
+// length N was transformed as the N/2 complex points z_n = x_2n + i x_2n+1;
+// with W = Z_(N/2-k), E = (Z_k + conj W)/2 and O = (Z_k - conj W)/(2i),
+// X_k = E + W_N^k O for 0 <= k <= N/2 (Z_(N/2) = Z_0).
+struct DirectView {
+  const double* re;
+  const double* im;
+  __device__ __forceinline__ void get(int k, double& xr, double& xi) const {
+    xr = re[k];
+    xi = im[k];
+  }
+};
+struct PackedView {
+  const double* re;
+  const double* im;
+  const double2* tw;  // W_N^m
+  int half;           // N / 2
+  __device__ __forceinline__ void get(int k, double& xr, double& xi) const {
+    const int kk = k == half ? 0 : k, j = k == 0 ? 0 : half - k;
+    const double zr = re[kk], zi = im[kk], wr = re[j], wi = im[j];
+    const double er = 0.5 * (zr + wr), ei = 0.5 * (zi - wi);
+    const double orr = 0.5 * (zi + wi), oi = 0.5 * (wr - zr);
+    const double2 t = tw[k];
+    xr = er + (orr * t.x - oi * t.y);
+    xi = ei + (orr * t.y + oi * t.x);
+  }
+};
+
+// The spectrum of bin k, as fft.rs:228-258 forms it.
 struct Bin {
   double nr, ni, unscaled, scale;
 };
-__device__ __forceinline__ Bin bin_of(const double* re, const double* im, int k, int N, int nbins,
-                                      double norm) {
+template <class View>
+__device__ __forceinline__ Bin bin_of(const View& v, int k, int N, int nbins, double inv_norm) {
+  double xr, xi;
+  v.get(k, xr, xi);
   Bin b;
-  b.nr = re[k] / norm;
-  b.ni = im[k] / norm;
+  b.nr = xr * inv_norm;
+  b.ni = xi * inv_norm;
   b.unscaled = hypot(b.nr, b.ni);
   b.scale = (k == 0 || ((N & 1) == 0 && k == nbins - 1)) ? 1.0 : 2.0;
   return b;
@@ -228,9 +263,9 @@ __device__ __forceinline__ void chirp_out(double& re, double& im, double2 c, dou
 }
 
 // Single-sided spectrum outputs of bin k (fft.rs:228-259); returns the magnitude.
-__device__ __forceinline__ double write_bin(const SpecArgs& a, const double* re, const double* im,
-                                            int64_t b, int k) {
-  const Bin bn = bin_of(re, im, k, a.fft_size, a.num_bins, a.norm);
+template <class View>
+__device__ __forceinline__ double write_bin(const SpecArgs& a, const View& v, int64_t b, int k) {
+  const Bin bn = bin_of(v, k, a.fft_size, a.num_bins, a.inv_norm);
   const double m = bn.unscaled * bn.scale;
   const int64_t o = b * a.num_bins + k;
   if (a.mag) a.mag[o] = m;
@@ -246,9 +281,9 @@ __device__ __forceinline__ double write_bin(const SpecArgs& a, const double* re,
 // magnitude > 0 within the search range (ties: lowest bin, thd.rs:132-153),
 // else the nearest bin (thd.rs:101-127).  Both lie within [c - 3, c + 4],
 // c the rounded (clamped) bin of the target.  Then thd.rs:212-311.
-template <int NT>
-__device__ void thd_stage(const SpecArgs& a, const double* re, const double* im, int64_t b,
-                          double* hf, double* hm, double* hp) {
+template <int NT, class View>
+__device__ void thd_stage(const SpecArgs& a, const View& v, int64_t b, double* hf, double* hm,
+                          double* hp) {
   const int tid = threadIdx.x, nb = a.num_bins;
   for (int g = tid >> 3; g <= a.found; g += NT >> 3) {
     const int lane8 = tid & 7;
@@ -259,7 +294,7 @@ __device__ void thd_stage(const SpecArgs& a, const double* re, const double* im,
     const bool inr = i >= 0 && i < nb;
     double fi = 0.0, m = 0.0, ph = 0.0;
     if (inr) {
-      const Bin bn = bin_of(re, im, i, a.fft_size, nb, a.norm);
+      const Bin bn = bin_of(v, i, a.fft_size, nb, a.inv_norm);
       fi = (double)i * a.fr;
       m = bn.unscaled * bn.scale;
       ph = atan2(bn.ni, bn.nr);
@@ -312,15 +347,19 @@ __device__ void thd_stage(const SpecArgs& a, const double* re, const double* im,
 
 enum { kModeSpectra = 0, kModeStats = 1, kModeThd = 2 };
 
-// One waveform per workgroup iteration, all in LDS (transforms up to 2^13).
-// kModeStats also folds every magnitude into per-thread Welford
-// accumulators for the bins the thread owns.
-template <int LOG2M, int MODE>
-__global__ __launch_bounds__(threads_for(LOG2M)) void k_spectrum(SpecArgs a) {
-  constexpr int M = 1 << LOG2M, NT = threads_for(LOG2M);
-  constexpr int KB = (M / 2 + 1 + NT - 1) / NT;  // bins per thread
+// One waveform per workgroup iteration, all in LDS.  PACKED: a real
+// waveform of power-of-two length N = 2L as an L-point complex transform
+// (twiddles W_N^2m from the length-N table), separated per bin; otherwise
+// the L-point complex transform directly, with the chirp-z wrap when the
+// plan needs it.  kModeStats also folds every magnitude into per-thread
+// Welford accumulators for the bins the thread owns.
+template <int LOG2L, int MODE, bool PACKED>
+__global__ __launch_bounds__(threads_for(LOG2L)) void k_spectrum(SpecArgs a) {
+  constexpr int L = 1 << LOG2L, NT = threads_for(LOG2L);
+  constexpr int NBMAX = PACKED ? L + 1 : L / 2 + 1;
+  constexpr int KB = (NBMAX + NT - 1) / NT;  // bins per thread
   constexpr int KS = MODE == kModeStats ? KB : 1;
-  __shared__ double re[M], im[M];
+  __shared__ double re[L], im[L];
   __shared__ double hf[MODE == kModeThd ? kMaxHarmonics + 1 : 1];
   __shared__ double hm[MODE == kModeThd ? kMaxHarmonics + 1 : 1];
   __shared__ double hp[MODE == kModeThd ? kMaxHarmonics + 1 : 1];
@@ -334,34 +373,50 @@ __global__ __launch_bounds__(threads_for(LOG2M)) void k_spectrum(SpecArgs a) {
 
   for (int64_t b = blockIdx.x; b < a.batch; b += gridDim.x) {
     const double* w = a.data + b * a.bstride;
-    for (int s = tid; s < M; s += NT) load_sample(a, w, s, re[s], im[s]);
+    for (int s = tid; s < L; s += NT) {
+      if constexpr (PACKED) {
+        double xr, xi, yr, yi;
+        load_sample(a, w, 2 * s, xr, xi);
+        load_sample(a, w, 2 * s + 1, yr, yi);
+        re[s] = xr;
+        im[s] = yr;
+      } else {
+        load_sample(a, w, s, re[s], im[s]);
+      }
+    }
     __syncthreads();
-    fft_lds<LOG2M, NT>(re, im, a.tw, tid, 1);
-    if (a.bluestein) {
-      for (int s = tid; s < M; s += NT) chirp_mul(re[s], im[s], a.bfft[s]);
+    fft_lds<LOG2L, NT>(re, im, a.tw, tid, PACKED ? 2 : 1);
+    if (!PACKED && a.bluestein) {
+      for (int s = tid; s < L; s += NT) chirp_mul(re[s], im[s], a.bfft[s]);
       __syncthreads();
-      fft_lds<LOG2M, NT>(re, im, a.tw, tid, 1);
-      for (int s = tid; s < a.fft_size; s += NT) chirp_out(re[s], im[s], a.chirp[s], 1.0 / (double)M);
+      fft_lds<LOG2L, NT>(re, im, a.tw, tid, 1);
+      for (int s = tid; s < a.fft_size; s += NT) chirp_out(re[s], im[s], a.chirp[s], 1.0 / (double)L);
       __syncthreads();
     }
+    using View = typename std::conditional<PACKED, PackedView, DirectView>::type;
+    View v;
+    if constexpr (PACKED) v = PackedView{re, im, a.tw, L};
+    else v = DirectView{re, im};
     if constexpr (MODE == kModeThd) {
-      thd_stage<NT>(a, re, im, b, hf, hm, hp);
-    } else {
+      thd_stage<NT>(a, v, b, hf, hm, hp);
+    } else if constexpr (MODE == kModeStats) {
       cnt += 1.0;
+      const double inv = 1.0 / cnt;
 #pragma unroll
       for (int q = 0; q < KB; ++q) {
         const int k = tid + q * NT;
         if (k < nb) {
-          const double m = write_bin(a, re, im, b, k);
-          if constexpr (MODE == kModeStats) {
-            const double d = m - wm[q];
-            wm[q] += d / cnt;
-            w2[q] += d * (m - wm[q]);
-            wmin[q] = fmin(wmin[q], m);
-            wmax[q] = fmax(wmax[q], m);
-          }
+          const double m = write_bin(a, v, b, k);
+          const double d = m - wm[q];
+          wm[q] += d * inv;
+          w2[q] += d * (m - wm[q]);
+          wmin[q] = fmin(wmin[q], m);
+          wmax[q] = fmax(wmax[q], m);
         }
       }
+    } else {
+#pragma unroll 1
+      for (int k = tid; k < nb; k += NT) (void)write_bin(a, v, b, k);
     }
     __syncthreads();  // LDS is refilled by the next waveform
   }
@@ -370,8 +425,8 @@ __global__ __launch_bounds__(threads_for(LOG2M)) void k_spectrum(SpecArgs a) {
     for (int q = 0; q < KB; ++q) {
       const int k = tid + q * NT;
       if (k < nb) {
-        double* o = a.part + ((int64_t)blockIdx.x * nb + k) * 4;
-        o[0] = wm[q]; o[1] = w2[q]; o[2] = wmin[q]; o[3] = wmax[q];
+        double* o = a.part + ((int64_t)blockIdx.x * nb + k) * 5;
+        o[0] = cnt; o[1] = wm[q]; o[2] = w2[q]; o[3] = wmin[q]; o[4] = wmax[q];
       }
     }
   }
@@ -468,12 +523,11 @@ __global__ __launch_bounds__(kGlobalThreads) void k_post_global(SpecArgs a, int6
   __shared__ double hf[THD ? kMaxHarmonics + 1 : 1], hm[THD ? kMaxHarmonics + 1 : 1],
       hp[THD ? kMaxHarmonics + 1 : 1];
   const int64_t bl = blockIdx.x, b = b0 + bl;
-  const double* re = z + bl * 2 * M;
-  const double* im = re + M;
+  const DirectView v{z + bl * 2 * M, z + bl * 2 * M + M};
   if constexpr (THD) {
-    thd_stage<kGlobalThreads>(a, re, im, b, hf, hm, hp);
+    thd_stage<kGlobalThreads>(a, v, b, hf, hm, hp);
   } else {
-    for (int k = threadIdx.x; k < a.num_bins; k += kGlobalThreads) (void)write_bin(a, re, im, b, k);
+    for (int k = threadIdx.x; k < a.num_bins; k += kGlobalThreads) (void)write_bin(a, v, b, k);
   }
 }
 
@@ -494,32 +548,50 @@ __global__ void k_bin_stats_cols(const double* __restrict__ mag, int64_t batch, 
     mn = fmin(mn, m);
     mx = fmax(mx, m);
   }
-  double* o = part + ((int64_t)g * nb + k) * 4;
-  o[0] = mean; o[1] = m2; o[2] = mn; o[3] = mx;
+  double* o = part + ((int64_t)g * nb + k) * 5;
+  o[0] = cnt; o[1] = mean; o[2] = m2; o[3] = mn; o[4] = mx;
 }
 
-// Per-bin merge of the workgroups' Welford partials in workgroup order
-// (Chan et al.'s pairwise update), then the population deviation.
-__global__ void k_bin_stats_merge(const double* __restrict__ part, int grid, int64_t batch, int nb,
-                                  double* mean_o, double* std_o, double* min_o, double* max_o) {
+// Per-bin merge of Welford partials [groups][nb][5] (count, mean, M2, min,
+// max) in group order, Chan et al.'s pairwise update: each level folds runs
+// of kMergeRun consecutive groups, until one group is left, which the last
+// kernel turns into the mean and population deviation.  The fold order is
+// fixed, so results do not depend on scheduling.
+constexpr int kMergeRun = 16;
+
+struct Welford {
+  double n = 0.0, mean = 0.0, m2 = 0.0, mn = INFINITY, mx = -INFINITY;
+  __device__ void fold(const double* p) {
+    if (p[0] <= 0.0) return;
+    const double c = p[0], tot = n + c, d = p[1] - mean;
+    mean += d * (c / tot);
+    m2 += p[2] + d * d * (n * c / tot);
+    n = tot;
+    mn = fmin(mn, p[3]);
+    mx = fmax(mx, p[4]);
+  }
+};
+
+__global__ void k_bin_stats_fold(const double* __restrict__ part, int groups, int nb,
+                                 double* __restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nb) return;
-  double n = 0.0, mean = 0.0, m2 = 0.0, mn = INFINITY, mx = -INFINITY;
-  for (int g = 0; g < grid; ++g) {
-    const int64_t cg = (batch - g + grid - 1) / grid;  // waveforms workgroup g visited
-    if (cg <= 0) continue;
-    const double* p = part + ((int64_t)g * nb + k) * 4;
-    const double c = (double)cg, tot = n + c, d = p[0] - mean;
-    mean += d * (c / tot);
-    m2 += p[1] + d * d * (n * c / tot);
-    n = tot;
-    mn = fmin(mn, p[2]);
-    mx = fmax(mx, p[3]);
-  }
-  if (mean_o) mean_o[k] = mean;
-  if (std_o) std_o[k] = sqrt(m2 / (double)batch);
-  if (min_o) min_o[k] = mn;
-  if (max_o) max_o[k] = mx;
+  const int g0 = blockIdx.y * kMergeRun, g1 = min(groups, g0 + kMergeRun);
+  Welford w;
+  for (int g = g0; g < g1; ++g) w.fold(part + ((int64_t)g * nb + k) * 5);
+  double* o = out + ((int64_t)blockIdx.y * nb + k) * 5;
+  o[0] = w.n; o[1] = w.mean; o[2] = w.m2; o[3] = w.mn; o[4] = w.mx;
+}
+
+__global__ void k_bin_stats_finish(const double* __restrict__ part, int64_t batch, int nb,
+                                   double* mean_o, double* std_o, double* min_o, double* max_o) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  const double* p = part + (int64_t)k * 5;
+  if (mean_o) mean_o[k] = p[1];
+  if (std_o) std_o[k] = sqrt(p[2] / (double)batch);
+  if (min_o) min_o[k] = p[3];
+  if (max_o) max_o[k] = p[4];
 }
 
 template <typename T>
@@ -681,12 +753,13 @@ void launch_fft_inplace(int log2m, double2* v, const double2* tw, hipStream_t st
   }
 }
 
-template <int MODE>
-void launch_spectrum_mode(int log2m, unsigned grid, const SpecArgs& a, hipStream_t st) {
-  switch (log2m) {
-#define SPX_SPEC_CASE(L)                                                                   \
-  case L:                                                                                  \
-    hipLaunchKernelGGL((k_spectrum<L, MODE>), dim3(grid), dim3(threads_for(L)), 0, st, a); \
+template <int MODE, bool PACKED>
+void launch_spectrum_mode(int log2l, unsigned grid, const SpecArgs& a, hipStream_t st) {
+  switch (log2l) {
+#define SPX_SPEC_CASE(L)                                                                    \
+  case L:                                                                                   \
+    hipLaunchKernelGGL((k_spectrum<L, MODE, PACKED>), dim3(grid), dim3(threads_for(L)), 0, \
+                       st, a);                                                              \
     break;
     SPX_SPEC_CASE(0) SPX_SPEC_CASE(1) SPX_SPEC_CASE(2) SPX_SPEC_CASE(3) SPX_SPEC_CASE(4)
     SPX_SPEC_CASE(5) SPX_SPEC_CASE(6) SPX_SPEC_CASE(7) SPX_SPEC_CASE(8) SPX_SPEC_CASE(9)
@@ -695,10 +768,25 @@ void launch_spectrum_mode(int log2m, unsigned grid, const SpecArgs& a, hipStream
   }
 }
 
-void launch_spectrum(int mode, int log2m, unsigned grid, const SpecArgs& a, hipStream_t st) {
-  if (mode == kModeStats) launch_spectrum_mode<kModeStats>(log2m, grid, a, st);
-  else if (mode == kModeThd) launch_spectrum_mode<kModeThd>(log2m, grid, a, st);
-  else launch_spectrum_mode<kModeSpectra>(log2m, grid, a, st);
+// Power-of-two lengths from 4 up to 2^14 points run packed (half-length
+// transform); the chirp-z plans and N <= 2 run the complex transform up to
+// 2^13 points; everything else takes the four-step path.
+bool packed_path(const SpectralPlan& p) {
+  return !p.bluestein && p.fft_size >= 4 && p.log2_run - 1 <= kMaxLog2;
+}
+bool lds_path(const SpectralPlan& p) { return packed_path(p) || p.log2_run <= kMaxLog2; }
+
+template <int MODE>
+void launch_spectrum_m(const SpectralPlan& p, unsigned grid, const SpecArgs& a, hipStream_t st) {
+  if (packed_path(p)) launch_spectrum_mode<MODE, true>(p.log2_run - 1, grid, a, st);
+  else launch_spectrum_mode<MODE, false>(p.log2_run, grid, a, st);
+}
+
+void launch_spectrum(int mode, const SpectralPlan& p, unsigned grid, const SpecArgs& a,
+                     hipStream_t st) {
+  if (mode == kModeStats) launch_spectrum_m<kModeStats>(p, grid, a, st);
+  else if (mode == kModeThd) launch_spectrum_m<kModeThd>(p, grid, a, st);
+  else launch_spectrum_m<kModeSpectra>(p, grid, a, st);
 }
 
 template <int LOG2V, bool COLS>
@@ -908,6 +996,7 @@ SpecArgs base_args(spx_handle* h, const spx_waveform_batch& wf, const SpectralPl
   a.chirp = static_cast<const double2*>(at(p.o_chirp));
   a.bfft = static_cast<const double2*>(at(p.o_bfft));
   a.norm = p.norm;
+  a.inv_norm = 1.0 / p.norm;
   return a;
 }
 
@@ -1029,13 +1118,16 @@ int spx_batch_fft_device(spx_handle* h, const spx_waveform_batch* wf,
   const int nb = (int)p->num_bins;
   const unsigned sgrid = grid_of(B, kStatsGrid);
   void* part = nullptr;
+  void* part2 = nullptr;
   if (stats) {
-    if (int rc = workspace(h, "sp_part", sizeof(double) * 4 * sgrid * nb, &part)) return rc;
+    const int64_t g2 = (sgrid + kMergeRun - 1) / kMergeRun;
+    if (int rc = workspace(h, "sp_part", sizeof(double) * 5 * sgrid * nb, &part)) return rc;
+    if (int rc = workspace(h, "sp_part2", sizeof(double) * 5 * g2 * nb, &part2)) return rc;
   }
-  if (p->log2_run <= kMaxLog2) {
+  if (lds_path(*p)) {
     if (stats) a.part = static_cast<double*>(part);
-    launch_spectrum(stats ? kModeStats : kModeSpectra, p->log2_run,
-                    stats ? sgrid : grid_of(B, kPlainGrid), a, h->stream);
+    launch_spectrum(stats ? kModeStats : kModeSpectra, *p, stats ? sgrid : grid_of(B, kPlainGrid),
+                    a, h->stream);
   } else {
     if (stats && !a.mag) {  // the statistics read every magnitude back
       void* mags = nullptr;
@@ -1047,10 +1139,23 @@ int spx_batch_fft_device(spx_handle* h, const spx_waveform_batch* wf,
       hipLaunchKernelGGL(k_bin_stats_cols, dim3((nb + 255) / 256, sgrid), dim3(256), 0, h->stream,
                          a.mag, B, nb, static_cast<double*>(part));
   }
-  if (stats)
-    hipLaunchKernelGGL(k_bin_stats_merge, dim3((nb + 255) / 256), dim3(256), 0, h->stream,
-                       static_cast<const double*>(part), (int)sgrid, B, nb, out->mean_magnitude,
+  if (stats) {
+    // fold levels alternate between the two partial buffers; the second is
+    // never smaller than a level's output
+    double* src = static_cast<double*>(part);
+    double* dst = static_cast<double*>(part2);
+    int groups = (int)sgrid;
+    while (groups > 1) {
+      const int next = (groups + kMergeRun - 1) / kMergeRun;
+      hipLaunchKernelGGL(k_bin_stats_fold, dim3((nb + 255) / 256, next), dim3(256), 0, h->stream,
+                         static_cast<const double*>(src), groups, nb, dst);
+      std::swap(src, dst);
+      groups = next;
+    }
+    hipLaunchKernelGGL(k_bin_stats_finish, dim3((nb + 255) / 256), dim3(256), 0, h->stream,
+                       static_cast<const double*>(src), B, nb, out->mean_magnitude,
                        out->std_magnitude, out->min_magnitude, out->max_magnitude);
+  }
   SPX_HIP(hipGetLastError(), SPX_ERR_BACKEND);
   return SPX_OK;
 }
@@ -1108,8 +1213,8 @@ int spx_batch_thd_device(spx_handle* h, const spx_waveform_batch* wf, double fun
   a.fund_mag = out->fundamental_magnitude;
   a.hrms = out->harmonic_rms;
   a.harm = out->harmonics;
-  if (p->log2_run <= kMaxLog2) {
-    launch_spectrum(kModeThd, p->log2_run, grid_of(B, kPlainGrid), a, h->stream);
+  if (lds_path(*p)) {
+    launch_spectrum(kModeThd, *p, grid_of(B, kPlainGrid), a, h->stream);
   } else if (int rc = run_global(h, *p, a, true)) {
     return rc;
   }
