@@ -325,7 +325,7 @@ int spx_sweep_dcop_device(spx_handle* h, const spx_circuit* ckt, const spx_sweep
                           const spx_nr_config* cfg, const double* d_x0, spx_sweep_result* d_out);
 
 /* ------------------------------------------------------------------------- */
-/* Backward-Euler transient sweep (CapacitorState::stamp_be / update,
+/* Transient sweep: Backward Euler by default (CapacitorState::stamp_be / update,
  * crates/spicier-solver/src/transient/companion.rs:41-66; solve_transient,
  * crates/spicier-solver/src/transient/solver.rs:41-160; PULSE sources by
  * eval_pulse, crates/spicier-devices/src/waveforms.rs:160-180).  Every
@@ -335,11 +335,45 @@ int spx_sweep_dcop_device(spx_handle* h, const spx_circuit* ckt, const spx_sweep
  * negative to its positive node.  The reference's transient is linear (one
  * solve per step); circuits with diodes or MOSFETs run plain Newton at every
  * step from the previous step's solution (no step cutting).  FP64. */
+/* Integration methods (IntegrationMethod, transient/types.rs:50-63) and
+ * their capacitor companions (companion.rs:41-130):
+ *   SPX_TRAN_BE      G = C/h, I = G*v_prev;
+ *   SPX_TRAN_TRAP    G = 2C/h, I = G*v_prev + i_prev;
+ *   SPX_TRAN_TRBDF2  a trapezoidal stage of gamma*h (gamma = 2 - sqrt 2) to an
+ *                    intermediate point, then a variable-step BDF2 stage of
+ *                    (1 - gamma)*h; both stages evaluate sources at the end
+ *                    of the step, and i_prev follows the reference's update
+ *                    rules (companion.rs:62-95), as solve_transient
+ *                    (solver.rs:184-248) does. */
+enum { SPX_TRAN_BE = 0, SPX_TRAN_TRAP = 1, SPX_TRAN_TRBDF2 = 2 };
+
+/* Adaptive trapezoidal stepping (AdaptiveTransientParams, types.rs:76-117;
+ * solve_transient_adaptive, solver.rs:461-676): each instance keeps its own
+ * time grid.  A step of h from t solves the trapezoidal companion with the
+ * sources at t (as the reference stamps them), estimates each capacitor's
+ * truncation error |i_trap - i_be| / 3 (estimate_lte, companion.rs:132-150),
+ * and is rejected when max LTE > max(abstol, reltol * max|v_cap|) while
+ * h > h_min (h *= clamp(sqrt(tol/lte), 0.1, 0.5)); an accepted step may grow
+ * h by up to 1.5x when max LTE < tol/2.  h is clamped to [h_min, h_max] and
+ * cut to end exactly at tstop.  An instance needing more than max_points
+ * time points (t = 0 included) stops with SPX_FAILED. */
+typedef struct spx_tran_adaptive {
+    double tstop;     /* > 0 */
+    double h_init;    /* > 0 */
+    double h_min;     /* > 0 */
+    double h_max;     /* >= h_min */
+    double reltol;    /* >= 0 */
+    double abstol;    /* >= 0 */
+    int32_t max_points;
+} spx_tran_adaptive;
+
 typedef struct spx_tran_config {
-    double tstep;            /* step h (s), > 0 */
-    int32_t steps;           /* number of steps after t = 0 */
+    double tstep;            /* step h (s), > 0 (fixed steps) */
+    int32_t steps;           /* number of steps after t = 0 (fixed steps) */
     int32_t num_probes;      /* unknowns recorded at every step */
     const int32_t* probes;   /* [num_probes] unknown indices (host array) */
+    int32_t method;          /* SPX_TRAN_*; ignored (trapezoidal) when adaptive */
+    const spx_tran_adaptive* adaptive;  /* NULL: fixed steps */
 } spx_tran_config;
 
 typedef struct spx_tran_result {
@@ -349,8 +383,16 @@ typedef struct spx_tran_result {
     int32_t* flags;      /* [B] SPX_FLAG_* (KCL test of the last step) */
     double* residual;    /* [B] KCL residual ratio of the last step */
     double* params;      /* [B][P] sampled parameters (may be NULL) */
-    double* waveforms;   /* [B][steps+1][num_probes] probed unknowns at t = 0, h, ... */
+    double* waveforms;   /* [B][points][num_probes] probed unknowns at every time point:
+                            points = steps + 1, or max_points when adaptive */
     int32_t* fail_step;  /* [B] first step that failed (0 = operating point), -1 if none */
+    /* adaptive only (each may be NULL): */
+    double* times;           /* [B][max_points] time of every point */
+    int32_t* num_points;     /* [B] points recorded, t = 0 included */
+    int32_t* total_steps;    /* [B] steps tried (accepted + rejected) */
+    int32_t* rejected_steps; /* [B] */
+    double* min_step;        /* [B] smallest accepted step (inf if none) */
+    double* max_step;        /* [B] largest accepted step (0 if none) */
 } spx_tran_result;
 
 int spx_transient_sweep(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,

@@ -374,14 +374,21 @@ double pnjlim(double vnew, double vold, double vt, double vcrit, int* limited) {
   return vnew;
 }
 
-// Backward-Euler transient context (transient/companion.rs:41-66): mode 1
-// is a step at time t of size h with capacitor voltages vprev[e] of the
-// previous step; without it (or mode 0) capacitors are open and PULSE
-// sources sit at V1 (waveforms.rs:150-160).
+// Transient context: mode 1 is a step ending at time t; without it (or mode
+// 0) capacitors are open and PULSE sources sit at V1 (waveforms.rs:150-160).
+// comp picks the capacitor companion (companion.rs:41-130):
+//   0 (BE, stamp_be)       G = C/h, I = G*vprev
+//   1 (trap, stamp_trap)   G = 2C/h, I = G*vprev + iprev
+//   2 (stamp_trbdf2_bdf2)  G = C/bden, I = G*(a1*vprev + a2*vpp)
+// with h the companion's step (gamma*h for the TR-BDF2 first stage).
 struct OTran {
   int mode;
   double t, h;
   const double* vprev;
+  int comp = 0;
+  const double* iprev = nullptr;
+  const double* vpp = nullptr;
+  double a1 = 0.0, a2 = 0.0, bden = 0.0;
 };
 
 // eval_pulse (waveforms.rs:160-180)
@@ -411,9 +418,19 @@ void stamp_impl(const spx_circuit* c, const double* params, const double* x, dou
         m.conductance(a, b, 1.0 / P(params, el.param[0]));
         break;
       case SPX_EL_CAPACITOR:  // open at DC (passive.rs:233-245); BE companion in a step
-        if (tc && tc->mode == 1) {  // stamp_be, companion.rs:41-48
-          const double geq = P(params, el.param[0]) / tc->h;
-          const double ieq = geq * tc->vprev[e];
+        if (tc && tc->mode == 1) {
+          const double C = P(params, el.param[0]);
+          double geq, ieq;
+          if (tc->comp == 1) {  // stamp_trap, companion.rs:53-60
+            geq = 2.0 * C / tc->h;
+            ieq = geq * tc->vprev[e] + tc->iprev[e];
+          } else if (tc->comp == 2) {  // stamp_trbdf2_bdf2, companion.rs:100-130
+            geq = C / tc->bden;
+            ieq = geq * (tc->a1 * tc->vprev[e] + tc->a2 * tc->vpp[e]);
+          } else {  // stamp_be, companion.rs:41-48
+            geq = C / tc->h;
+            ieq = geq * tc->vprev[e];
+          }
           m.conductance(a, b, geq);
           m.current(b, a, ieq);
         }
@@ -938,60 +955,213 @@ extern "C" int orc_gmres(int32_t n, int64_t nnz, const int32_t* rows, const int3
 }
 
 // ----------------------------------------------------------- transient
-// Backward-Euler transient (transient/solver.rs:41-160, companion.rs:41-66):
-// DC operating point (orc_dcop semantics, capacitors open, PULSE at V1),
-// then `steps` fixed steps at t = k*h.  Linear circuits take one solve per
-// step as the reference does; nonlinear ones plain Newton from the previous
-// step.  waves[(k)*np + p] = x[probes[p]] at step k (k = 0 the operating
-// point).  Returns the spx_conv_status of the run.
-extern "C" int orc_transient(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
-                             double h, int32_t steps, int32_t np, const int32_t* probes,
-                             double* x, double* waves, int32_t* iterations, int32_t* flags,
-                             double* residual, int32_t* fail_step) {
+namespace {
+
+// TRBDF2_GAMMA (types.rs:11) and the BDF2-stage coefficients of
+// stamp_trbdf2_bdf2 (companion.rs:100-124) for a full step h.
+const double kGamma = 2.0 - M_SQRT2;
+
+void bdf2_coeffs(double h, double* a1, double* a2, double* bden) {
+  const double h2 = (1.0 - kGamma) * h;
+  const double h1 = kGamma * h;
+  const double rho = h2 / h1;
+  const double denom = 1.0 + 2.0 * rho;
+  *a1 = (1.0 + rho) * (1.0 + rho) / denom;
+  *a2 = -rho * rho / denom;
+  const double b0 = (1.0 + rho) / denom;
+  *bden = b0 * h2;
+}
+
+struct CapState {
+  std::vector<double> vprev, iprev, vpp;
+  explicit CapState(int ne) : vprev(ne, 0.0), iprev(ne, 0.0), vpp(ne, 0.0) {}
+};
+
+double cap_volt(const spx_circuit* c, int e, const double* x) {
+  return volt(x, c->elements[e].node[0]) - volt(x, c->elements[e].node[1]);
+}
+
+// CapacitorState::update (companion.rs:62-85), rule 0 BE, 1 trapezoidal,
+// 2 TR-BDF2 final; rule 3 update_trbdf2_intermediate (companion.rs:87-98).
+void cap_update_rule(const spx_circuit* c, const double* params, const double* x, double h,
+                     int rule, CapState& cs) {
+  for (int e = 0; e < c->num_elements; ++e) {
+    if (c->elements[e].kind != SPX_EL_CAPACITOR) continue;
+    const double C = P(params, c->elements[e].param[0]);
+    const double v = cap_volt(c, e, x);
+    double& vp = cs.vprev[e];
+    double& ip = cs.iprev[e];
+    double& vq = cs.vpp[e];
+    if (rule == 3) {
+      const double h_gamma = kGamma * h;
+      vq = vp;
+      ip = 2.0 * C / h_gamma * (v - vq) - ip;
+      vp = v;
+      continue;
+    }
+    if (rule == 0) {
+      ip = C / h * (v - vp);
+    } else if (rule == 1) {
+      ip = 2.0 * C / h * (v - vp) - ip;
+    } else {
+      const double alpha = (1.0 - kGamma) / (kGamma * (2.0 - kGamma));
+      ip = C / h * ((1.0 + alpha) * v - (1.0 + 2.0 * alpha) * vp + alpha * vq);
+    }
+    vq = vp;
+    vp = v;
+  }
+}
+
+}  // namespace
+
+// One solve of a transient stage: linear circuits one LU solve (the
+// reference's), nonlinear ones plain Newton from x.  Returns 1 converged,
+// 0 failed, -1 singular; *it the iterations.
+static int tran_solve(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
+                      bool nonlin, const OTran& tc, double* x, int32_t* it) {
+  const int n = orc_circuit_size(c);
+  std::vector<double> J((size_t)n * n), rhs(n), xn(n);
+  if (!nonlin) {
+    std::vector<double> zero(n, 0.0);
+    stamp_impl(c, params, zero.data(), 0.0, 1.0, cfg->limiter, J.data(), rhs.data(), nullptr,
+               nullptr, &tc);
+    *it = 1;
+    return orc_lu_solve(n, J.data(), rhs.data(), x) ? -1 : 1;
+  }
+  const int st = newton_impl(c, params, cfg, 0.0, 1.0, x, xn.data(), it, &tc);
+  std::memcpy(x, xn.data(), sizeof(double) * n);
+  return st;
+}
+
+// Transient of one instance: DC operating point (orc_dcop semantics,
+// capacitors open, PULSE at V1), capacitor states from it (v_prev = its
+// voltage, i_prev = 0, solver.rs:49-55), then either fixed steps at
+// t = k*h with o->method (solve_transient, solver.rs:41-249) or adaptive
+// trapezoidal steps (solve_transient_adaptive, solver.rs:461-676).  Linear
+// circuits take one solve per stage as the reference does; nonlinear ones
+// plain Newton from the previous solution.  waves[k*np + p] = x[probes[p]]
+// at point k (k = 0 the operating point); times[k] (adaptive).  The KCL
+// residual is taken at the last point with the companion of its stage.
+extern "C" int orc_transient_ex(const spx_circuit* c, const double* params,
+                                const spx_nr_config* cfg, const orc_tran_opts* o, int32_t np,
+                                const int32_t* probes, double* x, double* waves, double* times,
+                                int32_t* num_points, int32_t* iterations, int32_t* flags,
+                                double* residual, int32_t* fail_step, int32_t* total_steps,
+                                int32_t* rejected_steps, double* min_step, double* max_step) {
   const int n = orc_circuit_size(c);
   int32_t it = 0, fl = 0;
   double res = 0.0;
   int status = orc_dcop(c, params, cfg, nullptr, x, &it, &fl, &res);
   fl &= ~SPX_FLAG_KCL_FAIL;
-  int32_t total = it, fail = -1;
+  int32_t total = it, fail = -1, points = 0, tried = 0, rejected = 0;
+  double hmin_used = INFINITY, hmax_used = 0.0;
+  auto probe = [&](int k, double t) {
+    if (waves)
+      for (int p = 0; p < np; ++p) waves[(size_t)k * np + p] = x[probes[p]];
+    if (times) times[k] = t;
+  };
   if (status != SPX_CONVERGED) {
     fail = 0;
   } else {
-    std::vector<double> vprev(c->num_elements, 0.0), xn(n), J((size_t)n * n), rhs(n);
-    auto cap_update = [&]() {
-      for (int e = 0; e < c->num_elements; ++e)
-        if (c->elements[e].kind == SPX_EL_CAPACITOR)
-          vprev[e] = volt(x, c->elements[e].node[0]) - volt(x, c->elements[e].node[1]);
-    };
-    auto probe = [&](int k) {
-      if (waves)
-        for (int p = 0; p < np; ++p) waves[(size_t)k * np + p] = x[probes[p]];
-    };
-    cap_update();
-    probe(0);
+    CapState cs(c->num_elements);
+    for (int e = 0; e < c->num_elements; ++e)
+      if (c->elements[e].kind == SPX_EL_CAPACITOR) cs.vprev[e] = cap_volt(c, e, x);
+    probe(0, 0.0);
+    points = 1;
     const bool nonlin = has_nonlinear(c);
-    for (int k = 1; k <= steps; ++k) {
-      OTran tc{1, k * h, h, vprev.data()};
-      int st;
-      if (!nonlin) {
-        std::vector<double> zero(n, 0.0);
-        stamp_impl(c, params, zero.data(), 0.0, 1.0, cfg->limiter, J.data(), rhs.data(), nullptr,
-                   nullptr, &tc);
-        st = orc_lu_solve(n, J.data(), rhs.data(), x) ? -1 : 1;
-        it = 1;
-      } else {
-        st = newton_impl(c, params, cfg, 0.0, 1.0, x, xn.data(), &it, &tc);
-        std::memcpy(x, xn.data(), sizeof(double) * n);
+    auto ctx = [&](double t, double h, int comp) {
+      OTran tc{1, t, h, cs.vprev.data()};
+      tc.comp = comp;
+      tc.iprev = cs.iprev.data();
+      tc.vpp = cs.vpp.data();
+      return tc;
+    };
+    if (!o->adaptive) {
+      const double h = o->h;
+      for (int k = 1; k <= o->steps; ++k) {
+        const double t = k * h;
+        int st;
+        if (o->method == SPX_TRAN_TRBDF2) {
+          const OTran t1 = ctx(t, kGamma * h, 1);
+          st = tran_solve(c, params, cfg, nonlin, t1, x, &it);
+          total += it;
+          if (st == 1) {
+            cap_update_rule(c, params, x, h, 3, cs);
+            OTran t2 = ctx(t, h, 2);
+            bdf2_coeffs(h, &t2.a1, &t2.a2, &t2.bden);
+            st = tran_solve(c, params, cfg, nonlin, t2, x, &it);
+            total += it;
+            if (st == 1 && k == o->steps) res = kcl_impl(c, params, cfg, x, 0.0, &t2);
+          }
+          if (st == 1) cap_update_rule(c, params, x, h, 2, cs);
+        } else {
+          const int comp = o->method == SPX_TRAN_TRAP ? 1 : 0;
+          const OTran tc = ctx(t, h, comp);
+          st = tran_solve(c, params, cfg, nonlin, tc, x, &it);
+          total += it;
+          if (st == 1) {
+            if (k == o->steps) res = kcl_impl(c, params, cfg, x, 0.0, &tc);
+            cap_update_rule(c, params, x, h, comp, cs);
+          }
+        }
+        if (st != 1) {
+          status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
+          fail = k;
+          break;
+        }
+        probe(k, t);
+        points = k + 1;
       }
-      total += it;
-      if (st != 1) {
-        status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
-        fail = k;
-        break;
+    } else {
+      // Newton (nonlinear circuits) starts from the latest solve's result,
+      // accepted or rejected; the capacitor states move only on acceptance.
+      double t = 0.0, h = o->h_init;
+      while (t < o->tstop) {
+        h = h < o->h_min ? o->h_min : (h > o->h_max ? o->h_max : h);
+        if (t + h > o->tstop) h = o->tstop - t;
+        const OTran tc = ctx(t, h, 1);  // sources at t, as the reference stamps them
+        const int st = tran_solve(c, params, cfg, nonlin, tc, x, &it);
+        total += it;
+        ++tried;
+        if (st != 1) {
+          status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
+          fail = points;
+          break;
+        }
+        double max_lte = 0.0, max_ref = 0.0;
+        for (int e = 0; e < c->num_elements; ++e) {
+          if (c->elements[e].kind != SPX_EL_CAPACITOR) continue;
+          const double C = P(params, c->elements[e].param[0]);
+          const double v = cap_volt(c, e, x);
+          const double i_trap = 2.0 * C / h * (v - cs.vprev[e]) - cs.iprev[e];
+          const double i_be = C / h * (v - cs.vprev[e]);
+          max_lte = std::fmax(max_lte, std::fabs(i_trap - i_be) / 3.0);
+          max_ref = std::fmax(max_ref, std::fabs(v));
+        }
+        const double tol = std::fmax(o->abstol, o->reltol * max_ref);
+        if (max_lte > tol && h > o->h_min) {
+          ++rejected;
+          const double factor = std::fmin(std::sqrt(tol / max_lte), 0.5);
+          h *= std::fmax(factor, 0.1);
+          continue;
+        }
+        if (points >= o->max_points) {
+          status = SPX_FAILED;
+          fail = points;
+          break;
+        }
+        t += h;
+        if (!(t < o->tstop)) res = kcl_impl(c, params, cfg, x, 0.0, &tc);
+        cap_update_rule(c, params, x, h, 1, cs);
+        hmin_used = std::fmin(hmin_used, h);
+        hmax_used = std::fmax(hmax_used, h);
+        probe(points, t);
+        ++points;
+        if (max_lte < tol * 0.5 && h < o->h_max) {
+          const double factor = std::fmin(std::sqrt(tol / std::fmax(max_lte, 1e-20)), 2.0);
+          h *= std::fmin(factor, 1.5);
+        }
       }
-      if (k == steps) res = kcl_impl(c, params, cfg, x, 0.0, &tc);
-      cap_update();
-      probe(k);
     }
   }
   if (status == SPX_CONVERGED && !(res <= 1.0)) fl |= SPX_FLAG_KCL_FAIL;
@@ -1001,7 +1171,48 @@ extern "C" int orc_transient(const spx_circuit* c, const double* params, const s
   if (flags) *flags = fl;
   if (residual) *residual = res;
   if (fail_step) *fail_step = fail;
+  if (num_points) *num_points = points;
+  if (total_steps) *total_steps = tried;
+  if (rejected_steps) *rejected_steps = rejected;
+  if (min_step) *min_step = hmin_used;
+  if (max_step) *max_step = hmax_used;
   return status;
+}
+
+// Backward-Euler fixed steps (the original entry; orc_transient_ex with
+// method BE).
+extern "C" int orc_transient(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
+                             double h, int32_t steps, int32_t np, const int32_t* probes,
+                             double* x, double* waves, int32_t* iterations, int32_t* flags,
+                             double* residual, int32_t* fail_step) {
+  orc_tran_opts o{};
+  o.method = SPX_TRAN_BE;
+  o.h = h;
+  o.steps = steps;
+  return orc_transient_ex(c, params, cfg, &o, np, probes, x, waves, nullptr, nullptr, iterations,
+                          flags, residual, fail_step, nullptr, nullptr, nullptr, nullptr);
+}
+
+extern "C" int orc_transient_ex_sweep(const spx_circuit* c, const double* params, int64_t B,
+                                      const spx_nr_config* cfg, const orc_tran_opts* o,
+                                      int32_t np, const int32_t* probes, double* x,
+                                      int32_t* status, double* waves, double* times,
+                                      int32_t* num_points, int32_t* iterations, int32_t* flags,
+                                      double* residual, int32_t* fail_step, int32_t* total_steps,
+                                      int32_t* rejected_steps, double* min_step,
+                                      double* max_step, int32_t nthreads) {
+  const int n = orc_circuit_size(c);
+  const int P = c->num_params;
+  const int64_t pts = o->adaptive ? o->max_points : o->steps + 1;
+  run_pool(B, nthreads, [&](int64_t i) {
+    status[i] = orc_transient_ex(
+        c, params + i * P, cfg, o, np, probes, x + i * n, waves ? waves + i * pts * np : nullptr,
+        times ? times + i * pts : nullptr, num_points ? num_points + i : nullptr, iterations + i,
+        flags + i, residual + i, fail_step + i, total_steps ? total_steps + i : nullptr,
+        rejected_steps ? rejected_steps + i : nullptr, min_step ? min_step + i : nullptr,
+        max_step ? max_step + i : nullptr);
+  });
+  return SPX_OK;
 }
 
 extern "C" int orc_transient_sweep(const spx_circuit* c, const double* params, int64_t B,

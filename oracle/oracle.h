@@ -97,6 +97,32 @@ void orc_stats_from_moments(int64_t count, double sum, double sum_sq, double* ou
  * out = mean, std_dev, min, max. */
 void orc_sweep_statistics(const double* samples, int64_t count, double* out4);
 
+/* Transient options: fixed steps of h with an integration method, or
+ * adaptive trapezoidal stepping (see spx_tran_adaptive). */
+typedef struct orc_tran_opts {
+    int32_t method;   /* SPX_TRAN_* (fixed steps) */
+    double h;
+    int32_t steps;
+    int32_t adaptive;
+    double tstop, h_init, h_min, h_max, reltol, abstol;
+    int32_t max_points;
+} orc_tran_opts;
+/* Transient of one instance with orc_tran_opts (transient/solver.rs:41-249,
+ * 461-676): waves [points][np] and times [points] (points = steps+1, or
+ * max_points when adaptive).  Returns the spx_conv_status of the run. */
+int orc_transient_ex(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
+                     const orc_tran_opts* o, int32_t np, const int32_t* probes, double* x,
+                     double* waves, double* times, int32_t* num_points, int32_t* iterations,
+                     int32_t* flags, double* residual, int32_t* fail_step, int32_t* total_steps,
+                     int32_t* rejected_steps, double* min_step, double* max_step);
+int orc_transient_ex_sweep(const spx_circuit* c, const double* params, int64_t B,
+                           const spx_nr_config* cfg, const orc_tran_opts* o, int32_t np,
+                           const int32_t* probes, double* x, int32_t* status, double* waves,
+                           double* times, int32_t* num_points, int32_t* iterations,
+                           int32_t* flags, double* residual, int32_t* fail_step,
+                           int32_t* total_steps, int32_t* rejected_steps, double* min_step,
+                           double* max_step, int32_t nthreads);
+
 /* Backward-Euler transient of one instance (transient/solver.rs:41-160);
  * waves is [(steps+1)][np].  Returns the spx_conv_status of the run. */
 int orc_transient(const spx_circuit* c, const double* params, const spx_nr_config* cfg, double h,

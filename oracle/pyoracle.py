@@ -168,6 +168,47 @@ def gmres(n, rows, cols, vals, b, restart=30, max_iter=500, tol=1e-8, jacobi=Tru
     return x, bool(conv), it.value, res.value
 
 
+class OrcTranOpts(C.Structure):
+    _fields_ = [("method", C.c_int32), ("h", C.c_double), ("steps", C.c_int32),
+                ("adaptive", C.c_int32), ("tstop", C.c_double), ("h_init", C.c_double),
+                ("h_min", C.c_double), ("h_max", C.c_double), ("reltol", C.c_double),
+                ("abstol", C.c_double), ("max_points", C.c_int32)]
+
+
+def transient_ex(circuit, params, cfg, tstep=0.0, steps=0, probes=None, method=0, adaptive=None,
+                 nthreads=8):
+    """orc_transient_ex_sweep over params [B][P]: fixed steps with an
+    integration method, or adaptive trapezoidal stepping (adaptive: an object
+    with tstop, h_init, h_min, h_max, reltol, abstol, max_points).  Returns a
+    dict like the engine's TransientSweepResult."""
+    ckt, keep = circuit.to_c()
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    B, n = params.shape[0], circuit.size
+    pr = np.ascontiguousarray(np.arange(n) if probes is None else probes, dtype=np.int32)
+    o = OrcTranOpts(method=int(method), h=tstep, steps=steps)
+    if adaptive is not None:
+        o.adaptive = 1
+        for k in ("tstop", "h_init", "h_min", "h_max", "reltol", "abstol", "max_points"):
+            setattr(o, k, getattr(adaptive, k))
+    pts = adaptive.max_points if adaptive is not None else steps + 1
+    out = dict(x=np.zeros((B, n)), status=np.zeros(B, np.int32),
+               waveforms=np.zeros((B, pts, pr.size)), times=np.zeros((B, pts)),
+               num_points=np.zeros(B, np.int32), iterations=np.zeros(B, np.int32),
+               flags=np.zeros(B, np.int32), residual=np.zeros(B), fail_step=np.zeros(B, np.int32),
+               total_steps=np.zeros(B, np.int32), rejected_steps=np.zeros(B, np.int32),
+               min_step=np.zeros(B), max_step=np.zeros(B))
+    f = lib().orc_transient_ex_sweep
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(A.SpxCircuit), C.c_void_p, C.c_int64, C.POINTER(A.SpxNrConfig),
+                  C.POINTER(OrcTranOpts), C.c_int32, C.c_void_p] + [C.c_void_p] * 13 + [C.c_int32]
+    f(C.byref(ckt), _p(params), B, C.byref(cfg), C.byref(o), pr.size, _p(pr), _p(out["x"]),
+      _p(out["status"]), _p(out["waveforms"]), _p(out["times"]), _p(out["num_points"]),
+      _p(out["iterations"]), _p(out["flags"]), _p(out["residual"]), _p(out["fail_step"]),
+      _p(out["total_steps"]), _p(out["rejected_steps"]), _p(out["min_step"]),
+      _p(out["max_step"]), nthreads)
+    return out
+
+
 def transient(circuit, params, cfg, tstep, steps, probes=None, nthreads=8):
     """orc_transient_sweep over params [B][P]: Backward-Euler transient per
     instance.  Returns a dict like the engine's TransientSweepResult."""

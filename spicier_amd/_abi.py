@@ -95,15 +95,28 @@ class SpxSweepResult(C.Structure):
                 ("refinements", C.c_void_p)]
 
 
+SPX_TRAN_BE, SPX_TRAN_TRAP, SPX_TRAN_TRBDF2 = 0, 1, 2
+
+
+class SpxTranAdaptive(C.Structure):
+    _fields_ = [("tstop", C.c_double), ("h_init", C.c_double), ("h_min", C.c_double),
+                ("h_max", C.c_double), ("reltol", C.c_double), ("abstol", C.c_double),
+                ("max_points", C.c_int32)]
+
+
 class SpxTranConfig(C.Structure):
     _fields_ = [("tstep", C.c_double), ("steps", C.c_int32), ("num_probes", C.c_int32),
-                ("probes", C.c_void_p)]
+                ("probes", C.c_void_p), ("method", C.c_int32),
+                ("adaptive", C.POINTER(SpxTranAdaptive))]
 
 
 class SpxTranResult(C.Structure):
     _fields_ = [("x", C.c_void_p), ("status", C.c_void_p), ("iterations", C.c_void_p),
                 ("flags", C.c_void_p), ("residual", C.c_void_p), ("params", C.c_void_p),
-                ("waveforms", C.c_void_p), ("fail_step", C.c_void_p)]
+                ("waveforms", C.c_void_p), ("fail_step", C.c_void_p), ("times", C.c_void_p),
+                ("num_points", C.c_void_p), ("total_steps", C.c_void_p),
+                ("rejected_steps", C.c_void_p), ("min_step", C.c_void_p),
+                ("max_step", C.c_void_p)]
 
 
 class SpxTranClass(C.Structure):

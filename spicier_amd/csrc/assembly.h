@@ -43,11 +43,24 @@ __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const d
     case SPX_EL_RESISTOR:
       vals[vid * ST] = 1.0 / PV(4);
       break;
-    case SPX_EL_CAPACITOR:  // companion.rs:41-48
+    case SPX_EL_CAPACITOR:  // open at DC; the stage's companion (companion.rs:41-130)
       if (vid >= 0) {
-        const double geq = (tc && tc->mode == 1) ? PV(4) / tc->h : 0.0;
+        double geq = 0.0, ieq = 0.0;
+        if (tc && tc->mode == 1) {
+          const double C = PV(4);
+          if (tc->comp == 1) {
+            geq = 2.0 * C / tc->h;
+            ieq = geq * tc->vprev[e * ST] + tc->iprev[e * ST];
+          } else if (tc->comp == 2) {
+            geq = C / tc->bden;
+            ieq = geq * (tc->a1 * tc->vprev[e * ST] + tc->a2 * tc->vpp[e * ST]);
+          } else {
+            geq = C / tc->h;
+            ieq = geq * tc->vprev[e * ST];
+          }
+        }
         vals[vid * ST] = geq;
-        vals[(vid + 1) * ST] = (tc && tc->mode == 1) ? geq * tc->vprev[e * ST] : 0.0;
+        vals[(vid + 1) * ST] = ieq;
       }
       break;
     case SPX_EL_VPULSE:

@@ -154,18 +154,43 @@ __device__ __forceinline__ void run_dcop(Inst& I, const DevProgram& g, const Dev
   I.write_result(out, inst, o.status, o.total, o.flags, res);
 }
 
-// Backward-Euler transient of one instance (transient/solver.rs:41-160):
-// the operating point (capacitors open, PULSE sources at V1), then `steps`
-// fixed steps of h, each solved at t = step * h with every capacitor's
-// companion built from the previous step.  The reference's transient is
-// linear (one solve per step); a circuit with diodes or MOSFETs runs plain
-// Newton at each step from the previous step's solution.  Probes record
-// t = 0 and every step; the result row carries the final step's solution,
-// the total Newton iterations (operating point included), and its FP64 KCL
-// residual.  An instance whose operating point or a step does not converge
-// stops there with status FAILED (or SINGULAR) and fail_step set.
-// The instance type also provides cap_update() (vprev from x) and
-// write_probes(row).
+// One stage of a transient step with the companion set in tc: linear
+// circuits one solve (the reference's), nonlinear ones plain Newton from
+// the current solution.  Returns 1 solved, -2 hand-over (FP32 factors
+// cannot resolve the instance), anything else a failure (< 0 singular).
+template <class Inst>
+__device__ __forceinline__ int tran_stage(Inst& I, const DevProgram& g, int& it) {
+  if (!g.has_nonlinear) {
+    const int r = I.linear_solve();
+    it = 1;
+    return r == 0 ? 1 : (r == 2 ? -2 : -1);
+  }
+  int esc = 0;
+  it = 0;
+  return I.newton(0.0, 1.0, it, esc);
+}
+
+// Transient of one instance (transient/solver.rs:41-249, 461-676): the
+// operating point (capacitors open, PULSE sources at V1) gives every
+// capacitor v_prev (i_prev = 0), then either
+//  * `steps` fixed steps of h solved at t = step * h with the method's
+//    companion: Backward Euler, trapezoidal, or TR-BDF2 (a trapezoidal
+//    stage of gamma*h then the BDF2 stage, both with the sources at t, the
+//    capacitor states updated after each stage as the reference does); or
+//  * adaptive trapezoidal steps with the sources at the start of the step,
+//    each instance on its own time grid: a step whose largest capacitor LTE
+//    exceeds max(abstol, reltol * max|v_cap|) while h > h_min is rejected
+//    and retried with h scaled by clamp(sqrt(tol/lte), 0.1, 0.5); an
+//    accepted step with LTE < tol/2 lets h grow by up to 1.5x; h stays in
+//    [h_min, h_max] and is cut to end exactly at tstop.
+// The reference's transient is linear (one solve per stage); a circuit
+// with diodes or MOSFETs runs plain Newton at each stage from the latest
+// solution.  Probes (and, adaptive, times) record t = 0 and every accepted
+// point; the result row carries the final solution, the total Newton
+// iterations (operating point included) and the FP64 KCL residual of the
+// last stage.  An instance whose operating point or a stage does not
+// converge, or that needs more than max_points points, stops with status
+// FAILED (or SINGULAR) and fail_step set.
 template <class Inst>
 __device__ __forceinline__ void run_transient(Inst& I, const DevProgram& g, const DevNrCfg& cfg,
                                               const DevTran& tr, TranCtx& tc,
@@ -179,38 +204,108 @@ __device__ __forceinline__ void run_transient(Inst& I, const DevProgram& g, cons
     return;
   }
   int status = o.status, total = o.total, flags = o.flags, fail = -1;
-  double res = 0.0;
-  const int64_t row0 = inst * (int64_t)(tr.steps + 1);
+  int points = 0, tried = 0, rejected = 0;
+  double res = 0.0, hmin_used = INFINITY, hmax_used = 0.0;
+  const int64_t stride = tr.adaptive ? tr.max_points : tr.steps + 1;
+  const int64_t row0 = inst * stride;
   if (status == SPX_CONVERGED) {
-    I.cap_update();
+    I.cap_update(-1, 0.0);
     if (tout.waves) I.write_probes(tr, tout.waves + row0 * tr.num_probes);
+    if (tout.times) I.write_scalar(tout.times + row0, 0.0);
+    points = 1;
     tc.mode = 1;
     I.set_reuse(true);
-    for (int step = 1; step <= tr.steps; ++step) {
-      tc.t = step * tr.h;
-      int it = 0, esc = 0, st;
-      if (!g.has_nonlinear) {
-        const int r = I.linear_solve();
-        st = r == 0 ? 1 : (r == 2 ? -2 : -1);
-        it = 1;
-      } else {
-        st = I.newton(0.0, 1.0, it, esc);
+    if (!tr.adaptive) {
+      const double h = tr.h;
+      for (int step = 1; step <= tr.steps; ++step) {
+        tc.t = step * h;
+        int it = 0, st;
+        if (tr.method == SPX_TRAN_TRBDF2) {
+          tc.comp = 1;
+          tc.h = kTrGamma * h;
+          st = tran_stage(I, g, it);
+          total += it;
+          if (st == 1) {
+            I.cap_update(3, h);
+            tc.comp = 2;
+            tc.h = h;
+            bdf2_coeffs(h, tc.a1, tc.a2, tc.bden);
+            st = tran_stage(I, g, it);
+            total += it;
+          }
+          if (st == 1) {
+            if (step == tr.steps) res = I.kcl(0.0);  // before the capacitors move on
+            I.cap_update(2, h);
+          }
+        } else {
+          tc.comp = tr.method == SPX_TRAN_TRAP ? 1 : 0;
+          tc.h = h;
+          st = tran_stage(I, g, it);
+          total += it;
+          if (st == 1) {
+            if (step == tr.steps) res = I.kcl(0.0);
+            I.cap_update(tc.comp, h);
+          }
+        }
+        if (st == -2) {
+          I.write_handover(out, inst, flags);
+          return;
+        }
+        if (st != 1) {
+          status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
+          fail = step;
+          break;
+        }
+        if (tout.waves) I.write_probes(tr, tout.waves + (row0 + step) * tr.num_probes);
+        points = step + 1;
       }
-      total += it;
-      if (st == -2) {
-        I.write_handover(out, inst, flags);
-        return;
+      if (tr.steps == 0) res = I.kcl(o.final_gmin);
+    } else {
+      double t = 0.0, h = tr.h_init;
+      tc.comp = 1;
+      while (t < tr.tstop) {
+        h = h < tr.h_min ? tr.h_min : (h > tr.h_max ? tr.h_max : h);
+        if (t + h > tr.tstop) h = tr.tstop - t;
+        tc.t = t;  // the reference stamps the sources at the start of the step
+        tc.h = h;
+        int it = 0;
+        const int st = tran_stage(I, g, it);
+        total += it;
+        ++tried;
+        if (st == -2) {
+          I.write_handover(out, inst, flags);
+          return;
+        }
+        if (st != 1) {
+          status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
+          fail = points;
+          break;
+        }
+        double lte, ref;
+        I.cap_lte_max(h, lte, ref);
+        const double tol = fmax(tr.abstol, tr.reltol * ref);
+        if (lte > tol && h > tr.h_min) {
+          ++rejected;
+          h *= fmax(fmin(sqrt(tol / lte), 0.5), 0.1);
+          continue;
+        }
+        if (points >= tr.max_points) {
+          status = SPX_FAILED;
+          fail = points;
+          break;
+        }
+        t += h;
+        if (!(t < tr.tstop)) res = I.kcl(0.0);
+        I.cap_update(1, h);
+        hmin_used = fmin(hmin_used, h);
+        hmax_used = fmax(hmax_used, h);
+        if (tout.waves) I.write_probes(tr, tout.waves + (row0 + points) * tr.num_probes);
+        if (tout.times) I.write_scalar(tout.times + row0 + points, t);
+        ++points;
+        if (lte < tol * 0.5 && h < tr.h_max)
+          h *= fmin(fmin(sqrt(tol / fmax(lte, 1e-20)), 2.0), 1.5);
       }
-      if (st != 1) {
-        status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
-        fail = step;
-        break;
-      }
-      if (step == tr.steps) res = I.kcl(0.0);  // before the capacitors move on
-      I.cap_update();
-      if (tout.waves) I.write_probes(tr, tout.waves + (row0 + step) * tr.num_probes);
     }
-    if (tr.steps == 0) res = I.kcl(o.final_gmin);
   } else {
     fail = 0;
     res = I.kcl(o.final_gmin);
@@ -218,6 +313,11 @@ __device__ __forceinline__ void run_transient(Inst& I, const DevProgram& g, cons
   if (status == SPX_CONVERGED && !(res <= 1.0)) flags |= SPX_FLAG_KCL_FAIL;
   I.write_result(out, inst, status, total, flags, res);
   if (tout.fail_step) I.write_scalar(tout.fail_step + inst, fail);
+  if (tout.num_points) I.write_scalar(tout.num_points + inst, points);
+  if (tout.total_steps) I.write_scalar(tout.total_steps + inst, tried);
+  if (tout.rejected_steps) I.write_scalar(tout.rejected_steps + inst, rejected);
+  if (tout.min_step) I.write_scalar(tout.min_step + inst, hmin_used);
+  if (tout.max_step) I.write_scalar(tout.max_step + inst, hmax_used);
 }
 
 }  // namespace spx

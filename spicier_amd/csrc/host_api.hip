@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -650,14 +651,69 @@ int gmres_locked(spx_handle* h, int64_t n, int64_t nnz, const int32_t* rows, con
 
 int check_tran(const spx_tran_config* t, int n) {
   if (!t) return fail(SPX_ERR_INVALID_ARGUMENT, "null transient configuration");
-  if (!(t->tstep > 0.0) || t->steps < 0 || t->steps > 1000000)
+  if (t->method < SPX_TRAN_BE || t->method > SPX_TRAN_TRBDF2)
+    return fail(SPX_ERR_INVALID_ARGUMENT, "unknown integration method " + std::to_string(t->method));
+  if (const spx_tran_adaptive* a = t->adaptive) {
+    auto pos = [](double v) { return std::isfinite(v) && v > 0.0; };
+    if (!pos(a->tstop) || !pos(a->h_init) || !pos(a->h_min) || !pos(a->h_max) ||
+        a->h_max < a->h_min)
+      return fail(SPX_ERR_INVALID_ARGUMENT,
+                  "adaptive transient needs finite tstop, h_init, h_min > 0 and h_max >= h_min");
+    if (!(a->reltol >= 0.0) || !(a->abstol >= 0.0) || !std::isfinite(a->reltol) ||
+        !std::isfinite(a->abstol))
+      return fail(SPX_ERR_INVALID_ARGUMENT, "adaptive transient needs reltol, abstol >= 0");
+    if (a->max_points < 1 || a->max_points > 10000000)
+      return fail(SPX_ERR_INVALID_ARGUMENT, "adaptive transient needs 1 <= max_points <= 1e7");
+  } else if (!(t->tstep > 0.0) || t->steps < 0 || t->steps > 1000000) {
     return fail(SPX_ERR_INVALID_ARGUMENT, "transient needs tstep > 0 and 0 <= steps <= 1e6");
+  }
   if (t->num_probes < 0 || (t->num_probes > 0 && !t->probes))
     return fail(SPX_ERR_INVALID_ARGUMENT, "bad probe list");
   for (int p = 0; p < t->num_probes; ++p)
     if (t->probes[p] < 0 || t->probes[p] >= n)
       return fail(SPX_ERR_INVALID_ARGUMENT, "probe " + std::to_string(p) + " outside the unknowns");
   return SPX_OK;
+}
+
+// Points per instance in the waveform and time arrays.
+int64_t tran_points(const spx_tran_config* t) {
+  return t->adaptive ? t->adaptive->max_points : (int64_t)t->steps + 1;
+}
+
+DevTran make_dev_tran(const spx_tran_config* t, const int* dprobe) {
+  DevTran tr{};
+  tr.h = t->tstep;
+  tr.steps = t->steps;
+  tr.num_probes = t->num_probes;
+  tr.probes = dprobe;
+  tr.method = t->method;
+  if (const spx_tran_adaptive* a = t->adaptive) {
+    tr.adaptive = 1;
+    tr.method = SPX_TRAN_TRAP;
+    tr.tstop = a->tstop;
+    tr.h_init = a->h_init;
+    tr.h_min = a->h_min;
+    tr.h_max = a->h_max;
+    tr.reltol = a->reltol;
+    tr.abstol = a->abstol;
+    tr.max_points = a->max_points;
+  }
+  return tr;
+}
+
+DevTranOut make_dev_tran_out(const spx_tran_result& r, bool adaptive) {
+  DevTranOut to{};
+  to.waves = r.waveforms;
+  to.fail_step = r.fail_step;
+  if (adaptive) {
+    to.times = r.times;
+    to.num_points = r.num_points;
+    to.total_steps = r.total_steps;
+    to.rejected_steps = r.rejected_steps;
+    to.min_step = r.min_step;
+    to.max_step = r.max_step;
+  }
+  return to;
 }
 
 int transient_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_spec* spec,
@@ -698,9 +754,9 @@ int transient_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_swe
             SPX_ERR_TRANSFER);
     SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
   }
-  DevTran tr{tran->tstep, tran->steps, tran->num_probes, static_cast<const int*>(dprobe)};
+  DevTran tr = make_dev_tran(tran, static_cast<const int*>(dprobe));
   DevOut o{out->x, out->status, out->iterations, out->flags, out->residual, nullptr, nullptr};
-  DevTranOut to{out->waveforms, out->fail_step};
+  DevTranOut to = make_dev_tran_out(*out, tran->adaptive != nullptr);
   DevNrCfg dc = to_dev(cfg);
   if (!o.status && hp.n <= kDenseMaxN && dc.precision == SPX_PREC_F32_IR) {
     void* st = nullptr;  // the FP64 hand-over pass reads the statuses
@@ -801,7 +857,12 @@ hipError_t enqueue_class(const MixedClass& mc, const DevNrCfg& dc, const DevTran
 int autotune_class(spx_handle* h, MixedClass& mc, const DevNrCfg& dc, int* best) {
   const int64_t Bs = std::min<int64_t>(mc.B, 1024);
   DevTran trs = mc.tr;
-  trs.steps = std::min(mc.tr.steps, 20);
+  if (trs.adaptive) {  // an adaptive class is timed on fixed trapezoidal steps of h_init
+    trs.adaptive = 0;
+    trs.h = trs.h_init;
+    trs.steps = 20;
+  }
+  trs.steps = std::min(trs.steps, 20);
   trs.num_probes = 0;
   void *dx, *dst, *dit, *dfl, *dres;
   int rc;
@@ -811,7 +872,7 @@ int autotune_class(spx_handle* h, MixedClass& mc, const DevNrCfg& dc, int* best)
   if ((rc = workspace(h, "tune_flags", sizeof(int) * Bs, &dfl))) return rc;
   if ((rc = workspace(h, "tune_res", sizeof(double) * Bs, &dres))) return rc;
   DevOut o{(double*)dx, (int*)dst, (int*)dit, (int*)dfl, (double*)dres, nullptr, nullptr};
-  DevTranOut to{nullptr, nullptr};
+  DevTranOut to{};
   std::vector<int> cands;
   if (mc.hp.n <= kDenseMaxN) cands.push_back(SPX_KERNEL_DENSE);
   cands.push_back(SPX_KERNEL_SPARSE64);
@@ -918,7 +979,7 @@ int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_
                              hipMemcpyHostToDevice, h->stream),
               SPX_ERR_TRANSFER);
     SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
-    mc.tr = DevTran{c.tran.tstep, c.tran.steps, c.tran.num_probes, static_cast<const int*>(dprobe)};
+    mc.tr = make_dev_tran(&c.tran, static_cast<const int*>(dprobe));
     mc.o = DevOut{c.result.x, c.result.status, c.result.iterations, c.result.flags,
                   c.result.residual, nullptr, nullptr};
     if (!mc.o.status && dc.precision == SPX_PREC_F32_IR) {
@@ -928,7 +989,7 @@ int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_
         return rc;
       mc.o.status = static_cast<int*>(st);
     }
-    mc.to = DevTranOut{c.result.waveforms, c.result.fail_step};
+    mc.to = make_dev_tran_out(c.result, c.tran.adaptive != nullptr);
     // the fastest kernel depends on the precision form too
     std::vector<int> tkey = h->programs[tag].blob;
     tkey.push_back(dc.precision);
@@ -953,13 +1014,19 @@ int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_
     const uint64_t words[] = {(uint64_t)kind, (uint64_t)mc.B, (uint64_t)mc.hp.n,
                               (uint64_t)mc.tr.steps, (uint64_t)mc.tr.num_probes,
                               (uint64_t)mc.gen, mc.seed, (uint64_t)mc.offset, (uint64_t)mc.grid,
-                              (uint64_t)mc.slot, (uint64_t)mc.smem, (uint64_t)mc.lu_lds};
+                              (uint64_t)mc.slot, (uint64_t)mc.smem, (uint64_t)mc.lu_lds,
+                              (uint64_t)mc.tr.method, (uint64_t)mc.tr.adaptive,
+                              (uint64_t)mc.tr.max_points};
     kput(words, sizeof(words));
-    kput(&mc.tr.h, sizeof(double));
+    const double dwords[] = {mc.tr.h,      mc.tr.tstop,  mc.tr.h_init, mc.tr.h_min,
+                             mc.tr.h_max,  mc.tr.reltol, mc.tr.abstol};
+    kput(dwords, sizeof(dwords));
     const void* ptrs[] = {mc.params, mc.sample_dst, mc.d_specs, dprobe, mc.ws, mc.prog,
                           mc.sp.perm, c.result.x, mc.o.status, c.result.iterations,
                           c.result.flags, c.result.residual, c.result.waveforms,
-                          c.result.fail_step};
+                          c.result.fail_step, mc.to.times, mc.to.num_points,
+                          mc.to.total_steps, mc.to.rejected_steps, mc.to.min_step,
+                          mc.to.max_step};
     kput(ptrs, sizeof(ptrs));
   }
   auto enqueue_all = [&](hipStream_t st) -> hipError_t {
@@ -1424,7 +1491,9 @@ int spx_transient_sweep(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
   const int64_t B = spec->batch;
   if (B == 0) return SPX_OK;
   const int n = hp.n, P = hp.P;
-  const int64_t W = B * (int64_t)(tran->steps + 1) * tran->num_probes;
+  const int64_t pts = tran_points(tran);
+  const int64_t W = B * pts * tran->num_probes;
+  const bool adaptive = tran->adaptive != nullptr;
   std::lock_guard<std::mutex> lk(h->mu);
   DeviceGuard dg(h->device);
   void *dx, *dst, *dit, *dfl, *dres, *dpar, *dwav = nullptr, *dfs, *dexp = nullptr;
@@ -1449,6 +1518,21 @@ int spx_transient_sweep(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
   }
   spx_tran_result d{(double*)dx,   (int*)dst,      (int*)dit, (int*)dfl, (double*)dres,
                     (double*)dpar, (double*)dwav,  (int*)dfs};
+  if (adaptive) {
+    void *dtm = nullptr, *dnp, *dts, *drj, *dmn, *dmx;
+    if (out->times && (rc = workspace(h, "tr_times", sizeof(double) * B * pts, &dtm))) return rc;
+    if ((rc = workspace(h, "tr_npoints", sizeof(int) * B, &dnp))) return rc;
+    if ((rc = workspace(h, "tr_tried", sizeof(int) * B, &dts))) return rc;
+    if ((rc = workspace(h, "tr_rejected", sizeof(int) * B, &drj))) return rc;
+    if ((rc = workspace(h, "tr_hmin", sizeof(double) * B, &dmn))) return rc;
+    if ((rc = workspace(h, "tr_hmax", sizeof(double) * B, &dmx))) return rc;
+    d.times = static_cast<double*>(dtm);
+    d.num_points = static_cast<int*>(dnp);
+    d.total_steps = static_cast<int*>(dts);
+    d.rejected_steps = static_cast<int*>(drj);
+    d.min_step = static_cast<double*>(dmn);
+    d.max_step = static_cast<double*>(dmx);
+  }
   if ((rc = transient_device_locked(h, ckt, &local, cfg, tran, &d))) return rc;
   auto d2h = [&](void* dst_h, const void* src_d, size_t bytes) -> int {
     if (!dst_h || !src_d || bytes == 0) return SPX_OK;
@@ -1464,6 +1548,14 @@ int spx_transient_sweep(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
   if ((rc = d2h(out->params, dpar, sizeof(double) * B * P))) return rc;
   if ((rc = d2h(out->waveforms, dwav, sizeof(double) * W))) return rc;
   if ((rc = d2h(out->fail_step, dfs, sizeof(int) * B))) return rc;
+  if (adaptive) {
+    if ((rc = d2h(out->times, d.times, sizeof(double) * B * pts))) return rc;
+    if ((rc = d2h(out->num_points, d.num_points, sizeof(int) * B))) return rc;
+    if ((rc = d2h(out->total_steps, d.total_steps, sizeof(int) * B))) return rc;
+    if ((rc = d2h(out->rejected_steps, d.rejected_steps, sizeof(int) * B))) return rc;
+    if ((rc = d2h(out->min_step, d.min_step, sizeof(double) * B))) return rc;
+    if ((rc = d2h(out->max_step, d.max_step, sizeof(double) * B))) return rc;
+  }
   SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
   return SPX_OK;
 }

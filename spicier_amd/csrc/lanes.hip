@@ -163,18 +163,38 @@ struct LaneInstance {
   }
 
   __device__ __forceinline__ void set_reuse(bool) {}  // FP64 factors: nothing carried over
-  __device__ __forceinline__ void cap_update() {
+  // Capacitor states from xs after a stage (cap_rule's rules; transient)
+  __device__ __forceinline__ void cap_update(int rule, double h) {
     double* vp = const_cast<double*>(tc->vprev);
+    double* ip = const_cast<double*>(tc->iprev);
+    double* vq = const_cast<double*>(tc->vpp);
     for (int e = 0; e < g.num_elems; ++e) {
       const int* el = g.elems + e * kElemInts;
-      if (el[0] == SPX_EL_CAPACITOR)
-        vp[e * ST] = (el[1] >= 0 ? xs[el[1] * ST] : 0.0) - (el[2] >= 0 ? xs[el[2] * ST] : 0.0);
+      if (el[0] == SPX_EL_CAPACITOR) {
+        const double v = (el[1] >= 0 ? xs[el[1] * ST] : 0.0) - (el[2] >= 0 ? xs[el[2] * ST] : 0.0);
+        cap_rule(rule, pv[el[4] * ST], v, h, vp[e * ST], ip[e * ST], vq[e * ST]);
+      }
     }
+  }
+  // Largest capacitor LTE and |v| of a trapezoidal step of h (adaptive)
+  __device__ __forceinline__ void cap_lte_max(double h, double& lte, double& ref) {
+    double l = 0.0, r = 0.0;
+    for (int e = 0; e < g.num_elems; ++e) {
+      const int* el = g.elems + e * kElemInts;
+      if (el[0] == SPX_EL_CAPACITOR) {
+        const double v = (el[1] >= 0 ? xs[el[1] * ST] : 0.0) - (el[2] >= 0 ? xs[el[2] * ST] : 0.0);
+        l = fmax(l, cap_lte(pv[el[4] * ST], v, h, tc->vprev[e * ST], tc->iprev[e * ST]));
+        r = fmax(r, fabs(v));
+      }
+    }
+    lte = l;
+    ref = r;
   }
   __device__ __forceinline__ void write_probes(const DevTran& tr, double* row) const {
     for (int p = 0; p < tr.num_probes; ++p) row[p] = xs[tr.probes[p] * ST];
   }
-  __device__ __forceinline__ void write_scalar(int* dst, int v) const { *dst = v; }
+  template <typename V>
+  __device__ __forceinline__ void write_scalar(V* dst, V v) const { *dst = v; }
 
   __device__ __forceinline__ void write_result(const DevOut& out, int64_t inst, int status,
                                                int total, int flags, double res) const {
@@ -196,7 +216,7 @@ struct LaneInstance {
 
 // Per-lane doubles of a slot (times kLanes for the slot).
 inline int64_t lanes_instance_doubles(const DevProgram& g, const DevSparse& sp) {
-  return (int64_t)g.P + 4 * g.n + g.num_vals + 2 * g.num_elems + g.num_pos + sp.nnz_lu;
+  return (int64_t)g.P + 4 * g.n + g.num_vals + 4 * g.num_elems + g.num_pos + sp.nnz_lu;
 }
 
 __global__ __launch_bounds__(kLanes) void k_transient_lanes(DevProgram g, DevSparse sp,
@@ -214,18 +234,19 @@ __global__ __launch_bounds__(kLanes) void k_transient_lanes(DevProgram g, DevSpa
   double* y = xw + g.n * kLanes;
   double* vals = y + g.n * kLanes;
   double* vst = vals + (int64_t)g.num_vals * kLanes;
-  double* vcap = vst + (int64_t)g.num_elems * kLanes;
-  double* av = vcap + (int64_t)g.num_elems * kLanes;
+  double* vcap = vst + (int64_t)g.num_elems * kLanes;  // vprev, iprev, vpp
+  double* av = vcap + 3 * (int64_t)g.num_elems * kLanes;
   double* lu = av + (int64_t)g.num_pos * kLanes;
   for (int64_t blk = blockIdx.x; blk * kLanes < B; blk += gridDim.x) {
     const int64_t inst = blk * kLanes + lane;
     if (inst >= B) continue;
     for (int p = 0; p < g.P; ++p) pv[p * kLanes] = params[inst * g.P + p];
     for (int i = 0; i < g.n; ++i) xs[i * kLanes] = 0.0;
-    for (int e = 0; e < g.num_elems; ++e) vcap[e * kLanes] = 0.0;
+    for (int e = 0; e < 3 * g.num_elems; ++e) vcap[e * kLanes] = 0.0;
     vals[kValOne * kLanes] = 1.0;
     vals[kValGmin * kLanes] = 0.0;
-    TranCtx tc{0, 0.0, tr.h, vcap};
+    TranCtx tc{0, 0.0, tr.h, vcap, vcap + (int64_t)g.num_elems * kLanes,
+               vcap + 2 * (int64_t)g.num_elems * kLanes, 0, 0.0, 0.0, 0.0};
     LaneInstance I{g, sp, cfg, pv, vals, vst, av, lu, rhs, xs, xw, y};
     I.tc = &tc;
     run_transient(I, g, cfg, tr, tc, out, tout, inst);

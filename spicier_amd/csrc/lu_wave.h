@@ -76,8 +76,12 @@ __device__ __forceinline__ double pull_from(double v, int src) {
   return *reinterpret_cast<double*>(&p);
 }
 
+// |v| with the sign bit cleared.  The pivot search ranks the bit patterns of
+// |v|, so a -0.0 (an elimination update fma(-u, l, +0.0) whose exact
+// product underflows rounds to -0.0) must rank as zero, not above every
+// positive value.
 template <typename T>
-__device__ __forceinline__ T tabs(T v) { return v < T(0) ? -v : v; }
+__device__ __forceinline__ T tabs(T v) { return fabs(v); }
 
 // Pivot-row broadcast: kBcastReadlane moves each element through an SGPR
 // (v_readlane, VALU), kBcastLds has the pivot lane store its row to LDS and

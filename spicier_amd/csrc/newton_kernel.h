@@ -288,20 +288,39 @@ struct Instance {
     }
   }
 
-  // vprev of every capacitor from xs (transient)
-  __device__ __forceinline__ void cap_update() {
+  // Capacitor states from xs after a stage (cap_rule's rules; transient)
+  __device__ __forceinline__ void cap_update(int rule, double h) {
     double* vp = const_cast<double*>(tc->vprev);
+    double* ip = const_cast<double*>(tc->iprev);
+    double* vq = const_cast<double*>(tc->vpp);
     for (int e = lane; e < g.num_elems; e += kWave) {
       const int* el = g.elems + e * kElemInts;
-      if (el[0] == SPX_EL_CAPACITOR)
-        vp[e] = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+      if (el[0] == SPX_EL_CAPACITOR) {
+        const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+        cap_rule(rule, pv[el[4]], v, h, vp[e], ip[e], vq[e]);
+      }
     }
     __syncthreads();
+  }
+  // Largest capacitor LTE and |v| of a trapezoidal step of h (adaptive)
+  __device__ __forceinline__ void cap_lte_max(double h, double& lte, double& ref) {
+    double l = 0.0, r = 0.0;
+    for (int e = lane; e < g.num_elems; e += kWave) {
+      const int* el = g.elems + e * kElemInts;
+      if (el[0] == SPX_EL_CAPACITOR) {
+        const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+        l = fmax(l, cap_lte(pv[el[4]], v, h, tc->vprev[e], tc->iprev[e]));
+        r = fmax(r, fabs(v));
+      }
+    }
+    lte = wave_max(l);
+    ref = wave_max(r);
   }
   __device__ __forceinline__ void write_probes(const DevTran& tr, double* row) const {
     for (int p = lane; p < tr.num_probes; p += kWave) row[p] = xs[tr.probes[p]];
   }
-  __device__ __forceinline__ void write_scalar(int* dst, int v) const {
+  template <typename V>
+  __device__ __forceinline__ void write_scalar(V* dst, V v) const {
     if (lane == 0) *dst = v;
   }
 
@@ -400,12 +419,14 @@ __global__ __launch_bounds__(64, IR ? 4 : 3) void k_transient_dense(DevProgram g
   double* vst = pv + g.P;
   double* prow = vst + g.num_elems;
   double* vcap = prow + row_buffer_doubles(N);
+  double* icap = vcap + g.num_elems;
+  double* vcap2 = icap + g.num_elems;
   Instance<N, IR> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
-  TranCtx tc{0, 0.0, tr.h, vcap};
+  TranCtx tc{0, 0.0, tr.h, vcap, icap, vcap2, 0, 0.0, 0.0, 0.0};
   I.tc = &tc;
   for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
   for (int p = lane; p < g.P; p += kWave) pv[p] = params[inst * g.P + p];
-  for (int e = lane; e < g.num_elems; e += kWave) vcap[e] = 0.0;
+  for (int e = lane; e < 3 * g.num_elems; e += kWave) vcap[e] = 0.0;
   if (lane == 0) { vals[kValOne] = 1.0; vals[kValGmin] = 0.0; }
   if (lane < N) xs[lane] = 0.0;
   __syncthreads();
@@ -422,7 +443,7 @@ template <int N>
 hipError_t launch_transient_impl(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
                                  const double* params, int64_t B, const DevOut& out,
                                  const DevTranOut& tout, hipStream_t st, int phase) {
-  size_t sm = sizeof(double) * (newton_smem_doubles(g, N) + g.num_elems);
+  size_t sm = sizeof(double) * (newton_smem_doubles(g, N) + 3 * g.num_elems);
   const bool ir = (phase & 4) != 0;
   if (phase & 1) {
     hipError_t e = hipFuncSetAttribute((const void*)k_transient_dense<N, false>,

@@ -266,19 +266,39 @@ struct SparseInstance {
     }
   }
   __device__ __forceinline__ void set_reuse(bool) {}  // FP64 factors: nothing carried over
-  __device__ __forceinline__ void cap_update() {
+  // Capacitor states from xs after a stage (cap_rule's rules; transient)
+  __device__ __forceinline__ void cap_update(int rule, double h) {
     double* vp = const_cast<double*>(tc->vprev);
+    double* ip = const_cast<double*>(tc->iprev);
+    double* vq = const_cast<double*>(tc->vpp);
     for (int e = tid; e < g.num_elems; e += T) {
       const int* el = g.elems + e * kElemInts;
-      if (el[0] == SPX_EL_CAPACITOR)
-        vp[e] = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+      if (el[0] == SPX_EL_CAPACITOR) {
+        const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+        cap_rule(rule, pv[el[4]], v, h, vp[e], ip[e], vq[e]);
+      }
     }
     __syncthreads();
+  }
+  // Largest capacitor LTE and |v| of a trapezoidal step of h (adaptive)
+  __device__ __forceinline__ void cap_lte_max(double h, double& lte, double& ref) {
+    double l = 0.0, r = 0.0;
+    for (int e = tid; e < g.num_elems; e += T) {
+      const int* el = g.elems + e * kElemInts;
+      if (el[0] == SPX_EL_CAPACITOR) {
+        const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+        l = fmax(l, cap_lte(pv[el[4]], v, h, tc->vprev[e], tc->iprev[e]));
+        r = fmax(r, fabs(v));
+      }
+    }
+    lte = block_max(l);
+    ref = block_max(r);
   }
   __device__ __forceinline__ void write_probes(const DevTran& tr, double* row) const {
     for (int p = tid; p < tr.num_probes; p += T) row[p] = xs[tr.probes[p]];
   }
-  __device__ __forceinline__ void write_scalar(int* dst, int v) const {
+  template <typename V>
+  __device__ __forceinline__ void write_scalar(V* dst, V v) const {
     if (tid == 0) *dst = v;
   }
 
@@ -379,8 +399,8 @@ __global__ __launch_bounds__(T) void k_transient_sparse(
   double* vals = w;
   double* vst = vals + g.num_vals;
   double* av = vst + g.num_elems;
-  double* vcap = av + g.num_pos;
-  double* lu = lu_in_lds ? lds_lu : vcap + g.num_elems;
+  double* vcap = av + g.num_pos;  // vprev, iprev, vpp: num_elems each
+  double* lu = lu_in_lds ? lds_lu : vcap + 3 * g.num_elems;
   // small plans are read from LDS: every elimination level reads its index
   // tables, and from global memory each level would cost L2 round trips
   if (plan_in_lds) {
@@ -395,9 +415,9 @@ __global__ __launch_bounds__(T) void k_transient_sparse(
       vals[kValGmin] = 0.0;
     }
     for (int i = tid; i < n; i += T) xs[i] = 0.0;
-    for (int e = tid; e < g.num_elems; e += T) vcap[e] = 0.0;
+    for (int e = tid; e < 3 * g.num_elems; e += T) vcap[e] = 0.0;
     __syncthreads();
-    TranCtx tc{0, 0.0, tr.h, vcap};
+    TranCtx tc{0, 0.0, tr.h, vcap, vcap + g.num_elems, vcap + 2 * g.num_elems, 0, 0.0, 0.0, 0.0};
     SparseInstance<T> I{g, sp, cfg, tid, params + inst * g.P, vals, vst, av, lu, rhs, xs, xw, y,
                      red, redd};
     I.tc = &tc;
@@ -417,7 +437,7 @@ hipError_t sparse_transient_shape_t(const DevProgram& g, const DevSparse& sp, in
   // the plan's tables follow in LDS while the whole stays within the budget
   const size_t plan_bytes = (size_t)sp.blob_ints * sizeof(int);
   if (*smem + plan_bytes <= lds_budget) *smem = (*smem + plan_bytes) | kPlanInLdsBit;
-  *slot = sparse_slot_doubles(g, sp, *lu_in_lds) + ((g.num_elems + 31) & ~31);
+  *slot = sparse_slot_doubles(g, sp, *lu_in_lds) + ((3 * g.num_elems + 31) & ~31);
   const size_t bytes = *smem & ~kPlanInLdsBit;
   hipError_t e = hipFuncSetAttribute((const void*)k_transient_sparse<T>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);

@@ -16,7 +16,10 @@
 // gather per pattern entry — no atomics.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <cmath>
 
 namespace spx {
 
@@ -57,29 +60,102 @@ struct DevNrCfg {
   double kcl_reltol, kcl_abstol, kcl_vntol;
 };
 
-// Backward-Euler transient context (transient/companion.rs:41-66): mode 0
-// is the operating point the transient starts from (capacitors open, PULSE
-// sources at V1, waveforms.rs:150-160); mode 1 a time step at t with step h
-// (capacitor C -> G = C/h with I = G * v_prev from its negative to its
-// positive node, PULSE sources at eval_pulse(t)).  vprev[e] holds the
-// previous step's voltage of capacitor element e.
+// Transient context: mode 0 is the operating point the transient starts
+// from (capacitors open, PULSE sources at V1, waveforms.rs:150-160); mode 1
+// a stage solved at time t (PULSE sources at eval_pulse(t)) with the
+// capacitor companion `comp` (companion.rs:41-130), each capacitor C
+// becoming G with I from its negative to its positive node:
+//   0 Backward Euler (stamp_be)      G = C/h, I = G*vprev
+//   1 trapezoidal (stamp_trap)       G = 2C/h, I = G*vprev + iprev
+//   2 BDF2 stage (stamp_trbdf2_bdf2) G = C/bden, I = G*(a1*vprev + a2*vpp)
+// h is the companion's step (gamma*h for the TR-BDF2 trapezoidal stage).
+// vprev / iprev / vpp[e] are capacitor element e's CapacitorState.
 struct TranCtx {
   int mode;
   double t, h;
   const double* vprev;
+  const double* iprev;
+  const double* vpp;
+  int comp;
+  double a1, a2, bden;
 };
 
-// Transient run: h, step count and the probed unknowns.
+// TRBDF2_GAMMA (transient/types.rs:11).
+constexpr double kTrGamma = 2.0 - M_SQRT2;
+
+// BDF2-stage coefficients of stamp_trbdf2_bdf2 (companion.rs:100-124) for
+// a full step h: bden = b0 * h2.
+__host__ __device__ inline void bdf2_coeffs(double h, double& a1, double& a2, double& bden) {
+  const double h2 = (1.0 - kTrGamma) * h;
+  const double h1 = kTrGamma * h;
+  const double rho = h2 / h1;
+  const double denom = 1.0 + 2.0 * rho;
+  a1 = (1.0 + rho) * (1.0 + rho) / denom;
+  a2 = -rho * rho / denom;
+  const double b0 = (1.0 + rho) / denom;
+  bden = b0 * h2;
+}
+
+// CapacitorState update of one capacitor with voltage v after a stage
+// (companion.rs:62-98): rule -1 takes the operating point (vprev only; iprev
+// and vpp start at 0), 0 Backward Euler, 1 trapezoidal, 2 the TR-BDF2 final
+// update, 3 update_trbdf2_intermediate; h is the full step.
+__device__ __forceinline__ void cap_rule(int rule, double C, double v, double h, double& vp,
+                                         double& ip, double& vq) {
+  if (rule < 0) {
+    vp = v;
+    return;
+  }
+  if (rule == 3) {
+    const double h_gamma = kTrGamma * h;
+    vq = vp;
+    ip = 2.0 * C / h_gamma * (v - vq) - ip;
+    vp = v;
+    return;
+  }
+  if (rule == 0) {
+    ip = C / h * (v - vp);
+  } else if (rule == 1) {
+    ip = 2.0 * C / h * (v - vp) - ip;
+  } else {
+    const double alpha = (1.0 - kTrGamma) / (kTrGamma * (2.0 - kTrGamma));
+    ip = C / h * ((1.0 + alpha) * v - (1.0 + 2.0 * alpha) * vp + alpha * vq);
+  }
+  vq = vp;
+  vp = v;
+}
+
+// estimate_lte of one capacitor (companion.rs:132-150) for a trapezoidal
+// step of h ending at voltage v.
+__device__ __forceinline__ double cap_lte(double C, double v, double h, double vp, double ip) {
+  const double i_trap = 2.0 * C / h * (v - vp) - ip;
+  const double i_be = C / h * (v - vp);
+  return fabs(i_trap - i_be) / 3.0;
+}
+
+// Transient run: fixed steps (h, steps, method) or adaptive trapezoidal
+// stepping (tstop, h_init, h_min, h_max, reltol, abstol, max_points), and
+// the probed unknowns.
 struct DevTran {
   double h;
   int steps;
   int num_probes;
   const int* probes;
+  int method;    // SPX_TRAN_*
+  int adaptive;  // 0: fixed steps
+  double tstop, h_init, h_min, h_max, reltol, abstol;
+  int max_points;
 };
 
 struct DevTranOut {
-  double* waves;   // [B][steps + 1][num_probes]
+  double* waves;   // [B][points][num_probes], points = steps + 1 or max_points
   int* fail_step;  // [B] first step that did not converge, -1 if none
+  double* times;   // adaptive: [B][max_points]
+  int* num_points; // adaptive: [B]
+  int* total_steps;
+  int* rejected_steps;
+  double* min_step;
+  double* max_step;
 };
 
 struct DevOut {

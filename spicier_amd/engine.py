@@ -550,11 +550,41 @@ def sweep_dcop(circuit, batch: int, seed: int = 42, generator: int = A.SPX_GEN_H
     return out
 
 
+class IntegrationMethod(enum.IntEnum):
+    """IntegrationMethod (crates/spicier-solver/src/transient/types.rs:50-63)."""
+    BackwardEuler = A.SPX_TRAN_BE
+    Trapezoidal = A.SPX_TRAN_TRAP
+    TrBdf2 = A.SPX_TRAN_TRBDF2
+
+
+@dataclass
+class AdaptiveTransientParams:
+    """AdaptiveTransientParams (transient/types.rs:76-117), with the defaults
+    of its Default impl, plus the per-instance capacity of time points."""
+    tstop: float = 1e-3
+    h_init: float = 1e-9
+    h_min: float = 1e-15
+    h_max: float = 1e-6
+    reltol: float = 1e-3
+    abstol: float = 1e-6
+    max_points: int = 4096
+
+    @classmethod
+    def for_tstop(cls, tstop: float, max_points: int = 4096) -> "AdaptiveTransientParams":
+        return cls(tstop=tstop, h_max=tstop / 100.0, max_points=max_points)
+
+    def to_c(self) -> A.SpxTranAdaptive:
+        return A.SpxTranAdaptive(self.tstop, self.h_init, self.h_min, self.h_max, self.reltol,
+                                 self.abstol, int(self.max_points))
+
+
 @dataclass
 class TransientSweepResult:
-    """Backward-Euler transient of every instance (TransientResult,
+    """Transient of every instance (TransientResult / AdaptiveTransientResult,
     crates/spicier-solver/src/transient/result.rs, restricted to the probed
-    unknowns): waveforms[i, k, p] = x[probes[p]] at t = k * tstep."""
+    unknowns): waveforms[i, k, p] = x[probes[p]] at time point k.  Fixed
+    steps: t = k * tstep.  Adaptive: point_times[i, k] for k < num_points[i],
+    with total_steps, rejected_steps, min_step and max_step per instance."""
     solutions: np.ndarray
     status: np.ndarray
     iterations: np.ndarray
@@ -564,20 +594,41 @@ class TransientSweepResult:
     waveforms: np.ndarray
     fail_step: np.ndarray
     tstep: float
+    point_times: np.ndarray | None = None
+    num_points: np.ndarray | None = None
+    total_steps: np.ndarray | None = None
+    rejected_steps: np.ndarray | None = None
+    min_step: np.ndarray | None = None
+    max_step: np.ndarray | None = None
 
     @property
     def times(self) -> np.ndarray:
+        """Fixed steps: the shared grid [points]; adaptive: [batch][max_points]."""
+        if self.point_times is not None:
+            return self.point_times
         return np.arange(self.waveforms.shape[1]) * self.tstep
 
     def accepted_mask(self) -> np.ndarray:
         return (self.status == A.SPX_CONVERGED) & ((self.flags & A.SPX_FLAG_KCL_FAIL) == 0)
 
 
+def _tran_config(tstep, steps, pr, method, adaptive):
+    ad = adaptive.to_c() if adaptive is not None else None
+    tr = A.SpxTranConfig(tstep=tstep, steps=steps, num_probes=pr.size, probes=A.ptr(pr),
+                         method=int(method), adaptive=C.pointer(ad) if ad is not None else None)
+    return tr, ad
+
+
 def transient_sweep(circuit, batch: int, tstep: float, steps: int, probes=None, seed: int = 42,
                     generator: int = A.SPX_GEN_HASH, sweep_offset: int = 0,
                     config: A.SpxNrConfig | None = None, explicit_params=None,
+                    method: IntegrationMethod = IntegrationMethod.BackwardEuler,
+                    adaptive: AdaptiveTransientParams | None = None,
                     device_id: int = 0) -> TransientSweepResult:
-    """Monte Carlo Backward-Euler transient on the GPU (host buffers)."""
+    """Monte Carlo transient on the GPU (host buffers): `steps` fixed steps
+    of `tstep` with `method` (solve_transient, transient/solver.rs:41-249),
+    or, with `adaptive`, trapezoidal steps under LTE control
+    (solve_transient_adaptive, solver.rs:461-676; tstep and steps unused)."""
     dev = Device.get(device_id)
     cfg = config or nr_config()
     ckt, keep = circuit.to_c()
@@ -587,15 +638,25 @@ def transient_sweep(circuit, batch: int, tstep: float, steps: int, probes=None, 
     spec, keep2 = circuit.sweep_spec(batch, seed, generator, sweep_offset, explicit_params)
     n, P = circuit.size, circuit.num_params
     pr = np.ascontiguousarray(np.arange(n) if probes is None else probes, dtype=np.int32)
-    tr = A.SpxTranConfig(tstep=tstep, steps=steps, num_probes=pr.size, probes=A.ptr(pr))
+    tr, ad = _tran_config(tstep, steps, pr, method, adaptive)
+    pts = adaptive.max_points if adaptive is not None else steps + 1
     out = TransientSweepResult(np.zeros((batch, n)), np.zeros(batch, np.int32),
                                np.zeros(batch, np.int32), np.zeros(batch, np.int32),
                                np.zeros(batch), np.zeros((batch, P)),
-                               np.zeros((batch, steps + 1, pr.size)), np.zeros(batch, np.int32),
+                               np.zeros((batch, pts, pr.size)), np.zeros(batch, np.int32),
                                tstep)
+    if adaptive is not None:
+        out.point_times = np.zeros((batch, pts))
+        out.num_points = np.zeros(batch, np.int32)
+        out.total_steps = np.zeros(batch, np.int32)
+        out.rejected_steps = np.zeros(batch, np.int32)
+        out.min_step = np.zeros(batch)
+        out.max_step = np.zeros(batch)
     res = A.SpxTranResult(A.ptr(out.solutions), A.ptr(out.status), A.ptr(out.iterations),
                           A.ptr(out.flags), A.ptr(out.residual), A.ptr(out.params),
-                          A.ptr(out.waveforms), A.ptr(out.fail_step))
+                          A.ptr(out.waveforms), A.ptr(out.fail_step), A.ptr(out.point_times),
+                          A.ptr(out.num_points), A.ptr(out.total_steps),
+                          A.ptr(out.rejected_steps), A.ptr(out.min_step), A.ptr(out.max_step))
     _check(dev.lib.spx_transient_sweep(dev.handle, C.byref(ckt), C.byref(spec), C.byref(cfg),
                                        C.byref(tr), C.byref(res)))
     return out
@@ -614,6 +675,8 @@ class TranClass:
     seed: int = 42
     sweep_offset: int = 0
     buffers: dict = field(default_factory=dict)
+    method: IntegrationMethod = IntegrationMethod.BackwardEuler
+    adaptive: AdaptiveTransientParams | None = None
 
     def allocate(self, device_id: int = 0):
         import torch
@@ -622,18 +685,27 @@ class TranClass:
         pr = np.arange(n) if self.probes is None else np.asarray(self.probes)
         f64 = dict(dtype=torch.float64, device=dev)
         i32 = dict(dtype=torch.int32, device=dev)
+        pts = self.adaptive.max_points if self.adaptive is not None else self.steps + 1
         self.buffers = dict(x=torch.empty(B, n, **f64), status=torch.empty(B, **i32),
                             iterations=torch.empty(B, **i32), flags=torch.empty(B, **i32),
                             residual=torch.empty(B, **f64), params=torch.empty(B, P, **f64),
-                            waveforms=torch.empty(B, self.steps + 1, pr.size, **f64),
+                            waveforms=torch.empty(B, pts, pr.size, **f64),
                             fail_step=torch.empty(B, **i32))
+        if self.adaptive is not None:
+            self.buffers.update(times=torch.empty(B, pts, **f64),
+                                num_points=torch.empty(B, **i32),
+                                total_steps=torch.empty(B, **i32),
+                                rejected_steps=torch.empty(B, **i32),
+                                min_step=torch.empty(B, **f64), max_step=torch.empty(B, **f64))
         return self
 
     def host_result(self) -> "TransientSweepResult":
         b = {k: v.cpu().numpy() for k, v in self.buffers.items()}
         return TransientSweepResult(b["x"], b["status"], b["iterations"], b["flags"],
                                     b["residual"], b["params"], b["waveforms"], b["fail_step"],
-                                    self.tstep)
+                                    self.tstep, b.get("times"), b.get("num_points"),
+                                    b.get("total_steps"), b.get("rejected_steps"),
+                                    b.get("min_step"), b.get("max_step"))
 
 
 class KernelKind(enum.IntEnum):
@@ -665,12 +737,16 @@ def transient_mixed_device(classes, config: A.SpxNrConfig | None = None, graph: 
         b = c.buffers
         arr[i].circuit = C.pointer(ckt)
         arr[i].spec = spec
-        arr[i].tran = A.SpxTranConfig(tstep=c.tstep, steps=c.steps, num_probes=pr.size,
-                                      probes=A.ptr(pr))
+        tr, ad = _tran_config(c.tstep, c.steps, pr, c.method, c.adaptive)
+        keep.append(ad)
+        arr[i].tran = tr
         arr[i].result = A.SpxTranResult(A.ptr(b["x"]), A.ptr(b["status"]),
                                         A.ptr(b["iterations"]), A.ptr(b["flags"]),
                                         A.ptr(b["residual"]), A.ptr(b["params"]),
-                                        A.ptr(b["waveforms"]), A.ptr(b["fail_step"]))
+                                        A.ptr(b["waveforms"]), A.ptr(b["fail_step"]),
+                                        *[A.ptr(b.get(k)) for k in
+                                          ("times", "num_points", "total_steps",
+                                           "rejected_steps", "min_step", "max_step")])
     kinds = np.array([int(k) for k in (kernels or [KernelKind.Auto] * len(classes))],
                      dtype=np.int32)
     opts = (A.SPX_MIXED_GRAPH if graph else 0) | (A.SPX_MIXED_AUTOTUNE if autotune else 0)
