@@ -191,7 +191,12 @@ __device__ __forceinline__ int tran_stage(Inst& I, const DevProgram& g, int& it)
 // last stage.  An instance whose operating point or a stage does not
 // converge, or that needs more than max_points points, stops with status
 // FAILED (or SINGULAR) and fail_step set.
-template <class Inst>
+//
+// GENERAL = false compiles the Backward-Euler fixed-step run alone (capacitor
+// state v_prev only): the kernels that serve it keep the register budget
+// and occupancy of a lean step loop; GENERAL = true adds the other methods
+// and adaptive stepping.
+template <bool GENERAL, class Inst>
 __device__ __forceinline__ void run_transient(Inst& I, const DevProgram& g, const DevNrCfg& cfg,
                                               const DevTran& tr, TranCtx& tc,
                                               const DevOut& out, const DevTranOut& tout,
@@ -215,7 +220,31 @@ __device__ __forceinline__ void run_transient(Inst& I, const DevProgram& g, cons
     points = 1;
     tc.mode = 1;
     I.set_reuse(true);
-    if (!tr.adaptive) {
+    if constexpr (!GENERAL) {
+      const double h = tr.h;
+      tc.comp = 0;
+      tc.h = h;
+      for (int step = 1; step <= tr.steps; ++step) {
+        tc.t = step * h;
+        int it = 0;
+        const int st = tran_stage(I, g, it);
+        total += it;
+        if (st == -2) {
+          I.write_handover(out, inst, flags);
+          return;
+        }
+        if (st != 1) {
+          status = st < 0 ? SPX_SINGULAR : SPX_FAILED;
+          fail = step;
+          break;
+        }
+        if (step == tr.steps) res = I.kcl(0.0);  // before the capacitors move on
+        I.cap_update(-1, h);                     // Backward Euler reads v_prev only
+        if (tout.waves) I.write_probes(tr, tout.waves + (row0 + step) * tr.num_probes);
+        points = step + 1;
+      }
+      if (tr.steps == 0) res = I.kcl(o.final_gmin);
+    } else if (!tr.adaptive) {
       const double h = tr.h;
       for (int step = 1; step <= tr.steps; ++step) {
         tc.t = step * h;

@@ -219,6 +219,7 @@ inline int64_t lanes_instance_doubles(const DevProgram& g, const DevSparse& sp) 
   return (int64_t)g.P + 4 * g.n + g.num_vals + 4 * g.num_elems + g.num_pos + sp.nnz_lu;
 }
 
+template <bool GENERAL>
 __global__ __launch_bounds__(kLanes) void k_transient_lanes(DevProgram g, DevSparse sp,
                                                             DevNrCfg cfg, DevTran tr,
                                                             const double* __restrict__ params,
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(kLanes) void k_transient_lanes(DevProgram g, DevSpa
                vcap + 2 * (int64_t)g.num_elems * kLanes, 0, 0.0, 0.0, 0.0};
     LaneInstance I{g, sp, cfg, pv, vals, vst, av, lu, rhs, xs, xw, y};
     I.tc = &tc;
-    run_transient(I, g, cfg, tr, tc, out, tout, inst);
+    run_transient<GENERAL>(I, g, cfg, tr, tc, out, tout, inst);
   }
 }
 
@@ -260,8 +261,8 @@ hipError_t lanes_transient_shape(const DevProgram& g, const DevSparse& sp, int64
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
     return e;
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_transient_lanes,
-                                                        kLanes, 0)) != hipSuccess)
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+           &per_cu, (const void*)k_transient_lanes<false>, kLanes, 0)) != hipSuccess)
     return e;
   if (per_cu < 1) per_cu = 1;
   const int64_t blocks = (B + kLanes - 1) / kLanes;
@@ -276,8 +277,12 @@ hipError_t launch_transient_lanes(const DevProgram& g, const DevSparse& sp, cons
                                   const DevOut& out, const DevTranOut& tout, double* ws, int grid,
                                   int64_t slot, hipStream_t st) {
   if (B == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_transient_lanes, dim3((unsigned)grid), dim3(kLanes), 0, st, g, sp, cfg, tr,
-                     params, B, out, tout, ws, slot);
+  if (tr.adaptive || tr.method != SPX_TRAN_BE)
+    hipLaunchKernelGGL(k_transient_lanes<true>, dim3((unsigned)grid), dim3(kLanes), 0, st, g, sp,
+                       cfg, tr, params, B, out, tout, ws, slot);
+  else
+    hipLaunchKernelGGL(k_transient_lanes<false>, dim3((unsigned)grid), dim3(kLanes), 0, st, g,
+                       sp, cfg, tr, params, B, out, tout, ws, slot);
   return hipGetLastError();
 }
 

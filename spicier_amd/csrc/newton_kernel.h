@@ -395,11 +395,13 @@ __global__ __launch_bounds__(64, IR ? 4 : 3) void k_newton_dense(DevProgram g, D
   run_dcop(I, g, cfg, x0 != nullptr, out, inst, escalated_only);
 }
 
-// Batched Backward-Euler transient, one wave per instance: LDS as
-// k_newton_dense plus the capacitor states.  IR: FP32 factors carried across
-// the time steps with FP64 refinement; an instance FP32 cannot resolve is left
-// SPX_ACTIVE for the FP64 pass (escalated_only), which reruns it from t = 0.
-template <int N, bool IR>
+// Batched transient, one wave per instance: LDS as k_newton_dense plus the
+// capacitor states (v_prev; with GENERAL also i_prev and v_prev_prev).  IR:
+// FP32 factors carried across the time steps with FP64 refinement; an
+// instance FP32 cannot resolve is left SPX_ACTIVE for the FP64 pass
+// (escalated_only), which reruns it from t = 0.  GENERAL: the trapezoidal /
+// TR-BDF2 / adaptive driver (run_transient<true>); otherwise Backward Euler.
+template <int N, bool IR, bool GENERAL>
 __global__ __launch_bounds__(64, IR ? 4 : 3) void k_transient_dense(DevProgram g, DevNrCfg cfg, DevTran tr,
                                                         const double* __restrict__ params,
                                                         int64_t B, DevOut out, DevTranOut tout,
@@ -419,51 +421,60 @@ __global__ __launch_bounds__(64, IR ? 4 : 3) void k_transient_dense(DevProgram g
   double* vst = pv + g.P;
   double* prow = vst + g.num_elems;
   double* vcap = prow + row_buffer_doubles(N);
-  double* icap = vcap + g.num_elems;
-  double* vcap2 = icap + g.num_elems;
+  double* icap = GENERAL ? vcap + g.num_elems : nullptr;
+  double* vcap2 = GENERAL ? icap + g.num_elems : nullptr;
   Instance<N, IR> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
   TranCtx tc{0, 0.0, tr.h, vcap, icap, vcap2, 0, 0.0, 0.0, 0.0};
   I.tc = &tc;
   for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
   for (int p = lane; p < g.P; p += kWave) pv[p] = params[inst * g.P + p];
-  for (int e = lane; e < 3 * g.num_elems; e += kWave) vcap[e] = 0.0;
+  for (int e = lane; e < (GENERAL ? 3 : 1) * g.num_elems; e += kWave) vcap[e] = 0.0;
   if (lane == 0) { vals[kValOne] = 1.0; vals[kValGmin] = 0.0; }
   if (lane < N) xs[lane] = 0.0;
   __syncthreads();
   I.init_row();
   __syncthreads();
-  run_transient(I, g, cfg, tr, tc, out, tout, inst);
+  run_transient<GENERAL>(I, g, cfg, tr, tc, out, tout, inst);
 }
 
-// phase bit 1: set the kernels' LDS attribute (not allowed while a stream is
-// being captured into a graph); bit 2: launch; bit 4: the FP32+IR form (the
-// FP32 kernel, then the FP64 pass over the instances it handed over; needs
-// out.status).
-template <int N>
-hipError_t launch_transient_impl(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
-                                 const double* params, int64_t B, const DevOut& out,
-                                 const DevTranOut& tout, hipStream_t st, int phase) {
-  size_t sm = sizeof(double) * (newton_smem_doubles(g, N) + 3 * g.num_elems);
+template <int N, bool GENERAL>
+hipError_t launch_transient_t(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
+                              const double* params, int64_t B, const DevOut& out,
+                              const DevTranOut& tout, hipStream_t st, int phase) {
+  size_t sm = sizeof(double) * (newton_smem_doubles(g, N) + (GENERAL ? 3 : 1) * g.num_elems);
   const bool ir = (phase & 4) != 0;
   if (phase & 1) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_transient_dense<N, false>,
+    hipError_t e = hipFuncSetAttribute((const void*)k_transient_dense<N, false, GENERAL>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e == hipSuccess && ir)
-      e = hipFuncSetAttribute((const void*)k_transient_dense<N, true>,
+      e = hipFuncSetAttribute((const void*)k_transient_dense<N, true, GENERAL>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return e;
   }
   if (!(phase & 2) || B == 0) return hipSuccess;
   if (ir) {
     if (!out.status) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_transient_dense<N, true>), dim3((unsigned)B), dim3(kWave), sm, st, g,
-                       cfg, tr, params, B, out, tout, 0);
+    hipLaunchKernelGGL((k_transient_dense<N, true, GENERAL>), dim3((unsigned)B), dim3(kWave), sm,
+                       st, g, cfg, tr, params, B, out, tout, 0);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((k_transient_dense<N, false>), dim3((unsigned)B), dim3(kWave), sm, st, g, cfg,
-                     tr, params, B, out, tout, ir ? 1 : 0);
+  hipLaunchKernelGGL((k_transient_dense<N, false, GENERAL>), dim3((unsigned)B), dim3(kWave), sm,
+                     st, g, cfg, tr, params, B, out, tout, ir ? 1 : 0);
   return hipGetLastError();
+}
+
+// phase bit 1: set the kernels' LDS attribute (not allowed while a stream is
+// being captured into a graph); bit 2: launch; bit 4: the FP32+IR form (the
+// FP32 kernel, then the FP64 pass over the instances it handed over; needs
+// out.status).  Backward-Euler fixed steps take the lean kernels.
+template <int N>
+hipError_t launch_transient_impl(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
+                                 const double* params, int64_t B, const DevOut& out,
+                                 const DevTranOut& tout, hipStream_t st, int phase) {
+  if (tr.adaptive || tr.method != SPX_TRAN_BE)
+    return launch_transient_t<N, true>(g, cfg, tr, params, B, out, tout, st, phase);
+  return launch_transient_t<N, false>(g, cfg, tr, params, B, out, tout, st, phase);
 }
 
 template <int N, bool IR>

@@ -380,7 +380,7 @@ inline hipError_t grid_for(const void* kernel, size_t smem, int64_t B, int* grid
 
 // Batched Backward-Euler transient on the sparse path; the workspace slot
 // adds the capacitor states after the sparse Newton slot.
-template <int T>
+template <int T, bool GENERAL>
 __global__ __launch_bounds__(T) void k_transient_sparse(
     DevProgram g, DevSparse sp, DevNrCfg cfg, DevTran tr, const double* __restrict__ params,
     int64_t B, DevOut out, DevTranOut tout, double* ws, int64_t slot, int lu_in_lds,
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(T) void k_transient_sparse(
     SparseInstance<T> I{g, sp, cfg, tid, params + inst * g.P, vals, vst, av, lu, rhs, xs, xw, y,
                      red, redd};
     I.tc = &tc;
-    run_transient(I, g, cfg, tr, tc, out, tout, inst);
+    run_transient<GENERAL>(I, g, cfg, tr, tc, out, tout, inst);
     __syncthreads();
   }
 }
@@ -439,10 +439,14 @@ hipError_t sparse_transient_shape_t(const DevProgram& g, const DevSparse& sp, in
   if (*smem + plan_bytes <= lds_budget) *smem = (*smem + plan_bytes) | kPlanInLdsBit;
   *slot = sparse_slot_doubles(g, sp, *lu_in_lds) + ((3 * g.num_elems + 31) & ~31);
   const size_t bytes = *smem & ~kPlanInLdsBit;
-  hipError_t e = hipFuncSetAttribute((const void*)k_transient_sparse<T>,
+  hipError_t e = hipFuncSetAttribute((const void*)k_transient_sparse<T, false>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_transient_sparse<T, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return e;
-  return grid_for((const void*)k_transient_sparse<T>, bytes, B, grid, T);
+  // sized for the lean kernel; the persistent loop covers any grid
+  return grid_for((const void*)k_transient_sparse<T, false>, bytes, B, grid, T);
 }
 
 // Shape of a sparse transient launch with `threads` (64 or 256) per
@@ -464,12 +468,19 @@ hipError_t launch_transient_sparse(const DevProgram& g, const DevSparse& sp, con
   if (B == 0) return hipSuccess;
   const int plan = (smem & kPlanInLdsBit) ? 1 : 0;
   smem &= ~kPlanInLdsBit;
-  if (threads == 64)
-    hipLaunchKernelGGL(k_transient_sparse<64>, dim3((unsigned)grid), dim3(64), smem, st, g, sp,
-                       cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
+  const bool general = tr.adaptive || tr.method != SPX_TRAN_BE;
+  if (threads == 64 && general)
+    hipLaunchKernelGGL((k_transient_sparse<64, true>), dim3((unsigned)grid), dim3(64), smem, st,
+                       g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
+  else if (threads == 64)
+    hipLaunchKernelGGL((k_transient_sparse<64, false>), dim3((unsigned)grid), dim3(64), smem, st,
+                       g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
+  else if (general)
+    hipLaunchKernelGGL((k_transient_sparse<256, true>), dim3((unsigned)grid), dim3(256), smem,
+                       st, g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
   else
-    hipLaunchKernelGGL(k_transient_sparse<256>, dim3((unsigned)grid), dim3(256), smem, st, g, sp,
-                       cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
+    hipLaunchKernelGGL((k_transient_sparse<256, false>), dim3((unsigned)grid), dim3(256), smem,
+                       st, g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
   return hipGetLastError();
 }
 
