@@ -178,8 +178,12 @@ typedef enum spx_element_kind {
     SPX_EL_DIODE = 4,     /* node[0]=anode,node[1]=cathode; param[0]=IS, param[1]=N (diode.rs:91-131) */
     SPX_EL_NMOS = 5,      /* node[0..2]=d,g,s; param[0]=VTO,[1]=KP,[2]=LAMBDA,[3]=W,[4]=L (level1.rs:154-248) */
     SPX_EL_PMOS = 6,      /* as NMOS, PMOS sign convention */
-    SPX_EL_VPULSE = 7     /* voltage source whose transient value is PULSE(param[0]=V1, [1]=V2, [2]=TD,
+    SPX_EL_VPULSE = 7,    /* voltage source whose transient value is PULSE(param[0]=V1, [1]=V2, [2]=TD,
                              [3]=TR, [4]=TF, [5]=PW, [6]=PER) (waveforms.rs:160-180); DC value V1 */
+    SPX_EL_VSIN = 8       /* voltage source whose transient value is SIN(param[0]=VO, [1]=VA,
+                             [2]=FREQ, [3]=TD, [4]=THETA, [5]=PHASE in degrees): VO before TD,
+                             then VO + VA e^(-THETA (t-TD)) sin(2 pi FREQ (t-TD) + PHASE)
+                             (eval_sin, waveforms.rs:185-201); DC value VO */
 } spx_element_kind;
 
 #define SPX_MAX_ELEMENT_PARAMS 7

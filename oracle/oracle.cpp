@@ -320,7 +320,8 @@ extern "C" void orc_mosfet_evaluate(int32_t pmos, double vto, double kp, double 
 extern "C" int32_t orc_circuit_size(const spx_circuit* c) {
   int32_t nv = 0;
   for (int e = 0; e < c->num_elements; ++e)
-    nv += (c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE);
+    nv += (c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE ||
+           c->elements[e].kind == SPX_EL_VSIN);
   return c->num_nodes + nv;
 }
 
@@ -402,6 +403,16 @@ double eval_pulse(double v1, double v2, double td, double tr, double tf, double 
   return v1;
 }
 
+// eval_sin (waveforms.rs:185-201)
+double eval_sin(double vo, double va, double freq, double td, double theta, double phase,
+                double t) {
+  if (t < td) return vo;
+  const double t_rel = t - td;
+  const double phase_rad = phase * M_PI / 180.0;
+  const double damping = theta > 0.0 ? std::exp(-theta * t_rel) : 1.0;
+  return vo + va * damping * std::sin(2.0 * M_PI * freq * t_rel + phase_rad);
+}
+
 void stamp_impl(const spx_circuit* c, const double* params, const double* x, double gmin,
                 double sf, int32_t limiter, double* J, double* rhs, double* vstate,
                 int* limited, const OTran* tc = nullptr) {
@@ -444,6 +455,15 @@ void stamp_impl(const spx_circuit* c, const double* params, const double* x, dou
           v = eval_pulse(P(params, el.param[0]), P(params, el.param[1]), P(params, el.param[2]),
                          P(params, el.param[3]), P(params, el.param[4]), P(params, el.param[5]),
                          P(params, el.param[6]), tc->t);
+        m.vsource(a, b, k++, v * sf);
+        break;
+      }
+      case SPX_EL_VSIN: {
+        double v = P(params, el.param[0]);
+        if (tc && tc->mode == 1)
+          v = eval_sin(P(params, el.param[0]), P(params, el.param[1]), P(params, el.param[2]),
+                       P(params, el.param[3]), P(params, el.param[4]), P(params, el.param[5]),
+                       tc->t);
         m.vsource(a, b, k++, v * sf);
         break;
       }

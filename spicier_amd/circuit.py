@@ -52,7 +52,8 @@ class Circuit:
 
     @property
     def num_vsources(self) -> int:
-        return sum(e[0] in (A.SPX_EL_VSOURCE, A.SPX_EL_VPULSE) for e in self.elements)
+        return sum(e[0] in (A.SPX_EL_VSOURCE, A.SPX_EL_VPULSE, A.SPX_EL_VSIN)
+                   for e in self.elements)
 
     @property
     def size(self) -> int:
@@ -115,6 +116,11 @@ class Circuit:
 
     def vpulse(self, pos, neg, v1, v2, td, tr, tf, pw, per):
         self._add(A.SPX_EL_VPULSE, (pos, neg), (v1, v2, td, tr, tf, pw, per))
+
+    def vsin(self, pos, neg, vo, va, freq, td=0.0, theta=0.0, phase=0.0):
+        """SIN(vo va freq td theta phase) source (waveforms.rs:34-49, 185-201);
+        phase in degrees; DC value vo."""
+        self._add(A.SPX_EL_VSIN, (pos, neg), (vo, va, freq, td, theta, phase))
 
     def isource(self, frm, to, i):
         """Current i flows from `frm` through the source into `to`."""

@@ -20,6 +20,16 @@ __device__ __forceinline__ double eval_pulse(double v1, double v2, double td, do
   return v1;
 }
 
+// eval_sin (waveforms.rs:185-201): phase in degrees, damping from TD on.
+__device__ __forceinline__ double eval_sin(double vo, double va, double freq, double td,
+                                           double theta, double phase, double t) {
+  if (t < td) return vo;
+  const double t_rel = t - td;
+  const double phase_rad = phase * M_PI / 180.0;
+  const double damping = theta > 0.0 ? exp(-theta * t_rel) : 1.0;
+  return vo + va * damping * sin(2.0 * M_PI * freq * t_rel + phase_rad);
+}
+
 // Companion values of element e at node voltages xs (unknown u -> xs[u],
 // ground -> 0): conductances, currents and couplings into vals[vid..].
 // sf scales independent sources (source stepping); pnjlim junction state
@@ -62,6 +72,12 @@ __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const d
         vals[vid * ST] = geq;
         vals[(vid + 1) * ST] = ieq;
       }
+      break;
+    case SPX_EL_VSIN:
+      vals[vid * ST] =
+          ((tc && tc->mode == 1) ? eval_sin(PV(4), PV(5), PV(6), PV(7), PV(8), PV(9), tc->t)
+                                 : PV(4)) *
+          sf;
       break;
     case SPX_EL_VPULSE:
       vals[vid * ST] =

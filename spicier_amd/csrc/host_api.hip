@@ -106,7 +106,8 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
   const int nn = c->num_nodes, P = c->num_params;
   int nv = 0;
   for (int e = 0; e < c->num_elements; ++e)
-    nv += c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE;
+    nv += c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE ||
+          c->elements[e].kind == SPX_EL_VSIN;
   const int n = nn + nv;
   if (n < 1) return fail(SPX_ERR_INVALID_DIMENSION, "circuit has no unknowns");
   if (n > max_n)
@@ -146,6 +147,7 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
       case SPX_EL_DIODE: need = 2; break;
       case SPX_EL_NMOS: case SPX_EL_PMOS: need = 5; break;
       case SPX_EL_VPULSE: need = 7; break;
+      case SPX_EL_VSIN: need = 6; break;
       default:
         return fail(SPX_ERR_INVALID_ARGUMENT, "element " + std::to_string(e) + " has unknown kind");
     }
@@ -168,7 +170,8 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
         }
         break;
       case SPX_EL_VSOURCE:
-      case SPX_EL_VPULSE: {  // mna.rs:124-145
+      case SPX_EL_VPULSE:
+      case SPX_EL_VSIN: {  // mna.rs:124-145
         vid = hp.num_vals++;
         const int row = nn + branch++;
         if (u[0] >= 0) { add(u[0], row, kValOne, false); add(row, u[0], kValOne, false); }
@@ -1270,7 +1273,8 @@ int32_t spx_circuit_size(const spx_circuit* c) {
   if (!c || !c->elements) return c ? c->num_nodes : -1;
   int32_t nv = 0;
   for (int e = 0; e < c->num_elements; ++e)
-    nv += c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE;
+    nv += c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE ||
+          c->elements[e].kind == SPX_EL_VSIN;
   return c->num_nodes + nv;
 }
 

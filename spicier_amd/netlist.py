@@ -9,8 +9,9 @@ Follows spicier-parser's conventions (crates/spicier-parser/src):
     line (lexer.rs:56-160);
   * values carry SPICE suffixes T G MEG K M MIL U N P F, case-insensitive
     (spicier-core/src/units.rs:15-50);
-  * R, C, V (DC value or PULSE(v1 v2 td tr tf pw per), missing PULSE
-    arguments 0 — parser/waveforms.rs:12-42), I (current flows from n+
+  * R, C, V (DC value, PULSE(v1 v2 td tr tf pw per) or SIN(vo va freq td
+    theta phase), missing trailing arguments 0 — parser/waveforms.rs:12-72;
+    PWL is an error), I (current flows from n+
     through the source to n-), D (optional model), M (d g s b model, W= L=
     instance overrides) (parser/elements.rs);
   * .MODEL name D(IS= N=) / NMOS(...) / PMOS(...) level 1 with VTO KP LAMBDA
@@ -294,7 +295,19 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
             need(3)
             rest = args[2:]
             pulse = next((t for t in rest if t.upper().startswith("PULSE")), None)
-            if pulse is not None:
+            sinw = next((t for t in rest if t.upper().startswith("SIN")), None)
+            if any(t.upper().startswith("PWL") for t in rest):
+                raise InvalidArgument(f"line {no}: PWL sources are not supported")
+            if sinw is not None:
+                if "(" not in sinw or not sinw.endswith(")"):
+                    raise InvalidArgument(f"line {no}: expected SIN(vo va freq [td theta phase])")
+                inner = _tokens(sinw[sinw.index("(") + 1:-1])
+                if len(inner) < 3 or len(inner) > 6:
+                    raise InvalidArgument(f"line {no}: SIN takes 3 to 6 values")
+                vals = [scope.value(t, no, name=f"{nm}.SIN{k}") for k, t in enumerate(inner)]
+                vals += [0.0] * (6 - len(vals))
+                c.vsin(args[0], args[1], *vals)
+            elif pulse is not None:
                 if "(" not in pulse or not pulse.endswith(")"):
                     raise InvalidArgument(f"line {no}: expected PULSE(v1 v2 [td tr tf pw per])")
                 inner = _tokens(pulse[pulse.index("(") + 1:-1])
