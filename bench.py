@@ -43,7 +43,7 @@ MI355X_FP64_PEAK_TFLOPS = 78.6   # vector and matrix FP64; FP32 non-packed issue
 # Algorithmic bytes per launch are 107.2 MB (FP32+IR warm start) and 73.7 MB
 # (FP64 zero start): parameters and x0 read once, x and per-instance status
 # written once.
-MEASURED_TRAFFIC_BYTES = {"fp64": 937.8e6, "fp32_ir_surrogate": 1304.2e6}
+MEASURED_TRAFFIC_BYTES = {"fp64": 1006.6e6, "fp32_ir_surrogate": 1270.6e6}
 
 
 def lu_flops(n: int) -> float:
