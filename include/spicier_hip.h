@@ -169,7 +169,8 @@ int spx_gmres_solve_batch_f64_device(spx_handle* h, int64_t n, int64_t nnz, cons
  * and stamp conventions follow spicier-core MnaSystem
  * (crates/spicier-core/src/mna.rs:87-169) and the device files cited below.
  * Nodes are numbered as spicier NodeId: 0 is ground, node k>0 is unknown k-1;
- * voltage sources get branch unknowns num_nodes + (order of appearance). */
+ * voltage sources and inductors get branch unknowns num_nodes + (order of
+ * appearance). */
 typedef enum spx_element_kind {
     SPX_EL_RESISTOR = 0,  /* node[0..1]; param[0] = resistance Ω (passive.rs:53-58) */
     SPX_EL_CAPACITOR = 1, /* node[0..1]; param[0] = C (F); open at DC, BE companion in transient */
@@ -180,10 +181,14 @@ typedef enum spx_element_kind {
     SPX_EL_PMOS = 6,      /* as NMOS, PMOS sign convention */
     SPX_EL_VPULSE = 7,    /* voltage source whose transient value is PULSE(param[0]=V1, [1]=V2, [2]=TD,
                              [3]=TR, [4]=TF, [5]=PW, [6]=PER) (waveforms.rs:160-180); DC value V1 */
-    SPX_EL_VSIN = 8       /* voltage source whose transient value is SIN(param[0]=VO, [1]=VA,
+    SPX_EL_VSIN = 8,      /* voltage source whose transient value is SIN(param[0]=VO, [1]=VA,
                              [2]=FREQ, [3]=TD, [4]=THETA, [5]=PHASE in degrees): VO before TD,
                              then VO + VA e^(-THETA (t-TD)) sin(2 pi FREQ (t-TD) + PHASE)
                              (eval_sin, waveforms.rs:185-201); DC value VO */
+    SPX_EL_INDUCTOR = 9   /* node[0..1]; param[0] = L (H); a branch unknown like a voltage source
+                             (passive.rs:283-375): 0 V source at DC; in a transient step its
+                             row is i - G (v0 - v1) = I with the method's companion G, I
+                             (InductorState, transient/companion.rs:158-302) */
 } spx_element_kind;
 
 #define SPX_MAX_ELEMENT_PARAMS 7

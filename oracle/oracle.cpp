@@ -321,7 +321,7 @@ extern "C" int32_t orc_circuit_size(const spx_circuit* c) {
   int32_t nv = 0;
   for (int e = 0; e < c->num_elements; ++e)
     nv += (c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE ||
-           c->elements[e].kind == SPX_EL_VSIN);
+           c->elements[e].kind == SPX_EL_VSIN || c->elements[e].kind == SPX_EL_INDUCTOR);
   return c->num_nodes + nv;
 }
 
@@ -465,6 +465,33 @@ void stamp_impl(const spx_circuit* c, const double* params, const double* x, dou
                        P(params, el.param[3]), P(params, el.param[4]), P(params, el.param[5]),
                        tc->t);
         m.vsource(a, b, k++, v * sf);
+        break;
+      }
+      case SPX_EL_INDUCTOR: {
+        const int kb = k++;
+        if (tc && tc->mode == 1) {
+          // InductorState companions (companion.rs:197-243): the conductance
+          // G with I from node 0 to node 1 and no branch current, as the
+          // reference's transient MNA has none (stampers.rs:515-535); the
+          // branch unknown stays as a decoupled row x = 0.
+          const double L = P(params, el.param[0]);
+          double geq, ieq;
+          if (tc->comp == 1) {  // stamp_trap
+            geq = tc->h / (2.0 * L);
+            ieq = tc->iprev[e] + geq * tc->vprev[e];
+          } else if (tc->comp == 2) {  // stamp_trbdf2_bdf2
+            geq = tc->bden / L;
+            ieq = tc->a1 * tc->iprev[e] + tc->a2 * tc->vpp[e];
+          } else {  // stamp_be
+            geq = tc->h / L;
+            ieq = tc->iprev[e];
+          }
+          m.conductance(a, b, geq);
+          m.current(a, b, ieq);
+          m.add(c->num_nodes + kb, c->num_nodes + kb, 1.0);
+        } else {
+          m.vsource(a, b, kb, 0.0);  // a 0 V source at DC (passive.rs:318-325)
+        }
         break;
       }
       case SPX_EL_ISOURCE:
@@ -1001,11 +1028,44 @@ double cap_volt(const spx_circuit* c, int e, const double* x) {
   return volt(x, c->elements[e].node[0]) - volt(x, c->elements[e].node[1]);
 }
 
+// InductorState::update (companion.rs:219-239) for rules 0-2 and
+// update_trbdf2_intermediate (companion.rs:241-250) for rule 3, followed by
+// the main loop's v_prev = v_gamma (solver.rs:199-203); vp = v_prev, ip =
+// i_prev, iq = i_prev_prev.  The TR-BDF2 final update reads v_prev in both
+// history terms, as the reference writes it.
+void ind_update_rule(double L, double v, double h, int rule, double& vp, double& ip,
+                     double& iq) {
+  if (rule == 3) {
+    const double h_gamma = kGamma * h;
+    iq = ip;
+    ip += h_gamma / (2.0 * L) * (v + vp);
+    vp = v;
+    return;
+  }
+  if (rule == 0) {
+    ip += h / L * v;
+  } else if (rule == 1) {
+    ip += h / (2.0 * L) * (v + vp);
+  } else {
+    const double gamma = kGamma;
+    const double alpha = (1.0 - gamma) / (gamma * (2.0 - gamma));
+    const double di = h / L * ((1.0 + alpha) * v - (1.0 + 2.0 * alpha) * vp + alpha * vp);
+    ip += di;
+  }
+  iq = ip;
+  vp = v;
+}
+
 // CapacitorState::update (companion.rs:62-85), rule 0 BE, 1 trapezoidal,
 // 2 TR-BDF2 final; rule 3 update_trbdf2_intermediate (companion.rs:87-98).
 void cap_update_rule(const spx_circuit* c, const double* params, const double* x, double h,
                      int rule, CapState& cs) {
   for (int e = 0; e < c->num_elements; ++e) {
+    if (c->elements[e].kind == SPX_EL_INDUCTOR) {
+      ind_update_rule(P(params, c->elements[e].param[0]), cap_volt(c, e, x), h, rule,
+                      cs.vprev[e], cs.iprev[e], cs.vpp[e]);
+      continue;
+    }
     if (c->elements[e].kind != SPX_EL_CAPACITOR) continue;
     const double C = P(params, c->elements[e].param[0]);
     const double v = cap_volt(c, e, x);
@@ -1075,18 +1135,34 @@ extern "C" int orc_transient_ex(const spx_circuit* c, const double* params,
   fl &= ~SPX_FLAG_KCL_FAIL;
   int32_t total = it, fail = -1, points = 0, tried = 0, rejected = 0;
   double hmin_used = INFINITY, hmax_used = 0.0;
-  auto probe = [&](int k, double t) {
+  // xr: x with each inductor's branch entry set to its current G v + I under
+  // the companion of the latest solved stage (the transient stages leave
+  // that entry 0)
+  std::vector<double> xr(x, x + n);
+  bool solved = false;  // a transient stage solved: x takes xr's branch currents
+  std::vector<int> brow(c->num_elements, -1);  // inductor branch rows
+  for (int e = 0, k = 0; e < c->num_elements; ++e) {
+    const int kind = c->elements[e].kind;
+    if (kind == SPX_EL_VSOURCE || kind == SPX_EL_VPULSE || kind == SPX_EL_VSIN) ++k;
+    if (kind == SPX_EL_INDUCTOR) brow[e] = c->num_nodes + k++;
+  }
+  auto probe = [&](int k, double t, const double* xp) {
     if (waves)
-      for (int p = 0; p < np; ++p) waves[(size_t)k * np + p] = x[probes[p]];
+      for (int p = 0; p < np; ++p) waves[(size_t)k * np + p] = xp[probes[p]];
     if (times) times[k] = t;
   };
   if (status != SPX_CONVERGED) {
     fail = 0;
   } else {
     CapState cs(c->num_elements);
-    for (int e = 0; e < c->num_elements; ++e)
+    for (int e = 0; e < c->num_elements; ++e) {
       if (c->elements[e].kind == SPX_EL_CAPACITOR) cs.vprev[e] = cap_volt(c, e, x);
-    probe(0, 0.0);
+      if (c->elements[e].kind == SPX_EL_INDUCTOR) {  // solver.rs:60-75
+        cs.vprev[e] = cap_volt(c, e, x);
+        cs.iprev[e] = x[brow[e]];
+      }
+    }
+    probe(0, 0.0, x);
     points = 1;
     const bool nonlin = has_nonlinear(c);
     auto ctx = [&](double t, double h, int comp) {
@@ -1095,6 +1171,26 @@ extern "C" int orc_transient_ex(const spx_circuit* c, const double* params,
       tc.iprev = cs.iprev.data();
       tc.vpp = cs.vpp.data();
       return tc;
+    };
+    auto branch_currents = [&](const OTran& tc) {
+      solved = true;
+      std::memcpy(xr.data(), x, sizeof(double) * n);
+      for (int e = 0; e < c->num_elements; ++e) {
+        if (brow[e] < 0) continue;
+        const double L = P(params, c->elements[e].param[0]);
+        double geq, ieq;
+        if (tc.comp == 1) {
+          geq = tc.h / (2.0 * L);
+          ieq = tc.iprev[e] + geq * tc.vprev[e];
+        } else if (tc.comp == 2) {
+          geq = tc.bden / L;
+          ieq = tc.a1 * tc.iprev[e] + tc.a2 * tc.vpp[e];
+        } else {
+          geq = tc.h / L;
+          ieq = tc.iprev[e];
+        }
+        xr[brow[e]] = geq * cap_volt(c, e, x) + ieq;
+      }
     };
     if (!o->adaptive) {
       const double h = o->h;
@@ -1106,12 +1202,14 @@ extern "C" int orc_transient_ex(const spx_circuit* c, const double* params,
           st = tran_solve(c, params, cfg, nonlin, t1, x, &it);
           total += it;
           if (st == 1) {
+            branch_currents(t1);
             cap_update_rule(c, params, x, h, 3, cs);
             OTran t2 = ctx(t, h, 2);
             bdf2_coeffs(h, &t2.a1, &t2.a2, &t2.bden);
             st = tran_solve(c, params, cfg, nonlin, t2, x, &it);
             total += it;
             if (st == 1 && k == o->steps) res = kcl_impl(c, params, cfg, x, 0.0, &t2);
+            if (st == 1) branch_currents(t2);
           }
           if (st == 1) cap_update_rule(c, params, x, h, 2, cs);
         } else {
@@ -1121,6 +1219,7 @@ extern "C" int orc_transient_ex(const spx_circuit* c, const double* params,
           total += it;
           if (st == 1) {
             if (k == o->steps) res = kcl_impl(c, params, cfg, x, 0.0, &tc);
+            branch_currents(tc);
             cap_update_rule(c, params, x, h, comp, cs);
           }
         }
@@ -1129,7 +1228,7 @@ extern "C" int orc_transient_ex(const spx_circuit* c, const double* params,
           fail = k;
           break;
         }
-        probe(k, t);
+        probe(k, t, xr.data());
         points = k + 1;
       }
     } else {
@@ -1148,6 +1247,7 @@ extern "C" int orc_transient_ex(const spx_circuit* c, const double* params,
           fail = points;
           break;
         }
+        branch_currents(tc);
         double max_lte = 0.0, max_ref = 0.0;
         for (int e = 0; e < c->num_elements; ++e) {
           if (c->elements[e].kind != SPX_EL_CAPACITOR) continue;
@@ -1157,6 +1257,15 @@ extern "C" int orc_transient_ex(const spx_circuit* c, const double* params,
           const double i_be = C / h * (v - cs.vprev[e]);
           max_lte = std::fmax(max_lte, std::fabs(i_trap - i_be) / 3.0);
           max_ref = std::fmax(max_ref, std::fabs(v));
+        }
+        for (int e = 0; e < c->num_elements; ++e) {  // solver.rs:612-617
+          if (c->elements[e].kind != SPX_EL_INDUCTOR) continue;
+          const double L = P(params, c->elements[e].param[0]);
+          const double v = cap_volt(c, e, x);
+          const double di_trap = h / (2.0 * L) * (v + cs.vprev[e]);
+          const double di_be = h / L * v;
+          max_lte = std::fmax(max_lte, std::fabs(di_trap - di_be) / 3.0);
+          max_ref = std::fmax(max_ref, std::fabs(cs.iprev[e]));
         }
         const double tol = std::fmax(o->abstol, o->reltol * max_ref);
         if (max_lte > tol && h > o->h_min) {
@@ -1175,7 +1284,7 @@ extern "C" int orc_transient_ex(const spx_circuit* c, const double* params,
         cap_update_rule(c, params, x, h, 1, cs);
         hmin_used = std::fmin(hmin_used, h);
         hmax_used = std::fmax(hmax_used, h);
-        probe(points, t);
+        probe(points, t, xr.data());
         ++points;
         if (max_lte < tol * 0.5 && h < o->h_max) {
           const double factor = std::fmin(std::sqrt(tol / std::fmax(max_lte, 1e-20)), 2.0);
@@ -1185,6 +1294,9 @@ extern "C" int orc_transient_ex(const spx_circuit* c, const double* params,
     }
   }
   if (status == SPX_CONVERGED && !(res <= 1.0)) fl |= SPX_FLAG_KCL_FAIL;
+  if (status != SPX_SINGULAR && solved)
+    for (int e = 0; e < c->num_elements; ++e)
+      if (brow[e] >= 0) x[brow[e]] = xr[brow[e]];
   if (status == SPX_SINGULAR)
     for (int i = 0; i < n; ++i) x[i] = 0.0;
   if (iterations) *iterations = total;

@@ -36,6 +36,7 @@ SPX_FLAG_WARM_START = 16
 SPX_EL_RESISTOR, SPX_EL_CAPACITOR, SPX_EL_VSOURCE, SPX_EL_ISOURCE = 0, 1, 2, 3
 SPX_EL_DIODE, SPX_EL_NMOS, SPX_EL_PMOS, SPX_EL_VPULSE = 4, 5, 6, 7
 SPX_EL_VSIN = 8
+SPX_EL_INDUCTOR = 9
 SPX_MAX_ELEMENT_PARAMS = 7
 
 # parameter kinds

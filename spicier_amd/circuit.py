@@ -52,7 +52,8 @@ class Circuit:
 
     @property
     def num_vsources(self) -> int:
-        return sum(e[0] in (A.SPX_EL_VSOURCE, A.SPX_EL_VPULSE, A.SPX_EL_VSIN)
+        return sum(e[0] in (A.SPX_EL_VSOURCE, A.SPX_EL_VPULSE, A.SPX_EL_VSIN,
+                            A.SPX_EL_INDUCTOR)
                    for e in self.elements)
 
     @property
@@ -110,6 +111,12 @@ class Circuit:
 
     def capacitor(self, a, b, c):
         self._add(A.SPX_EL_CAPACITOR, (a, b), (c,))
+
+    def inductor(self, a, b, l):
+        """Inductor of l henries from a to b (passive.rs:283-375); its branch
+        current (a -> b) is an unknown, after the node voltages in element
+        order with the voltage sources."""
+        self._add(A.SPX_EL_INDUCTOR, (a, b), (l,))
 
     def vsource(self, pos, neg, v):
         self._add(A.SPX_EL_VSOURCE, (pos, neg), (v,))
@@ -386,6 +393,33 @@ def ladder(n: int, seed: int = 0) -> Circuit:
                    kp=c.gauss(110e-6, 0.05 * 110e-6, name=f"KP{k}"), lam=lam, w=w, l=l)
         prev = node
     assert c.size == n
+    return c
+
+
+def rlc_ladder(sections: int, seed: int = 0, diodes: bool = False) -> Circuit:
+    """Low-pass RLC ladder driven by the ladder's PULSE source: section k
+    runs s(k-1) (or in) -> R~U(10,100) Ohm -> m(k) -> L~U(1,10) nH -> s(k),
+    with C~U(0.1,1) pF from s(k) to ground; a 1 kOhm load ends the line.
+    With `diodes`, every other section's output also clamps to ground
+    through a diode (IS~logU(1e-15,1e-13) A).  2 + 3 * sections unknowns:
+    in, m(k), s(k), the source branch and one branch per inductor.  `seed`
+    names the circuit; instances differ only in parameter values."""
+    if sections < 1:
+        raise ValueError("an RLC ladder needs at least one section")
+    c = Circuit(f"rlc_ladder_{sections}_{seed}{'_d' if diodes else ''}")
+    pulse = [c.const(TRAN_PULSE[k], k.upper()) for k in ("v1", "v2", "td", "tr", "tf", "pw", "per")]
+    c.vpulse("in", "0", *pulse)
+    nd = c.const(1.0, "N")
+    prev = "in"
+    for k in range(1, sections + 1):
+        c.resistor(prev, f"m{k}", c.uniform(10.0, 100.0, name=f"R{k}"))
+        c.inductor(f"m{k}", f"s{k}", c.uniform(1e-9, 10e-9, name=f"L{k}"))
+        c.capacitor(f"s{k}", "0", c.uniform(0.1e-12, 1e-12, name=f"C{k}"))
+        if diodes and k % 2 == 0:
+            c.diode(f"s{k}", "0", c.loguniform(1e-15, 1e-13, name=f"IS{k}"), nd)
+        prev = f"s{k}"
+    c.resistor(prev, "0", c.const(1e3, "RLOAD"))
+    assert c.size == 2 + 3 * sections
     return c
 
 

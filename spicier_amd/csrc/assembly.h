@@ -73,6 +73,29 @@ __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const d
         vals[(vid + 1) * ST] = ieq;
       }
       break;
+    case SPX_EL_INDUCTOR:  // branch row c (v0 - v1) + d i = r
+      if (tc && tc->mode == 1) {
+        const double L = PV(4);
+        double geq, ieq;
+        if (tc->comp == 1) {
+          geq = tc->h / (2.0 * L);
+          ieq = tc->iprev[e * ST] + geq * tc->vprev[e * ST];
+        } else if (tc->comp == 2) {
+          geq = tc->bden / L;
+          ieq = tc->a1 * tc->iprev[e * ST] + tc->a2 * tc->vpp[e * ST];
+        } else {
+          geq = tc->h / L;
+          ieq = tc->iprev[e * ST];
+        }
+        vals[vid * ST] = -geq;
+        vals[(vid + 1) * ST] = 1.0;
+        vals[(vid + 2) * ST] = ieq;
+      } else {  // 0 V source
+        vals[vid * ST] = 1.0;
+        vals[(vid + 1) * ST] = 0.0;
+        vals[(vid + 2) * ST] = 0.0;
+      }
+      break;
     case SPX_EL_VSIN:
       vals[vid * ST] =
           ((tc && tc->mode == 1) ? eval_sin(PV(4), PV(5), PV(6), PV(7), PV(8), PV(9), tc->t)

@@ -192,10 +192,15 @@ __device__ __forceinline__ int tran_stage(Inst& I, const DevProgram& g, int& it)
 // converge, or that needs more than max_points points, stops with status
 // FAILED (or SINGULAR) and fail_step set.
 //
-// GENERAL = false compiles the Backward-Euler fixed-step run alone (capacitor
-// state v_prev only): the kernels that serve it keep the register budget
-// and occupancy of a lean step loop; GENERAL = true adds the other methods
-// and adaptive stepping.
+// Inductors (InductorState, companion.rs:158-302) start from their DC
+// branch current and voltage and follow the same stage order with their own
+// companions and updates; in adaptive runs their LTE joins the capacitors'
+// and their |i_prev| the reference magnitude.
+//
+// GENERAL = false compiles the Backward-Euler fixed-step run of a circuit
+// without inductors alone (capacitor state v_prev only): the kernels that
+// serve it keep the register budget and occupancy of a lean step loop;
+// GENERAL = true adds inductors, the other methods and adaptive stepping.
 template <bool GENERAL, class Inst>
 __device__ __forceinline__ void run_transient(Inst& I, const DevProgram& g, const DevNrCfg& cfg,
                                               const DevTran& tr, TranCtx& tc,

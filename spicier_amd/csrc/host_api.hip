@@ -91,7 +91,7 @@ int ir_phase(const DevNrCfg& dc) { return dc.precision == SPX_PREC_F32_IR ? 4 : 
 
 // ------------------------------------------------------------- circuit compile
 struct HostProgram {
-  int n = 0, nn = 0, P = 0, num_elems = 0, num_vals = 2, has_nonlinear = 0;
+  int n = 0, nn = 0, P = 0, num_elems = 0, num_vals = 2, has_nonlinear = 0, has_inductor = 0;
   bool want_sparse = false;  // build the sparse plan even when n fits the register LU
   std::vector<int> elems, order, pos_rc, pos_off, row_ptr, contrib, rhs_off, rhs_contrib;
   int max_row_nnz = 0;
@@ -107,7 +107,7 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
   int nv = 0;
   for (int e = 0; e < c->num_elements; ++e)
     nv += c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE ||
-          c->elements[e].kind == SPX_EL_VSIN;
+          c->elements[e].kind == SPX_EL_VSIN || c->elements[e].kind == SPX_EL_INDUCTOR;
   const int n = nn + nv;
   if (n < 1) return fail(SPX_ERR_INVALID_DIMENSION, "circuit has no unknowns");
   if (n > max_n)
@@ -143,6 +143,7 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
     int need = 0;
     switch (el.kind) {
       case SPX_EL_RESISTOR: case SPX_EL_CAPACITOR: case SPX_EL_VSOURCE: case SPX_EL_ISOURCE:
+      case SPX_EL_INDUCTOR:
         need = 1; break;
       case SPX_EL_DIODE: need = 2; break;
       case SPX_EL_NMOS: case SPX_EL_PMOS: need = 5; break;
@@ -178,6 +179,25 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
         if (u[1] >= 0) { add(u[1], row, kValOne, true); add(row, u[1], kValOne, true); }
         rhs[row].clear();  // rhs[row] = voltage (assignment)
         rhs[row].push_back(vid << 1);
+        break;
+      }
+      case SPX_EL_INDUCTOR: {
+        // KCL: +i at node 0, -i at node 1, as a voltage source (mna.rs:124-145).
+        // Branch row: c (v0 - v1) + d i = r with (c, d, r) = (1, 0, 0) at DC
+        // (the reference's 0 V source, passive.rs:318-325) and (-G, 1, I) in a
+        // transient step, which is the reference's conductance companion
+        // (G, I from node 0 to node 1) with i kept as an unknown.
+        vid = hp.num_vals;
+        hp.num_vals += 3;
+        const int row = nn + branch++;
+        u[2] = row;  // the element record carries its branch row
+        if (u[0] >= 0) { add(u[0], row, kValOne, false); add(row, u[0], vid, false); }
+        if (u[1] >= 0) { add(u[1], row, kValOne, true); add(row, u[1], vid, true); }
+        if (transient) {
+          add(row, row, vid + 1, false);
+          rhs[row].push_back((vid + 2) << 1);
+        }
+        hp.has_inductor = 1;
         break;
       }
       case SPX_EL_ISOURCE:
@@ -310,6 +330,7 @@ int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g, DevSpars
   g.num_vals = hp.num_vals;
   g.num_pos = (int)hp.pos_rc.size();
   g.has_nonlinear = hp.has_nonlinear;
+  g.has_inductor = hp.has_inductor;
   g.max_row_nnz = hp.max_row_nnz;
   g.elems = base + o_el;
   g.order = base + o_or;
@@ -1274,7 +1295,7 @@ int32_t spx_circuit_size(const spx_circuit* c) {
   int32_t nv = 0;
   for (int e = 0; e < c->num_elements; ++e)
     nv += c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE ||
-          c->elements[e].kind == SPX_EL_VSIN;
+          c->elements[e].kind == SPX_EL_VSIN || c->elements[e].kind == SPX_EL_INDUCTOR;
   return c->num_nodes + nv;
 }
 

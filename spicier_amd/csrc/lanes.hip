@@ -163,7 +163,8 @@ struct LaneInstance {
   }
 
   __device__ __forceinline__ void set_reuse(bool) {}  // FP64 factors: nothing carried over
-  // Capacitor states from xs after a stage (cap_rule's rules; transient)
+  // Capacitor and inductor states from xs after a stage (cap_rule's and
+  // ind_rule's rules; transient)
   __device__ __forceinline__ void cap_update(int rule, double h) {
     double* vp = const_cast<double*>(tc->vprev);
     double* ip = const_cast<double*>(tc->iprev);
@@ -173,10 +174,14 @@ struct LaneInstance {
       if (el[0] == SPX_EL_CAPACITOR) {
         const double v = (el[1] >= 0 ? xs[el[1] * ST] : 0.0) - (el[2] >= 0 ? xs[el[2] * ST] : 0.0);
         cap_rule(rule, pv[el[4] * ST], v, h, vp[e * ST], ip[e * ST], vq[e * ST]);
+      } else if (el[0] == SPX_EL_INDUCTOR && ip) {  // el[3]: its branch row
+        const double v = (el[1] >= 0 ? xs[el[1] * ST] : 0.0) - (el[2] >= 0 ? xs[el[2] * ST] : 0.0);
+        ind_rule(rule, pv[el[4] * ST], v, xs[el[3] * ST], h, vp[e * ST], ip[e * ST], vq[e * ST]);
       }
     }
   }
-  // Largest capacitor LTE and |v| of a trapezoidal step of h (adaptive)
+  // Largest LTE of a trapezoidal step of h over the capacitors and inductors,
+  // and the largest capacitor |v| and inductor |i_prev| (adaptive)
   __device__ __forceinline__ void cap_lte_max(double h, double& lte, double& ref) {
     double l = 0.0, r = 0.0;
     for (int e = 0; e < g.num_elems; ++e) {
@@ -185,6 +190,10 @@ struct LaneInstance {
         const double v = (el[1] >= 0 ? xs[el[1] * ST] : 0.0) - (el[2] >= 0 ? xs[el[2] * ST] : 0.0);
         l = fmax(l, cap_lte(pv[el[4] * ST], v, h, tc->vprev[e * ST], tc->iprev[e * ST]));
         r = fmax(r, fabs(v));
+      } else if (el[0] == SPX_EL_INDUCTOR) {  // the reference takes |i_prev| here
+        const double v = (el[1] >= 0 ? xs[el[1] * ST] : 0.0) - (el[2] >= 0 ? xs[el[2] * ST] : 0.0);
+        l = fmax(l, ind_lte(pv[el[4] * ST], v, h, tc->vprev[e * ST]));
+        r = fmax(r, fabs(tc->iprev[e * ST]));
       }
     }
     lte = l;
@@ -277,7 +286,7 @@ hipError_t launch_transient_lanes(const DevProgram& g, const DevSparse& sp, cons
                                   const DevOut& out, const DevTranOut& tout, double* ws, int grid,
                                   int64_t slot, hipStream_t st) {
   if (B == 0) return hipSuccess;
-  if (tr.adaptive || tr.method != SPX_TRAN_BE)
+  if (tr.adaptive || tr.method != SPX_TRAN_BE || g.has_inductor)
     hipLaunchKernelGGL(k_transient_lanes<true>, dim3((unsigned)grid), dim3(kLanes), 0, st, g, sp,
                        cfg, tr, params, B, out, tout, ws, slot);
   else

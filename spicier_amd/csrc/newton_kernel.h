@@ -288,7 +288,8 @@ struct Instance {
     }
   }
 
-  // Capacitor states from xs after a stage (cap_rule's rules; transient)
+  // Capacitor and inductor states from xs after a stage (cap_rule's and
+  // ind_rule's rules; transient)
   __device__ __forceinline__ void cap_update(int rule, double h) {
     double* vp = const_cast<double*>(tc->vprev);
     double* ip = const_cast<double*>(tc->iprev);
@@ -298,11 +299,15 @@ struct Instance {
       if (el[0] == SPX_EL_CAPACITOR) {
         const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
         cap_rule(rule, pv[el[4]], v, h, vp[e], ip[e], vq[e]);
+      } else if (el[0] == SPX_EL_INDUCTOR && ip) {  // el[3]: its branch row
+        const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+        ind_rule(rule, pv[el[4]], v, xs[el[3]], h, vp[e], ip[e], vq[e]);
       }
     }
     __syncthreads();
   }
-  // Largest capacitor LTE and |v| of a trapezoidal step of h (adaptive)
+  // Largest LTE of a trapezoidal step of h over the capacitors and inductors,
+  // and the largest capacitor |v| and inductor |i_prev| (adaptive)
   __device__ __forceinline__ void cap_lte_max(double h, double& lte, double& ref) {
     double l = 0.0, r = 0.0;
     for (int e = lane; e < g.num_elems; e += kWave) {
@@ -311,6 +316,10 @@ struct Instance {
         const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
         l = fmax(l, cap_lte(pv[el[4]], v, h, tc->vprev[e], tc->iprev[e]));
         r = fmax(r, fabs(v));
+      } else if (el[0] == SPX_EL_INDUCTOR) {  // the reference takes |i_prev| here
+        const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+        l = fmax(l, ind_lte(pv[el[4]], v, h, tc->vprev[e]));
+        r = fmax(r, fabs(tc->iprev[e]));
       }
     }
     lte = wave_max(l);
@@ -467,12 +476,13 @@ hipError_t launch_transient_t(const DevProgram& g, const DevNrCfg& cfg, const De
 // phase bit 1: set the kernels' LDS attribute (not allowed while a stream is
 // being captured into a graph); bit 2: launch; bit 4: the FP32+IR form (the
 // FP32 kernel, then the FP64 pass over the instances it handed over; needs
-// out.status).  Backward-Euler fixed steps take the lean kernels.
+// out.status).  Backward-Euler fixed steps of circuits without inductors take
+// the lean kernels.
 template <int N>
 hipError_t launch_transient_impl(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
                                  const double* params, int64_t B, const DevOut& out,
                                  const DevTranOut& tout, hipStream_t st, int phase) {
-  if (tr.adaptive || tr.method != SPX_TRAN_BE)
+  if (tr.adaptive || tr.method != SPX_TRAN_BE || g.has_inductor)
     return launch_transient_t<N, true>(g, cfg, tr, params, B, out, tout, st, phase);
   return launch_transient_t<N, false>(g, cfg, tr, params, B, out, tout, st, phase);
 }

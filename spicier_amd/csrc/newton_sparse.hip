@@ -266,7 +266,8 @@ struct SparseInstance {
     }
   }
   __device__ __forceinline__ void set_reuse(bool) {}  // FP64 factors: nothing carried over
-  // Capacitor states from xs after a stage (cap_rule's rules; transient)
+  // Capacitor and inductor states from xs after a stage (cap_rule's and
+  // ind_rule's rules; transient)
   __device__ __forceinline__ void cap_update(int rule, double h) {
     double* vp = const_cast<double*>(tc->vprev);
     double* ip = const_cast<double*>(tc->iprev);
@@ -276,11 +277,15 @@ struct SparseInstance {
       if (el[0] == SPX_EL_CAPACITOR) {
         const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
         cap_rule(rule, pv[el[4]], v, h, vp[e], ip[e], vq[e]);
+      } else if (el[0] == SPX_EL_INDUCTOR && ip) {  // el[3]: its branch row
+        const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+        ind_rule(rule, pv[el[4]], v, xs[el[3]], h, vp[e], ip[e], vq[e]);
       }
     }
     __syncthreads();
   }
-  // Largest capacitor LTE and |v| of a trapezoidal step of h (adaptive)
+  // Largest LTE of a trapezoidal step of h over the capacitors and inductors,
+  // and the largest capacitor |v| and inductor |i_prev| (adaptive)
   __device__ __forceinline__ void cap_lte_max(double h, double& lte, double& ref) {
     double l = 0.0, r = 0.0;
     for (int e = tid; e < g.num_elems; e += T) {
@@ -289,6 +294,10 @@ struct SparseInstance {
         const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
         l = fmax(l, cap_lte(pv[el[4]], v, h, tc->vprev[e], tc->iprev[e]));
         r = fmax(r, fabs(v));
+      } else if (el[0] == SPX_EL_INDUCTOR) {  // the reference takes |i_prev| here
+        const double v = (el[1] >= 0 ? xs[el[1]] : 0.0) - (el[2] >= 0 ? xs[el[2]] : 0.0);
+        l = fmax(l, ind_lte(pv[el[4]], v, h, tc->vprev[e]));
+        r = fmax(r, fabs(tc->iprev[e]));
       }
     }
     lte = block_max(l);
@@ -468,7 +477,7 @@ hipError_t launch_transient_sparse(const DevProgram& g, const DevSparse& sp, con
   if (B == 0) return hipSuccess;
   const int plan = (smem & kPlanInLdsBit) ? 1 : 0;
   smem &= ~kPlanInLdsBit;
-  const bool general = tr.adaptive || tr.method != SPX_TRAN_BE;
+  const bool general = tr.adaptive || tr.method != SPX_TRAN_BE || g.has_inductor;
   if (threads == 64 && general)
     hipLaunchKernelGGL((k_transient_sparse<64, true>), dim3((unsigned)grid), dim3(64), smem, st,
                        g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);

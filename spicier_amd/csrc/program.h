@@ -35,6 +35,7 @@ struct DevProgram {
   int num_vals;
   int num_pos;
   int has_nonlinear;
+  int has_inductor;  // transients then carry each inductor's current history
   int max_row_nnz;
   const int* elems;
   const int* order;
@@ -69,7 +70,13 @@ struct DevNrCfg {
 //   1 trapezoidal (stamp_trap)       G = 2C/h, I = G*vprev + iprev
 //   2 BDF2 stage (stamp_trbdf2_bdf2) G = C/bden, I = G*(a1*vprev + a2*vpp)
 // h is the companion's step (gamma*h for the TR-BDF2 trapezoidal stage).
-// vprev / iprev / vpp[e] are capacitor element e's CapacitorState.
+// vprev / iprev / vpp[e] are capacitor element e's CapacitorState.  An
+// inductor L takes the companion of InductorState (companion.rs:197-243):
+//   0  G = h/L,      I = iprev
+//   1  G = h/(2L),   I = iprev + G*vprev
+//   2  G = bden/L,   I = a1*iprev + a2*vpp
+// with I flowing from its first node to its second, and vprev / iprev /
+// vpp[e] its v_prev, i_prev and i_prev_prev.
 struct TranCtx {
   int mode;
   double t, h;
@@ -131,6 +138,46 @@ __device__ __forceinline__ double cap_lte(double C, double v, double h, double v
   const double i_trap = 2.0 * C / h * (v - vp) - ip;
   const double i_be = C / h * (v - vp);
   return fabs(i_trap - i_be) / 3.0;
+}
+
+// InductorState update (companion.rs:219-262) of one inductor with branch
+// voltage v after a stage, the same rules as cap_rule: vp = v_prev, ip =
+// i_prev, iq = i_prev_prev.  Rule -1 takes the operating point (v_prev and
+// the DC branch current ib, solver.rs:60-75).  The TR-BDF2 final update reads
+// v_prev in both history terms, and the intermediate one leaves v_prev at the
+// stage voltage (solver.rs:199-203), as the reference does.
+__device__ __forceinline__ void ind_rule(int rule, double L, double v, double ib, double h,
+                                         double& vp, double& ip, double& iq) {
+  if (rule < 0) {
+    vp = v;
+    ip = ib;
+    return;
+  }
+  if (rule == 3) {
+    const double h_gamma = kTrGamma * h;
+    iq = ip;
+    ip += h_gamma / (2.0 * L) * (v + vp);
+    vp = v;
+    return;
+  }
+  if (rule == 0) {
+    ip += h / L * v;
+  } else if (rule == 1) {
+    ip += h / (2.0 * L) * (v + vp);
+  } else {
+    const double alpha = (1.0 - kTrGamma) / (kTrGamma * (2.0 - kTrGamma));
+    ip += h / L * ((1.0 + alpha) * v - (1.0 + 2.0 * alpha) * vp + alpha * vp);
+  }
+  iq = ip;
+  vp = v;
+}
+
+// estimate_lte of one inductor (companion.rs:264-277) for a trapezoidal
+// step of h ending at branch voltage v.
+__device__ __forceinline__ double ind_lte(double L, double v, double h, double vp) {
+  const double di_trap = h / (2.0 * L) * (v + vp);
+  const double di_be = h / L * v;
+  return fabs(di_trap - di_be) / 3.0;
 }
 
 // Transient run: fixed steps (h, steps, method) or adaptive trapezoidal

@@ -9,7 +9,7 @@ Follows spicier-parser's conventions (crates/spicier-parser/src):
     line (lexer.rs:56-160);
   * values carry SPICE suffixes T G MEG K M MIL U N P F, case-insensitive
     (spicier-core/src/units.rs:15-50);
-  * R, C, V (DC value, PULSE(v1 v2 td tr tf pw per) or SIN(vo va freq td
+  * R, C, L (parse_inductor, parser/elements.rs:188-202), V (DC value, PULSE(v1 v2 td tr tf pw per) or SIN(vo va freq td
     theta phase), missing trailing arguments 0 — parser/waveforms.rs:12-72;
     PWL is an error), I (current flows from n+
     through the source to n-), D (optional model), M (d g s b model, W= L=
@@ -26,7 +26,7 @@ braces in an element or model value:
   unif(nom, rvar)           -> U(nom*(1 - rvar), nom*(1 + rvar))
 A .PARAM is one sampled parameter shared by every use (correlated); a
 function written in a value is a parameter of its own.  Arithmetic on sampled
-parameters, and elements outside the engine's models (L, E, F, G, H, Q, J,
+parameters, and elements outside the engine's models (K, E, F, G, H, Q, J,
 X, B), are errors, not approximations.
 """
 from __future__ import annotations
@@ -280,17 +280,17 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
         kind = toks[0][0].upper()
         nm = toks[0]
         args = toks[1:]
-        nn = {"R": 2, "C": 2, "V": 2, "I": 2, "D": 2, "M": 4}.get(kind, 0)
+        nn = {"R": 2, "C": 2, "L": 2, "V": 2, "I": 2, "D": 2, "M": 4}.get(kind, 0)
         args = [nodes.id(a) if k < nn else a for k, a in enumerate(args)]
 
         def need(k):
             if len(args) < k:
                 raise InvalidArgument(f"line {no}: {nm} needs {k} fields")
 
-        if kind in "RC":
+        if kind in "RCL":  # parse_inductor (elements.rs:188-202): L name n+ n- value
             need(3)
             v = scope.value(args[2], no, name=nm)
-            (c.resistor if kind == "R" else c.capacitor)(args[0], args[1], v)
+            {"R": c.resistor, "C": c.capacitor, "L": c.inductor}[kind](args[0], args[1], v)
         elif kind == "V":
             need(3)
             rest = args[2:]

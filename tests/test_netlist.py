@@ -5,6 +5,7 @@ Monte Carlo functions and the errors for what the engine does not model."""
 import pytest
 
 from spicier_amd import _abi as A
+from spicier_amd import circuit as CK
 from spicier_amd import engine as E
 from spicier_amd.netlist import parse_netlist, parse_value
 
@@ -105,7 +106,7 @@ M1 1 2 0 0 nch
 
 
 @pytest.mark.parametrize("text", [
-    "t\nL1 1 0 1n\n",                                   # inductor
+    "t\nL1 1 0 1n\nL2 2 0 1n\nK1 L1 L2 0.9\n",          # mutual inductance
     "t\nX1 1 0 sub\n",                                  # subcircuit
     "t\n.param a=agauss(1,0.1,1)\nR1 1 0 {2*a}\n",      # arithmetic on a sampled value
     "t\nV1 1 0 PULSE 0 1\nR1 1 0 1k\n",                 # PULSE without parentheses
@@ -115,3 +116,13 @@ M1 1 2 0 0 nch
 def test_errors_for_what_the_engine_does_not_model(text):
     with pytest.raises(E.InvalidArgument):
         parse_netlist(text)
+
+
+def test_inductor_parses_as_reference_test():
+    # parser/mod.rs:640-652: V1 and L1 each carry a branch current
+    p = parse_netlist("Inductor Test\nV1 1 0 10\nL1 1 2 1m\nR1 2 0 100\n.end\n")
+    c = p.circuit
+    assert len(c.elements) == 3 and c.num_nodes == 2 and c.size == 4
+    kind, nodes, slots = c.elements[1]
+    assert kind == A.SPX_EL_INDUCTOR and nodes[:2] == [1, 2]
+    assert CK.nominal_values(c)[slots[0]] == pytest.approx(1e-3)
