@@ -169,8 +169,8 @@ int spx_gmres_solve_batch_f64_device(spx_handle* h, int64_t n, int64_t nnz, cons
  * and stamp conventions follow spicier-core MnaSystem
  * (crates/spicier-core/src/mna.rs:87-169) and the device files cited below.
  * Nodes are numbered as spicier NodeId: 0 is ground, node k>0 is unknown k-1;
- * voltage sources and inductors get branch unknowns num_nodes + (order of
- * appearance). */
+ * voltage sources, inductors, VCVS and CCVS get branch unknowns num_nodes +
+ * (order of appearance). */
 typedef enum spx_element_kind {
     SPX_EL_RESISTOR = 0,  /* node[0..1]; param[0] = resistance Ω (passive.rs:53-58) */
     SPX_EL_CAPACITOR = 1, /* node[0..1]; param[0] = C (F); open at DC, BE companion in transient */
@@ -185,18 +185,32 @@ typedef enum spx_element_kind {
                              [2]=FREQ, [3]=TD, [4]=THETA, [5]=PHASE in degrees): VO before TD,
                              then VO + VA e^(-THETA (t-TD)) sin(2 pi FREQ (t-TD) + PHASE)
                              (eval_sin, waveforms.rs:185-201); DC value VO */
-    SPX_EL_INDUCTOR = 9   /* node[0..1]; param[0] = L (H); a branch unknown like a voltage source
+    SPX_EL_INDUCTOR = 9,  /* node[0..1]; param[0] = L (H); a branch unknown like a voltage source
                              (passive.rs:283-375): 0 V source at DC; in a transient step its
                              row is i - G (v0 - v1) = I with the method's companion G, I
                              (InductorState, transient/companion.rs:158-302) */
+    SPX_EL_VCVS = 10,     /* E: node[0..1] = out+/out-, node[2..3] = ctrl+/ctrl-; param[0] = gain;
+                             a branch unknown, row v(out+) - v(out-) - gain (v(ctrl+) - v(ctrl-))
+                             = 0 (controlled.rs:58-90) */
+    SPX_EL_VCCS = 11,     /* G: nodes as VCVS; param[0] = gm; gm v(ctrl+, ctrl-) enters out+ and
+                             leaves out- (controlled.rs:172-201) */
+    SPX_EL_CCCS = 12,     /* F: node[0..1] = out+/out-; `ctrl` = the controlling element; param[0]
+                             = gain; KCL +gain i(ctrl) at out+, -gain i(ctrl) at out-
+                             (controlled.rs:267-285) */
+    SPX_EL_CCVS = 13      /* H: nodes and `ctrl` as CCCS; param[0] = gain; a branch unknown, row
+                             v(out+) - v(out-) - gain i(ctrl) = 0 (controlled.rs:351-375) */
 } spx_element_kind;
 
 #define SPX_MAX_ELEMENT_PARAMS 7
 
 typedef struct spx_element {
     int32_t kind;                           /* spx_element_kind */
-    int32_t node[3];                        /* NodeId numbers, 0 = ground */
+    int32_t node[4];                        /* NodeId numbers, 0 = ground */
     int32_t param[SPX_MAX_ELEMENT_PARAMS];  /* parameter slot per quantity, -1 = unused */
+    int32_t ctrl;                           /* CCCS / CCVS: index of the controlling element, an
+                                               earlier voltage source, inductor, VCVS or CCVS
+                                               (find_vsource_branch_index, netlist.rs:412-420);
+                                               -1 otherwise */
 } spx_element;
 
 typedef struct spx_circuit {

@@ -317,11 +317,25 @@ extern "C" void orc_mosfet_evaluate(int32_t pmos, double vto, double kp, double 
 }
 
 // ----------------------------------------------------------- MNA assembly
+namespace {
+// Devices with num_current_vars() == 1: voltage sources, Inductor, Vcvs, Ccvs
+// (sources.rs, passive.rs:341-350, controlled.rs:111-113, 400-402).
+bool has_branch(int kind) {
+  return kind == SPX_EL_VSOURCE || kind == SPX_EL_VPULSE || kind == SPX_EL_VSIN ||
+         kind == SPX_EL_INDUCTOR || kind == SPX_EL_VCVS || kind == SPX_EL_CCVS;
+}
+// The branch unknown of element e (which has one): its current_index, the
+// count of branch devices before it (parser/elements.rs next_current_index).
+int branch_row(const spx_circuit* c, int e) {
+  int k = 0;
+  for (int i = 0; i < e; ++i) k += has_branch(c->elements[i].kind);
+  return c->num_nodes + k;
+}
+}  // namespace
+
 extern "C" int32_t orc_circuit_size(const spx_circuit* c) {
   int32_t nv = 0;
-  for (int e = 0; e < c->num_elements; ++e)
-    nv += (c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE ||
-           c->elements[e].kind == SPX_EL_VSIN || c->elements[e].kind == SPX_EL_INDUCTOR);
+  for (int e = 0; e < c->num_elements; ++e) nv += has_branch(c->elements[e].kind);
   return c->num_nodes + nv;
 }
 
@@ -492,6 +506,48 @@ void stamp_impl(const spx_circuit* c, const double* params, const double* x, dou
         } else {
           m.vsource(a, b, kb, 0.0);  // a 0 V source at DC (passive.rs:318-325)
         }
+        break;
+      }
+      case SPX_EL_VCVS: {  // Vcvs::stamp, controlled.rs:58-90
+        const int br = c->num_nodes + k++;
+        const int cp = idx(el.node[2]), cn = idx(el.node[3]);
+        const double gain = P(params, el.param[0]);
+        if (a >= 0) m.add(a, br, 1.0);
+        if (b >= 0) m.add(b, br, -1.0);
+        if (a >= 0) m.add(br, a, 1.0);
+        if (b >= 0) m.add(br, b, -1.0);
+        if (cp >= 0) m.add(br, cp, -gain);
+        if (cn >= 0) m.add(br, cn, gain);
+        break;
+      }
+      case SPX_EL_VCCS: {  // Vccs::stamp, controlled.rs:172-201
+        const int cp = idx(el.node[2]), cn = idx(el.node[3]);
+        const double gm = P(params, el.param[0]);
+        if (a >= 0) {
+          if (cp >= 0) m.add(a, cp, -gm);
+          if (cn >= 0) m.add(a, cn, gm);
+        }
+        if (b >= 0) {
+          if (cp >= 0) m.add(b, cp, gm);
+          if (cn >= 0) m.add(b, cn, -gm);
+        }
+        break;
+      }
+      case SPX_EL_CCCS: {  // Cccs::stamp, controlled.rs:267-285
+        const int br = branch_row(c, el.ctrl);
+        const double gain = P(params, el.param[0]);
+        if (a >= 0) m.add(a, br, gain);
+        if (b >= 0) m.add(b, br, -gain);
+        break;
+      }
+      case SPX_EL_CCVS: {  // Ccvs::stamp, controlled.rs:351-375
+        const int br = c->num_nodes + k++;
+        const int ctrl_br = branch_row(c, el.ctrl);
+        if (a >= 0) m.add(a, br, 1.0);
+        if (b >= 0) m.add(b, br, -1.0);
+        if (a >= 0) m.add(br, a, 1.0);
+        if (b >= 0) m.add(br, b, -1.0);
+        m.add(br, ctrl_br, -P(params, el.param[0]));
         break;
       }
       case SPX_EL_ISOURCE:
@@ -1141,11 +1197,8 @@ extern "C" int orc_transient_ex(const spx_circuit* c, const double* params,
   std::vector<double> xr(x, x + n);
   bool solved = false;  // a transient stage solved: x takes xr's branch currents
   std::vector<int> brow(c->num_elements, -1);  // inductor branch rows
-  for (int e = 0, k = 0; e < c->num_elements; ++e) {
-    const int kind = c->elements[e].kind;
-    if (kind == SPX_EL_VSOURCE || kind == SPX_EL_VPULSE || kind == SPX_EL_VSIN) ++k;
-    if (kind == SPX_EL_INDUCTOR) brow[e] = c->num_nodes + k++;
-  }
+  for (int e = 0; e < c->num_elements; ++e)
+    if (c->elements[e].kind == SPX_EL_INDUCTOR) brow[e] = branch_row(c, e);
   auto probe = [&](int k, double t, const double* xp) {
     if (waves)
       for (int p = 0; p < np; ++p) waves[(size_t)k * np + p] = xp[probes[p]];

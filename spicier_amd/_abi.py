@@ -37,6 +37,7 @@ SPX_EL_RESISTOR, SPX_EL_CAPACITOR, SPX_EL_VSOURCE, SPX_EL_ISOURCE = 0, 1, 2, 3
 SPX_EL_DIODE, SPX_EL_NMOS, SPX_EL_PMOS, SPX_EL_VPULSE = 4, 5, 6, 7
 SPX_EL_VSIN = 8
 SPX_EL_INDUCTOR = 9
+SPX_EL_VCVS, SPX_EL_VCCS, SPX_EL_CCCS, SPX_EL_CCVS = 10, 11, 12, 13
 SPX_MAX_ELEMENT_PARAMS = 7
 
 # parameter kinds
@@ -51,8 +52,8 @@ SPX_WARM_ZERO, SPX_WARM_GIVEN, SPX_WARM_SURROGATE = 0, 1, 2
 
 
 class SpxElement(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("node", C.c_int32 * 3),
-                ("param", C.c_int32 * SPX_MAX_ELEMENT_PARAMS)]
+    _fields_ = [("kind", C.c_int32), ("node", C.c_int32 * 4),
+                ("param", C.c_int32 * SPX_MAX_ELEMENT_PARAMS), ("ctrl", C.c_int32)]
 
 
 class SpxCircuit(C.Structure):

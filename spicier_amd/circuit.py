@@ -22,6 +22,11 @@ import numpy as np
 from . import _abi as A
 
 
+# element kinds that carry a branch current unknown, numbered in element order
+BRANCH_KINDS = (A.SPX_EL_VSOURCE, A.SPX_EL_VPULSE, A.SPX_EL_VSIN, A.SPX_EL_INDUCTOR,
+                A.SPX_EL_VCVS, A.SPX_EL_CCVS)
+
+
 @dataclass(frozen=True)
 class Param:
     """Handle of a parameter slot."""
@@ -35,6 +40,7 @@ class Circuit:
     specs: list = field(default_factory=list)   # (kind, rng_index, a, b, c, d)
     node_names: dict = field(default_factory=lambda: {"0": 0})
     slot_names: list = field(default_factory=list)
+    ctrl: dict = field(default_factory=dict)    # element index -> controlling element index
 
     # ---------------------------------------------------------------- nodes
     def node(self, name) -> int:
@@ -52,9 +58,7 @@ class Circuit:
 
     @property
     def num_vsources(self) -> int:
-        return sum(e[0] in (A.SPX_EL_VSOURCE, A.SPX_EL_VPULSE, A.SPX_EL_VSIN,
-                            A.SPX_EL_INDUCTOR)
-                   for e in self.elements)
+        return sum(e[0] in BRANCH_KINDS for e in self.elements)
 
     @property
     def size(self) -> int:
@@ -101,46 +105,75 @@ class Circuit:
         return self.const(v).slot
 
     # ------------------------------------------------------------- elements
-    def _add(self, kind, nodes, params):
-        n = [self.node(x) for x in nodes] + [0] * (3 - len(nodes))
+    def _add(self, kind, nodes, params, ctrl=None) -> int:
+        """Appends an element; returns its index."""
+        n = [self.node(x) for x in nodes] + [0] * (4 - len(nodes))
         p = [self._p(v) for v in params] + [-1] * (A.SPX_MAX_ELEMENT_PARAMS - len(params))
         self.elements.append((kind, n, p))
+        if ctrl is not None:
+            self.ctrl[len(self.elements) - 1] = ctrl
+        return len(self.elements) - 1
+
+    def _branch_element(self, ctrl) -> int:
+        if not (isinstance(ctrl, int) and 0 <= ctrl < len(self.elements)
+                and self.elements[ctrl][0] in BRANCH_KINDS):
+            raise ValueError("a current-controlled source needs an earlier element with a "
+                             "branch current (voltage source, inductor, VCVS or CCVS)")
+        return ctrl
+
+    def vcvs(self, op, on, cp, cn, gain):
+        """E: v(op) - v(on) = gain (v(cp) - v(cn)) (controlled.rs:58-90)."""
+        return self._add(A.SPX_EL_VCVS, (op, on, cp, cn), (gain,))
+
+    def vccs(self, op, on, cp, cn, gm):
+        """G: gm (v(cp) - v(cn)) enters op and leaves on (controlled.rs:172-201)."""
+        return self._add(A.SPX_EL_VCCS, (op, on, cp, cn), (gm,))
+
+    def cccs(self, op, on, ctrl: int, gain):
+        """F: gain times element `ctrl`'s branch current, KCL +at op, -at on
+        (controlled.rs:267-285)."""
+        return self._add(A.SPX_EL_CCCS, (op, on), (gain,), ctrl=self._branch_element(ctrl))
+
+    def ccvs(self, op, on, ctrl: int, gain):
+        """H: v(op) - v(on) = gain times element `ctrl`'s branch current
+        (controlled.rs:351-375)."""
+        return self._add(A.SPX_EL_CCVS, (op, on), (gain,), ctrl=self._branch_element(ctrl))
 
     def resistor(self, a, b, r):
-        self._add(A.SPX_EL_RESISTOR, (a, b), (r,))
+        return self._add(A.SPX_EL_RESISTOR, (a, b), (r,))
 
     def capacitor(self, a, b, c):
-        self._add(A.SPX_EL_CAPACITOR, (a, b), (c,))
+        return self._add(A.SPX_EL_CAPACITOR, (a, b), (c,))
 
     def inductor(self, a, b, l):
         """Inductor of l henries from a to b (passive.rs:283-375); its branch
         current (a -> b) is an unknown, after the node voltages in element
         order with the voltage sources."""
-        self._add(A.SPX_EL_INDUCTOR, (a, b), (l,))
+        return self._add(A.SPX_EL_INDUCTOR, (a, b), (l,))
 
     def vsource(self, pos, neg, v):
-        self._add(A.SPX_EL_VSOURCE, (pos, neg), (v,))
+        return self._add(A.SPX_EL_VSOURCE, (pos, neg), (v,))
 
     def vpulse(self, pos, neg, v1, v2, td, tr, tf, pw, per):
-        self._add(A.SPX_EL_VPULSE, (pos, neg), (v1, v2, td, tr, tf, pw, per))
+        return self._add(A.SPX_EL_VPULSE, (pos, neg), (v1, v2, td, tr, tf, pw, per))
 
     def vsin(self, pos, neg, vo, va, freq, td=0.0, theta=0.0, phase=0.0):
         """SIN(vo va freq td theta phase) source (waveforms.rs:34-49, 185-201);
         phase in degrees; DC value vo."""
-        self._add(A.SPX_EL_VSIN, (pos, neg), (vo, va, freq, td, theta, phase))
+        return self._add(A.SPX_EL_VSIN, (pos, neg), (vo, va, freq, td, theta, phase))
 
     def isource(self, frm, to, i):
         """Current i flows from `frm` through the source into `to`."""
-        self._add(A.SPX_EL_ISOURCE, (frm, to), (i,))
+        return self._add(A.SPX_EL_ISOURCE, (frm, to), (i,))
 
     def diode(self, anode, cathode, is_=1e-14, n=1.0):
-        self._add(A.SPX_EL_DIODE, (anode, cathode), (is_, n))
+        return self._add(A.SPX_EL_DIODE, (anode, cathode), (is_, n))
 
     def nmos(self, d, g, s, vto=0.7, kp=2e-5, lam=0.0, w=10e-6, l=1e-6):
-        self._add(A.SPX_EL_NMOS, (d, g, s), (vto, kp, lam, w, l))
+        return self._add(A.SPX_EL_NMOS, (d, g, s), (vto, kp, lam, w, l))
 
     def pmos(self, d, g, s, vto=-0.7, kp=1e-5, lam=0.0, w=10e-6, l=1e-6):
-        self._add(A.SPX_EL_PMOS, (d, g, s), (vto, kp, lam, w, l))
+        return self._add(A.SPX_EL_PMOS, (d, g, s), (vto, kp, lam, w, l))
 
     # ------------------------------------------------------------ C structs
     def to_c(self):
@@ -149,10 +182,11 @@ class Circuit:
         arr = (A.SpxElement * max(1, len(self.elements)))()
         for i, (k, n, p) in enumerate(self.elements):
             arr[i].kind = k
-            for j in range(3):
+            for j in range(4):
                 arr[i].node[j] = n[j]
             for j in range(A.SPX_MAX_ELEMENT_PARAMS):
                 arr[i].param[j] = p[j]
+            arr[i].ctrl = self.ctrl.get(i, -1)
         ckt = A.SpxCircuit(num_nodes=self.num_nodes, num_elements=len(self.elements),
                            elements=C.cast(arr, C.POINTER(A.SpxElement)),
                            num_params=self.num_params)
@@ -420,6 +454,38 @@ def rlc_ladder(sections: int, seed: int = 0, diodes: bool = False) -> Circuit:
         prev = f"s{k}"
     c.resistor(prev, "0", c.const(1e3, "RLOAD"))
     assert c.size == 2 + 3 * sections
+    return c
+
+
+def controlled_ladder(sections: int, seed: int = 0) -> Circuit:
+    """Cascade of buffered RC sections with every controlled-source kind,
+    driven by the ladder's PULSE source.  Section k: a 0 V sense source
+    from the previous buffer output (or in) to s(k); R~U(1,10) kOhm from
+    s(k) to a(k) and C~U(0.1,1) pF from a(k) to ground; F injects 0.1 times
+    the sense current into a(k); E buffers b(k) = g a(k), g~U(0.5,0.9), with
+    a 10 kOhm load; H drives h(k) = 1 kOhm times the sense current into
+    10 kOhm; G sinks gm a(k), gm~U(0.1,1) mS, from node q(k) loaded by 1 kOhm.
+    8 * sections + 2 unknowns; instances differ only in parameter values."""
+    if sections < 1:
+        raise ValueError("a controlled ladder needs at least one section")
+    c = Circuit(f"controlled_ladder_{sections}_{seed}")
+    pulse = [c.const(TRAN_PULSE[k], k.upper()) for k in ("v1", "v2", "td", "tr", "tf", "pw", "per")]
+    c.vpulse("in", "0", *pulse)
+    zero, rl, rm, fk = c.const(0.0, "ZERO"), c.const(10e3, "RL"), c.const(1e3, "RM"), c.const(0.1, "FK")
+    prev = "in"
+    for k in range(1, sections + 1):
+        vs = c.vsource(prev, f"s{k}", zero)
+        c.resistor(f"s{k}", f"a{k}", c.uniform(1e3, 10e3, name=f"R{k}"))
+        c.capacitor(f"a{k}", "0", c.uniform(0.1e-12, 1e-12, name=f"C{k}"))
+        c.cccs("0", f"a{k}", vs, fk)
+        c.vcvs(f"b{k}", "0", f"a{k}", "0", c.uniform(0.5, 0.9, name=f"E{k}"))
+        c.resistor(f"b{k}", "0", rl)
+        c.ccvs(f"h{k}", "0", vs, rm)
+        c.resistor(f"h{k}", "0", rl)
+        c.vccs("0", f"q{k}", f"a{k}", "0", c.uniform(1e-4, 1e-3, name=f"G{k}"))
+        c.resistor(f"q{k}", "0", rm)
+        prev = f"b{k}"
+    assert c.size == 8 * sections + 2
     return c
 
 

@@ -113,6 +113,12 @@ __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const d
     case SPX_EL_ISOURCE:
       vals[vid * ST] = PV(4) * sf;
       break;
+    case SPX_EL_VCVS:  // gains: dependent sources do not take source stepping
+    case SPX_EL_VCCS:
+    case SPX_EL_CCCS:
+    case SPX_EL_CCVS:
+      vals[vid * ST] = PV(4);
+      break;
     case SPX_EL_DIODE: {
       double vd = X(el[1]) - X(el[2]);
       double gd, ieq;

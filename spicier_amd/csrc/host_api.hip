@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -90,10 +91,17 @@ constexpr int kSparseMaxN = 4096;
 int ir_phase(const DevNrCfg& dc) { return dc.precision == SPX_PREC_F32_IR ? 4 : 0; }
 
 // ------------------------------------------------------------- circuit compile
+// Element kinds with a branch current unknown (numbered in element order).
+inline bool has_branch(int kind) {
+  return kind == SPX_EL_VSOURCE || kind == SPX_EL_VPULSE || kind == SPX_EL_VSIN ||
+         kind == SPX_EL_INDUCTOR || kind == SPX_EL_VCVS || kind == SPX_EL_CCVS;
+}
+
 struct HostProgram {
   int n = 0, nn = 0, P = 0, num_elems = 0, num_vals = 2, has_nonlinear = 0, has_inductor = 0;
   bool want_sparse = false;  // build the sparse plan even when n fits the register LU
   std::vector<int> elems, order, pos_rc, pos_off, row_ptr, contrib, rhs_off, rhs_contrib;
+  std::vector<char> weak_diag;  // nodes with no diagonal entry besides gmin
   int max_row_nnz = 0;
 };
 
@@ -106,8 +114,7 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
   const int nn = c->num_nodes, P = c->num_params;
   int nv = 0;
   for (int e = 0; e < c->num_elements; ++e)
-    nv += c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE ||
-          c->elements[e].kind == SPX_EL_VSIN || c->elements[e].kind == SPX_EL_INDUCTOR;
+    nv += has_branch(c->elements[e].kind);
   const int n = nn + nv;
   if (n < 1) return fail(SPX_ERR_INVALID_DIMENSION, "circuit has no unknowns");
   if (n > max_n)
@@ -131,19 +138,31 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
     if (j >= 0) rhs[j].push_back(vid << 1);
   };
   int branch = 0;
+  std::vector<int> brow(c->num_elements, -1);  // branch row of each branch element
+  std::set<int> dc_zero_vid;                   // value slots that are 0 at DC
   for (int e = 0; e < c->num_elements; ++e) {
     const spx_element& el = c->elements[e];
-    int u[3];
-    for (int k = 0; k < 3; ++k) {
+    int u[4];
+    for (int k = 0; k < 4; ++k) {
       if (el.node[k] < 0 || el.node[k] > nn)
         return fail(SPX_ERR_INVALID_ARGUMENT, "element " + std::to_string(e) +
                                                   " references node " + std::to_string(el.node[k]));
       u[k] = el.node[k] - 1;
     }
+    if (has_branch(el.kind)) brow[e] = nn + branch;  // the case below takes branch++
+    int cb = -1;  // controlling branch row (CCCS / CCVS)
+    if (el.kind == SPX_EL_CCCS || el.kind == SPX_EL_CCVS) {
+      if (el.ctrl < 0 || el.ctrl >= e || brow[el.ctrl] < 0)
+        return fail(SPX_ERR_INVALID_ARGUMENT,
+                    "element " + std::to_string(e) +
+                        " needs an earlier controlling element with a branch current");
+      cb = brow[el.ctrl];
+    }
     int need = 0;
     switch (el.kind) {
       case SPX_EL_RESISTOR: case SPX_EL_CAPACITOR: case SPX_EL_VSOURCE: case SPX_EL_ISOURCE:
-      case SPX_EL_INDUCTOR:
+      case SPX_EL_INDUCTOR: case SPX_EL_VCVS: case SPX_EL_VCCS: case SPX_EL_CCCS:
+      case SPX_EL_CCVS:
         need = 1; break;
       case SPX_EL_DIODE: need = 2; break;
       case SPX_EL_NMOS: case SPX_EL_PMOS: need = 5; break;
@@ -165,6 +184,7 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
       case SPX_EL_CAPACITOR:  // open at DC; BE companion in a transient (companion.rs:41-48)
         if (transient) {
           vid = hp.num_vals;
+          dc_zero_vid.insert(vid);
           hp.num_vals += 2;
           cond(u[0], u[1], vid);
           cur(u[1], u[0], vid + 1);
@@ -200,6 +220,43 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
         hp.has_inductor = 1;
         break;
       }
+      case SPX_EL_VCVS: {  // controlled.rs:58-90, in its push order
+        vid = hp.num_vals++;
+        const int row = nn + branch++;
+        if (u[0] >= 0) add(u[0], row, kValOne, false);
+        if (u[1] >= 0) add(u[1], row, kValOne, true);
+        if (u[0] >= 0) add(row, u[0], kValOne, false);
+        if (u[1] >= 0) add(row, u[1], kValOne, true);
+        if (u[2] >= 0) add(row, u[2], vid, true);
+        if (u[3] >= 0) add(row, u[3], vid, false);
+        break;
+      }
+      case SPX_EL_VCCS:  // controlled.rs:172-201
+        vid = hp.num_vals++;
+        if (u[0] >= 0) {
+          if (u[2] >= 0) add(u[0], u[2], vid, true);
+          if (u[3] >= 0) add(u[0], u[3], vid, false);
+        }
+        if (u[1] >= 0) {
+          if (u[2] >= 0) add(u[1], u[2], vid, false);
+          if (u[3] >= 0) add(u[1], u[3], vid, true);
+        }
+        break;
+      case SPX_EL_CCCS:  // controlled.rs:267-285
+        vid = hp.num_vals++;
+        if (u[0] >= 0) add(u[0], cb, vid, false);
+        if (u[1] >= 0) add(u[1], cb, vid, true);
+        break;
+      case SPX_EL_CCVS: {  // controlled.rs:351-375
+        vid = hp.num_vals++;
+        const int row = nn + branch++;
+        if (u[0] >= 0) add(u[0], row, kValOne, false);
+        if (u[1] >= 0) add(u[1], row, kValOne, true);
+        if (u[0] >= 0) add(row, u[0], kValOne, false);
+        if (u[1] >= 0) add(row, u[1], kValOne, true);
+        add(row, cb, vid, true);
+        break;
+      }
       case SPX_EL_ISOURCE:
         vid = hp.num_vals++;
         cur(u[0], u[1], vid);
@@ -233,6 +290,16 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
     int rec[kElemInts] = {el.kind, u[0], u[1], u[2], el.param[0], el.param[1], el.param[2],
                           el.param[3], el.param[4], el.param[5], el.param[6], vid};
     hp.elems.insert(hp.elems.end(), rec, rec + kElemInts);
+  }
+  // weak: a node diagonal with nothing but capacitor companions (zero at the
+  // operating point) and, below, gmin
+  hp.weak_diag.assign(n, 0);
+  for (int i = 0; i < nn; ++i) {
+    auto it = pos.find({i, i});
+    bool weak = true;
+    if (it != pos.end())
+      for (int t : it->second) weak = weak && dc_zero_vid.count(t >> 1);
+    hp.weak_diag[i] = weak;
   }
   for (int i = 0; i < nn; ++i) add(i, i, kValGmin, false);  // GminStamper, newton.rs:393-398
   std::vector<int> rownnz(n, 0);
@@ -307,7 +374,7 @@ int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g, DevSpars
   }
   if (hp.n > kDenseMaxN || sp_always(hp)) {
     if (!ps.has_sparse) {
-      const char* why = build_sparse_plan(hp.n, hp.nn, hp.pos_rc, ps.plan);
+      const char* why = build_sparse_plan(hp.n, hp.nn, hp.pos_rc, ps.plan, &hp.weak_diag);
       if (*why) return fail(SPX_ERR_INVALID_DIMENSION, why);
       void* ds = nullptr;
       const auto& pb = ps.plan.blob;
@@ -1294,8 +1361,7 @@ int32_t spx_circuit_size(const spx_circuit* c) {
   if (!c || !c->elements) return c ? c->num_nodes : -1;
   int32_t nv = 0;
   for (int e = 0; e < c->num_elements; ++e)
-    nv += c->elements[e].kind == SPX_EL_VSOURCE || c->elements[e].kind == SPX_EL_VPULSE ||
-          c->elements[e].kind == SPX_EL_VSIN || c->elements[e].kind == SPX_EL_INDUCTOR;
+    nv += has_branch(c->elements[e].kind);
   return c->num_nodes + nv;
 }
 

@@ -122,8 +122,11 @@ struct HostSparsePlan {
 
 // n unknowns, the first num_nodes of them node voltages; pattern entries
 // (row | col << 16) sorted by (row, col).
+// weak_diag (optional, num_nodes flags): nodes whose diagonal carries gmin
+// alone, ordered like branch unknowns.
 const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_rc,
-                              HostSparsePlan& plan);
+                              HostSparsePlan& plan,
+                              const std::vector<char>* weak_diag = nullptr);
 
 // Pointers of a plan whose blob sits at `base` on the device.
 DevSparse device_sparse(const HostSparsePlan& plan, const int* base);

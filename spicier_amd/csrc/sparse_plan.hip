@@ -42,10 +42,41 @@ void eliminate_in(const Adj& adj0, const std::vector<int>& order, std::vector<in
   }
 }
 
-// Minimum degree over the plain unknowns, ties to the lowest index, each
-// branch unknown right after its first neighbour.
+// Branch placement after unknown v is eliminated: one unplaced branch
+// unknown coupled to v both ways follows it, the one with the fewest other
+// unplaced two-way neighbours (providers_of) left, and then provides in turn
+// for the next, along a chain.  Each unknown provides for one branch unknown
+// at most: a second one on the same node would find its diagonal cancelled
+// (a node pinned by one voltage-type row leaves another attached row with a
+// zero Schur complement).  `place` eliminates and marks an unknown.
+template <class Place>
+void place_branches(int v, const std::vector<std::vector<int>>& branches_of,
+                    const std::vector<std::vector<int>>& providers_of,
+                    const std::vector<char>& placed, Place&& place) {
+  for (int cur = v;;) {
+    int best = -1, bfree = 0;
+    for (int b : branches_of[cur]) {
+      if (placed[b]) continue;
+      int nfree = 0;
+      for (int q : providers_of[b]) nfree += !placed[q] && q != cur;
+      if (best < 0 || nfree < bfree) {
+        best = b;
+        bfree = nfree;
+      }
+    }
+    if (best < 0) return;
+    place(best);
+    cur = best;
+  }
+}
+
+// Minimum degree over the plain unknowns, ties to the lowest index, the
+// branch unknowns placed by place_branches (branches_of lists the branch
+// unknowns each unknown is coupled to both ways, providers_of the two-way
+// neighbours of each branch unknown).
 std::vector<int> md_order(const Adj& adj0, const std::vector<char>& branch,
-                          const std::vector<std::vector<int>>& branches_of) {
+                          const std::vector<std::vector<int>>& branches_of,
+                          const std::vector<std::vector<int>>& providers_of) {
   const int n = (int)adj0.size();
   Adj adj = adj0;
   std::vector<char> done(n, 0);
@@ -72,8 +103,7 @@ std::vector<int> md_order(const Adj& adj0, const std::vector<char>& branch,
         bdeg = adj[v].size();
       }
     elim(best);
-    for (int b : branches_of[best])
-      if (!done[b]) elim(b);
+    place_branches(best, branches_of, providers_of, done, elim);
   }
   for (int v = 0; v < n; ++v)
     if (!done[v]) elim(v);
@@ -81,12 +111,13 @@ std::vector<int> md_order(const Adj& adj0, const std::vector<char>& branch,
 }
 
 // Nested dissection over the plain unknowns (recursive bisection at the
-// middle breadth-first level from a pseudo-peripheral vertex), then each
-// branch unknown right after its first neighbour.  A path or ladder becomes
-// a cyclic-reduction order: an elimination tree of depth ~2 log2 n instead
-// of the chain of length n that minimum degree yields.
+// middle breadth-first level from a pseudo-peripheral vertex), the branch
+// unknowns placed by place_branches.  A path or ladder becomes a
+// cyclic-reduction order: an elimination tree of depth ~2 log2 n instead of
+// the chain of length n that minimum degree yields.
 std::vector<int> nd_order(const Adj& adj0, const std::vector<char>& branch,
-                          const std::vector<std::vector<int>>& branches_of) {
+                          const std::vector<std::vector<int>>& branches_of,
+                          const std::vector<std::vector<int>>& providers_of) {
   const int n = (int)adj0.size();
   std::vector<int> plain_order;
   std::vector<int> mark(n, -1), level(n, -1);
@@ -155,14 +186,13 @@ std::vector<int> nd_order(const Adj& adj0, const std::vector<char>& branch,
   rec(plain);
   std::vector<char> placed(n, 0);
   std::vector<int> order;
+  auto put = [&](int u) {
+    order.push_back(u);
+    placed[u] = 1;
+  };
   for (int v : plain_order) {
-    order.push_back(v);
-    placed[v] = 1;
-    for (int b : branches_of[v])
-      if (!placed[b]) {
-        order.push_back(b);
-        placed[b] = 1;
-      }
+    put(v);
+    place_branches(v, branches_of, providers_of, placed, put);
   }
   for (int v = 0; v < n; ++v)
     if (!placed[v]) order.push_back(v);
@@ -191,7 +221,7 @@ double plan_cost(const std::vector<std::vector<int>>& S) {
 }  // namespace
 
 const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_rc,
-                              HostSparsePlan& plan) {
+                              HostSparsePlan& plan, const std::vector<char>* weak_diag) {
   // symmetrised adjacency without the diagonal
   Adj adj(n);
   for (int rc : pos_rc) {
@@ -201,26 +231,41 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
       adj[c].insert(r);
     }
   }
-  // "Branch" unknowns: index >= num_nodes, or no diagonal entry in the
-  // pattern (an MNA voltage-source row seen through a plain matrix).  Each
-  // follows the first of its neighbours to be eliminated.
+  // "Branch" unknowns: index >= num_nodes, no diagonal entry in the pattern
+  // (an MNA voltage-source row seen through a plain matrix), or a node whose
+  // diagonal holds gmin alone (weak_diag: a node between voltage sources).
+  // Their diagonals are zero until an elimination fills them, so each one
+  // follows an eliminated unknown it is coupled to both ways: that
+  // elimination puts -a_bu a_ub / pivot_u on its diagonal.  A one-way
+  // coupling (a VCVS row's control node, a CCVS row's controlling branch)
+  // fills nothing and does not count.  A placed branch unknown provides for
+  // the next (node, branch, node, ... along voltage sources in series).
   std::vector<char> has_diag(n, 0), branch(n, 0);
   for (int rc : pos_rc)
     if ((rc & 0xffff) == (rc >> 16)) has_diag[rc & 0xffff] = 1;
-  for (int v = 0; v < n; ++v) branch[v] = v >= num_nodes || !has_diag[v];
-  std::vector<std::vector<int>> branches_of(n);
+  for (int v = 0; v < n; ++v)
+    branch[v] = v >= num_nodes || !has_diag[v] || (weak_diag && (*weak_diag)[v]);
+  std::set<int64_t> entries;
+  for (int rc : pos_rc) entries.insert(((int64_t)(rc & 0xffff) << 32) | (rc >> 16));
+  auto two_way = [&](int a, int b) {
+    return entries.count(((int64_t)a << 32) | b) && entries.count(((int64_t)b << 32) | a);
+  };
+  std::vector<std::vector<int>> branches_of(n), providers_of(n);
   for (int b = 0; b < n; ++b)
     if (branch[b])
       for (int v : adj[b])
-        if (!branch[v]) branches_of[v].push_back(b);
+        if (two_way(v, b)) {
+          branches_of[v].push_back(b);
+          providers_of[b].push_back(v);
+        }
   // two orderings, the cheaper kept
   std::vector<int> perm, iperm;
   std::vector<std::vector<int>> S;
-  eliminate_in(adj, md_order(adj, branch, branches_of), perm, iperm, S);
+  eliminate_in(adj, md_order(adj, branch, branches_of, providers_of), perm, iperm, S);
   {
     std::vector<int> perm2, iperm2;
     std::vector<std::vector<int>> S2;
-    eliminate_in(adj, nd_order(adj, branch, branches_of), perm2, iperm2, S2);
+    eliminate_in(adj, nd_order(adj, branch, branches_of, providers_of), perm2, iperm2, S2);
     if (plan_cost(S2) < plan_cost(S)) {
       perm.swap(perm2);
       iperm.swap(iperm2);

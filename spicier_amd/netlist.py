@@ -9,6 +9,8 @@ Follows spicier-parser's conventions (crates/spicier-parser/src):
     line (lexer.rs:56-160);
   * values carry SPICE suffixes T G MEG K M MIL U N P F, case-insensitive
     (spicier-core/src/units.rs:15-50);
+  * E, G (4 nodes and a gain), F, H (2 nodes, the name of an earlier V, L, E or H
+    element and a gain) — parser/elements.rs:716-830;
   * R, C, L (parse_inductor, parser/elements.rs:188-202), V (DC value, PULSE(v1 v2 td tr tf pw per) or SIN(vo va freq td
     theta phase), missing trailing arguments 0 — parser/waveforms.rs:12-72;
     PWL is an error), I (current flows from n+
@@ -26,8 +28,8 @@ braces in an element or model value:
   unif(nom, rvar)           -> U(nom*(1 - rvar), nom*(1 + rvar))
 A .PARAM is one sampled parameter shared by every use (correlated); a
 function written in a value is a parameter of its own.  Arithmetic on sampled
-parameters, and elements outside the engine's models (K, E, F, G, H, Q, J,
-X, B), are errors, not approximations.
+parameters, and elements outside the engine's models (K, Q, J, X, B), are
+errors, not approximations.
 """
 from __future__ import annotations
 
@@ -36,7 +38,7 @@ import operator
 import re
 from dataclasses import dataclass, field
 
-from .circuit import Circuit, Param
+from .circuit import BRANCH_KINDS, Circuit, Param
 from .engine import InvalidArgument
 
 # units.rs:37-49
@@ -276,11 +278,13 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
         else:
             pending.append((no, s, toks))
     nodes = _Nodes()
+    by_name = {}  # element name -> index (find_vsource_branch_index, netlist.rs:412-420)
     for no, s, toks in pending:
         kind = toks[0][0].upper()
         nm = toks[0]
         args = toks[1:]
-        nn = {"R": 2, "C": 2, "L": 2, "V": 2, "I": 2, "D": 2, "M": 4}.get(kind, 0)
+        nn = {"R": 2, "C": 2, "L": 2, "V": 2, "I": 2, "D": 2, "M": 4, "E": 4, "G": 4, "F": 2,
+              "H": 2}.get(kind, 0)
         args = [nodes.id(a) if k < nn else a for k, a in enumerate(args)]
 
         def need(k):
@@ -319,6 +323,18 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
             else:
                 vtok = rest[1] if rest[0].upper() == "DC" and len(rest) > 1 else rest[0]
                 c.vsource(args[0], args[1], scope.value(vtok, no, name=nm))
+        elif kind in "EG":  # parse_vcvs / parse_vccs (elements.rs:716-760): 4 nodes, gain
+            need(5)
+            v = scope.value(args[4], no, name=nm)
+            (c.vcvs if kind == "E" else c.vccs)(args[0], args[1], args[2], args[3], v)
+        elif kind in "FH":  # parse_cccs / parse_ccvs (elements.rs:762-830): 2 nodes, source, gain
+            need(4)
+            ctrl = by_name.get(args[2].upper())
+            if ctrl is None or c.elements[ctrl][0] not in BRANCH_KINDS:
+                raise InvalidArgument(f"line {no}: {nm}: controlling source {args[2]} not found "
+                                      "(it must be an earlier V, L, E or H element)")
+            v = scope.value(args[3], no, name=nm)
+            (c.cccs if kind == "F" else c.ccvs)(args[0], args[1], ctrl, v)
         elif kind == "I":
             need(3)
             rest = args[2:]
@@ -352,6 +368,7 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
         else:
             raise InvalidArgument(f"line {no}: element {nm} ({kind}) is not supported by the "
                                   "sweep engine")
+        by_name[nm.upper()] = len(c.elements) - 1
     # unknown k - 1 is node k; a node number without elements would be an
     # empty (singular) row of every instance's system
     names = {k: v for k, v in nodes.map.items() if v != 0}
