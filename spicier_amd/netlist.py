@@ -28,8 +28,10 @@ braces in an element or model value:
   unif(nom, rvar)           -> U(nom*(1 - rvar), nom*(1 + rvar))
 A .PARAM is one sampled parameter shared by every use (correlated); a
 function written in a value is a parameter of its own.  Arithmetic on sampled
-parameters, and elements outside the engine's models (K, Q, J, X, B), are
-errors, not approximations.
+parameters, and elements outside the engine's models (K, Q, J, B), are
+errors, not approximations.  .SUBCKT definitions with PARAMS: defaults and X
+instances (nested, with PARAMS: overrides) are flattened first
+(parser/subcircuit.rs).
 """
 from __future__ import annotations
 
@@ -106,6 +108,150 @@ _TOK = re.compile(r"\{[^}]*\}|[A-Za-z_][\w.]*\s*\([^()]*\)|[()=,]|[^\s()=,]+")
 
 def _tokens(s: str):
     return [t for t in _TOK.findall(s) if t != ","]
+
+
+# ------------------------------------------------------------ subcircuits
+# node tokens after the element name, per element letter
+_NODE_COUNT = {"R": 2, "C": 2, "L": 2, "V": 2, "I": 2, "D": 2, "M": 4, "E": 4, "G": 4, "F": 2,
+               "H": 2}
+_IDENT = re.compile(r"(?<![\w.])[A-Za-z_]\w*")
+
+
+def _split_params(toks, line):
+    """(tokens before PARAMS:, {NAME: value text}) of a .SUBCKT or X line."""
+    for i, t in enumerate(toks):
+        if t.upper().startswith("PARAMS:"):
+            rest = ([t[7:]] if len(t) > 7 else []) + toks[i + 1:]
+            kv, j = {}, 0
+            while j < len(rest):
+                if j + 2 < len(rest) and rest[j + 1] == "=":
+                    kv[rest[j].upper()] = rest[j + 2]
+                    j += 3
+                else:
+                    raise InvalidArgument(f"line {line}: expected name=value after PARAMS:")
+            return toks[:i], kv
+    return toks, {}
+
+
+def _subst(text, ctx):
+    """Replaces subcircuit parameter names in a value by their values."""
+    def rep(m):
+        v = ctx.get(m.group(0).upper())
+        if v is None:
+            return m.group(0)
+        return v if re.fullmatch(r"[\w.+-]+", v) else f"({v})"
+    return _IDENT.sub(rep, text)
+
+
+def _flatten(lines):
+    """Expands X instances of .SUBCKT definitions (parser/subcircuit.rs): an
+    element R1 of instance X1 becomes RX1_1, an internal node n becomes
+    X1_n, ports take the instance's connections and 0 / gnd stay ground;
+    parameters resolve global .PARAM < the definition's PARAMS: defaults <
+    the instance's PARAMS:, innermost instance last.  Nested instances
+    expand recursively.  A current-controlled source names its control
+    inside the definition first (renamed with it), else outside."""
+    defs, top, cur = {}, [], None
+    for no, s in lines:
+        toks = _tokens(s)
+        cmd = toks[0].upper()
+        if cmd == ".SUBCKT":
+            if cur is not None:
+                raise InvalidArgument(f"line {no}: .SUBCKT inside .SUBCKT {cur['name']}")
+            head, params = _split_params(toks[1:], no)
+            if not head:
+                raise InvalidArgument(f"line {no}: .SUBCKT needs a name")
+            cur = {"name": head[0].upper(), "ports": [p.upper() for p in head[1:]],
+                   "params": params, "body": [], "line": no}
+        elif cmd == ".ENDS":
+            if cur is None:
+                raise InvalidArgument(f"line {no}: .ENDS without .SUBCKT")
+            defs[cur["name"]] = cur
+            cur = None
+        elif cur is not None:
+            cur["body"].append((no, s))
+        else:
+            top.append((no, s))
+    if cur is not None:
+        raise InvalidArgument(f"line {cur['line']}: .SUBCKT {cur['name']} has no .ENDS")
+
+    def expand(no, toks, ctx, depth):
+        inst = toks[0]
+        head, params = _split_params(toks[1:], no)
+        if len(head) < 1:
+            raise InvalidArgument(f"line {no}: {inst} needs a subcircuit name")
+        conns, sub = head[:-1], head[-1].upper()
+        d = defs.get(sub)
+        if d is None:
+            raise InvalidArgument(f"line {no}: unknown subcircuit {head[-1]}")
+        if len(conns) != len(d["ports"]):
+            raise InvalidArgument(f"line {no}: subcircuit {head[-1]} expects {len(d['ports'])} "
+                                  f"ports but {len(conns)} are given")
+        if depth > 32:
+            raise InvalidArgument(f"line {no}: subcircuits nested more than 32 deep")
+        local = dict(ctx)
+        for k, v in d["params"].items():
+            local[k] = _subst(v.strip("{}"), ctx)
+        for k, v in params.items():
+            local[k] = _subst(v.strip("{}"), ctx)
+        ports = dict(zip(d["ports"], conns))
+        names = {t[0].upper() for _, b in d["body"] for t in [_tokens(b)] if t}
+
+        def node(t):
+            if t.upper() in ports:
+                return ports[t.upper()]
+            if t in ("0", "gnd", "GND"):
+                return t
+            return f"{inst}_{t}"
+
+        def val(t):
+            if t.startswith("{") and t.endswith("}"):
+                return "{" + _subst(t[1:-1], local) + "}"
+            if t.upper() in local:
+                return "{" + local[t.upper()] + "}"
+            m = re.match(r"^([A-Za-z_][\w.]*\s*\()(.*)\)$", t)
+            if m:
+                return m.group(1) + _subst(m.group(2), local) + ")"
+            return t
+
+        out = []
+        for bno, b in d["body"]:
+            bt = _tokens(b)
+            if bt[0].startswith("."):
+                out.append((bno, b))
+                continue
+            kind = bt[0][0].upper()
+            name = f"{bt[0][0]}{inst}_{bt[0][1:]}"
+            if kind == "X":
+                bh, bp = _split_params(bt[1:], bno)
+                sub_t = [name] + [node(t) for t in bh[:-1]] + [bh[-1]]
+                if bp:
+                    sub_t += ["PARAMS:"] + [x for k, v in bp.items()
+                                            for x in (k, "=", "{" + _subst(v.strip("{}"), local) + "}")]
+                out.extend(expand(no, sub_t, local, depth + 1))
+                continue
+            nn = _NODE_COUNT.get(kind, 0)
+            toks2 = [name]
+            rest = bt[1:]
+            for i, t in enumerate(rest):
+                if i < nn:
+                    toks2.append(node(t))
+                elif kind in "FH" and i == nn and t.upper() in names:
+                    toks2.append(f"{t[0]}{inst}_{t[1:]}")
+                elif i + 1 < len(rest) and rest[i + 1] == "=":
+                    toks2.append(t)  # the key of an instance override such as W=...
+                else:
+                    toks2.append(val(t))
+            out.append((bno, " ".join(toks2)))
+        return out
+
+    flat = []
+    for no, s in top:
+        if s[0].upper() == "X":
+            flat.extend(expand(no, _tokens(s), {}, 0))
+        else:
+            flat.append((no, s))
+    return flat
 
 
 # ------------------------------------------------------------ expressions
@@ -232,6 +378,7 @@ def _kv(tokens, line, scope, prefix=""):
 def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
     """Compile a SPICE netlist into a Circuit (see the module docstring)."""
     title, lines = _logical_lines(text)
+    lines = _flatten(lines)
     c = Circuit(name or (title or "netlist"))
     scope = _Scope(c)
     models: dict = {}

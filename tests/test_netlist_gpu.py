@@ -84,3 +84,41 @@ C1 out 0 {Cap}
     out = p.circuit.node_names["out"] - 1
     for i in range(3):
         np.testing.assert_allclose(got.waveforms[i, :, out], want, rtol=1e-12, atol=1e-15)
+
+
+def test_subcircuit_inverter_chain_monte_carlo(oracle):
+    # a self-biased CMOS inverter .SUBCKT instanced eight times, nested in a
+    # pair, with sampled threshold voltages shared through .PARAM and
+    # per-instance widths
+    text = """subckt chain
+.param vtn=agauss(0.5, 0.03, 1) vtp=agauss(-0.5, 0.03, 1)
+.model nch NMOS(VTO={vtn} KP=120u LAMBDA=0.02)
+.model pch PMOS(VTO={vtp} KP=40u LAMBDA=0.02)
+.SUBCKT INV in out vdd PARAMS: WN=2u WP=6u
+M1 out in 0 0 nch W={WN} L=1u
+M2 out in vdd vdd pch W={WP} L=1u
+RF out in 100k
+.ENDS INV
+.SUBCKT PAIR a b vdd PARAMS: K=1
+X1 a m vdd INV PARAMS: WN={2u*K} WP={6u*K}
+X2 m b vdd INV
+.ENDS PAIR
+VDD vdd 0 1.8
+VIN vin 0 0.9
+RIN vin in 100k
+X1 in n1 vdd PAIR PARAMS: K=1
+X2 n1 n2 vdd PAIR PARAMS: K=2
+X3 n2 n3 vdd PAIR PARAMS: K=1.5
+X4 n3 out vdd PAIR
+.end
+"""
+    c = parse_netlist(text).circuit
+    assert sum(e[0] in (A.SPX_EL_NMOS, A.SPX_EL_PMOS) for e in c.elements) == 16
+    cfg, acfg = E.nr_config(limiter=E.Limiter.Pnjlim), A.default_nr_config(limiter=A.SPX_LIMIT_PNJLIM)
+    got = E.sweep_dcop(c, 2048, seed=5, config=cfg)
+    want = oracle.sweep_dcop(c, got.params, acfg, nthreads=16)
+    np.testing.assert_array_equal(got.status, want["status"])
+    same = got.iterations == want["iterations"]
+    assert same.mean() >= 0.95
+    assert _close(got.solutions[same], want["x"][same])
+    assert np.all(got.status == A.SPX_CONVERGED)
