@@ -197,8 +197,11 @@ typedef enum spx_element_kind {
     SPX_EL_CCCS = 12,     /* F: node[0..1] = out+/out-; `ctrl` = the controlling element; param[0]
                              = gain; KCL +gain i(ctrl) at out+, -gain i(ctrl) at out-
                              (controlled.rs:267-285) */
-    SPX_EL_CCVS = 13      /* H: nodes and `ctrl` as CCCS; param[0] = gain; a branch unknown, row
+    SPX_EL_CCVS = 13,     /* H: nodes and `ctrl` as CCCS; param[0] = gain; a branch unknown, row
                              v(out+) - v(out-) - gain i(ctrl) = 0 (controlled.rs:351-375) */
+    SPX_EL_VPWL = 14      /* voltage source PWL(t1 v1 ... tk vk): `ctrl` = k points (1..4096),
+                             param[0] = the slot of t1, the 2k values in consecutive slots;
+                             eval_pwl (waveforms.rs:204-232); DC value v1 */
 } spx_element_kind;
 
 #define SPX_MAX_ELEMENT_PARAMS 7
@@ -210,7 +213,7 @@ typedef struct spx_element {
     int32_t ctrl;                           /* CCCS / CCVS: index of the controlling element, an
                                                earlier voltage source, inductor, VCVS or CCVS
                                                (find_vsource_branch_index, netlist.rs:412-420);
-                                               -1 otherwise */
+                                               VPWL: its number of points; -1 otherwise */
 } spx_element;
 
 typedef struct spx_circuit {

@@ -322,7 +322,8 @@ namespace {
 // (sources.rs, passive.rs:341-350, controlled.rs:111-113, 400-402).
 bool has_branch(int kind) {
   return kind == SPX_EL_VSOURCE || kind == SPX_EL_VPULSE || kind == SPX_EL_VSIN ||
-         kind == SPX_EL_INDUCTOR || kind == SPX_EL_VCVS || kind == SPX_EL_CCVS;
+         kind == SPX_EL_VPWL || kind == SPX_EL_INDUCTOR || kind == SPX_EL_VCVS ||
+         kind == SPX_EL_CCVS;
 }
 // The branch unknown of element e (which has one): its current_index, the
 // count of branch devices before it (parser/elements.rs next_current_index).
@@ -427,6 +428,21 @@ double eval_sin(double vo, double va, double freq, double td, double theta, doub
   return vo + va * damping * std::sin(2.0 * M_PI * freq * t_rel + phase_rad);
 }
 
+// eval_pwl (waveforms.rs:204-232) over np (t, v) pairs in params[first..]
+double eval_pwl(const double* params, int first, int np, double t) {
+  const double* p = params + first;
+  if (t <= p[0]) return p[1];
+  if (t >= p[2 * (np - 1)]) return p[2 * (np - 1) + 1];
+  for (int i = 0; i < np - 1; ++i) {
+    const double t0 = p[2 * i], v0 = p[2 * i + 1], t1 = p[2 * i + 2], v1 = p[2 * i + 3];
+    if (t >= t0 && t <= t1) {
+      const double frac = (t - t0) / (t1 - t0);
+      return v0 + frac * (v1 - v0);
+    }
+  }
+  return p[2 * (np - 1) + 1];
+}
+
 void stamp_impl(const spx_circuit* c, const double* params, const double* x, double gmin,
                 double sf, int32_t limiter, double* J, double* rhs, double* vstate,
                 int* limited, const OTran* tc = nullptr) {
@@ -469,6 +485,12 @@ void stamp_impl(const spx_circuit* c, const double* params, const double* x, dou
           v = eval_pulse(P(params, el.param[0]), P(params, el.param[1]), P(params, el.param[2]),
                          P(params, el.param[3]), P(params, el.param[4]), P(params, el.param[5]),
                          P(params, el.param[6]), tc->t);
+        m.vsource(a, b, k++, v * sf);
+        break;
+      }
+      case SPX_EL_VPWL: {  // DC value v1 (waveforms.rs:153)
+        double v = params[el.param[0] + 1];
+        if (tc && tc->mode == 1) v = eval_pwl(params, el.param[0], el.ctrl, tc->t);
         m.vsource(a, b, k++, v * sf);
         break;
       }

@@ -38,6 +38,7 @@ SPX_EL_DIODE, SPX_EL_NMOS, SPX_EL_PMOS, SPX_EL_VPULSE = 4, 5, 6, 7
 SPX_EL_VSIN = 8
 SPX_EL_INDUCTOR = 9
 SPX_EL_VCVS, SPX_EL_VCCS, SPX_EL_CCCS, SPX_EL_CCVS = 10, 11, 12, 13
+SPX_EL_VPWL = 14
 SPX_MAX_ELEMENT_PARAMS = 7
 
 # parameter kinds

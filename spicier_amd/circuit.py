@@ -23,8 +23,8 @@ from . import _abi as A
 
 
 # element kinds that carry a branch current unknown, numbered in element order
-BRANCH_KINDS = (A.SPX_EL_VSOURCE, A.SPX_EL_VPULSE, A.SPX_EL_VSIN, A.SPX_EL_INDUCTOR,
-                A.SPX_EL_VCVS, A.SPX_EL_CCVS)
+BRANCH_KINDS = (A.SPX_EL_VSOURCE, A.SPX_EL_VPULSE, A.SPX_EL_VSIN, A.SPX_EL_VPWL,
+                A.SPX_EL_INDUCTOR, A.SPX_EL_VCVS, A.SPX_EL_CCVS)
 
 
 @dataclass(frozen=True)
@@ -161,6 +161,21 @@ class Circuit:
         """SIN(vo va freq td theta phase) source (waveforms.rs:34-49, 185-201);
         phase in degrees; DC value vo."""
         return self._add(A.SPX_EL_VSIN, (pos, neg), (vo, va, freq, td, theta, phase))
+
+    def vpwl(self, pos, neg, points):
+        """PWL((t1, v1), (t2, v2), ...) source (waveforms.rs:116-118, 204-232):
+        the 2k constant values take consecutive parameter slots; DC value v1."""
+        pts = [(float(t), float(v)) for t, v in points]
+        if not pts:
+            raise ValueError("PWL needs at least one point")
+        first = len(self.specs)
+        for j, (t, v) in enumerate(pts):
+            self.const(t, name=f"PWL{first}_T{j}")
+            self.const(v, name=f"PWL{first}_V{j}")
+        i = self._add(A.SPX_EL_VPWL, (pos, neg), ())
+        self.elements[i][2][0] = first
+        self.ctrl[i] = len(pts)
+        return i
 
     def isource(self, frm, to, i):
         """Current i flows from `frm` through the source into `to`."""

@@ -30,6 +30,26 @@ __device__ __forceinline__ double eval_sin(double vo, double va, double freq, do
   return vo + va * damping * sin(2.0 * M_PI * freq * t_rel + phase_rad);
 }
 
+// eval_pwl (waveforms.rs:204-232) over np (t, v) pairs in consecutive
+// parameter slots from `first`: the first value up to t1, the last from tk
+// on, else linear interpolation in the first interval [t_i, t_i+1] that
+// holds t.
+template <int ST>
+__device__ __forceinline__ double eval_pwl(const double* pv, int first, int np, double t) {
+  auto T = [&](int i) { return pv[(first + 2 * i) * ST]; };
+  auto V = [&](int i) { return pv[(first + 2 * i + 1) * ST]; };
+  if (t <= T(0)) return V(0);
+  if (t >= T(np - 1)) return V(np - 1);
+  for (int i = 0; i < np - 1; ++i) {
+    const double t0 = T(i), t1 = T(i + 1);
+    if (t >= t0 && t <= t1) {
+      const double frac = (t - t0) / (t1 - t0);
+      return V(i) + frac * (V(i + 1) - V(i));
+    }
+  }
+  return V(np - 1);
+}
+
 // Companion values of element e at node voltages xs (unknown u -> xs[u],
 // ground -> 0): conductances, currents and couplings into vals[vid..].
 // sf scales independent sources (source stepping); pnjlim junction state
@@ -95,6 +115,11 @@ __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const d
         vals[(vid + 1) * ST] = 0.0;
         vals[(vid + 2) * ST] = 0.0;
       }
+      break;
+    case SPX_EL_VPWL:  // el[3]: number of points, el[4]: the slot of t1
+      vals[vid * ST] = ((tc && tc->mode == 1) ? eval_pwl<ST>(pv, el[4], el[3], tc->t)
+                                              : pv[(el[4] + 1) * ST]) *
+                       sf;
       break;
     case SPX_EL_VSIN:
       vals[vid * ST] =

@@ -91,10 +91,13 @@ constexpr int kSparseMaxN = 4096;
 int ir_phase(const DevNrCfg& dc) { return dc.precision == SPX_PREC_F32_IR ? 4 : 0; }
 
 // ------------------------------------------------------------- circuit compile
+constexpr int kMaxPwlPoints = 4096;
+
 // Element kinds with a branch current unknown (numbered in element order).
 inline bool has_branch(int kind) {
   return kind == SPX_EL_VSOURCE || kind == SPX_EL_VPULSE || kind == SPX_EL_VSIN ||
-         kind == SPX_EL_INDUCTOR || kind == SPX_EL_VCVS || kind == SPX_EL_CCVS;
+         kind == SPX_EL_VPWL || kind == SPX_EL_INDUCTOR || kind == SPX_EL_VCVS ||
+         kind == SPX_EL_CCVS;
 }
 
 struct HostProgram {
@@ -168,6 +171,14 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
       case SPX_EL_NMOS: case SPX_EL_PMOS: need = 5; break;
       case SPX_EL_VPULSE: need = 7; break;
       case SPX_EL_VSIN: need = 6; break;
+      case SPX_EL_VPWL:
+        if (el.ctrl < 1 || el.ctrl > kMaxPwlPoints || el.param[0] < 0 ||
+            (int64_t)el.param[0] + 2 * (int64_t)el.ctrl > P)
+          return fail(SPX_ERR_INVALID_ARGUMENT,
+                      "element " + std::to_string(e) + ": PWL needs 1 to " +
+                          std::to_string(kMaxPwlPoints) + " points in parameter slots");
+        need = 1;
+        break;
       default:
         return fail(SPX_ERR_INVALID_ARGUMENT, "element " + std::to_string(e) + " has unknown kind");
     }
@@ -192,7 +203,9 @@ int compile_circuit(const spx_circuit* c, HostProgram& hp, int max_n, bool trans
         break;
       case SPX_EL_VSOURCE:
       case SPX_EL_VPULSE:
-      case SPX_EL_VSIN: {  // mna.rs:124-145
+      case SPX_EL_VSIN:
+      case SPX_EL_VPWL: {  // mna.rs:124-145
+        if (el.kind == SPX_EL_VPWL) u[2] = el.ctrl;  // the record carries the point count
         vid = hp.num_vals++;
         const int row = nn + branch++;
         if (u[0] >= 0) { add(u[0], row, kValOne, false); add(row, u[0], kValOne, false); }

@@ -12,8 +12,8 @@ Follows spicier-parser's conventions (crates/spicier-parser/src):
   * E, G (4 nodes and a gain), F, H (2 nodes, the name of an earlier V, L, E or H
     element and a gain) — parser/elements.rs:716-830;
   * R, C, L (parse_inductor, parser/elements.rs:188-202), V (DC value, PULSE(v1 v2 td tr tf pw per) or SIN(vo va freq td
-    theta phase), missing trailing arguments 0 — parser/waveforms.rs:12-72;
-    PWL is an error), I (current flows from n+
+    theta phase), missing trailing arguments 0, or PWL(t1 v1 t2 v2 ...) with
+    constant points — parser/waveforms.rs:12-106), I (current flows from n+
     through the source to n-), D (optional model), M (d g s b model, W= L=
     instance overrides) (parser/elements.rs);
   * .MODEL name D(IS= N=) / NMOS(...) / PMOS(...) level 1 with VTO KP LAMBDA
@@ -447,9 +447,18 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
             rest = args[2:]
             pulse = next((t for t in rest if t.upper().startswith("PULSE")), None)
             sinw = next((t for t in rest if t.upper().startswith("SIN")), None)
-            if any(t.upper().startswith("PWL") for t in rest):
-                raise InvalidArgument(f"line {no}: PWL sources are not supported")
-            if sinw is not None:
+            pwl = next((t for t in rest if t.upper().startswith("PWL")), None)
+            if pwl is not None:  # parse_pwl_waveform (parser/waveforms.rs:74-106)
+                if "(" not in pwl or not pwl.endswith(")"):
+                    raise InvalidArgument(f"line {no}: expected PWL(t1 v1 t2 v2 ...)")
+                inner = _tokens(pwl[pwl.index("(") + 1:-1])
+                if not inner or len(inner) % 2:
+                    raise InvalidArgument(f"line {no}: PWL needs time-value pairs")
+                vals = [scope.value(t, no, name=f"{nm}.PWL{k}") for k, t in enumerate(inner)]
+                if any(isinstance(v, Param) for v in vals):
+                    raise InvalidArgument(f"line {no}: PWL points must be constants")
+                c.vpwl(args[0], args[1], list(zip(vals[0::2], vals[1::2])))
+            elif sinw is not None:
                 if "(" not in sinw or not sinw.endswith(")"):
                     raise InvalidArgument(f"line {no}: expected SIN(vo va freq [td theta phase])")
                 inner = _tokens(sinw[sinw.index("(") + 1:-1])

@@ -81,5 +81,5 @@ R1 in 0 1k
     assert [CK.nominal_values(p.circuit)[s] for s in el[2][:6]] == [0.0, 2.0, 50.0, 0.0, 0.0, 0.0]
     with pytest.raises(InvalidArgument):
         parse_netlist("title\nV1 in 0 SIN(0 1)\nR1 in 0 1\n", name="x")
-    with pytest.raises(InvalidArgument):
-        parse_netlist("title\nV1 in 0 PWL(0 0 1m 1)\nR1 in 0 1\n", name="x")
+    with pytest.raises(InvalidArgument):  # PWL takes pairs
+        parse_netlist("title\nV1 in 0 PWL(0 0 1m)\nR1 in 0 1\n", name="x")
