@@ -11,6 +11,7 @@ Follows spicier-parser's conventions (crates/spicier-parser/src):
     (spicier-core/src/units.rs:15-50);
   * E, G (4 nodes and a gain), F, H (2 nodes, the name of an earlier V, L, E or H
     element and a gain) — parser/elements.rs:716-830;
+  * T (Z0= TD= [NL=], the reference's LC-section expansion, tline.rs);
   * R, C, L (parse_inductor, parser/elements.rs:188-202), V (DC value, PULSE(v1 v2 td tr tf pw per) or SIN(vo va freq td
     theta phase), missing trailing arguments 0, or PWL(t1 v1 t2 v2 ...) with
     constant points — parser/waveforms.rs:12-106), I (current flows from n+
@@ -431,7 +432,7 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
         nm = toks[0]
         args = toks[1:]
         nn = {"R": 2, "C": 2, "L": 2, "V": 2, "I": 2, "D": 2, "M": 4, "E": 4, "G": 4, "F": 2,
-              "H": 2}.get(kind, 0)
+              "H": 2, "T": 4}.get(kind, 0)
         args = [nodes.id(a) if k < nn else a for k, a in enumerate(args)]
 
         def need(k):
@@ -491,6 +492,27 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
                                       "(it must be an earlier V, L, E or H element)")
             v = scope.value(args[3], no, name=nm)
             (c.cccs if kind == "F" else c.ccvs)(args[0], args[1], ctrl, v)
+        elif kind == "T":  # lossless line: elements.rs:963-1068, tline.rs:30-236
+            need(4)
+            kv = _kv(args[4:], no, scope, prefix=f"{nm}.")
+            if "Z0" not in kv or "TD" not in kv:
+                raise InvalidArgument(f"line {no}: {nm} needs Z0= and TD=")
+            if any(isinstance(v, Param) for v in kv.values()):
+                raise InvalidArgument(f"line {no}: {nm}: Z0, TD and NL must be constants")
+            z0, td = kv["Z0"], kv["TD"]
+            if not (z0 > 0 and td > 0):
+                raise InvalidArgument(f"line {no}: {nm} needs Z0 > 0 and TD > 0")
+            sections = max(int(kv.get("NL", 10)), 2)  # DEFAULT_NUM_SECTIONS, MIN_SECTIONS
+            # port1+ -L- int1 -L- ... -L- port2+, C from each section's right
+            # node to ground (build_transient_state, cli/stampers.rs:575-615):
+            # the inductors take consecutive branch unknowns as the line's
+            # current variables do; at DC they are the line's 0 V sources
+            chain = ([args[0]] + [nodes.id(f"{nm}_int{k}") for k in range(1, sections)]
+                     + [args[2]])
+            for k in range(sections):
+                c.inductor(chain[k], chain[k + 1], z0 * td / sections)
+            for k in range(sections):
+                c.capacitor(chain[k + 1], 0, td / (z0 * sections))
         elif kind == "I":
             need(3)
             rest = args[2:]

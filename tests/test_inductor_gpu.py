@@ -152,3 +152,18 @@ def test_fp32_ir_with_inductors_matches_fp64_oracle(oracle):
                                method=int(M.Trapezoidal), nthreads=16)
     np.testing.assert_array_equal(got.status, want["status"])
     assert np.all(close(got.waveforms, want["waveforms"]))
+
+
+@pytest.mark.parametrize("method", [M.BackwardEuler, M.Trapezoidal, M.TrBdf2])
+def test_transmission_line_netlist_on_gpu(oracle, method):
+    # the reference's 50-section LC expansion of a matched 50 Ohm, 5 ns line
+    # (103 unknowns: the shared-symbolic sparse path), source resistance varied
+    from spicier_amd.netlist import parse_netlist
+    c = parse_netlist("t\nV1 1 0 PULSE(0 1 0 0 0 1 0)\nRS 1 in {aunif(50, 5)}\n"
+                      "T1 in 0 out 0 Z0=50 TD=5n NL=50\nRL out 0 50\n.end\n").circuit
+    probes = [c.node_names["in"] - 1, c.node_names["out"] - 1, c.num_nodes + 1]
+    h, steps = 5e-9 / 200, 400
+    got = E.transient_sweep(c, 24, h, steps, probes=probes, seed=3, method=method)
+    want = oracle.transient_ex(c, got.params, A.default_nr_config(), h, steps, probes=probes,
+                               method=int(method), nthreads=16)
+    _check_fixed(got, want)

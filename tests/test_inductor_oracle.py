@@ -222,3 +222,38 @@ def test_adaptive_rl_accuracy_and_inductor_lte(oracle):
     assert out["total_steps"][0] == (n - 1) + out["rejected_steps"][0]
     exact = V / R * (1 - np.exp(-t / TAU))
     assert np.abs(i - exact).max() < 1e-2 * V / R
+
+
+def test_reference_transmission_line_dc(oracle):
+    # parse_and_simulate.rs:597-638 and 641-680: a line is a short at DC; NL=3
+    # gives three branch currents beside V1's
+    from spicier_amd.netlist import parse_netlist
+    c = parse_netlist("T DC\nV1 1 0 DC 10\nT1 1 0 2 0 Z0=50 TD=1n NL=3\nR1 2 0 100\n.end\n").circuit
+    assert c.size - c.num_nodes == 4
+    x = oracle.sweep_dcop(c, CK.nominal_values(c).reshape(1, -1))["x"][0]
+    assert x[c.node_names["2"] - 1] == pytest.approx(10.0, rel=1e-12)
+    c = parse_netlist("T matched\nV1 1 0 DC 10\nR_source 1 in 50\nT1 in 0 out 0 Z0=50 TD=5n NL=5\n"
+                      "R_load out 0 50\n.end\n").circuit
+    x = oracle.sweep_dcop(c, CK.nominal_values(c).reshape(1, -1))["x"][0]
+    assert x[c.node_names["in"] - 1] == pytest.approx(5.0, rel=1e-12)
+    assert x[c.node_names["out"] - 1] == pytest.approx(5.0, rel=1e-12)
+
+
+def test_matched_line_delays_a_step(oracle):
+    # a 50 Ohm line of TD = 5 ns, 50 sections, matched at both ends: the far
+    # end stays near 0 until about TD, then settles at half the step
+    # (trapezoidal, h = TD/200); the LC sections give L = Z0 TD / N and
+    # C = TD / (Z0 N) per section
+    from spicier_amd.netlist import parse_netlist
+    c = parse_netlist("t\nV1 1 0 PULSE(0 1 0 0 0 1 0)\nRS 1 in 50\n"
+                      "T1 in 0 out 0 Z0=50 TD=5n NL=50\nRL out 0 50\n.end\n").circuit
+    ls = [CK.nominal_values(c)[e[2][0]] for e in c.elements if e[0] == A.SPX_EL_INDUCTOR]
+    assert len(ls) == 50 and ls[0] == pytest.approx(50 * 5e-9 / 50)
+    h, steps = 5e-9 / 200, 600
+    out = _run(oracle, c, A.SPX_TRAN_TRAP, h, steps)
+    w = out["waveforms"][0, :, c.node_names["out"] - 1]
+    t = np.arange(steps + 1) * h
+    assert np.abs(w[t < 0.8 * 5e-9]).max() < 0.02
+    assert abs(w[-1] - 0.5) < 0.02
+    k = np.argmax(w > 0.25)
+    assert abs(t[k] - 5e-9) < 0.15 * 5e-9
