@@ -29,8 +29,9 @@ braces in an element or model value:
   unif(nom, rvar)           -> U(nom*(1 - rvar), nom*(1 + rvar))
 A .PARAM is one sampled parameter shared by every use (correlated); a
 function written in a value is a parameter of its own.  Arithmetic on sampled
-parameters, and elements outside the engine's models (K, Q, J, B), are
-errors, not approximations.  .SUBCKT definitions with PARAMS: defaults and X
+parameters, and elements outside the engine's models (Q, J, B), are
+errors, not approximations.  K couplings are read and checked and, as in the
+reference's DC and transient analyses, change nothing (ParsedNetlist.couplings).  .SUBCKT definitions with PARAMS: defaults and X
 instances (nested, with PARAMS: overrides) are flattened first
 (parser/subcircuit.rs).
 """
@@ -41,6 +42,7 @@ import operator
 import re
 from dataclasses import dataclass, field
 
+from ._abi import SPX_EL_INDUCTOR as A_EL_INDUCTOR
 from .circuit import BRANCH_KINDS, Circuit, Param
 from .engine import InvalidArgument
 
@@ -72,6 +74,9 @@ class ParsedNetlist:
     title: str = ""
     tran: tuple | None = None        # (tstep, tstop) of a .TRAN line
     params: dict = field(default_factory=dict)   # .PARAM name -> value or Param
+    # K couplings (name, inductors, k): read and checked but without effect
+    # here, as in the reference's DC and transient analyses
+    couplings: list = field(default_factory=list)
 
 
 # ------------------------------------------------------------------ lines
@@ -427,6 +432,7 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
             pending.append((no, s, toks))
     nodes = _Nodes()
     by_name = {}  # element name -> index (find_vsource_branch_index, netlist.rs:412-420)
+    couplings = []
     for no, s, toks in pending:
         kind = toks[0][0].upper()
         nm = toks[0]
@@ -492,6 +498,24 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
                                       "(it must be an earlier V, L, E or H element)")
             v = scope.value(args[3], no, name=nm)
             (c.cccs if kind == "F" else c.ccvs)(args[0], args[1], ctrl, v)
+        elif kind == "K":  # parse_mutual_inductance (elements.rs:633-700)
+            # Kname L1 L2 [L3 ...] k.  The reference's MutualInductance stamps
+            # nothing at DC (mutual.rs:108-114) and its transient state builder
+            # skips it (cli/stampers.rs:541-620): the coupling reaches only AC
+            # analysis, so it changes no DC or transient result there either
+            need(3)
+            inds = args[:-1]
+            if len(inds) < 2:
+                raise InvalidArgument(f"line {no}: {nm} needs at least 2 inductors")
+            for ln in inds:
+                i = by_name.get(ln.upper())
+                if i is None or c.elements[i][0] != A_EL_INDUCTOR:
+                    raise InvalidArgument(f"line {no}: {nm}: {ln} is not an earlier inductor")
+            kval = scope.value(args[-1], no, name=nm)
+            if isinstance(kval, Param) or not (0.0 < kval <= 1.0):
+                raise InvalidArgument(f"line {no}: {nm}: coupling must be a constant in (0, 1]")
+            couplings.append((nm, [x.upper() for x in inds], kval))
+            continue
         elif kind == "T":  # lossless line: elements.rs:963-1068, tline.rs:30-236
             need(4)
             kv = _kv(args[4:], no, scope, prefix=f"{nm}.")
@@ -556,4 +580,4 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
         raise InvalidArgument(f"node numbers {missing} have no element (every system would be "
                               "singular)")
     c.node_names = {"0": 0, **names}
-    return ParsedNetlist(c, title, tran, dict(scope.params))
+    return ParsedNetlist(c, title, tran, dict(scope.params), couplings)

@@ -106,7 +106,8 @@ M1 1 2 0 0 nch
 
 
 @pytest.mark.parametrize("text", [
-    "t\nL1 1 0 1n\nL2 2 0 1n\nK1 L1 L2 0.9\n",          # mutual inductance
+    "t\nL1 1 0 1n\nR1 1 0 1\nK1 L1 R1 0.9\n",            # K naming a resistor
+    "t\nL1 1 0 1n\nL2 2 0 1n\nK1 L1 L2 1.5\n",          # coupling above 1
     "t\nX1 1 0 sub\n",                                  # subcircuit
     "t\n.param a=agauss(1,0.1,1)\nR1 1 0 {2*a}\n",      # arithmetic on a sampled value
     "t\nV1 1 0 PULSE 0 1\nR1 1 0 1k\n",                 # PULSE without parentheses
@@ -126,3 +127,14 @@ def test_inductor_parses_as_reference_test():
     kind, nodes, slots = c.elements[1]
     assert kind == A.SPX_EL_INDUCTOR and nodes[:2] == [1, 2]
     assert CK.nominal_values(c)[slots[0]] == pytest.approx(1e-3)
+
+
+def test_mutual_inductance_read_without_effect():
+    # a coupled LC filter like parser/mod.rs:2066's: K is read and checked, and
+    # the circuit program is the one without it (mutual.rs:108-114 stamps nothing)
+    text = ("t\nV1 1 0 DC 1\nL1 1 2 1m\nC1 2 0 1u\nL2 3 0 1m\nR1 3 0 50\nK1 L1 L2 0.99\n"
+            ".end\n")
+    p = parse_netlist(text)
+    assert p.couplings == [("K1", ["L1", "L2"], 0.99)]
+    q = parse_netlist(text.replace("K1 L1 L2 0.99\n", ""))
+    assert p.circuit.elements == q.circuit.elements
