@@ -117,9 +117,10 @@ def _tokens(s: str):
 
 
 # ------------------------------------------------------------ subcircuits
-# node tokens after the element name, per element letter
+# node tokens after the element name, per element letter (subcircuit
+# flattening and the element reader share it)
 _NODE_COUNT = {"R": 2, "C": 2, "L": 2, "V": 2, "I": 2, "D": 2, "M": 4, "E": 4, "G": 4, "F": 2,
-               "H": 2}
+               "H": 2, "T": 4}
 _IDENT = re.compile(r"(?<![\w.])[A-Za-z_]\w*")
 
 
@@ -244,6 +245,8 @@ def _flatten(lines):
                     toks2.append(node(t))
                 elif kind in "FH" and i == nn and t.upper() in names:
                     toks2.append(f"{t[0]}{inst}_{t[1:]}")
+                elif kind == "K" and i < len(rest) - 1 and t.upper() in names:
+                    toks2.append(f"{t[0]}{inst}_{t[1:]}")  # an inductor of this definition
                 elif i + 1 < len(rest) and rest[i + 1] == "=":
                     toks2.append(t)  # the key of an instance override such as W=...
                 else:
@@ -437,8 +440,7 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
         kind = toks[0][0].upper()
         nm = toks[0]
         args = toks[1:]
-        nn = {"R": 2, "C": 2, "L": 2, "V": 2, "I": 2, "D": 2, "M": 4, "E": 4, "G": 4, "F": 2,
-              "H": 2, "T": 4}.get(kind, 0)
+        nn = _NODE_COUNT.get(kind, 0)
         args = [nodes.id(a) if k < nn else a for k, a in enumerate(args)]
 
         def need(k):
@@ -507,14 +509,11 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
             inds = args[:-1]
             if len(inds) < 2:
                 raise InvalidArgument(f"line {no}: {nm} needs at least 2 inductors")
-            for ln in inds:
-                i = by_name.get(ln.upper())
-                if i is None or c.elements[i][0] != A_EL_INDUCTOR:
-                    raise InvalidArgument(f"line {no}: {nm}: {ln} is not an earlier inductor")
             kval = scope.value(args[-1], no, name=nm)
             if isinstance(kval, Param) or not (0.0 < kval <= 1.0):
                 raise InvalidArgument(f"line {no}: {nm}: coupling must be a constant in (0, 1]")
-            couplings.append((nm, [x.upper() for x in inds], kval))
+            # the inductors may come later in the deck: checked after every element
+            couplings.append((nm, [x.upper() for x in inds], kval, no))
             continue
         elif kind == "T":  # lossless line: elements.rs:963-1068, tline.rs:30-236
             need(4)
@@ -571,6 +570,12 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
             raise InvalidArgument(f"line {no}: element {nm} ({kind}) is not supported by the "
                                   "sweep engine")
         by_name[nm.upper()] = len(c.elements) - 1
+    for nm, inds, kval, no in couplings:
+        for ln in inds:
+            i = by_name.get(ln)
+            if i is None or c.elements[i][0] != A_EL_INDUCTOR:
+                raise InvalidArgument(f"line {no}: {nm}: {ln} is not an inductor of the circuit")
+    couplings = [(nm, inds, kval) for nm, inds, kval, _ in couplings]
     # unknown k - 1 is node k; a node number without elements would be an
     # empty (singular) row of every instance's system
     names = {k: v for k, v in nodes.map.items() if v != 0}

@@ -111,3 +111,32 @@ def test_current_control_inside_a_definition(oracle):
 def test_subckt_errors(text):
     with pytest.raises(E.InvalidArgument):
         parse_netlist(text)
+
+
+def test_transmission_line_inside_a_definition(oracle):
+    # two instances of a subcircuit holding a T line: each line's ports take
+    # the instance's connections, its internal section nodes are named after
+    # the flattened line (TX1_1_int1), so the lines short 1 -> 2 -> 3 at DC
+    # and R1 sees V1 directly
+    c = parse_netlist("t\n.SUBCKT line a b\nT1 a 0 b 0 Z0=50 TD=1n NL=2\n.ENDS\n"
+                      "V1 1 0 DC 10\nX1 1 2 line\nX2 2 3 line\nR1 3 0 100\n.end\n").circuit
+    assert set(c.node_names) == {"0", "1", "2", "3", "TX1_1_int1", "TX2_1_int1"}
+    assert c.size - c.num_nodes == 1 + 2 * 2  # V1's branch and two sections per line
+    x = _x(oracle, c)
+    for node in ("1", "2", "3", "TX1_1_int1", "TX2_1_int1"):
+        assert _v(c, x, node) == pytest.approx(10.0, rel=1e-12)
+    assert x[c.num_nodes] == pytest.approx(-0.1, rel=1e-12)  # V1 supplies 10 V / 100 Ohm
+
+
+def test_couplings_inside_a_definition_and_before_their_inductors(oracle):
+    # a K may name inductors that come later in the deck; inside a definition
+    # it names that instance's own inductors
+    p = parse_netlist("t\nK1 L1 L2 0.5\nV1 1 0 DC 1\nL1 1 2 1m\nR1 2 0 1k\nL2 3 0 1m\n"
+                      "R2 3 0 1k\n.end\n")
+    assert p.couplings == [("K1", ["L1", "L2"], 0.5)]
+    p = parse_netlist("t\n.SUBCKT xf a b\nLa a 0 1m\nLb b 0 1m\nK1 La Lb 0.9\n.ENDS\n"
+                      "V1 1 0 DC 1\nR1 1 2 1k\nX1 2 3 xf\nX2 3 4 xf\nR2 4 0 1k\n.end\n")
+    assert p.couplings == [("KX1_1", ["LX1_A", "LX1_B"], 0.9),
+                           ("KX2_1", ["LX2_A", "LX2_B"], 0.9)]
+    with pytest.raises(E.InvalidArgument):
+        parse_netlist("t\nK1 L1 L9 0.5\nL1 1 0 1m\nR1 1 0 1k\n.end\n")
