@@ -764,8 +764,10 @@ extern "C" int orc_dcop(const spx_circuit* c, const double* params, const spx_nr
 
     // Gmin stepping (newton.rs:373-459).  Retries at one level restart from the
     // same solution and are therefore identical replays of the first attempt;
-    // the first attempt decides the level.  On failure the stage's solution
-    // (last converged level, zeros if none) is what it returns.
+    // the first attempt decides the level.  A level that does not converge
+    // adds its iterations once per attempt (max_attempts_per_level of them,
+    // newton.rs:411-426), as total_iterations does.  On failure the stage's
+    // solution (last converged level, zeros if none) is what it returns.
     if (status != SPX_CONVERGED && cfg->gmin_stepping) {
       fl |= SPX_FLAG_GMIN_STEPPING;
       std::vector<double> sol(n, 0.0), trial(n);
@@ -775,6 +777,7 @@ extern "C" int orc_dcop(const spx_circuit* c, const double* params, const spx_nr
       while (g >= cfg->gmin_final * 0.99) {
         last = orc_newton(c, params, cfg, g, 1.0, sol.data(), trial.data(), &it);
         total += it;
+        if (last == 0) total += (cfg->gmin_max_attempts - 1) * it;
         if (last != 1) { ok = false; break; }
         sol = trial;
         final_gmin = g;
@@ -791,7 +794,8 @@ extern "C" int orc_dcop(const spx_circuit* c, const double* params, const spx_nr
       }
     }
 
-    // Source stepping (newton.rs:261-356), same replay remark.
+    // Source stepping (newton.rs:261-356), same replay and counting remarks
+    // (newton.rs:300-315).
     if (status != SPX_CONVERGED && cfg->source_stepping) {
       fl |= SPX_FLAG_SOURCE_STEPPING;
       std::vector<double> sol(n, 0.0), trial(n);
@@ -802,6 +806,7 @@ extern "C" int orc_dcop(const spx_circuit* c, const double* params, const spx_nr
         double cur = std::fmin(factor, 1.0);
         last = orc_newton(c, params, cfg, 0.0, cur, sol.data(), trial.data(), &it);
         total += it;
+        if (last == 0) total += (cfg->src_max_attempts - 1) * it;
         if (last != 1) {
           if (step > cfg->src_min_step) {
             step /= 2.0;

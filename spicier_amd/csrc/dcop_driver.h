@@ -38,8 +38,11 @@ __device__ __forceinline__ DcopOutcome solve_dcop(Inst& I, const DevProgram& g,
     // (newton.rs:261-356), as one state machine so the Newton loop has a
     // single, inlined call site.  Retries at one level restart from the same
     // solution and so replay the first attempt exactly; the first attempt
-    // decides the level.  A stage that fails returns its last converged
-    // level's solution (zeros if none).
+    // decides the level, and a level that does not converge counts its
+    // iterations once per attempt, max_attempts_per_level times, as the
+    // reference's total_iterations does (newton.rs:300-315, 411-426).  A
+    // stage that fails returns its last converged level's solution (zeros if
+    // none).
     enum { kPlain, kGmin, kSource, kDone };
     int stage = kPlain;
     double gm = 0.0, factor = 0.0, step = 0.0;
@@ -78,6 +81,8 @@ __device__ __forceinline__ DcopOutcome solve_dcop(Inst& I, const DevProgram& g,
       }
       const int st = I.newton(lg, lsf, it, esc);
       total += it;
+      if (st == 0 && stage != kPlain)  // the level's replayed attempts
+        total += (stage == kGmin ? cfg.gmin_attempts - 1 : cfg.src_attempts - 1) * it;
       if (st == -2) {
         handover = true;
         stage = kDone;

@@ -487,10 +487,12 @@ DevNrCfg to_dev(const spx_nr_config* c) {
   d.gmin_initial = c->gmin_initial;
   d.gmin_final = c->gmin_final;
   d.gmin_factor = c->gmin_factor;
+  d.gmin_attempts = c->gmin_max_attempts;
   d.source_stepping = c->source_stepping;
   d.src_initial = c->src_initial;
   d.src_step = c->src_step;
   d.src_min_step = c->src_min_step;
+  d.src_attempts = c->src_max_attempts;
   d.precision = c->precision;
   d.limiter = c->limiter;
   d.warm = c->warm_start;
@@ -518,6 +520,9 @@ int check_cfg(const spx_nr_config* c) {
     return fail(SPX_ERR_INVALID_ARGUMENT, "Gmin stepping needs factor > 1 and final > 0");
   if (c->source_stepping && !(c->src_min_step > 0.0 && c->src_step > 0.0))
     return fail(SPX_ERR_INVALID_ARGUMENT, "source stepping needs positive steps");
+  if ((c->gmin_stepping && (c->gmin_max_attempts < 1 || c->gmin_max_attempts > 1000)) ||
+      (c->source_stepping && (c->src_max_attempts < 1 || c->src_max_attempts > 1000)))
+    return fail(SPX_ERR_INVALID_ARGUMENT, "max attempts per stepping level must be 1..1000");
   return SPX_OK;
 }
 

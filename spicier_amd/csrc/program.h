@@ -53,8 +53,10 @@ struct DevNrCfg {
   int max_iterations;
   int gmin_stepping;
   double gmin_initial, gmin_final, gmin_factor;
+  int gmin_attempts;
   int source_stepping;
   double src_initial, src_step, src_min_step;
+  int src_attempts;
   int precision, limiter, warm;
   int ir_max_steps;
   double ir_tol;

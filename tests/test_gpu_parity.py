@@ -185,6 +185,13 @@ def test_stepping_fallbacks_match_oracle(oracle):
         want = oracle.sweep_dcop(c, got.params, ocfg, nthreads=8)
         _parity(got, want, str(kw))
         assert np.all(got.flags & (A.SPX_FLAG_GMIN_STEPPING | A.SPX_FLAG_SOURCE_STEPPING))
+    # every level fails: each failed level counts all its attempts
+    # (newton.rs:300-315, 411-426), 1 + 5 + 5 x 5 = 31 iterations
+    got = E.sweep_dcop(c, B, config=E.nr_config(E.ConvergenceCriteria(max_iterations=1)))
+    want = oracle.sweep_dcop(c, got.params, A.default_nr_config(max_iterations=1), nthreads=8)
+    np.testing.assert_array_equal(got.status, want["status"])
+    np.testing.assert_array_equal(got.iterations, want["iterations"])
+    np.testing.assert_array_equal(got.iterations, 31)
 
 
 def test_linear_sweep_single_solve(oracle):

@@ -373,6 +373,33 @@ def test_gmin_and_source_stepping_reach_the_same_point(oracle):
     np.testing.assert_allclose(o2["x"][0], plain, rtol=1e-5, atol=1e-9)
 
 
+def test_stepping_counts_every_attempt_of_a_failed_level(oracle):
+    # newton.rs:300-315 / 411-426: a level whose Newton solve does not
+    # converge is attempted max_attempts_per_level times, and every attempt's
+    # iterations go into total_iterations.  With a 1-iteration budget no solve
+    # from zero can pass the dx test (x moves by volts), so by hand: plain NR
+    # 1, the first Gmin level 5 x 1 and the stage gives up, then source
+    # stepping at factors 0.1, 0.05, 0.025, 0.0125, 0.00625 (the step halves
+    # while it exceeds 0.01) 5 x 1 each: 1 + 5 + 25 = 31.
+    c = CK.cs_amp()
+    p = CK.instance_params(c, 1)
+    o = oracle.sweep_dcop(c, p, A.default_nr_config(max_iterations=1))
+    assert o["status"][0] == A.SPX_FAILED and o["iterations"][0] == 31
+    o = oracle.sweep_dcop(c, p, A.default_nr_config(max_iterations=1, gmin_max_attempts=2,
+                                                    src_max_attempts=3))
+    assert o["iterations"][0] == 1 + 2 + 5 * 3
+    # a 3-iteration budget fails plain NR and converges later: the attempts
+    # change only the count, by (attempts - 1) x 3 per failed level, never the
+    # solution (the retries are replays)
+    base = oracle.sweep_dcop(c, p, A.default_nr_config(max_iterations=3, gmin_max_attempts=1,
+                                                       src_max_attempts=1))
+    five = oracle.sweep_dcop(c, p, A.default_nr_config(max_iterations=3))
+    assert base["status"][0] == five["status"][0]
+    np.testing.assert_array_equal(base["x"], five["x"])
+    d = int(five["iterations"][0] - base["iterations"][0])
+    assert d > 0 and d % (4 * 3) == 0
+
+
 def test_source_stepping_overshoot_is_flagged(oracle):
     # newton.rs:312-350: after a halved step the factor can jump past 1.0
     # (0.975 + 0.05) and the loop ends "converged" without a level at 1.0.
