@@ -91,7 +91,7 @@ void* spx_stream(spx_handle* h);
  * is singular iff an exact zero pivot column occurs or the solution is not
  * finite; singular systems get an all-zero solution and singular[i] = 1
  * (singular may be NULL).  *num_singular receives the count (may be NULL).
- * batch == 0 succeeds and touches nothing.  n must be 1..256. */
+ * batch == 0 succeeds and touches nothing.  n must be 1..4096. */
 int spx_solve_batch_f64(spx_handle* h, const double* matrices, const double* rhs,
                         int64_t n, int64_t batch, double* solutions,
                         uint8_t* singular, int64_t* num_singular);
@@ -99,6 +99,25 @@ int spx_solve_batch_f64(spx_handle* h, const double* matrices, const double* rhs
 int spx_solve_batch_f64_device(spx_handle* h, const double* d_matrices,
                                const double* d_rhs, int64_t n, int64_t batch,
                                double* d_solutions, uint8_t* d_singular);
+/* Batched LU factorisation alone — the getrf half of the CUDA backend's
+ * solve (cublasDgetrfBatched, crates/spicier-backend-cuda/src/batched_lu.rs:
+ * 404-428), with the CPU path's pivot rule (nalgebra LU::new,
+ * crates/spicier-batched-sweep/src/solver.rs:370): the largest |a| of the
+ * column, the first row on exact ties, every bit compared.
+ * matrices: batch*n*n column-major; factors out in LAPACK getrf's format
+ * (column-major P A = L U, unit-lower L strictly below the diagonal, U on and
+ * above); pivots: batch*n int32, pivots[i*n + k] = the 0-based row swapped
+ * with row k at step k (>= k).  A system with a zero pivot column or a
+ * non-finite factor is singular: singular[i] = 1 (may be NULL) and its
+ * factors and pivots are zero; *num_singular receives the count (may be
+ * NULL).  batch == 0 succeeds and touches nothing.  n must be 1..4096. */
+int spx_lu_factor_batch_f64(spx_handle* h, const double* matrices, int64_t n, int64_t batch,
+                            double* factors, int32_t* pivots, uint8_t* singular,
+                            int64_t* num_singular);
+/* Same contract on device pointers; enqueued on the handle's stream. */
+int spx_lu_factor_batch_f64_device(spx_handle* h, const double* d_matrices, int64_t n,
+                                   int64_t batch, double* d_factors, int32_t* d_pivots,
+                                   uint8_t* d_singular);
 /* Largest batch one launch of the solve accepts (GpuBatchConfig
  * max_batch_per_launch, solver.rs:6-41). Larger batches are split. */
 int64_t spx_max_batch_per_launch(void);

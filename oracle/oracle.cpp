@@ -202,11 +202,11 @@ extern "C" int orc_generate_params(const spx_sweep_spec* spec, int32_t P, double
 // CpuBatchedSolver (solver.rs:370-380) and solve_dense (linear.rs:506).
 // The multiplier is formed as a * (1/pivot) and the update as (-u)*l + a,
 // the order nalgebra's gauss_step / axpy use.
-extern "C" int orc_lu_solve(int32_t n, const double* a_cm, const double* b, double* x) {
-  std::vector<double> a(a_cm, a_cm + (size_t)n * n);  // column-major
+namespace {
+// nalgebra LU::new (gauss_step): in place on the column-major a, perm_from[i]
+// the row swapped with row i at step i; a zero pivot column is skipped.
+void lu_factor_inplace(int n, double* a, int* perm_from) {
   auto A = [&](int r, int c) -> double& { return a[(size_t)c * n + r]; };
-  std::vector<double> y(b, b + n);
-  std::vector<int> perm_from(n);
   for (int i = 0; i < n; ++i) {
     int piv = i;
     double best = std::fabs(A(i, i));
@@ -227,6 +227,31 @@ extern "C" int orc_lu_solve(int32_t n, const double* a_cm, const double* b, doub
       for (int r = i + 1; r < n; ++r) A(r, c) = (-u) * A(r, i) + A(r, c);
     }
   }
+}
+}  // namespace
+
+// The factorisation alone, in LAPACK getrf's format: lu column-major with the
+// unit-lower L strictly below the diagonal and U on and above it, ipiv[i] the
+// 0-based row swapped with row i at step i.  Returns 1 (singular) if a pivot
+// is zero or a factor is not finite; lu and ipiv are then zeroed.
+extern "C" int orc_lu_factor(int32_t n, const double* a_cm, double* lu, int32_t* ipiv) {
+  std::vector<double> a(a_cm, a_cm + (size_t)n * n);
+  std::vector<int> perm_from(n);
+  lu_factor_inplace(n, a.data(), perm_from.data());
+  bool singular = false;
+  for (int i = 0; i < n; ++i) singular |= !(a[(size_t)i * n + i] != 0.0);
+  for (size_t t = 0; t < a.size(); ++t) singular |= !std::isfinite(a[t]);
+  for (size_t t = 0; t < a.size(); ++t) lu[t] = singular ? 0.0 : a[t];
+  for (int i = 0; i < n; ++i) ipiv[i] = singular ? 0 : perm_from[i];
+  return singular ? 1 : 0;
+}
+
+extern "C" int orc_lu_solve(int32_t n, const double* a_cm, const double* b, double* x) {
+  std::vector<double> a(a_cm, a_cm + (size_t)n * n);  // column-major
+  auto A = [&](int r, int c) -> double& { return a[(size_t)c * n + r]; };
+  std::vector<double> y(b, b + n);
+  std::vector<int> perm_from(n);
+  lu_factor_inplace(n, a.data(), perm_from.data());
   // apply the row transpositions to b in order
   for (int i = 0; i < n; ++i)
     if (perm_from[i] != i) std::swap(y[i], y[perm_from[i]]);

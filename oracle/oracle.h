@@ -42,6 +42,9 @@ int orc_generate_params(const spx_sweep_spec* spec, int32_t num_params, double* 
 
 /* nalgebra LU::new + solve (solver.rs:370-380): returns 1 if singular (x zeroed). */
 int orc_lu_solve(int32_t n, const double* a_colmajor, const double* b, double* x);
+/* Its factorisation alone in LAPACK getrf format (L strictly below the
+ * diagonal, U on and above, 0-based ipiv); returns 1 if singular (zeroed). */
+int orc_lu_factor(int32_t n, const double* a_colmajor, double* lu, int32_t* ipiv);
 /* CpuBatchedSolver::solve_batch (solver.rs:327-385) plus the non-finite rule of
  * FaerBatchedSolver (faer_solver.rs:74-86). */
 int orc_solve_batch(const double* matrices, const double* rhs, int64_t n, int64_t batch,

@@ -43,6 +43,7 @@ def lib():
         "orc_monte_carlo_lcg": (None, [u64, i64, i32, vp, vp, vp, vp, vp]),
         "orc_generate_params": (C.c_int, [C.POINTER(A.SpxSweepSpec), i32, vp]),
         "orc_lu_solve": (C.c_int, [i32, vp, vp, vp]),
+        "orc_lu_factor": (C.c_int, [i32, vp, vp, vp]),
         "orc_solve_batch": (C.c_int, [vp, vp, i64, i64, vp, vp, C.POINTER(C.c_int64)]),
         "orc_thermal_voltage": (d, [d]),
         "orc_diode_limit": (d, [d, d]),
@@ -84,6 +85,16 @@ def lu_solve(a, b):
     x = np.zeros(len(b))
     s = lib().orc_lu_solve(len(b), _p(a), _p(b), _p(x))
     return x, bool(s)
+
+
+def lu_factor(a):
+    """(lu, ipiv, singular) of one matrix in LAPACK getrf format (0-based ipiv)."""
+    a = np.asfortranarray(a, dtype=np.float64)
+    n = a.shape[0]
+    lu = np.zeros((n, n), order="F")
+    piv = np.zeros(n, dtype=np.int32)
+    s = lib().orc_lu_factor(n, _p(a), _p(lu), _p(piv))
+    return lu, piv, bool(s)
 
 
 def solve_batch(matrices, rhs, n, batch):

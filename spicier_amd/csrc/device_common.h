@@ -212,57 +212,43 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// Lane holding the partial-pivot row: the largest |a| over eligible lanes,
-// the smallest `pos` among exact ties (nalgebra's first-maximum rule).  For
-// non-negative IEEE values the bit pattern orders like the value, so the max
-// runs on integer keys: first on the high 32 bits (one v_max_u32_dpp per
-// level), then — only if several lanes share that word — on the low 32 bits,
-// then on the position.  Returns -1 if every eligible |a| is zero.
-__device__ __forceinline__ int pivot_lane(double absval, bool eligible, int pos) {
-  const uint64_t bits = eligible ? (uint64_t)__double_as_longlong(absval) : 0ull;
-  const uint32_t hi = (uint32_t)(bits >> 32);
-  const uint32_t mhi = wave_max_u32(hi);
-  uint64_t m = __ballot(eligible && hi == mhi);
-  if (mhi == 0 || __popcll(m) != 1) {
-    const bool cand = (m >> __lane_id()) & 1;
-    const uint32_t lo = cand ? (uint32_t)bits : 0u;
-    const uint32_t mlo = wave_max_u32(lo);
-    if (mhi == 0 && mlo == 0) return -1;
-    m = __ballot(cand && lo == mlo);
-    if (__popcll(m) != 1) {  // exact tie: smallest position wins
-      const uint32_t best = wave_max_u32(((m >> __lane_id()) & 1) ? (uint32_t)(64 - pos) : 0u);
-      m = __ballot(pos == (int)(64 - best));
-    }
-  }
-  return (int)__ffsll((unsigned long long)m) - 1;
-}
-
-__device__ __forceinline__ int pivot_lane(float absval, bool eligible, int pos) {
-  uint32_t key = eligible ? (uint32_t)__float_as_uint(absval) : 0u;
-  uint32_t mx = wave_max_u32(key);
-  if (mx == 0) return -1;
-  uint64_t m = __ballot(eligible && key == mx);
-  if (__popcll(m) != 1) {
-    uint32_t best = wave_max_u32(((m >> __lane_id()) & 1) ? (uint32_t)(64 - pos) : 0u);
-    m = __ballot(pos == (int)(64 - best));
-  }
-  return (int)__ffsll((unsigned long long)m) - 1;
-}
-
-// Branch-free pivot choice for the register LU: one 32-bit DPP max over
+// Pivot choice for the register LU: one 32-bit DPP max over
 // key = (high word of |a| with its 6 low bits cleared) | (63 - pos).  The
 // largest |a| wins; candidates equal in sign, exponent and the top 14
 // mantissa bits — exact ties included — go to the smallest position, which is
 // nalgebra's first-maximum rule for exact ties.  Returns the winning position
 // (pos < 64) and whether the column held a non-zero value.
+//
+// FP64 is exact: when several eligible lanes share the winning truncated
+// key (equal to 14 mantissa bits, exact ties included, or every candidate
+// below 2^-1008), the tied lanes compare the remaining 38 bits of |a| and
+// then the position, in two more DPP max passes (the 6 remaining bits of the
+// high word with the top 26 of the low word; the low word's last 6 bits with
+// 63 - pos).  So the pivot is the strictly largest |a|, first position on
+// exact ties — nalgebra's rule — and a denormal pivot counts as non-zero.  The
+// extra passes run only on such ties (a wave-uniform branch).
 __device__ __forceinline__ int pivot_pos(double absval, bool eligible, int pos, bool& nonzero) {
-  const uint32_t hi = (uint32_t)((uint64_t)__double_as_longlong(absval) >> 32);
+  const uint64_t bits = (uint64_t)__double_as_longlong(absval);
+  const uint32_t hi = (uint32_t)(bits >> 32), lo = (uint32_t)bits;
   const uint32_t key = eligible ? ((hi & ~63u) | (uint32_t)(63 - pos)) : 0u;
   const uint32_t mx = wave_max_u32(key);
+  const bool tied = eligible && (key >> 6) == (mx >> 6);
+  if (__popcll(__ballot(tied)) > 1) {
+    const uint32_t ka = tied ? (((hi & 63u) << 26) | (lo >> 6)) : 0u;
+    const uint32_t ma = wave_max_u32(ka);
+    const bool tied2 = tied && ka == ma;
+    const uint32_t kb = tied2 ? (((lo & 63u) << 6) | (uint32_t)(63 - pos)) : 0u;
+    const uint32_t mb = wave_max_u32(kb);
+    nonzero = (mx >> 6) != 0 || ma != 0 || (mb >> 6) != 0;
+    return 63 - (int)(mb & 63u);
+  }
   nonzero = (mx >> 6) != 0;
   return 63 - (int)(mx & 63u);
 }
 
+// FP32 keeps the truncated comparison (17 mantissa bits): FP32 factors serve
+// only as the preconditioner of FP64 iterative refinement, whose stopping
+// test, not the pivot order, decides the result.
 __device__ __forceinline__ int pivot_pos(float absval, bool eligible, int pos, bool& nonzero) {
   const uint32_t bits = __float_as_uint(absval);
   const uint32_t key = eligible ? ((bits & ~63u) | (uint32_t)(63 - pos)) : 0u;

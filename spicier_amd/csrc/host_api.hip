@@ -68,6 +68,8 @@ hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const 
                                 const DevOut& out, double* ws, int grid, int64_t slot,
                                 size_t smem, int lu_in_lds, hipStream_t st);
 size_t solve_generic_workspace(int n, int64_t chunk);
+hipError_t launch_factor_batch(const double* M, int n, int64_t B, double* F, int32_t* piv,
+                               uint8_t* sing, hipStream_t st);
 }  // namespace spx
 
 namespace spx {
@@ -1270,6 +1272,63 @@ int spx_solve_batch_f64_device(spx_handle* h, const double* dM, const double* dr
   }
   SPX_HIP(launch_solve_batch(dM, drhs, (int)n, B, dx, dsing, work, chunk, h->stream),
           SPX_ERR_BACKEND);
+  return SPX_OK;
+}
+
+int spx_lu_factor_batch_f64_device(spx_handle* h, const double* dM, int64_t n, int64_t B,
+                                   double* dF, int32_t* dpiv, uint8_t* dsing) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  if (B < 0 || n < 0) return fail(SPX_ERR_INVALID_DIMENSION, "negative size");
+  if (B == 0) return SPX_OK;
+  if (!dM || !dF || !dpiv) return fail(SPX_ERR_INVALID_ARGUMENT, "null buffer");
+  if (n < 1 || n > 4096)
+    return fail(SPX_ERR_INVALID_DIMENSION, "n must be 1..4096, got " + std::to_string(n));
+  if (B > spx_max_batch_per_launch())
+    return fail(SPX_ERR_BATCH_TOO_LARGE, "batch " + std::to_string(B) + " exceeds " +
+                                             std::to_string(spx_max_batch_per_launch()));
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  SPX_HIP(launch_factor_batch(dM, (int)n, B, dF, dpiv, dsing, h->stream), SPX_ERR_BACKEND);
+  return SPX_OK;
+}
+
+int spx_lu_factor_batch_f64(spx_handle* h, const double* M, int64_t n, int64_t B, double* F,
+                            int32_t* piv, uint8_t* sing, int64_t* num_singular) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  if (B < 0 || n < 0) return fail(SPX_ERR_INVALID_DIMENSION, "negative size");
+  if (num_singular) *num_singular = 0;
+  if (B == 0) return SPX_OK;
+  if (!M || !F || !piv) return fail(SPX_ERR_INVALID_ARGUMENT, "null buffer");
+  if (n < 1 || n > 4096)
+    return fail(SPX_ERR_INVALID_DIMENSION, "n must be 1..4096, got " + std::to_string(n));
+  void *dM, *dF, *dp, *ds;
+  {
+    std::lock_guard<std::mutex> lk(h->mu);
+    DeviceGuard dg(h->device);
+    int rc;
+    if ((rc = workspace(h, "sb_M", sizeof(double) * B * n * n, &dM))) return rc;
+    if ((rc = workspace(h, "lf_F", sizeof(double) * B * n * n, &dF))) return rc;
+    if ((rc = workspace(h, "lf_piv", sizeof(int32_t) * B * n, &dp))) return rc;
+    if ((rc = workspace(h, "sb_sing", B, &ds))) return rc;
+    SPX_HIP(hipMemcpyAsync(dM, M, sizeof(double) * B * n * n, hipMemcpyHostToDevice, h->stream),
+            SPX_ERR_TRANSFER);
+  }
+  int rc = spx_lu_factor_batch_f64_device(h, (double*)dM, n, B, (double*)dF, (int32_t*)dp,
+                                          (uint8_t*)ds);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  std::vector<uint8_t> hs(B);
+  SPX_HIP(hipMemcpyAsync(F, dF, sizeof(double) * B * n * n, hipMemcpyDeviceToHost, h->stream),
+          SPX_ERR_TRANSFER);
+  SPX_HIP(hipMemcpyAsync(piv, dp, sizeof(int32_t) * B * n, hipMemcpyDeviceToHost, h->stream),
+          SPX_ERR_TRANSFER);
+  SPX_HIP(hipMemcpyAsync(hs.data(), ds, B, hipMemcpyDeviceToHost, h->stream), SPX_ERR_TRANSFER);
+  SPX_HIP(hipStreamSynchronize(h->stream), SPX_ERR_TRANSFER);
+  int64_t ns = 0;
+  for (int64_t i = 0; i < B; ++i) ns += hs[i];
+  if (sing) std::memcpy(sing, hs.data(), B);
+  if (num_singular) *num_singular = ns;
   return SPX_OK;
 }
 

@@ -43,25 +43,49 @@ __global__ __launch_bounds__(64) void k_solve_rows(const double* __restrict__ M,
   if (lane == 0 && sing) sing[i] = ok ? 0 : 1;
 }
 
-__global__ __launch_bounds__(256) void k_solve_generic(const double* __restrict__ M,
-                                                       const double* __restrict__ rhs, int n,
-                                                       int64_t base, double* __restrict__ W,
-                                                       double* __restrict__ Y,
-                                                       double* __restrict__ x,
-                                                       uint8_t* __restrict__ sing) {
-  __shared__ double sv[256];
-  __shared__ int si[256];
-  __shared__ int s_p;
-  __shared__ double s_pv;
-  const int64_t li = blockIdx.x, gi = base + li;
+// getrf of one system (n <= N) in the register LU: factors in LAPACK's
+// format (column-major, unit-lower L strictly below the diagonal, U on and
+// above) and the 0-based row swapped with row k at step k.
+template <int N>
+__global__ __launch_bounds__(64) void k_factor_rows(const double* __restrict__ M, int n,
+                                                    double* __restrict__ F,
+                                                    int32_t* __restrict__ piv,
+                                                    uint8_t* __restrict__ sing) {
+  __shared__ double row[row_buffer_doubles(N)];
+  const int64_t i = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double* A = M + i * (int64_t)n * n;
+  RowLU<N, double> lu;
+  lu.row = row;
+  static_for<0, N>([&](auto J) __attribute__((always_inline)) {
+    constexpr int j = J;
+    lu.a[j] = (j < n && lane < n) ? A[(int64_t)j * n + lane] : (j == lane ? 1.0 : 0.0);
+  });
+  bool ok = lu.template factor<true>(lane);
+  bool fin = true;
+  static_for<0, N>([&](auto J) __attribute__((always_inline)) {
+    constexpr int j = J;
+    fin = fin && isfinite(lu.a[j]);
+  });
+  ok = ok && __all(lane >= n || fin);
+  double* out = F + i * (int64_t)n * n;
+  static_for<0, N>([&](auto J) __attribute__((always_inline)) {
+    constexpr int j = J;
+    if (j < n && lane < n) out[(int64_t)j * n + lane] = ok ? lu.a[j] : 0.0;
+  });
+  if (lane < n) piv[i * n + lane] = ok ? lu.ipiv : 0;
+  if (lane == 0 && sing) sing[i] = ok ? 0 : 1;
+}
+
+// Right-looking unblocked partial-pivot LU of the column-major n x n matrix
+// A in global memory by one 256-thread workgroup (column k contiguous, so the
+// pivot search and the multiplier column are unit-stride): first maximum
+// |a| on ties, physical row swaps (applied to y too when given), multipliers
+// as a * (1/pivot), ipiv[k] the row swapped with row k when given.  Returns
+// false at the first zero pivot column.  sv/si: 256 doubles / ints of LDS.
+__device__ bool generic_factor(double* A, int n, double* y, int32_t* ipiv, double* sv, int* si,
+                               int& s_p, double& s_pv) {
   const int tid = threadIdx.x;
-  const int64_t nn = (int64_t)n * n;
-  double* A = W + li * nn;
-  double* y = Y + li * n;
-  for (int64_t t = tid; t < nn; t += 256) A[t] = M[gi * nn + t];
-  for (int t = tid; t < n; t += 256) y[t] = rhs[gi * n + t];
-  __syncthreads();
-  bool ok = true;
   for (int k = 0; k < n; ++k) {
     double best = -1.0;
     int bi = n;
@@ -83,22 +107,23 @@ __global__ __launch_bounds__(256) void k_solve_generic(const double* __restrict_
     if (tid == 0) { s_p = si[0]; s_pv = sv[0]; }
     __syncthreads();
     const int p = s_p;
-    if (!(s_pv > 0.0)) { ok = false; break; }
+    if (!(s_pv > 0.0)) return false;
+    if (tid == 0 && ipiv) ipiv[k] = p;
     if (p != k) {
       for (int c = tid; c < n; c += 256) {
         double t = A[(int64_t)c * n + k];
         A[(int64_t)c * n + k] = A[(int64_t)c * n + p];
         A[(int64_t)c * n + p] = t;
       }
-      if (tid == 0) { double t = y[k]; y[k] = y[p]; y[p] = t; }
+      if (tid == 0 && y) { double t = y[k]; y[k] = y[p]; y[p] = t; }
     }
     __syncthreads();
     const double inv = 1.0 / A[(int64_t)k * n + k];
-    const double yk = y[k];
+    const double yk = y ? y[k] : 0.0;
     for (int r = k + 1 + tid; r < n; r += 256) {
       double l = A[(int64_t)k * n + r] * inv;
       A[(int64_t)k * n + r] = l;
-      y[r] = fma(-yk, l, y[r]);
+      if (y) y[r] = fma(-yk, l, y[r]);
     }
     __syncthreads();
     const int m = n - k - 1;
@@ -110,6 +135,28 @@ __global__ __launch_bounds__(256) void k_solve_generic(const double* __restrict_
     }
     __syncthreads();
   }
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_solve_generic(const double* __restrict__ M,
+                                                       const double* __restrict__ rhs, int n,
+                                                       int64_t base, double* __restrict__ W,
+                                                       double* __restrict__ Y,
+                                                       double* __restrict__ x,
+                                                       uint8_t* __restrict__ sing) {
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  __shared__ int s_p;
+  __shared__ double s_pv;
+  const int64_t li = blockIdx.x, gi = base + li;
+  const int tid = threadIdx.x;
+  const int64_t nn = (int64_t)n * n;
+  double* A = W + li * nn;
+  double* y = Y + li * n;
+  for (int64_t t = tid; t < nn; t += 256) A[t] = M[gi * nn + t];
+  for (int t = tid; t < n; t += 256) y[t] = rhs[gi * n + t];
+  __syncthreads();
+  bool ok = generic_factor(A, n, y, nullptr, sv, si, s_p, s_pv);
   if (ok) {
     for (int k = n - 1; k >= 0; --k) {
       if (tid == 0) y[k] = y[k] / A[(int64_t)k * n + k];
@@ -124,6 +171,53 @@ __global__ __launch_bounds__(256) void k_solve_generic(const double* __restrict_
   }
   for (int r = tid; r < n; r += 256) x[gi * n + r] = ok ? y[r] : 0.0;
   if (tid == 0 && sing) sing[gi] = ok ? 0 : 1;
+}
+
+// getrf in place on the output F (copied from M): factors and 0-based
+// pivots of LAPACK's format, zeros for a singular system.
+__global__ __launch_bounds__(256) void k_factor_generic(const double* __restrict__ M, int n,
+                                                        double* __restrict__ F,
+                                                        int32_t* __restrict__ piv,
+                                                        uint8_t* __restrict__ sing) {
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  __shared__ int s_p;
+  __shared__ double s_pv;
+  const int64_t i = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t nn = (int64_t)n * n;
+  double* A = F + i * nn;
+  int32_t* ip = piv + i * n;
+  for (int64_t t = tid; t < nn; t += 256) A[t] = M[i * nn + t];
+  __syncthreads();
+  bool ok = generic_factor(A, n, nullptr, ip, sv, si, s_p, s_pv);
+  int bad = 0;
+  if (ok)
+    for (int64_t t = tid; t < nn; t += 256) bad |= !isfinite(A[t]);
+  ok = !__syncthreads_or(bad || !ok);
+  if (!ok) {
+    for (int64_t t = tid; t < nn; t += 256) A[t] = 0.0;
+    for (int t = tid; t < n; t += 256) ip[t] = 0;
+  }
+  if (tid == 0 && sing) sing[i] = ok ? 0 : 1;
+}
+
+template <int N>
+static void launch_factor_rows(const double* M, int n, int64_t B, double* F, int32_t* piv,
+                               uint8_t* sing, hipStream_t st) {
+  hipLaunchKernelGGL(k_factor_rows<N>, dim3((unsigned)B), dim3(kWave), 0, st, M, n, F, piv, sing);
+}
+
+hipError_t launch_factor_batch(const double* M, int n, int64_t B, double* F, int32_t* piv,
+                               uint8_t* sing, hipStream_t st) {
+  if (B == 0) return hipSuccess;
+  if (n <= 8) launch_factor_rows<8>(M, n, B, F, piv, sing, st);
+  else if (n <= 16) launch_factor_rows<16>(M, n, B, F, piv, sing, st);
+  else if (n <= 32) launch_factor_rows<32>(M, n, B, F, piv, sing, st);
+  else if (n <= 64) launch_factor_rows<64>(M, n, B, F, piv, sing, st);
+  else
+    hipLaunchKernelGGL(k_factor_generic, dim3((unsigned)B), dim3(256), 0, st, M, n, F, piv, sing);
+  return hipGetLastError();
 }
 
 template <int N>

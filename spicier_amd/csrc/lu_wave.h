@@ -8,12 +8,12 @@
 // physical row swap of the reference's LU.  After the last step every row is
 // pushed to the lane of its final position (one cross-lane permute per
 // register), so lane r holds row r of P A = L U and the solves address the
-// row of step k as lane k, a compile-time constant.  The pivot of step k is the largest |A[.][k]| over rows at
-// positions >= k, smallest position on ties — nalgebra's icamax rule (first
-// maximum), which CpuBatchedSolver inherits (spicier-batched-sweep
-// src/solver.rs:370-380); the comparison runs on the top 14 mantissa bits,
-// so candidates that close count as tied (the pivot then differs from the
-// reference's only between values equal to 6e-5 relative).  Multipliers are formed as a * (1/pivot) and
+// row of step k as lane k, a compile-time constant.  The pivot of step k is
+// the largest |A[.][k]| over rows at positions >= k, smallest position on
+// exact ties — nalgebra's icamax rule (first maximum), which CpuBatchedSolver
+// inherits (spicier-batched-sweep src/solver.rs:370-380).  FP64 compares
+// every bit of |a| (pivot_pos); FP32 factors compare 17 mantissa bits, as
+// they only precondition FP64 refinement.  Multipliers are formed as a * (1/pivot) and
 // updates as fma(-u, l, a).  The caller's workgroup is exactly one wave, so
 // __syncthreads() orders LDS traffic without a real barrier.
 //
@@ -101,14 +101,17 @@ struct RowLU {
   T piv;       // U[lane][lane] once factored
   T dinv;      // 1 / U[lane][lane]
   T* row;      // N elements of LDS: broadcast buffer of the LDS variant
+  int ipiv;    // factor<true>: lane k holds the position swapped with k at step k
 
   // Factor in place.  Returns false if a pivot column is zero (or NaN): the
   // reference's LU then reports the system singular.  Every step runs over
   // all N columns with no per-column guard, so the update is straight-line
   // code; rows and columns past n are identity padding, whose multipliers
   // are zero.
+  template <bool Record = false>
   __device__ __forceinline__ bool factor(int lane) {
     pos = lane;
+    if constexpr (Record) ipiv = lane;
     dinv = T(0);
     piv = T(0);
     bool ok = true;
@@ -122,6 +125,9 @@ struct RowLU {
       bool nz;
       const int bp = __builtin_amdgcn_readfirstlane(pivot_pos(tabs(a[k]), pos >= k, pos, nz));
       ok = ok && nz;
+      if constexpr (Record) {
+        if (lane == k) ipiv = bp;
+      }
       const int p = __builtin_amdgcn_readfirstlane(
           (int)__ffsll((unsigned long long)__ballot(pos == bp)) - 1);
       if (pos == k) pos = bp;  // row at position k moves to the pivot's position

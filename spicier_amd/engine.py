@@ -228,6 +228,24 @@ class HipBatchedSolver(BatchedLuSolver):
             self.device.handle, A.ptr(d_matrices), A.ptr(d_rhs), n, batch_size, A.ptr(d_x),
             A.ptr(d_sing)))
 
+    def factor_batch(self, matrices, n: int, batch_size: int):
+        """The getrf half (cublasDgetrfBatched, batched_lu.rs:404-428) with the
+        CPU path's pivot rule: (factors [B][n][n], factors[i] the n x n matrix
+        of LAPACK's getrf format, 0-based pivots [B][n], singular indices)."""
+        m = np.ascontiguousarray(matrices, dtype=np.float64).reshape(-1)
+        if m.size != batch_size * n * n:
+            raise InvalidDimension(f"Expected {batch_size * n * n} matrix elements, got {m.size}")
+        f = np.zeros(batch_size * n * n)
+        piv = np.zeros(max(batch_size * n, 1), dtype=np.int32)
+        sing = np.zeros(max(batch_size, 1), dtype=np.uint8)
+        ns = C.c_int64(0)
+        _check(self.device.lib.spx_lu_factor_batch_f64(self.device.handle, A.ptr(m), n,
+                                                       batch_size, A.ptr(f), A.ptr(piv),
+                                                       A.ptr(sing), C.byref(ns)))
+        idx = [int(i) for i in np.nonzero(sing[:batch_size])[0]]
+        return (f.reshape(batch_size, n, n).transpose(0, 2, 1),
+                piv[:batch_size * n].reshape(batch_size, n), idx)
+
     def should_use_gpu(self, matrix_size: int, batch_size: int) -> bool:
         return self._config.should_use_gpu(matrix_size, batch_size)
 
