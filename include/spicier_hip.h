@@ -99,6 +99,19 @@ int spx_solve_batch_f64(spx_handle* h, const double* matrices, const double* rhs
 int spx_solve_batch_f64_device(spx_handle* h, const double* d_matrices,
                                const double* d_rhs, int64_t n, int64_t batch,
                                double* d_solutions, uint8_t* d_singular);
+/* No-allocation variant of spx_solve_batch_f64_device.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same inputs, outputs, semantics and kernels; the
+ * LU workspace of n > 64 is a 64 MiB array that the library holds on each
+ * device from its load until the process ends, so the call makes no
+ * allocation.  One system must fit it: n <= 2895 (n > 2895 is
+ * SPX_ERR_INVALID_DIMENSION).  Every call on a device shares that device's
+ * array, so calls on one device that overlap in time (handles on different
+ * streams) race; calls on one handle are stream-ordered and do not. */
+int spx_solve_batch_f64_device_noalloc(spx_handle* h, const double* d_matrices,
+                                       const double* d_rhs, int64_t n, int64_t batch,
+                                       double* d_solutions, uint8_t* d_singular);
 /* Batched LU factorisation alone — the getrf half of the CUDA backend's
  * solve (cublasDgetrfBatched, crates/spicier-backend-cuda/src/batched_lu.rs:
  * 404-428), with the CPU path's pivot rule (nalgebra LU::new,

@@ -231,6 +231,7 @@ def load_library():
         "spx_stream": (C.c_void_p, [vp]),
         "spx_solve_batch_f64": (C.c_int, [vp, vp, vp, i64, i64, vp, u8p, C.POINTER(C.c_int64)]),
         "spx_solve_batch_f64_device": (C.c_int, [vp, vp, vp, i64, i64, vp, u8p]),
+        "spx_solve_batch_f64_device_noalloc": (C.c_int, [vp, vp, vp, i64, i64, vp, u8p]),
         "spx_lu_factor_batch_f64": (C.c_int, [vp, vp, i64, i64, vp, vp, u8p,
                                               C.POINTER(C.c_int64)]),
         "spx_lu_factor_batch_f64_device": (C.c_int, [vp, vp, i64, i64, vp, vp, u8p]),

@@ -70,6 +70,7 @@ hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const 
 size_t solve_generic_workspace(int n, int64_t chunk);
 hipError_t launch_factor_batch(const double* M, int n, int64_t B, double* F, int32_t* piv,
                                uint8_t* sing, hipStream_t st);
+int noalloc_scratch_init(int device);
 }  // namespace spx
 
 namespace spx {
@@ -1209,6 +1210,7 @@ int spx_create(int32_t device_id, spx_handle** out) {
   if (device_id < 0 || device_id >= count)
     return fail(SPX_ERR_BACKEND_INIT, "device " + std::to_string(device_id) + " out of range");
   DeviceGuard dg(device_id);
+  if (int rc = noalloc_scratch_init(device_id)) return rc;
   auto* h = new spx_handle;
   h->device = device_id;
   e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);

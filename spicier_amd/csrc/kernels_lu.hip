@@ -12,7 +12,11 @@
 // pivot column or non-finite solution) get zeros and a flag.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "device_common.h"
+#include "handle.h"
 #include "lu_wave.h"
 
 namespace spx {
@@ -251,3 +255,79 @@ hipError_t launch_solve_batch(const double* M, const double* rhs, int n, int64_t
 }
 
 }  // namespace spx
+
+// ---------------------------------------------------------------------------
+// The no-allocation dense solve: the generic path's LU workspace lives in a
+// file-scope device array instead of a handle workspace, so the call makes
+// no device allocation.  Each GPU holds its own copy of the array (from the
+// library's load until the process ends); host code reaches it only through
+// the address hipGetSymbolAddress returns on that device, taken once per
+// device when a handle is created there.
+
+namespace spx {
+
+namespace {
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;  // 64 MiB per device
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// With `device` current: records the device address of this device's
+// scratch array (once per device).
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(SPX_ERR_BACKEND_INIT, "device " + std::to_string(device) + " out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return SPX_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(SPX_ERR_BACKEND_INIT,
+                std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[device] = p;
+  return SPX_OK;
+}
+
+}  // namespace spx
+
+using namespace spx;
+
+extern "C" int spx_solve_batch_f64_device_noalloc(spx_handle* h, const double* dM,
+                                                  const double* drhs, int64_t n, int64_t B,
+                                                  double* dx, uint8_t* dsing) {
+  if (!h) return fail(SPX_ERR_INVALID_ARGUMENT, "null handle");
+  if (B < 0 || n < 0) return fail(SPX_ERR_INVALID_DIMENSION, "negative size");
+  if (B == 0) return SPX_OK;
+  if (!dM || !drhs || !dx) return fail(SPX_ERR_INVALID_ARGUMENT, "null buffer");
+  if (n < 1 || n > 4096)
+    return fail(SPX_ERR_INVALID_DIMENSION, "n must be 1..4096, got " + std::to_string(n));
+  if (B > spx_max_batch_per_launch())
+    return fail(SPX_ERR_BATCH_TOO_LARGE, "batch " + std::to_string(B) + " exceeds " +
+                                             std::to_string(spx_max_batch_per_launch()));
+  int64_t chunk = 1;
+  if (n > 64) {  // one system's matrix and rhs must fit the scratch array
+    const size_t per = solve_generic_workspace((int)n, 1);
+    if (per > kNoAllocScratchBytes)
+      return fail(SPX_ERR_INVALID_DIMENSION,
+                  "the no-allocation solve holds one system of at most " +
+                      std::to_string(kNoAllocScratchBytes) + " bytes (n <= 2895), got n = " +
+                      std::to_string(n));
+    chunk = std::min<int64_t>(B, (int64_t)(kNoAllocScratchBytes / per));
+  }
+  std::lock_guard<std::mutex> lk(h->mu);
+  DeviceGuard dg(h->device);
+  void* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> sl(g_scratch_mu);
+    ptr = (h->device >= 0 && h->device < kMaxDevices) ? g_scratch[h->device] : nullptr;
+  }
+  if (!ptr) return fail(SPX_ERR_BACKEND_INIT, "scratch address of the device is not known");
+  SPX_HIP(launch_solve_batch(dM, drhs, (int)n, B, dx, dsing, static_cast<double*>(ptr), chunk,
+                             h->stream),
+          SPX_ERR_BACKEND);
+  return SPX_OK;
+}
+
