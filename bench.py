@@ -34,16 +34,57 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-MI355X_FP64_PEAK_TFLOPS = 78.6   # vector and matrix FP64; FP32 non-packed issues at the same rate
+# MI355X published dense peaks (TFLOP/s), vector and matrix alike
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 
-# HBM bytes per launch of the form's Newton kernel, from rocprofv3 --pmc
+# HBM bytes per launch of the form's Newton kernel: rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the counter's
-# reading for a 1 GiB device copy (FETCH x 2.0, WRITE x 1.0); the runs and
-# figures are in profiles/newton_dense_64_traffic_inverter_chain_65536.txt.
-# Algorithmic bytes per launch are 107.2 MB (FP32+IR warm start) and 73.7 MB
-# (FP64 zero start): parameters and x0 read once, x and per-instance status
-# written once.
-MEASURED_TRAFFIC_BYTES = {"fp64": 1006.6e6, "fp32_ir_surrogate": 1270.6e6}
+# reading for a 1 GiB device copy.  The runs, the scaling and the figures are
+# in this file, whose "traffic_bytes_per_launch" line bench.py reads.
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "newton_dense_64_traffic_inverter_chain_65536.txt")
+
+
+def measured_traffic():
+    """{form: HBM bytes per launch} from the committed counter summary."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch"):
+                return {k: float(v) for k, v in (t.split("=") for t in line.split()[1:])}
+    except OSError:
+        pass
+    return {}
+
+
+def host_cpu():
+    """Host CPU model and the cores this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"model": model, "cpus_available": avail, "nproc": os.cpu_count()}
+
+
+def fp64_error(x, ref):
+    """Per-node error of solutions x [B][n] against FP64 reference solutions:
+    max |dx| (V), max and 99.9th-percentile (over instances) of the largest
+    per-node relative error, and the instances outside the 1e-9 relative /
+    1e-12 V bound."""
+    import numpy as np
+    d = np.abs(x - ref)
+    rel = d / np.maximum(np.abs(ref), 1e-300)
+    inst = rel.max(axis=1)
+    outside = np.count_nonzero((d > np.maximum(1e-9 * np.abs(ref), 1e-12)).any(axis=1))
+    return {"max_abs_V": float(d.max()), "max_rel": float(inst.max()),
+            "p999_rel": float(np.quantile(inst, 0.999)), "instances": int(len(x)),
+            "outside_1e-9_bound": int(outside)}
 
 
 def lu_flops(n: int) -> float:
@@ -199,6 +240,82 @@ def other_workloads(E, A, CK, torch, np, dev, stream):
     return out
 
 
+def _oracle_rate(pyoracle, A, np, circuit, params, cfg, threads, min_seconds, max_passes=100):
+    """Accepted solves/s of the oracle over params, repeated until min_seconds;
+    returns (rate, passes, seconds, first pass's outputs)."""
+    acc, passes, dt, first = 0, 0, 0.0, None
+    t0 = time.perf_counter()
+    while passes < max_passes and (passes == 0 or dt < min_seconds):
+        out = pyoracle.sweep_dcop(circuit, params, cfg, nthreads=threads)
+        first = first or out
+        acc += int(np.count_nonzero((out["status"] == A.SPX_CONVERGED)
+                                    & ((out["flags"] & A.SPX_FLAG_KCL_FAIL) == 0)))
+        passes += 1
+        dt = time.perf_counter() - t0
+    return acc / dt, passes, dt, first
+
+
+def cpu_baselines(args, A, CK, np, params, ckt_py, B, report, solutions, warm_x0=None):
+    """The restated reference CPU path (the oracle: per-instance Newton with
+    Gmin/source stepping, nalgebra-style dense LU below 50 unknowns and the
+    CachedSparseLu restatement from 50 on, as solve_newton_raphson picks,
+    newton.rs:92-108), one task per instance on a thread pool as rayon's
+    par_iter (parallel.rs:150-155), on all available host cores and on one
+    core, for the headline sweep and the two other DC-OP configurations.
+    The headline's all-core pass over every instance also gives the FP64
+    reference solutions that each GPU form's error is stated against; the
+    surrogate form's error is also stated against the FP64 oracle started
+    from the same surrogate prediction (a warm-started Newton stops at its
+    dx test on another iterate than a zero-started one)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    host = host_cpu()
+    allc = max(1, min(16, host["cpus_available"]))
+    cfg = A.default_nr_config()
+    S = min(args.cpu_sample, B)
+    rate, passes, dt, first = _oracle_rate(pyoracle, A, np, ckt_py, params[:S], cfg, allc,
+                                           args.cpu_seconds)
+    n1 = min(S, 4096)
+    rate1, passes1, dt1, _ = _oracle_rate(pyoracle, A, np, ckt_py, params[:n1], cfg, 1, 2.0)
+    if S == B:
+        for name, x in solutions.items():
+            report[name]["fp64_oracle_error"] = fp64_error(x, first["x"])
+            if name != "fp64" and "fp64" in solutions:
+                report[name]["fp64_gpu_error"] = fp64_error(x, solutions["fp64"])
+        if warm_x0 is not None and "fp32_ir_surrogate" in solutions:
+            same = pyoracle.sweep_dcop(ckt_py, params[:S],
+                                       A.default_nr_config(warm_start=A.SPX_WARM_GIVEN),
+                                       x0=warm_x0[:S], nthreads=allc)
+            report["fp32_ir_surrogate"]["fp64_oracle_same_start_error"] = fp64_error(
+                solutions["fp32_ir_surrogate"], same["x"])
+    others = {}
+    for key, mk, count, sample_all, sample_one, ocfg in (
+            ("cs_amp_1024", CK.cs_amp, 1024, 1024, 1024, A.default_nr_config()),
+            ("grid32_diodes_4096", lambda: CK.grid_diodes(32), 4096, 256, 16,
+             A.default_nr_config(limiter=A.SPX_LIMIT_PNJLIM))):
+        c = mk()
+        p = pyoracle.generate_params(c, sample_all, seed=42)
+        ra, pa, da, _ = _oracle_rate(pyoracle, A, np, c, p, ocfg, allc, 1.0)
+        r1, p1, d1, _ = _oracle_rate(pyoracle, A, np, c, p[:sample_one], ocfg, 1, 1.0)
+        others[key] = {
+            "all_cores": {"value": ra, "cores": allc,
+                          "sample": f"{pa} passes over the first {sample_all} of {count} "
+                                    f"instances, {da:.2f} s"},
+            "one_core": {"value": r1, "cores": 1,
+                         "sample": f"{p1} passes over the first {sample_one} instances, "
+                                   f"{d1:.2f} s"},
+            "unit": "solves/s", "unknowns": c.size}
+    return {"value": rate, "unit": "solves/s", "cores": allc, "kind": "port",
+            "sample": f"{passes} passes over the first {S} of the {B} instances (same seeded "
+                      f"parameters), restated reference CPU path (per-instance NR + Gmin/source "
+                      f"stepping; sparse LU from 50 unknowns as the reference's Newton), one task "
+                      f"per instance on {allc} host threads; {dt:.2f} s wall",
+            "one_core": {"value": rate1, "cores": 1,
+                         "sample": f"{passes1} passes over the first {n1} instances, "
+                                   f"{dt1:.2f} s wall"},
+            "host": host, "configs": others}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,7 +323,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--cpu-sample", type=int, default=65536)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32_ir,fp32_ir_surrogate")
     ap.add_argument("--train", type=int, default=262144)
@@ -266,6 +383,8 @@ def main():
     forms["fp32_ir_surrogate"].warm_start = A.SPX_WARM_SURROGATE
     wanted = [f for f in args.forms.split(",") if f in forms]
     report = {}
+    solutions = {}
+    warm_x0 = None
     training = None
     sur = None
     if "fp32_ir_surrogate" in wanted:
@@ -290,6 +409,7 @@ def main():
         dev.synchronize()
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
+        solutions[name] = buf["x"].cpu().numpy()
         st = buf["status"].cpu().numpy()
         fl = buf["flags"].cpu().numpy()
         it = buf["iters"].cpu().numpy()
@@ -309,6 +429,8 @@ def main():
             x0buf = torch.empty(B, n, **f64)
             torch.cuda.synchronize()
             sur.predict(buf["params"], n, x0buf)
+            torch.cuda.synchronize()
+            warm_x0 = x0buf.cpu().numpy()
             kcfg = E.nr_config(precision=E.Precision.F32_IR)
             kcfg.warm_start = A.SPX_WARM_GIVEN
             x0 = A.ptr(x0buf)
@@ -327,12 +449,14 @@ def main():
         else:  # FP32 factorisations + per refinement step 2n^2 FP64 residual and 2n^2 FP32 solve
             flops32 = fac.sum() * getrf_flops(n) + ref.sum() * 2.0 * n * n
             flops64 = ref.sum() * 2.0 * n * n
+        peak_s = flops32 / (PEAK_TFLOPS["fp32"] * 1e12) + flops64 / (PEAK_TFLOPS["fp64"] * 1e12)
         report[name] = dict(
             solves_per_s=accepted / wall, ms_per_step=wall * 1e3, accepted=accepted,
             flagged=B - accepted, mean_iterations=float(it.mean()),
             mean_factorizations=float(fac.mean()), mean_refinements=float(ref.mean()),
             newton_kernel_ms=k_s * 1e3, flops_fp64=float(flops64), flops_fp32=float(flops32),
             tflops=(flops64 + flops32) / k_s / 1e12,
+            peak_fraction=peak_s / k_s,
             fallback_rate=float(np.count_nonzero(fl & (A.SPX_FLAG_GMIN_STEPPING
                                                      | A.SPX_FLAG_SOURCE_STEPPING)) / B),
             fp64_escalations=int(np.count_nonzero(fl & A.SPX_FLAG_FP64_ESCALATED)),
@@ -354,38 +478,36 @@ def main():
             "ms_per_step": host_s * 1e3,
             "note": "spx_sweep_dcop with host result buffers (x, params, status, iterations, "
                     "flags, residual, counters, node statistics copied back per sweep)"}
+    # The Newton kernel's flops split by precision, each against its own
+    # published peak: the time the work would take at peak is
+    # f32 / 157.3 + f64 / 78.6 TFLOP/s, and frac is that time over the
+    # kernel's measured time (achieved / peak with peak the blend of the two).
+    traffic = measured_traffic()
+    f32, f64_ = h["flops_fp32"], h["flops_fp64"]
+    k_s = h["newton_kernel_ms"] * 1e-3
+    peak_s = f32 / (PEAK_TFLOPS["fp32"] * 1e12) + f64_ / (PEAK_TFLOPS["fp64"] * 1e12)
     roofline = {
-        "bound": "mfma",
-        "compute": "FP64/FP32 vector FMA (LU in VGPRs); packed FP32 disabled, so FP32 issues at "
-                   "the FP64 rate",
-        "achieved": h["tflops"], "peak": MI355X_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-        "frac": h["tflops"] / MI355X_FP64_PEAK_TFLOPS,
-        "traffic": MEASURED_TRAFFIC_BYTES.get(head) if args.batch == 65536 else None,
+        "bound": "valu",
+        "compute": "LU, refinement and residuals on vector FMA (VGPR-resident rows); packed "
+                   "FP32 is disabled, so FP32 issues at the FP64 vector rate",
+        "achieved": (f32 + f64_) / k_s / 1e12, "peak": (f32 + f64_) / peak_s / 1e12,
+        "unit": "TFLOP/s", "frac": peak_s / k_s,
+        "by_precision": {
+            "fp32": {"tflop": f32 / 1e12, "achieved": f32 / k_s / 1e12,
+                     "peak": PEAK_TFLOPS["fp32"], "frac": f32 / k_s / 1e12 / PEAK_TFLOPS["fp32"]},
+            "fp64": {"tflop": f64_ / 1e12, "achieved": f64_ / k_s / 1e12,
+                     "peak": PEAK_TFLOPS["fp64"], "frac": f64_ / k_s / 1e12 / PEAK_TFLOPS["fp64"]},
+        },
+        "traffic": traffic.get(head) if args.batch == 65536 else None,
+        "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
         "kernel": f"k_newton_dense<64,{'false' if head == 'fp64' else 'true'}>",
         "kernel_ms": h["newton_kernel_ms"],
     }
 
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-        S = min(args.cpu_sample, B)
-        params = buf["params"][:S].cpu().numpy()
-        threads = max(1, min(16, os.cpu_count() or 1))
-        # passes over the same instances until about --cpu-seconds of work
-        acc, passes, dt = 0, 0, 0.0
-        t0 = time.perf_counter()
-        while passes < 100 and (passes == 0 or dt < args.cpu_seconds):
-            out = pyoracle.sweep_dcop(ckt_py, params, A.default_nr_config(), nthreads=threads)
-            acc += int(np.count_nonzero((out["status"] == A.SPX_CONVERGED)
-                                        & ((out["flags"] & A.SPX_FLAG_KCL_FAIL) == 0)))
-            passes += 1
-            dt = time.perf_counter() - t0
-        cpu = {"value": acc / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-               "sample": f"{passes} passes over the first {S} of the {B} instances (same seeded "
-                         f"parameters), plain FP64 restated reference path (nalgebra-style LU, "
-                         f"NR + Gmin/source stepping), one task per instance on {threads} host "
-                         f"threads; {dt:.2f} s wall"}
+        cpu = cpu_baselines(args, A, CK, np, buf["params"].cpu().numpy(), ckt_py, B, report,
+                            solutions, warm_x0)
 
     workloads = None if args.no_workloads else other_workloads(E, A, CK, torch, np, dev, stream)
 

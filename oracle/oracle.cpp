@@ -9,6 +9,7 @@
 // 0.33.2 LU, faer 0.24.0 sparse LU) the crate's published algorithm is
 // restated and the deviation noted.
 #include "oracle.h"
+#include "sparse_lu.h"
 
 #include <atomic>
 #include <cmath>
@@ -374,7 +375,13 @@ struct Mna {
   int n, nn;
   double* J;
   double* rhs;
-  void add(int r, int c, double v) { J[(size_t)c * n + r] += v; }
+  std::vector<orc_sparse::Triplet>* trips;  // non-null: the triplet list instead of J
+  void add(int r, int c, double v) {
+    if (trips)
+      trips->push_back({r, c, v});
+    else
+      J[(size_t)c * n + r] += v;
+  }
   void conductance(int i, int j, double g) {  // mna.rs:87-98
     if (i >= 0) add(i, i, g);
     if (j >= 0) add(j, j, g);
@@ -393,6 +400,12 @@ struct Mna {
 };
 
 inline int idx(int32_t node) { return node - 1; }  // NodeId -> unknown, ground -> -1
+
+// Linear solver inside Newton and the transient: 0 the reference's rule
+// (SPARSE_THRESHOLD = 50, linear.rs:22: dense LU below, CachedSparseLu at and
+// above), 1 dense LU at every size.
+std::atomic<int> g_linear_mode{0};
+inline bool use_sparse(int n) { return g_linear_mode.load() == 0 && n >= 50; }
 inline double volt(const double* x, int32_t node) { return node > 0 ? x[node - 1] : 0.0; }
 inline double P(const double* params, int32_t slot) { return slot >= 0 ? params[slot] : 0.0; }
 
@@ -470,11 +483,15 @@ double eval_pwl(const double* params, int first, int np, double t) {
 
 void stamp_impl(const spx_circuit* c, const double* params, const double* x, double gmin,
                 double sf, int32_t limiter, double* J, double* rhs, double* vstate,
-                int* limited, const OTran* tc = nullptr) {
+                int* limited, const OTran* tc = nullptr,
+                std::vector<orc_sparse::Triplet>* trips = nullptr) {
   const int n = orc_circuit_size(c);
-  std::memset(J, 0, sizeof(double) * (size_t)n * n);
+  if (trips)
+    trips->clear();
+  else
+    std::memset(J, 0, sizeof(double) * (size_t)n * n);
   std::memset(rhs, 0, sizeof(double) * (size_t)n);
-  Mna m{n, c->num_nodes, J, rhs};
+  Mna m{n, c->num_nodes, J, rhs, trips};
   int k = 0;  // voltage-source branch counter
   for (int e = 0; e < c->num_elements; ++e) {
     const spx_element& el = c->elements[e];
@@ -655,6 +672,17 @@ void stamp_impl(const spx_circuit* c, const double* params, const double* x, dou
 
 }  // namespace
 
+extern "C" void orc_set_linear_solver(int32_t mode) { g_linear_mode.store(mode); }
+
+extern "C" int orc_sparse_lu_solve(int32_t n, int64_t nnz, const int32_t* rows,
+                                   const int32_t* cols, const double* vals, const double* b,
+                                   double* x) {
+  std::vector<orc_sparse::Triplet> t((size_t)nnz);
+  for (int64_t i = 0; i < nnz; ++i) t[i] = {rows[i], cols[i], vals[i]};
+  const orc_sparse::Csc a = orc_sparse::from_triplets(n, t);
+  return orc_sparse::solve(orc_sparse::analyse(a), a, b, x);
+}
+
 extern "C" void orc_stamp(const spx_circuit* c, const double* params, const double* x,
                           double gmin, double sf, int32_t limiter, double* J, double* rhs) {
   stamp_impl(c, params, x, gmin, sf, limiter, J, rhs, nullptr, nullptr);
@@ -696,14 +724,28 @@ namespace {
 int newton_impl(const spx_circuit* c, const double* params, const spx_nr_config* cfg, double gmin,
                 double sf, const double* x0, double* x, int32_t* iterations, const OTran* tc) {
   const int n = orc_circuit_size(c);
-  std::vector<double> J((size_t)n * n), rhs(n), xn(n), sol(n, 0.0);
+  // the linear solve by size, as solve_newton_raphson (newton.rs:92-108):
+  // dense LU below SPARSE_THRESHOLD, the cached sparse LU from the first
+  // iteration's pattern at and above it
+  const bool sparse = use_sparse(n);
+  std::vector<double> J(sparse ? 0 : (size_t)n * n), rhs(n), xn(n), sol(n, 0.0);
+  std::vector<orc_sparse::Triplet> trips;
+  orc_sparse::Symbolic sym;
   std::vector<double> vstate(c->num_elements, 0.0);  // pnjlim junction voltages
   if (x0) std::memcpy(sol.data(), x0, sizeof(double) * n);
   for (int it = 0; it < cfg->max_iterations; ++it) {
     int limited = 0;
     stamp_impl(c, params, sol.data(), gmin, sf, cfg->limiter, J.data(), rhs.data(),
-               vstate.data(), &limited, tc);
-    if (orc_lu_solve(n, J.data(), rhs.data(), xn.data())) {
+               vstate.data(), &limited, tc, sparse ? &trips : nullptr);
+    int singular;
+    if (sparse) {
+      const orc_sparse::Csc a = orc_sparse::from_triplets(n, trips);
+      if (it == 0) sym = orc_sparse::analyse(a);
+      singular = orc_sparse::solve(sym, a, rhs.data(), xn.data());
+    } else {
+      singular = orc_lu_solve(n, J.data(), rhs.data(), xn.data());
+    }
+    if (singular) {
       std::memcpy(x, sol.data(), sizeof(double) * n);
       *iterations = it + 1;
       return -1;
@@ -1208,12 +1250,20 @@ void cap_update_rule(const spx_circuit* c, const double* params, const double* x
 static int tran_solve(const spx_circuit* c, const double* params, const spx_nr_config* cfg,
                       bool nonlin, const OTran& tc, double* x, int32_t* it) {
   const int n = orc_circuit_size(c);
-  std::vector<double> J((size_t)n * n), rhs(n), xn(n);
-  if (!nonlin) {
+  std::vector<double> rhs(n), xn(n);
+  if (!nonlin) {  // solve_transient's solve by size (transient/solver.rs:131-141)
     std::vector<double> zero(n, 0.0);
+    *it = 1;
+    if (use_sparse(n)) {
+      std::vector<orc_sparse::Triplet> trips;
+      stamp_impl(c, params, zero.data(), 0.0, 1.0, cfg->limiter, nullptr, rhs.data(), nullptr,
+                 nullptr, &tc, &trips);
+      const orc_sparse::Csc a = orc_sparse::from_triplets(n, trips);
+      return orc_sparse::solve(orc_sparse::analyse(a), a, rhs.data(), x) ? -1 : 1;
+    }
+    std::vector<double> J((size_t)n * n);
     stamp_impl(c, params, zero.data(), 0.0, 1.0, cfg->limiter, J.data(), rhs.data(), nullptr,
                nullptr, &tc);
-    *it = 1;
     return orc_lu_solve(n, J.data(), rhs.data(), x) ? -1 : 1;
   }
   const int st = newton_impl(c, params, cfg, 0.0, 1.0, x, xn.data(), it, &tc);

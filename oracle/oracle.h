@@ -45,6 +45,15 @@ int orc_lu_solve(int32_t n, const double* a_colmajor, const double* b, double* x
 /* Its factorisation alone in LAPACK getrf format (L strictly below the
  * diagonal, U on and above, 0-based ipiv); returns 1 if singular (zeroed). */
 int orc_lu_factor(int32_t n, const double* a_colmajor, double* lu, int32_t* ipiv);
+/* Linear solver of the Newton and transient solves: 0 (default) the
+ * reference's size rule (dense LU for n < 50, the CachedSparseLu restatement
+ * of sparse_lu.h for n >= 50, linear.rs:22 / newton.rs:92-108), 1 dense LU
+ * at every size. */
+void orc_set_linear_solver(int32_t mode);
+/* CachedSparseLu::new + solve on triplets (duplicates summed); returns 1 if
+ * singular (x zeroed). */
+int orc_sparse_lu_solve(int32_t n, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                        const double* vals, const double* b, double* x);
 /* CpuBatchedSolver::solve_batch (solver.rs:327-385) plus the non-finite rule of
  * FaerBatchedSolver (faer_solver.rs:74-86). */
 int orc_solve_batch(const double* matrices, const double* rhs, int64_t n, int64_t batch,

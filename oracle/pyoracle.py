@@ -44,6 +44,8 @@ def lib():
         "orc_generate_params": (C.c_int, [C.POINTER(A.SpxSweepSpec), i32, vp]),
         "orc_lu_solve": (C.c_int, [i32, vp, vp, vp]),
         "orc_lu_factor": (C.c_int, [i32, vp, vp, vp]),
+        "orc_set_linear_solver": (None, [i32]),
+        "orc_sparse_lu_solve": (C.c_int, [i32, i64, vp, vp, vp, vp, vp]),
         "orc_solve_batch": (C.c_int, [vp, vp, i64, i64, vp, vp, C.POINTER(C.c_int64)]),
         "orc_thermal_voltage": (d, [d]),
         "orc_diode_limit": (d, [d, d]),
@@ -95,6 +97,22 @@ def lu_factor(a):
     piv = np.zeros(n, dtype=np.int32)
     s = lib().orc_lu_factor(n, _p(a), _p(lu), _p(piv))
     return lu, piv, bool(s)
+
+
+def sparse_lu_solve(n, rows, cols, vals, b):
+    """(x, singular) of the CachedSparseLu restatement on triplets."""
+    r = np.ascontiguousarray(rows, dtype=np.int32)
+    c = np.ascontiguousarray(cols, dtype=np.int32)
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(n)
+    s = lib().orc_sparse_lu_solve(n, r.size, _p(r), _p(c), _p(v), _p(b), _p(x))
+    return x, bool(s)
+
+
+def set_linear_solver(mode):
+    """0: the reference's size rule (sparse LU from n = 50), 1: dense LU always."""
+    lib().orc_set_linear_solver(int(mode))
 
 
 def solve_batch(matrices, rhs, n, batch):

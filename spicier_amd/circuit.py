@@ -64,6 +64,13 @@ class Circuit:
     def size(self) -> int:
         return self.num_nodes + self.num_vsources
 
+    def branch_unknown(self, element: int) -> int:
+        """Unknown index of a branch element's current (V, L, E, H sources;
+        branch rows follow the nodes in element order, mna.rs:124-145)."""
+        if self.elements[element][0] not in BRANCH_KINDS:
+            raise ValueError(f"element {element} has no branch current")
+        return self.num_nodes + sum(e[0] in BRANCH_KINDS for e in self.elements[:element])
+
     @property
     def num_params(self) -> int:
         return len(self.specs)

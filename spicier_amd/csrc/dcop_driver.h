@@ -23,7 +23,8 @@ __device__ __forceinline__ DcopOutcome solve_dcop(Inst& I, const DevProgram& g,
                                                   const DevNrCfg& cfg, bool warm,
                                                   int escalated_only) {
   int total = 0, it = 0, esc = 0, status = SPX_FAILED;
-  int flags = (warm ? SPX_FLAG_WARM_START : 0) | (escalated_only ? SPX_FLAG_FP64_ESCALATED : 0);
+  int flags = (warm ? SPX_FLAG_WARM_START : 0) |
+              (escalated_only == 1 ? SPX_FLAG_FP64_ESCALATED : 0);
   double final_gmin = 0.0;
   bool handover = false;  // FP32+IR pass: leave this instance to the FP64 pass
 

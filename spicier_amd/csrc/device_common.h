@@ -218,16 +218,34 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // mantissa bits — exact ties included — go to the smallest position, which is
 // nalgebra's first-maximum rule for exact ties.  Returns the winning position
 // (pos < 64) and whether the column held a non-zero value.
-//
-// FP64 is exact: when several eligible lanes share the winning truncated
-// key (equal to 14 mantissa bits, exact ties included, or every candidate
-// below 2^-1008), the tied lanes compare the remaining 38 bits of |a| and
-// then the position, in two more DPP max passes (the 6 remaining bits of the
-// high word with the top 26 of the low word; the low word's last 6 bits with
-// 63 - pos).  So the pivot is the strictly largest |a|, first position on
-// exact ties — nalgebra's rule — and a denormal pivot counts as non-zero.  The
-// extra passes run only on such ties (a wave-uniform branch).
 __device__ __forceinline__ int pivot_pos(double absval, bool eligible, int pos, bool& nonzero) {
+  const uint32_t hi = (uint32_t)((uint64_t)__double_as_longlong(absval) >> 32);
+  const uint32_t key = eligible ? ((hi & ~63u) | (uint32_t)(63 - pos)) : 0u;
+  const uint32_t mx = wave_max_u32(key);
+  nonzero = (mx >> 6) != 0;
+  return 63 - (int)(mx & 63u);
+}
+
+// 1 if this lane's candidate a could change the pivot pv that pivot_pos
+// chose under the exact rule: eligible, equal to |pv| in the compared bits
+// (sign cleared, the high word's 6 low bits dropped) but not in value.
+// Integer compares only, so that the caller accumulates the result in a VGPR
+// across the elimination without holding lane masks.
+__device__ __forceinline__ uint32_t pivot_inexact(double a, double pv, bool eligible) {
+  const uint64_t ba = (uint64_t)__double_as_longlong(a), bp = (uint64_t)__double_as_longlong(pv);
+  const uint32_t hi = (uint32_t)(ba >> 32) & 0x7fffffffu, hp = (uint32_t)(bp >> 32) & 0x7fffffffu;
+  const uint32_t lo = (uint32_t)ba, lp = (uint32_t)bp;
+  return (eligible && (hi ^ hp) < 64u && (hi != hp || lo != lp)) ? 1u : 0u;
+}
+
+// The exact rule: the strictly largest |a| over every bit, then the
+// smallest position.  After the truncated pass, lanes tied on its key
+// compare the remaining 38 bits of |a| (the high word's 6 low bits with the
+// low word's top 26, then the low word's last 6 bits with 63 - pos) in two
+// more DPP max passes, which run only on such ties (a wave-uniform branch).
+// A denormal pivot counts as non-zero.
+__device__ __forceinline__ int pivot_pos_exact(double absval, bool eligible, int pos,
+                                               bool& nonzero) {
   const uint64_t bits = (uint64_t)__double_as_longlong(absval);
   const uint32_t hi = (uint32_t)(bits >> 32), lo = (uint32_t)bits;
   const uint32_t key = eligible ? ((hi & ~63u) | (uint32_t)(63 - pos)) : 0u;

@@ -568,7 +568,7 @@ int sweep_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
     params = dst;
   }
   int* status = out->status;
-  if (!status && (out->node_stats || cfg->precision == SPX_PREC_F32_IR)) {
+  if (!status) {  // the dense kernels' second pass and the statistics read it
     void* p = nullptr;
     if ((rc = workspace(h, "status", sizeof(int) * B, &p))) return rc;
     status = static_cast<int*>(p);
@@ -870,8 +870,8 @@ int transient_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_swe
   DevOut o{out->x, out->status, out->iterations, out->flags, out->residual, nullptr, nullptr};
   DevTranOut to = make_dev_tran_out(*out, tran->adaptive != nullptr);
   DevNrCfg dc = to_dev(cfg);
-  if (!o.status && hp.n <= kDenseMaxN && dc.precision == SPX_PREC_F32_IR) {
-    void* st = nullptr;  // the FP64 hand-over pass reads the statuses
+  if (!o.status && hp.n <= kDenseMaxN) {
+    void* st = nullptr;  // the exact FP64 hand-over pass reads the statuses
     if ((rc = workspace(h, "tran_status_ir", sizeof(int) * B, &st))) return rc;
     o.status = static_cast<int*>(st);
   }
@@ -1094,8 +1094,8 @@ int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_
     mc.tr = make_dev_tran(&c.tran, static_cast<const int*>(dprobe));
     mc.o = DevOut{c.result.x, c.result.status, c.result.iterations, c.result.flags,
                   c.result.residual, nullptr, nullptr};
-    if (!mc.o.status && dc.precision == SPX_PREC_F32_IR) {
-      void* st = nullptr;  // the FP64 hand-over pass reads the statuses
+    if (!mc.o.status) {
+      void* st = nullptr;  // the exact FP64 hand-over pass reads the statuses
       if ((rc = workspace(h, ("mx_status" + tag).c_str(), sizeof(int) * std::max<int64_t>(1, mc.B),
                           &st)))
         return rc;

@@ -32,12 +32,19 @@ __global__ __launch_bounds__(64) void k_solve_rows(const double* __restrict__ M,
   const double* A = M + i * (int64_t)n * n;
   RowLU<N, double> lu;
   lu.row = row;
-  static_for<0, N>([&](auto J) __attribute__((always_inline)) {
-    constexpr int j = J;
-    lu.a[j] = (j < n && lane < n) ? A[(int64_t)j * n + lane] : (j == lane ? 1.0 : 0.0);
-  });
+  auto load = [&]() __attribute__((always_inline)) {
+    static_for<0, N>([&](auto J) __attribute__((always_inline)) {
+      constexpr int j = J;
+      lu.a[j] = (j < n && lane < n) ? A[(int64_t)j * n + lane] : (j == lane ? 1.0 : 0.0);
+    });
+  };
+  load();
   const double b = lane < n ? rhs[i * n + lane] : 0.0;
   bool ok = lu.factor(lane);
+  if (!ok && lu.inexact) {  // a near-tie: the exact pivot rule on the reloaded rows
+    load();
+    ok = lu.template factor<false, true>(lane);
+  }
   double xv = 0.0;
   if (ok) {
     xv = lu.template solve<true>(b, lane);
@@ -61,11 +68,18 @@ __global__ __launch_bounds__(64) void k_factor_rows(const double* __restrict__ M
   const double* A = M + i * (int64_t)n * n;
   RowLU<N, double> lu;
   lu.row = row;
-  static_for<0, N>([&](auto J) __attribute__((always_inline)) {
-    constexpr int j = J;
-    lu.a[j] = (j < n && lane < n) ? A[(int64_t)j * n + lane] : (j == lane ? 1.0 : 0.0);
-  });
+  auto load = [&]() __attribute__((always_inline)) {
+    static_for<0, N>([&](auto J) __attribute__((always_inline)) {
+      constexpr int j = J;
+      lu.a[j] = (j < n && lane < n) ? A[(int64_t)j * n + lane] : (j == lane ? 1.0 : 0.0);
+    });
+  };
+  load();
   bool ok = lu.template factor<true>(lane);
+  if (!ok && lu.inexact) {
+    load();
+    ok = lu.template factor<true, true>(lane);
+  }
   bool fin = true;
   static_for<0, N>([&](auto J) __attribute__((always_inline)) {
     constexpr int j = J;

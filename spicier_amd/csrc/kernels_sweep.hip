@@ -150,10 +150,15 @@ SPX_DECLARE_NEWTON(8, true, launch_newton_ir_8)
 SPX_DECLARE_NEWTON(16, true, launch_newton_ir_16)
 SPX_DECLARE_NEWTON(32, true, launch_newton_ir_32)
 SPX_DECLARE_NEWTON(64, true, launch_newton_ir_64)
+SPX_DECLARE_NEWTON(8, false, launch_newton_f64x_8)
+SPX_DECLARE_NEWTON(16, false, launch_newton_f64x_16)
+SPX_DECLARE_NEWTON(32, false, launch_newton_f64x_32)
+SPX_DECLARE_NEWTON(64, false, launch_newton_f64x_64)
 
-// FP32+IR runs in two passes: the FP32 kernel leaves instances it cannot
-// resolve at SPX_ACTIVE, and the FP64 kernel, launched over the same grid,
-// solves only those (every other workgroup exits on its first load).
+// Two passes over the same grid: the first (FP32+IR, or FP64 with the fast
+// pivot comparison) leaves every instance it cannot resolve at SPX_ACTIVE,
+// and the FP64 exact-pivot kernel solves only those (every other workgroup
+// exits on its first load).  Needs out.status.
 hipError_t launch_newton(const DevProgram& g, const DevNrCfg& cfg, const double* params,
                          const double* x0, int64_t B, const DevOut& out, hipStream_t st) {
   const bool ir = cfg.precision == SPX_PREC_F32_IR;
@@ -163,10 +168,12 @@ hipError_t launch_newton(const DevProgram& g, const DevNrCfg& cfg, const double*
          : g.n <= 32 ? launch_newton_f64_32 : launch_newton_f64_64;
   Fn f32 = g.n <= 8 ? launch_newton_ir_8 : g.n <= 16 ? launch_newton_ir_16
          : g.n <= 32 ? launch_newton_ir_32 : launch_newton_ir_64;
-  if (!ir) return f64(g, cfg, params, x0, B, out, 0, st);
-  hipError_t e = f32(g, cfg, params, x0, B, out, 0, st);
+  Fn exact = g.n <= 8 ? launch_newton_f64x_8 : g.n <= 16 ? launch_newton_f64x_16
+           : g.n <= 32 ? launch_newton_f64x_32 : launch_newton_f64x_64;
+  if (!out.status) return hipErrorInvalidValue;
+  hipError_t e = (ir ? f32 : f64)(g, cfg, params, x0, B, out, 0, st);
   if (e != hipSuccess) return e;
-  return f64(g, cfg, params, x0, B, out, 1, st);
+  return exact(g, cfg, params, x0, B, out, ir ? 1 : 2, st);
 }
 
 hipError_t launch_sample(const spx_param_spec* d_specs, int P, int gen, uint64_t seed, int64_t B,

@@ -11,9 +11,14 @@
 // row of step k as lane k, a compile-time constant.  The pivot of step k is
 // the largest |A[.][k]| over rows at positions >= k, smallest position on
 // exact ties — nalgebra's icamax rule (first maximum), which CpuBatchedSolver
-// inherits (spicier-batched-sweep src/solver.rs:370-380).  FP64 compares
-// every bit of |a| (pivot_pos); FP32 factors compare 17 mantissa bits, as
-// they only precondition FP64 refinement.  Multipliers are formed as a * (1/pivot) and
+// inherits (spicier-batched-sweep src/solver.rs:370-380).  The fast pass
+// compares 14 mantissa bits (FP64) or 17 (FP32) with the position; in FP64
+// it also notes whether a candidate equal to the pivot in those bits but not
+// in value existed (off the steps' critical path), and `inexact` then tells
+// the caller to reload the rows and factor again with Exact = true, which
+// compares every bit — so FP64 pivots follow the rule exactly.  FP32
+// factors keep the fast pass: they only precondition FP64 refinement.
+// Multipliers are formed as a * (1/pivot) and
 // updates as fma(-u, l, a).  The caller's workgroup is exactly one wave, so
 // __syncthreads() orders LDS traffic without a real barrier.
 //
@@ -102,19 +107,22 @@ struct RowLU {
   T dinv;      // 1 / U[lane][lane]
   T* row;      // N elements of LDS: broadcast buffer of the LDS variant
   int ipiv;    // factor<true>: lane k holds the position swapped with k at step k
+  bool inexact;  // FP64 fast pass: a near-tie needs factor<*, true> on reloaded rows
 
   // Factor in place.  Returns false if a pivot column is zero (or NaN): the
   // reference's LU then reports the system singular.  Every step runs over
   // all N columns with no per-column guard, so the update is straight-line
   // code; rows and columns past n are identity padding, whose multipliers
   // are zero.
-  template <bool Record = false>
+  template <bool Record = false, bool Exact = false>
   __device__ __forceinline__ bool factor(int lane) {
+    constexpr bool kF64 = std::is_same<T, double>::value;
     pos = lane;
     if constexpr (Record) ipiv = lane;
     dinv = T(0);
     piv = T(0);
     bool ok = true;
+    uint32_t nearby = 0;  // FP64 fast pass: a near-tie the exact rule might decide otherwise
     // One basic block for all N steps (no early exit, no data-dependent
     // branch), so the scheduler can start step k+1's pivot search while step
     // k's trailing updates are still in flight.  A zero pivot column clears
@@ -123,7 +131,12 @@ struct RowLU {
     static_for<0, N>([&](auto K) __attribute__((always_inline)) {
       constexpr int k = K;
       bool nz;
-      const int bp = __builtin_amdgcn_readfirstlane(pivot_pos(tabs(a[k]), pos >= k, pos, nz));
+      int bp0;
+      if constexpr (kF64 && Exact)
+        bp0 = pivot_pos_exact(tabs(a[k]), pos >= k, pos, nz);
+      else
+        bp0 = pivot_pos(tabs(a[k]), pos >= k, pos, nz);
+      const int bp = __builtin_amdgcn_readfirstlane(bp0);
       ok = ok && nz;
       if constexpr (Record) {
         if (lane == k) ipiv = bp;
@@ -141,6 +154,7 @@ struct RowLU {
         }
         __syncthreads();
         const T pv = row[k];
+        if constexpr (kF64 && !Exact) nearby |= pivot_inexact(a[k], pv, pos >= k);
         const T inv = pv != T(0) ? T(1) / pv : T(0);
         if (lane == p) {
           dinv = inv;
@@ -155,6 +169,7 @@ struct RowLU {
         __syncthreads();
       } else {
         const T pv = bcast(a[k], p);
+        if constexpr (kF64 && !Exact) nearby |= pivot_inexact(a[k], pv, pos >= k);
         const T inv = pv != T(0) ? T(1) / pv : T(0);
         if (lane == p) {
           dinv = inv;
@@ -168,6 +183,8 @@ struct RowLU {
         });
       }
     });
+    inexact = kF64 && !Exact && __any(nearby != 0u);
+    if (inexact) return false;  // the caller reloads the rows and runs factor<*, true>
     if (ok) {  // pos is a permutation of the 64 lanes: push every row home
       orig = push_to(lane, pos);
       static_for<0, N>([&](auto J) __attribute__((always_inline)) {

@@ -1,10 +1,11 @@
-// Newton kernel instantiation: systems of up to 32 unknowns, FP64 LU.
+// Newton kernel instantiation: systems of up to 32 unknowns, FP64 LU with the
+// fast pivot comparison (near-ties go to the exact pass, newton_f64x_32.hip).
 #include "newton_kernel.h"
 
 namespace spx {
 hipError_t launch_newton_f64_32(const DevProgram& g, const DevNrCfg& cfg, const double* params,
                               const double* x0, int64_t B, const DevOut& out, int escalated_only,
                               hipStream_t st) {
-  return launch_newton_impl<32, false>(g, cfg, params, x0, B, out, escalated_only, st);
+  return launch_newton_impl<32, false, false>(g, cfg, params, x0, B, out, escalated_only, st);
 }
 }  // namespace spx

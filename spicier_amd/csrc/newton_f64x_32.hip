@@ -1,0 +1,12 @@
+// Newton kernel instantiation: systems of up to 32 unknowns, FP64 LU with the
+// exact pivot rule — the pass that reruns the instances a first pass handed
+// over (FP32+IR escalations, FP64 near-tie pivots).
+#include "newton_kernel.h"
+
+namespace spx {
+hipError_t launch_newton_f64x_32(const DevProgram& g, const DevNrCfg& cfg, const double* params,
+                               const double* x0, int64_t B, const DevOut& out, int escalated_only,
+                               hipStream_t st) {
+  return launch_newton_impl<32, false, true>(g, cfg, params, x0, B, out, escalated_only, st);
+}
+}  // namespace spx

@@ -5,6 +5,6 @@ namespace spx {
 hipError_t launch_newton_ir_16(const DevProgram& g, const DevNrCfg& cfg, const double* params,
                               const double* x0, int64_t B, const DevOut& out, int escalated_only,
                               hipStream_t st) {
-  return launch_newton_impl<16, true>(g, cfg, params, x0, B, out, escalated_only, st);
+  return launch_newton_impl<16, true, false>(g, cfg, params, x0, B, out, escalated_only, st);
 }
 }  // namespace spx

@@ -34,7 +34,10 @@ inline size_t newton_smem_doubles(const DevProgram& g, int N) {
          row_buffer_doubles(N);
 }
 
-template <int N, bool IR>
+// Exact: the FP64 LU compares every bit of the pivot candidates (the exact
+// pass); otherwise the FP64 LU runs the fast comparison and hands an instance
+// whose pivot a near-tie could change to the exact pass.
+template <int N, bool IR, bool Exact = false>
 struct Instance {
   const DevProgram& g;
   const DevNrCfg& cfg;
@@ -127,15 +130,16 @@ struct Instance {
     return r;
   }
 
-  // x_new = J^{-1} rhs in FP64.  Returns false if singular.
-  __device__ __forceinline__ bool solve_f64(double& xnew) {
+  // x_new = J^{-1} rhs in FP64.  Returns 1 solved, 0 singular, 2 a near-tie
+  // the exact pivot rule must decide (the instance goes to the exact pass).
+  __device__ __forceinline__ int solve_f64(double& xnew) {
     ++nfact;
     RowLU<N, double> lu;
     lu.row = prow;
     load_rows(lu.a);
-    if (!lu.factor(lane)) return false;
+    if (!lu.template factor<false, Exact>(lane)) return (!Exact && lu.inexact) ? 2 : 0;
     xnew = lu.template solve<true>(lane < g.n ? rhs[lane] : 0.0, lane);
-    return __all(lane >= g.n || isfinite(xnew));
+    return __all(lane >= g.n || isfinite(xnew)) ? 1 : 0;
   }
 
   // FP32 factors + FP64 iterative refinement, starting from the current x.
@@ -203,7 +207,12 @@ struct Instance {
           return -2;
         }
       } else {
-        ok = solve_f64(xnew);
+        const int r = solve_f64(xnew);
+        if (r == 2) {  // a near-tie pivot: the exact pass reruns the instance
+          iters = it + 1;
+          return -2;
+        }
+        ok = r == 1;
       }
       if (!ok) {
         iters = it + 1;
@@ -267,7 +276,8 @@ struct Instance {
       }
       rc = r == 0 ? 0 : 2;
     } else {
-      rc = solve_f64(xn) ? 0 : 1;
+      const int r = solve_f64(xn);
+      rc = r == 1 ? 0 : (r == 2 ? 2 : 1);
     }
     if (lane < g.n) xs[lane] = rc == 0 ? xn : 0.0;
     __syncthreads();
@@ -366,13 +376,13 @@ struct Instance {
 
 namespace spx {
 
-// escalated_only: run only the instances an FP32+IR pass handed over (status
-// left SPX_ACTIVE), in FP64.
+// escalated_only: run only the instances a first pass handed over (status
+// left SPX_ACTIVE), in FP64 with the exact pivot rule (Exact).
 // Occupancy: the FP64 form at three waves per SIMD and the FP32+IR form at
 // four, measured best on the 64-unknown sweep (two and five were slower).  The
 // register budget this sets spills a few hundred bytes per wave outside the
 // LU's inner steps; latency hiding across the extra waves is worth more.
-template <int N, bool IR>
+template <int N, bool IR, bool Exact>
 __global__ __launch_bounds__(64, IR ? 4 : 3) void k_newton_dense(DevProgram g, DevNrCfg cfg,
                                                      const double* __restrict__ params,
                                                      const double* __restrict__ x0, int64_t B,
@@ -391,7 +401,7 @@ __global__ __launch_bounds__(64, IR ? 4 : 3) void k_newton_dense(DevProgram g, D
   double* pv = vals + g.num_vals;
   double* vst = pv + g.P;
   double* prow = vst + g.num_elems;
-  Instance<N, IR> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
+  Instance<N, IR, Exact> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
   const int n = g.n;
 
   for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
@@ -410,7 +420,7 @@ __global__ __launch_bounds__(64, IR ? 4 : 3) void k_newton_dense(DevProgram g, D
 // instance FP32 cannot resolve is left SPX_ACTIVE for the FP64 pass
 // (escalated_only), which reruns it from t = 0.  GENERAL: the trapezoidal /
 // TR-BDF2 / adaptive driver (run_transient<true>); otherwise Backward Euler.
-template <int N, bool IR, bool GENERAL>
+template <int N, bool IR, bool GENERAL, bool Exact>
 __global__ __launch_bounds__(64, IR ? 4 : 3) void k_transient_dense(DevProgram g, DevNrCfg cfg, DevTran tr,
                                                         const double* __restrict__ params,
                                                         int64_t B, DevOut out, DevTranOut tout,
@@ -432,7 +442,7 @@ __global__ __launch_bounds__(64, IR ? 4 : 3) void k_transient_dense(DevProgram g
   double* vcap = prow + row_buffer_doubles(N);
   double* icap = GENERAL ? vcap + g.num_elems : nullptr;
   double* vcap2 = GENERAL ? icap + g.num_elems : nullptr;
-  Instance<N, IR> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
+  Instance<N, IR, Exact> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
   TranCtx tc{0, 0.0, tr.h, vcap, icap, vcap2, 0, 0.0, 0.0, 0.0};
   I.tc = &tc;
   for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
@@ -446,37 +456,44 @@ __global__ __launch_bounds__(64, IR ? 4 : 3) void k_transient_dense(DevProgram g
   run_transient<GENERAL>(I, g, cfg, tr, tc, out, tout, inst);
 }
 
+// Two passes over the same grid: the first (FP32+IR, or FP64 with the fast
+// pivot comparison) leaves every instance it hands over at SPX_ACTIVE, and
+// the FP64 exact-pivot kernel reruns only those from t = 0 (every other
+// workgroup exits on its first load).  Needs out.status.
 template <int N, bool GENERAL>
 hipError_t launch_transient_t(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
                               const double* params, int64_t B, const DevOut& out,
                               const DevTranOut& tout, hipStream_t st, int phase) {
   size_t sm = sizeof(double) * (newton_smem_doubles(g, N) + (GENERAL ? 3 : 1) * g.num_elems);
   const bool ir = (phase & 4) != 0;
+  const void* first = ir ? (const void*)k_transient_dense<N, true, GENERAL, false>
+                         : (const void*)k_transient_dense<N, false, GENERAL, false>;
+  const void* exact = (const void*)k_transient_dense<N, false, GENERAL, true>;
   if (phase & 1) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_transient_dense<N, false, GENERAL>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    if (e == hipSuccess && ir)
-      e = hipFuncSetAttribute((const void*)k_transient_dense<N, true, GENERAL>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    hipError_t e = hipFuncSetAttribute(first, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(exact, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return e;
   }
   if (!(phase & 2) || B == 0) return hipSuccess;
-  if (ir) {
-    if (!out.status) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_transient_dense<N, true, GENERAL>), dim3((unsigned)B), dim3(kWave), sm,
-                       st, g, cfg, tr, params, B, out, tout, 0);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL((k_transient_dense<N, false, GENERAL>), dim3((unsigned)B), dim3(kWave), sm,
-                     st, g, cfg, tr, params, B, out, tout, ir ? 1 : 0);
+  if (!out.status) return hipErrorInvalidValue;
+  if (ir)
+    hipLaunchKernelGGL((k_transient_dense<N, true, GENERAL, false>), dim3((unsigned)B), dim3(kWave),
+                       sm, st, g, cfg, tr, params, B, out, tout, 0);
+  else
+    hipLaunchKernelGGL((k_transient_dense<N, false, GENERAL, false>), dim3((unsigned)B),
+                       dim3(kWave), sm, st, g, cfg, tr, params, B, out, tout, 0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_transient_dense<N, false, GENERAL, true>), dim3((unsigned)B), dim3(kWave),
+                     sm, st, g, cfg, tr, params, B, out, tout, 1);
   return hipGetLastError();
 }
 
 // phase bit 1: set the kernels' LDS attribute (not allowed while a stream is
 // being captured into a graph); bit 2: launch; bit 4: the FP32+IR form (the
-// FP32 kernel, then the FP64 pass over the instances it handed over; needs
-// out.status).  Backward-Euler fixed steps of circuits without inductors take
+// FP32 kernel, then the FP64 pass over the instances it handed over);
+// without it the fast FP64 kernel, then the exact-pivot pass.  Backward-Euler fixed steps of circuits without inductors take
 // the lean kernels.
 template <int N>
 hipError_t launch_transient_impl(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
@@ -487,16 +504,20 @@ hipError_t launch_transient_impl(const DevProgram& g, const DevNrCfg& cfg, const
   return launch_transient_t<N, false>(g, cfg, tr, params, B, out, tout, st, phase);
 }
 
-template <int N, bool IR>
+// escalated_only: 0 every instance; 1 the instances an FP32+IR pass handed
+// over (flagged SPX_FLAG_FP64_ESCALATED); 2 those a fast FP64 pass handed
+// over on a near-tie pivot (not flagged).  Only the exact kernel runs the
+// second passes.
+template <int N, bool IR, bool Exact>
 hipError_t launch_newton_impl(const DevProgram& g, const DevNrCfg& cfg, const double* params,
                               const double* x0, int64_t B, const DevOut& out,
                               int escalated_only, hipStream_t st) {
   size_t sm = sizeof(double) * newton_smem_doubles(g, N);
-  hipError_t e = hipFuncSetAttribute((const void*)k_newton_dense<N, IR>,
+  hipError_t e = hipFuncSetAttribute((const void*)k_newton_dense<N, IR, Exact>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_newton_dense<N, IR>), dim3((unsigned)B), dim3(kWave), sm, st, g, cfg,
-                     params, x0, B, out, escalated_only);
+  hipLaunchKernelGGL((k_newton_dense<N, IR, Exact>), dim3((unsigned)B), dim3(kWave), sm, st, g,
+                     cfg, params, x0, B, out, escalated_only);
   return hipGetLastError();
 }
 

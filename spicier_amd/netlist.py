@@ -77,6 +77,28 @@ class ParsedNetlist:
     # K couplings (name, inductors, k): read and checked but without effect
     # here, as in the reference's DC and transient analyses
     couplings: list = field(default_factory=list)
+    # element name (upper case) -> index in circuit.elements (a T line: its
+    # last section's capacitor)
+    elements: dict = field(default_factory=dict)
+
+    def unknown(self, quantity: str) -> int:
+        """Unknown index of a SPICE output name: V(node) or I(source) of a V,
+        L, E or H element (MNA branch current, ngspice's sign)."""
+        q = quantity.strip()
+        kind, arg = q[0].upper(), q[2:-1].strip()
+        if q[1] != "(" or q[-1] != ")" or kind not in "VI":
+            raise InvalidArgument(f"expected V(node) or I(source), got {quantity!r}")
+        if kind == "V":
+            if arg not in self.circuit.node_names or self.circuit.node_names[arg] == 0:
+                raise InvalidArgument(f"no node {arg!r}")
+            return self.circuit.node_names[arg] - 1
+        i = self.elements.get(arg.upper())
+        if i is None:
+            raise InvalidArgument(f"no element {arg!r}")
+        try:
+            return self.circuit.branch_unknown(i)
+        except ValueError:
+            raise InvalidArgument(f"{arg} carries no branch current") from None
 
 
 # ------------------------------------------------------------------ lines
@@ -585,4 +607,4 @@ def parse_netlist(text: str, name: str | None = None) -> ParsedNetlist:
         raise InvalidArgument(f"node numbers {missing} have no element (every system would be "
                               "singular)")
     c.node_names = {"0": 0, **names}
-    return ParsedNetlist(c, title, tran, dict(scope.params), couplings)
+    return ParsedNetlist(c, title, tran, dict(scope.params), couplings, dict(by_name))

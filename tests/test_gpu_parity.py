@@ -220,6 +220,7 @@ def test_full_size_headline_properties():
     same = f64.iterations == ir.iterations
     assert same.mean() > 0.999
     assert np.all(close(ir.solutions[same], f64.solutions[same], rtol=1e-9, atol=1e-12))
+    assert np.all(close(ir.solutions[~same], f64.solutions[~same], rtol=1e-3, atol=1e-6))
     np.testing.assert_array_equal(f64.node_stats[:, 0], B)
     assert np.allclose(f64.node_stats[:, 1], f64.solutions.sum(0), rtol=1e-12)
 

@@ -105,9 +105,14 @@ def test_mixed_ir_matches_oracle(oracle):
                                 A.default_nr_config(limiter=A.SPX_LIMIT_PNJLIM), c.tstep,
                                 c.steps, probes=c.probes, nthreads=16)
         np.testing.assert_array_equal(got.status, want["status"])
+        # every instance is compared: those whose Newton counts equal the
+        # oracle's to 1e-9, any other (a borderline dx test on an FP32+IR
+        # solve) at the SPICE tolerance, reltol 1e-3 and vntol 1 uV
         same = got.iterations == want["iterations"]
-        assert same.mean() >= 0.9
+        assert same.mean() >= 0.999, (c.circuit.name, np.count_nonzero(~same))
         assert _close(got.waveforms[same], want["waveforms"][same])
+        assert np.all(np.abs(got.waveforms[~same] - want["waveforms"][~same])
+                      <= 1e-3 * np.abs(want["waveforms"][~same]) + 1e-6)
         assert np.all(got.accepted_mask())
     for rep in range(2):
         for c in cls:

@@ -110,9 +110,11 @@ def test_trained_surrogate_saves_newton_iterations(oracle):
                              x0=x0, nthreads=8)
     np.testing.assert_array_equal(warm.status, want["status"])
     same = warm.iterations == want["iterations"]
-    assert same.mean() > 0.99
+    assert same.mean() >= 0.999, np.count_nonzero(~same)
     err = np.abs(warm.solutions - want["x"])
     assert np.all(err[same] <= np.maximum(1e-12, 1e-9 * np.abs(want["x"][same])))
+    # any other instance (a borderline dx test) at the SPICE tolerance
+    assert np.all(err[~same] <= 1e-3 * np.abs(want["x"][~same]) + 1e-6)
     # and the warm-started result agrees with the zero-start one at SPICE tolerance
     assert np.all(np.abs(warm.solutions - base.solutions)
                   <= 2e-3 * np.abs(base.solutions) + 2e-6)

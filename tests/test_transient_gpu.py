@@ -102,8 +102,11 @@ def test_ladder_transient_ir_matches_fp64_oracle(oracle, n):
     np.testing.assert_array_equal(got.fail_step, want["fail_step"])
     np.testing.assert_array_equal(got.flags & ~A.SPX_FLAG_FP64_ESCALATED, want["flags"])
     same = got.iterations == want["iterations"]
-    assert same.mean() >= 0.95, (n, same.mean())
+    assert same.mean() >= 0.999, (n, np.count_nonzero(~same))
     ok = close(got.waveforms[same], want["waveforms"][same])
     assert np.all(ok), (n, np.abs(got.waveforms[same] - want["waveforms"][same]).max())
     assert np.all(close(got.solutions[same], want["x"][same]))
+    # any other instance (a borderline dx test at some step) at the SPICE tolerance
+    assert np.all(np.abs(got.waveforms[~same] - want["waveforms"][~same])
+                  <= 1e-3 * np.abs(want["waveforms"][~same]) + 1e-6)
     assert np.all(got.accepted_mask())

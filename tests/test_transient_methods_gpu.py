@@ -4,9 +4,12 @@ oracle's restatement on identical seeded parameters, through the C ABI, on
 every kernel kind (register LU, shared-symbolic sparse with 64 and 256
 threads, one lane per instance).  Bar: waveforms, times and final
 solutions within 1e-9 relative (1e-12 absolute for waveforms); status, flags, iteration
-counts, point counts and step counts equal.  Adaptive grids branch on LTE
-comparisons, so an instance whose grid differs from the oracle's by a
-rounding-level decision is allowed (at most 5 % of instances)."""
+counts, point counts and step counts equal.  Every instance is compared.
+Adaptive grids branch on LTE comparisons, and FP32+IR Newton solves on dx
+tests, so an instance whose grid or iteration count differs from the
+oracle's by a rounding-level decision (at most 0.1 % of instances) is
+compared at the SPICE tolerance (reltol 1e-3, vntol 1 uV) instead: its final
+solution, and its waveforms interpolated onto the oracle's time grid."""
 import math
 
 import numpy as np
@@ -45,11 +48,23 @@ def _check_fixed(got, want):
     assert np.all(close(got.solutions, want["x"]))
 
 
+def spice_close(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b) <= 1e-3 * np.abs(b) + 1e-6
+
+
 def _check_adaptive(got, want):
     np.testing.assert_array_equal(got.status, want["status"])
     same = ((got.num_points == want["num_points"]) & (got.total_steps == want["total_steps"])
             & (got.rejected_steps == want["rejected_steps"]))
-    assert same.mean() >= 0.95, same.mean()
+    assert same.mean() >= 0.999, np.count_nonzero(~same)
+    for i in np.flatnonzero(~same):  # another grid: compared at the SPICE tolerance
+        n, m = got.num_points[i], want["num_points"][i]
+        assert got.point_times[i, n - 1] == want["times"][i, m - 1]
+        assert np.all(spice_close(got.solutions[i], want["x"][i])), i
+        for p in range(got.waveforms.shape[2]):
+            w = np.interp(want["times"][i, :m], got.point_times[i, :n], got.waveforms[i, :n, p])
+            assert np.all(spice_close(w, want["waveforms"][i, :m, p])), (i, p)
     for i in np.flatnonzero(same):
         n = got.num_points[i]
         # step sizes scaled by sqrt(tol/lte) carry the LTE's rounding (the
@@ -186,8 +201,9 @@ def test_methods_ir_match_fp64_oracle(oracle, method):
                                nthreads=16)
     np.testing.assert_array_equal(got.status, want["status"])
     same = got.iterations == want["iterations"]
-    assert same.mean() >= 0.95
+    assert same.mean() >= 0.999, np.count_nonzero(~same)
     assert np.all(close(got.waveforms[same], want["waveforms"][same]))
+    assert np.all(spice_close(got.waveforms[~same], want["waveforms"][~same]))
 
 
 def test_invalid_transient_settings():
