@@ -25,7 +25,13 @@ FLAGS = [
 
 
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    """Every HIP source, the slowest to compile (the 64- and 32-unknown
+    register-LU kernels) first so that the pool's last jobs are short."""
+    def cost(f):
+        heavy = ("newton" in f or "transient" in f) and ("_64" in f or "_32" in f)
+        return (0 if heavy else 1, 0 if "_64" in f else 1, f)
+    return [os.path.join(CSRC, f) for f in sorted(
+        (f for f in os.listdir(CSRC) if f.endswith(".hip")), key=cost)]
 
 
 def _headers():

@@ -59,16 +59,18 @@ __device__ __forceinline__ double eval_pwl(const double* pv, int first, int np, 
 //
 // ST is the stride of every per-instance array (xs, pv, vals, vst and the
 // transient's vprev): 1 when one workgroup owns an instance, 64 when the 64
-// lanes of a wave are 64 instances with their values interleaved.
-template <int ST = 1>
-__device__ __forceinline__ void eval_element(const DevProgram& g, int e, const double* xs,
-                                             const double* pv, double* vals, double* vst,
-                                             double sf, int limiter, bool& lim,
-                                             const TranCtx* tc = nullptr) {
-  const int* el = g.elems + e * kElemInts;
-  const int kind = el[0], vid = el[11];
+// lanes of a wave are 64 instances with their values interleaved.  ES is
+// the stride between the element record's fields: 1 in elems, 64 in the
+// lane-major el_lm.
+template <int ST = 1, int ES = 1>
+__device__ __forceinline__ void eval_element_at(const int* rec, int e, const double* xs,
+                                                const double* pv, double* vals, double* vst,
+                                                double sf, int limiter, bool& lim,
+                                                const TranCtx* tc = nullptr) {
+  auto F = [&](int k) { return rec[k * ES]; };
+  const int kind = F(0), vid = F(11);
   auto X = [&](int u) { return u >= 0 ? xs[u * ST] : 0.0; };
-  auto PV = [&](int k) { return pv[el[k] * ST]; };
+  auto PV = [&](int k) { return pv[F(k) * ST]; };
   switch (kind) {
     case SPX_EL_RESISTOR:
       vals[vid * ST] = 1.0 / PV(4);
@@ -116,9 +118,9 @@ __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const d
         vals[(vid + 2) * ST] = 0.0;
       }
       break;
-    case SPX_EL_VPWL:  // el[3]: number of points, el[4]: the slot of t1
-      vals[vid * ST] = ((tc && tc->mode == 1) ? eval_pwl<ST>(pv, el[4], el[3], tc->t)
-                                              : pv[(el[4] + 1) * ST]) *
+    case SPX_EL_VPWL:  // field 3: number of points, field 4: the slot of t1
+      vals[vid * ST] = ((tc && tc->mode == 1) ? eval_pwl<ST>(pv, F(4), F(3), tc->t)
+                                              : pv[(F(4) + 1) * ST]) *
                        sf;
       break;
     case SPX_EL_VSIN:
@@ -145,7 +147,7 @@ __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const d
       vals[vid * ST] = PV(4);
       break;
     case SPX_EL_DIODE: {
-      double vd = X(el[1]) - X(el[2]);
+      double vd = X(F(1)) - X(F(2));
       double gd, ieq;
       diode_companion(PV(4), PV(5), vd, limiter, vst[e * ST], lim, gd, ieq);
       vals[vid * ST] = gd;
@@ -154,7 +156,7 @@ __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const d
     }
     case SPX_EL_NMOS:
     case SPX_EL_PMOS: {
-      double vd = X(el[1]), vg = X(el[2]), vs = X(el[3]);
+      double vd = X(F(1)), vg = X(F(2)), vs = X(F(3));
       double gds, gm, ieq;
       mosfet_companion(kind == SPX_EL_PMOS, PV(4), PV(5), PV(6), PV(7), PV(8), vg - vs, vd - vs,
                        gds, gm, ieq);
@@ -165,6 +167,39 @@ __device__ __forceinline__ void eval_element(const DevProgram& g, int e, const d
     }
     default:
       break;
+  }
+}
+
+template <int ST = 1>
+__device__ __forceinline__ void eval_element(const DevProgram& g, int e, const double* xs,
+                                             const double* pv, double* vals, double* vst,
+                                             double sf, int limiter, bool& lim,
+                                             const TranCtx* tc = nullptr) {
+  eval_element_at<ST, 1>(g.elems + e * kElemInts, e, xs, pv, vals, vst, sf, limiter, lim, tc);
+}
+
+// The pattern and rhs sums of one 64-lane wave from the lane-major gather
+// program (program.h): the same contributions in the same order as
+// gather_sum, with the op loads independent of each other.
+__device__ __forceinline__ void gather_lanes(const DevProgram& g, int lane, double* J,
+                                             double* rhs, const double* vals) {
+  double acc = 0.0;
+  const int* ops = g.gath + lane;
+#pragma unroll 4
+  for (int k = 0; k < g.gath_k; ++k) {
+    const int op = ops[k * 64];
+    if (op != -1) {
+      const double v = vals[op & 0x3fff];
+      acc = (op & 0x4000) ? acc - v : acc + v;
+      if (op & 0x8000) {
+        const unsigned d = (unsigned)op >> 16;
+        if (d & 0x8000u)
+          rhs[d & 0x7fffu] = acc;
+        else
+          J[d] = acc;
+        acc = 0.0;
+      }
+    }
   }
 }
 

@@ -369,6 +369,55 @@ int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g, DevSpars
   size_t o_el = put(hp.elems), o_or = put(hp.order), o_rc = put(hp.pos_rc),
          o_po = put(hp.pos_off), o_co = put(hp.contrib), o_ro = put(hp.rhs_off),
          o_rh = put(hp.rhs_contrib), o_rp = put(hp.row_ptr);
+  // lane-major element table and gather program of one 64-lane wave
+  const int rounds = (hp.num_elems + 63) / 64;
+  std::vector<int> el_lm((size_t)rounds * kElemLm * 64, 0);
+  for (int t = 0; t < rounds; ++t)
+    for (int l = 0; l < 64; ++l) {
+      const int i = t * 64 + l;
+      int* dst = el_lm.data() + (size_t)t * kElemLm * 64 + l;
+      if (i >= hp.num_elems) {
+        dst[0] = -1;
+        continue;
+      }
+      const int e = hp.order[i];
+      for (int f = 0; f < kElemInts; ++f) dst[f * 64] = hp.elems[(size_t)e * kElemInts + f];
+      dst[kElemInts * 64] = e;
+    }
+  const int num_pos = (int)hp.pos_rc.size();
+  std::vector<std::vector<int>> lane_ops(64);
+  bool fits = hp.num_vals <= 0x3fff && num_pos < 0x8000 && hp.n < 0x8000;
+  auto emit = [&](int lane, const int* c0, const int* c1, int dst) {
+    auto& ops = lane_ops[lane];
+    if (c0 == c1) {  // an empty sum: +1 - 1 leaves +0.0, as the loop over nothing
+      ops.push_back(kValOne);
+      ops.push_back(kValOne | 1 << 14 | 1 << 15 | dst << 16);
+      return;
+    }
+    for (const int* c = c0; c < c1; ++c) {
+      const int slot = *c >> 1, neg = *c & 1;
+      fits = fits && slot <= 0x3fff;
+      ops.push_back(slot | neg << 14 | (c + 1 == c1 ? 1 << 15 : 0) | dst << 16);
+    }
+  };
+  for (int q = 0; q < num_pos && fits; ++q)
+    emit(q % 64, hp.contrib.data() + hp.pos_off[q], hp.contrib.data() + hp.pos_off[q + 1], q);
+  for (int r = 0; r < hp.n && fits; ++r)
+    emit(r % 64, hp.rhs_contrib.data() + hp.rhs_off[r], hp.rhs_contrib.data() + hp.rhs_off[r + 1],
+         0x8000 | r);
+  size_t gk = 0;
+  for (auto& v : lane_ops) gk = std::max(gk, v.size());
+  if (!fits) gk = 0;
+  std::vector<int> gath(std::max<size_t>(1, gk * 64), -1);
+  for (int l = 0; l < 64 && fits; ++l)
+    for (size_t k = 0; k < lane_ops[l].size(); ++k) gath[k * 64 + l] = lane_ops[l][k];
+  std::vector<int> rowmask(2 * (size_t)hp.n, 0);
+  for (int r = 0; r < hp.n && r < 64; ++r)
+    for (int q = hp.row_ptr[r]; q < hp.row_ptr[r + 1]; ++q) {
+      const int col = hp.pos_rc[q] >> 16;
+      if (col < 64) rowmask[2 * r + (col >> 5)] |= (int)(1u << (col & 31));
+    }
+  size_t o_lm = put(el_lm), o_ga = put(gath), o_rm = put(rowmask);
   // A sweep loop over one circuit uploads its program once: the upload (and
   // the stream wait its pageable source needs) is skipped while the program
   // in the workspace is byte-identical.
@@ -423,6 +472,11 @@ int upload_program(spx_handle* h, const HostProgram& hp, DevProgram& g, DevSpars
   g.contrib = base + o_co;
   g.rhs_off = base + o_ro;
   g.rhs_contrib = base + o_rh;
+  g.el_rounds = rounds;
+  g.el_lm = base + o_lm;
+  g.gath_k = (int)gk;
+  g.gath = base + o_ga;
+  g.rowmask = base + o_rm;
   return SPX_OK;
 }
 

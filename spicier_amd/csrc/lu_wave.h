@@ -24,6 +24,7 @@
 //
 // Lanes >= n carry identity rows (a[j] = [j == lane]) so that padding never
 // wins a pivot search and solves to zero.
+
 #pragma once
 
 #include <type_traits>
@@ -123,6 +124,7 @@ struct RowLU {
     piv = T(0);
     bool ok = true;
     uint32_t nearby = 0;  // FP64 fast pass: a near-tie the exact rule might decide otherwise
+
     // One basic block for all N steps (no early exit, no data-dependent
     // branch), so the scheduler can start step k+1's pivot search while step
     // k's trailing updates are still in flight.  A zero pivot column clears
@@ -170,7 +172,13 @@ struct RowLU {
       } else {
         const T pv = bcast(a[k], p);
         if constexpr (kF64 && !Exact) nearby |= pivot_inexact(a[k], pv, pos >= k);
-        const T inv = pv != T(0) ? T(1) / pv : T(0);
+        // FP32 factors only precondition FP64 refinement: the hardware
+        // reciprocal (1 ulp) serves; FP64 keeps the correctly rounded 1/pivot
+        T inv;
+        if constexpr (kF64)
+          inv = pv != T(0) ? T(1) / pv : T(0);
+        else
+          inv = pv != T(0) ? __builtin_amdgcn_rcpf(pv) : T(0);
         if (lane == p) {
           dinv = inv;
           piv = pv;

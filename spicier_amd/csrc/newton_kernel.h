@@ -66,31 +66,44 @@ struct Instance {
 
   __device__ __forceinline__ void set_reuse(bool v) { reuse32 = v; }
 
-  // after jc is in LDS; publishes the row masks (a barrier follows)
+  // publishes the row masks (a barrier follows)
   __device__ __forceinline__ void init_row() {
     lu32.row = reinterpret_cast<float*>(prow);
-    uint64_t m = 0;
+    uint32_t m0 = 0, m1 = 0;
     if (lane < g.n) {
       q0 = g.row_ptr[lane];
       q1 = g.row_ptr[lane + 1];
-      for (int q = q0; q < q1; ++q) m |= 1ull << jc[q];
+      m0 = (uint32_t)g.rowmask[2 * lane];
+      m1 = (uint32_t)g.rowmask[2 * lane + 1];
     }
     if (lane < N) {
-      rmask[2 * lane] = (uint32_t)m;
-      rmask[2 * lane + 1] = (uint32_t)(m >> 32);
+      rmask[2 * lane] = m0;
+      rmask[2 * lane + 1] = m1;
     }
   }
 
   // device evaluation + pattern assembly at xs; returns true if a pnjlim
   // limiter acted on any diode
+  // from the lane-major tables: coalesced element loads, independent gather
+  // ops (the same sums in the same order as the pos_off / contrib tables)
   __device__ __forceinline__ bool assemble(double gmin, double sf, int limiter) {
     bool lim = false;
-    for (int i = lane; i < g.num_elems; i += kWave)
-      eval_element(g, g.order[i], xs, pv, vals, vst, sf, limiter, lim, tc);
+    for (int t = 0; t < g.el_rounds; ++t) {
+      const int* rec = g.el_lm + t * kElemLm * kWave + lane;
+      if (rec[0] >= 0)
+        eval_element_at<1, kWave>(rec, rec[kElemInts * kWave], xs, pv, vals, vst, sf, limiter,
+                                  lim, tc);
+    }
     if (lane == 0) vals[kValGmin] = gmin;
     __syncthreads();
-    for (int q = lane; q < g.num_pos; q += kWave) J[q] = gather_sum(g.pos_off, g.contrib, q, vals);
-    for (int r = lane; r < g.n; r += kWave) rhs[r] = gather_sum(g.rhs_off, g.rhs_contrib, r, vals);
+    if (g.gath_k > 0) {
+      gather_lanes(g, lane, J, rhs, vals);
+    } else {
+      for (int q = lane; q < g.num_pos; q += kWave)
+        J[q] = gather_sum(g.pos_off, g.contrib, q, vals);
+      for (int r = lane; r < g.n; r += kWave)
+        rhs[r] = gather_sum(g.rhs_off, g.rhs_contrib, r, vals);
+    }
     __syncthreads();
     return __any(lim);
   }
@@ -456,58 +469,65 @@ __global__ __launch_bounds__(64, IR ? 4 : 3) void k_transient_dense(DevProgram g
   run_transient<GENERAL>(I, g, cfg, tr, tc, out, tout, inst);
 }
 
-// Two passes over the same grid: the first (FP32+IR, or FP64 with the fast
-// pivot comparison) leaves every instance it hands over at SPX_ACTIVE, and
-// the FP64 exact-pivot kernel reruns only those from t = 0 (every other
-// workgroup exits on its first load).  Needs out.status.
-template <int N, bool GENERAL>
-hipError_t launch_transient_t(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
-                              const double* params, int64_t B, const DevOut& out,
-                              const DevTranOut& tout, hipStream_t st, int phase) {
-  size_t sm = sizeof(double) * (newton_smem_doubles(g, N) + (GENERAL ? 3 : 1) * g.num_elems);
-  const bool ir = (phase & 4) != 0;
-  const void* first = ir ? (const void*)k_transient_dense<N, true, GENERAL, false>
-                         : (const void*)k_transient_dense<N, false, GENERAL, false>;
-  const void* exact = (const void*)k_transient_dense<N, false, GENERAL, true>;
+// One dense transient kernel variant: op 1 sets its LDS attribute (not
+// allowed while a stream is being captured into a graph), op 2 launches it.
+template <int N, bool IR, bool GENERAL, bool Exact>
+hipError_t transient_variant(int op, const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
+                             const double* params, int64_t B, const DevOut& out,
+                             const DevTranOut& tout, hipStream_t st, int escalated_only) {
+  const size_t sm =
+      sizeof(double) * (newton_smem_doubles(g, N) + (GENERAL ? 3 : 1) * g.num_elems);
+  if (op == 1)
+    return hipFuncSetAttribute((const void*)k_transient_dense<N, IR, GENERAL, Exact>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+  hipLaunchKernelGGL((k_transient_dense<N, IR, GENERAL, Exact>), dim3((unsigned)B), dim3(kWave),
+                     sm, st, g, cfg, tr, params, B, out, tout, escalated_only);
+  return hipGetLastError();
+}
+
+typedef hipError_t (*TranVariantFn)(int, const DevProgram&, const DevNrCfg&, const DevTran&,
+                                    const double*, int64_t, const DevOut&, const DevTranOut&,
+                                    hipStream_t, int);
+
+// A variant as a plain function, so that each can be compiled in a file of
+// its own (the 64-column register LU makes each variant slow to compile).
+#define SPX_TRAN_VARIANT_FN(NAME, N, IR, GENERAL, EXACT)                                    \
+  hipError_t NAME(int op, const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,      \
+                  const double* params, int64_t B, const DevOut& out, const DevTranOut& tout, \
+                  hipStream_t st, int esc) {                                                 \
+    return transient_variant<N, IR, GENERAL, EXACT>(op, g, cfg, tr, params, B, out, tout, st, \
+                                                    esc);                                    \
+  }
+#define SPX_TRAN_VARIANT_DECL(NAME)                                                         \
+  hipError_t NAME(int, const DevProgram&, const DevNrCfg&, const DevTran&, const double*,   \
+                  int64_t, const DevOut&, const DevTranOut&, hipStream_t, int);
+
+// The transient of one size class in two passes over the same grid: the
+// first kernel (FP32+IR, or FP64 with the fast pivot comparison) leaves every
+// instance it hands over at SPX_ACTIVE, and the FP64 exact-pivot kernel
+// reruns only those from t = 0 (every other workgroup exits on its first
+// load); needs out.status.  v[general][0 FP64, 1 FP32+IR, 2 exact].
+// phase bit 1: set the kernels' LDS attributes; bit 2: launch; bit 4: the
+// FP32+IR form.  Backward-Euler fixed steps of circuits without inductors
+// take the lean (general = 0) kernels.
+inline hipError_t launch_transient_variants(const TranVariantFn (&v)[2][3], const DevProgram& g,
+                                            const DevNrCfg& cfg, const DevTran& tr,
+                                            const double* params, int64_t B, const DevOut& out,
+                                            const DevTranOut& tout, hipStream_t st, int phase) {
+  const int general = (tr.adaptive || tr.method != SPX_TRAN_BE || g.has_inductor) ? 1 : 0;
+  const TranVariantFn first = v[general][(phase & 4) ? 1 : 0], exact = v[general][2];
   if (phase & 1) {
-    hipError_t e = hipFuncSetAttribute(first, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-    if (e == hipSuccess)
-      e = hipFuncSetAttribute(exact, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    hipError_t e = first(1, g, cfg, tr, params, B, out, tout, st, 0);
+    if (e == hipSuccess) e = exact(1, g, cfg, tr, params, B, out, tout, st, 0);
     if (e != hipSuccess) return e;
   }
   if (!(phase & 2) || B == 0) return hipSuccess;
   if (!out.status) return hipErrorInvalidValue;
-  if (ir)
-    hipLaunchKernelGGL((k_transient_dense<N, true, GENERAL, false>), dim3((unsigned)B), dim3(kWave),
-                       sm, st, g, cfg, tr, params, B, out, tout, 0);
-  else
-    hipLaunchKernelGGL((k_transient_dense<N, false, GENERAL, false>), dim3((unsigned)B),
-                       dim3(kWave), sm, st, g, cfg, tr, params, B, out, tout, 0);
-  hipError_t e = hipGetLastError();
+  hipError_t e = first(2, g, cfg, tr, params, B, out, tout, st, 0);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_transient_dense<N, false, GENERAL, true>), dim3((unsigned)B), dim3(kWave),
-                     sm, st, g, cfg, tr, params, B, out, tout, 1);
-  return hipGetLastError();
+  return exact(2, g, cfg, tr, params, B, out, tout, st, 1);
 }
 
-// phase bit 1: set the kernels' LDS attribute (not allowed while a stream is
-// being captured into a graph); bit 2: launch; bit 4: the FP32+IR form (the
-// FP32 kernel, then the FP64 pass over the instances it handed over);
-// without it the fast FP64 kernel, then the exact-pivot pass.  Backward-Euler fixed steps of circuits without inductors take
-// the lean kernels.
-template <int N>
-hipError_t launch_transient_impl(const DevProgram& g, const DevNrCfg& cfg, const DevTran& tr,
-                                 const double* params, int64_t B, const DevOut& out,
-                                 const DevTranOut& tout, hipStream_t st, int phase) {
-  if (tr.adaptive || tr.method != SPX_TRAN_BE || g.has_inductor)
-    return launch_transient_t<N, true>(g, cfg, tr, params, B, out, tout, st, phase);
-  return launch_transient_t<N, false>(g, cfg, tr, params, B, out, tout, st, phase);
-}
-
-// escalated_only: 0 every instance; 1 the instances an FP32+IR pass handed
-// over (flagged SPX_FLAG_FP64_ESCALATED); 2 those a fast FP64 pass handed
-// over on a near-tie pivot (not flagged).  Only the exact kernel runs the
-// second passes.
 template <int N, bool IR, bool Exact>
 hipError_t launch_newton_impl(const DevProgram& g, const DevNrCfg& cfg, const double* params,
                               const double* x0, int64_t B, const DevOut& out,

@@ -45,21 +45,66 @@ __device__ __forceinline__ bool block_all_t(bool p, int tid, int* red) {
 template <int T, class RHS>
 __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, double* lu, RHS b,
                                                     double* y, int tid, int* red) {
-  // one level's factor work, threads [t0, t0 + stride)
+  // One level's factor work, threads [t0, t0 + stride).  A thread takes four
+  // multipliers, then four targets, at a time and issues their index and
+  // value loads together (the values are in L2 or LDS, the plan in L2): a
+  // level is a handful of dependent load rounds, not one per entry.  Each
+  // target still sums its contributions in column order.
   auto factor_level = [&](int L, int t0, int stride, bool sync) {
+    constexpr int U = 4;
     const int a1 = sp.pa_ptr[L + 1];
-    for (int t = sp.pa_ptr[L] + t0; t < a1; t += stride) {
-      const int li = sp.pa_l[t];
-      lu[li] = lu[li] * (1.0 / lu[sp.pa_d[t]]);
+    for (int t = sp.pa_ptr[L] + t0; t < a1; t += U * stride) {
+      int li[U], di[U];
+      double lv[U], dv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int tt = t + u * stride;
+        li[u] = tt < a1 ? sp.pa_l[tt] : -1;
+        di[u] = tt < a1 ? sp.pa_d[tt] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (li[u] >= 0) {
+          lv[u] = lu[li[u]];
+          dv[u] = lu[di[u]];
+        }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (li[u] >= 0) lu[li[u]] = lv[u] * (1.0 / dv[u]);
     }
     if (sync) __syncthreads();
     const int b1 = sp.tb_ptr[L + 1];
-    for (int t = sp.tb_ptr[L] + t0; t < b1; t += stride) {
-      const int ti = sp.tb_t[t];
-      double v = lu[ti];
-      const int c1 = sp.tb_c[t + 1];
-      for (int c = sp.tb_c[t]; c < c1; ++c) v = fma(-lu[sp.cb_l[c]], lu[sp.cb_u[c]], v);
-      lu[ti] = v;
+    for (int t = sp.tb_ptr[L] + t0; t < b1; t += U * stride) {
+      int ti[U], c0[U], c1[U];
+      double v[U];
+      int cmax = 0;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int tt = t + u * stride;
+        const bool in = tt < b1;
+        ti[u] = in ? sp.tb_t[tt] : -1;
+        c0[u] = in ? sp.tb_c[tt] : 0;
+        c1[u] = in ? sp.tb_c[tt + 1] : 0;
+        cmax = max(cmax, c1[u] - c0[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (ti[u] >= 0) v[u] = lu[ti[u]];
+      for (int q = 0; q < cmax; ++q) {
+        int cl[U], cu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool in = c0[u] + q < c1[u];
+          cl[u] = in ? sp.cb_l[c0[u] + q] : -1;
+          cu[u] = in ? sp.cb_u[c0[u] + q] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (cl[u] >= 0) v[u] = fma(-lu[cl[u]], lu[cu[u]], v[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (ti[u] >= 0) lu[ti[u]] = v[u];
     }
     if (sync) __syncthreads();
   };
@@ -70,6 +115,7 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
       const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
       const double* li = lu + sp.diag[i] - nl;
       double v = y[i];
+#pragma unroll 4
       for (int t = 0; t < nl; ++t) v = fma(-li[t], y[sp.ut_row[u0 + t]], v);
       y[i] = v;
     }
@@ -81,6 +127,7 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
       const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
       const double* ui = lu + sp.diag[i];
       double v = y[i];
+#pragma unroll 4
       for (int t = 0; t < ns; ++t) v = fma(-ui[1 + t], y[sp.s_row[s0 + t]], v);
       y[i] = v * (1.0 / ui[0]);
     }

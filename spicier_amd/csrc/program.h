@@ -11,6 +11,16 @@
 //             formed in the same order as to_dense_matrix
 //   row_ptr : CSR row starts into the (row, col)-sorted pattern
 //   rhs_*   : the same for the right-hand side
+//   el_lm   : the elements again, lane-major for one 64-lane wave: element
+//             order[t*64 + l] in lane l of round t, field f at
+//             [(t*kElemLm + f)*64 + l] (field 12: its element index; kind -1
+//             pads), so each field load is one coalesced 256-byte access
+//   gath    : the pattern and rhs sums as one fixed-length program per lane
+//             for one 64-lane wave: lane l sums entries q = l, l+64, ... then
+//             rhs rows r = l, l+64, ...; op k of lane l at [k*64 + l] is
+//             slot | neg << 14 | last << 15 | dst << 16 (dst: pattern entry,
+//             or 0x8000 | rhs row), -1 pads; gath_k = 0 when a table would
+//             not fit those fields
 // Value slots 0 and 1 hold the constant 1.0 (voltage-source couplings) and
 // the current Gmin.  Per-iteration values live in LDS; assembly is one
 // gather per pattern entry — no atomics.
@@ -24,6 +34,7 @@
 namespace spx {
 
 constexpr int kElemInts = 12;
+constexpr int kElemLm = 13;  // lane-major element fields: the 12 above + element index
 constexpr int kValOne = 0;
 constexpr int kValGmin = 1;
 
@@ -45,6 +56,11 @@ struct DevProgram {
   const int* contrib;  // vid << 1 | negate
   const int* rhs_off;  // n + 1
   const int* rhs_contrib;
+  int el_rounds;       // ceil(num_elems / 64)
+  const int* el_lm;    // el_rounds * kElemLm * 64
+  int gath_k;          // ops per lane (0: use pos_off / contrib)
+  const int* gath;     // gath_k * 64
+  const int* rowmask;  // 2n words: columns present in row r (rows < 64)
 };
 
 // Newton / stepping configuration as the kernels read it.

@@ -1,0 +1,7 @@
+// transient_32_exact_gen.hip — dense transient kernel, systems of up to 32 unknowns:
+// FP64 exact-pivot hand-over pass, every method and adaptive stepping.
+#include "newton_kernel.h"
+
+namespace spx {
+SPX_TRAN_VARIANT_FN(tran_32_exact_gen, 32, false, true, true)
+}  // namespace spx

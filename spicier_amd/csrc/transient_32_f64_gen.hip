@@ -1,0 +1,7 @@
+// transient_32_f64_gen.hip — dense transient kernel, systems of up to 32 unknowns:
+// FP64 (fast pivot comparison), every method and adaptive stepping.
+#include "newton_kernel.h"
+
+namespace spx {
+SPX_TRAN_VARIANT_FN(tran_32_f64_gen, 32, false, true, false)
+}  // namespace spx

@@ -1,0 +1,7 @@
+// transient_32_ir_gen.hip — dense transient kernel, systems of up to 32 unknowns:
+// FP32 LU + FP64 refinement, every method and adaptive stepping.
+#include "newton_kernel.h"
+
+namespace spx {
+SPX_TRAN_VARIANT_FN(tran_32_ir_gen, 32, true, true, false)
+}  // namespace spx
