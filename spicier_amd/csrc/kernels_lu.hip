@@ -26,12 +26,10 @@ __global__ __launch_bounds__(64) void k_solve_rows(const double* __restrict__ M,
                                                    const double* __restrict__ rhs, int n,
                                                    double* __restrict__ x,
                                                    uint8_t* __restrict__ sing) {
-  __shared__ double row[row_buffer_doubles(N)];
   const int64_t i = blockIdx.x;
   const int lane = threadIdx.x;
   const double* A = M + i * (int64_t)n * n;
   RowLU<N, double> lu;
-  lu.row = row;
   auto load = [&]() __attribute__((always_inline)) {
     static_for<0, N>([&](auto J) __attribute__((always_inline)) {
       constexpr int j = J;
@@ -40,10 +38,12 @@ __global__ __launch_bounds__(64) void k_solve_rows(const double* __restrict__ M,
   };
   load();
   const double b = lane < n ? rhs[i * n + lane] : 0.0;
-  bool ok = lu.factor(lane);
+  // MNA matrices are sparse: the chunk masks of the loaded rows let the
+  // factorization skip the zero chunks of each pivot row
+  bool ok = lu.factor(lane, chunk_mask_of(lu.a));
   if (!ok && lu.inexact) {  // a near-tie: the exact pivot rule on the reloaded rows
     load();
-    ok = lu.template factor<false, true>(lane);
+    ok = lu.template factor<false, true>(lane, chunk_mask_of(lu.a));
   }
   double xv = 0.0;
   if (ok) {
@@ -62,12 +62,10 @@ __global__ __launch_bounds__(64) void k_factor_rows(const double* __restrict__ M
                                                     double* __restrict__ F,
                                                     int32_t* __restrict__ piv,
                                                     uint8_t* __restrict__ sing) {
-  __shared__ double row[row_buffer_doubles(N)];
   const int64_t i = blockIdx.x;
   const int lane = threadIdx.x;
   const double* A = M + i * (int64_t)n * n;
   RowLU<N, double> lu;
-  lu.row = row;
   auto load = [&]() __attribute__((always_inline)) {
     static_for<0, N>([&](auto J) __attribute__((always_inline)) {
       constexpr int j = J;
@@ -75,10 +73,10 @@ __global__ __launch_bounds__(64) void k_factor_rows(const double* __restrict__ M
     });
   };
   load();
-  bool ok = lu.template factor<true>(lane);
+  bool ok = lu.template factor<true>(lane, chunk_mask_of(lu.a));
   if (!ok && lu.inexact) {
     load();
-    ok = lu.template factor<true, true>(lane);
+    ok = lu.template factor<true, true>(lane, chunk_mask_of(lu.a));
   }
   bool fin = true;
   static_for<0, N>([&](auto J) __attribute__((always_inline)) {

@@ -89,34 +89,121 @@ __device__ __forceinline__ double pull_from(double v, int src) {
 template <typename T>
 __device__ __forceinline__ T tabs(T v) { return fabs(v); }
 
-// Pivot-row broadcast: kBcastReadlane moves each element through an SGPR
-// (v_readlane, VALU), kBcastLds has the pivot lane store its row to LDS and
-// every lane read it back with same-address loads (LDS pipe).  The first is
-// VALU-bound, the second LDS-bound; which wins depends on n and occupancy.
-enum { kBcastReadlane = 0, kBcastLds = 1 };
+// Columns are grouped in chunks of kChunk for the structural skip of the
+// elimination updates (one bit per chunk in RowLU::factor's masks).
+constexpr int kChunk = 8;
 
-// Doubles of the LDS buffer a RowLU<N, *> needs behind `row`: N for the
-// pivot-row broadcast of the LDS variant (the callers reserve at least 32).
-__host__ __device__ constexpr int row_buffer_doubles(int N) { return N < 32 ? 32 : N; }
+// Chunk mask of a row held in registers: bit c set if any of columns
+// [c*kChunk, (c+1)*kChunk) is non-zero.
+template <int N, typename T>
+__device__ __forceinline__ uint32_t chunk_mask_of(const T (&a)[N]) {
+  uint32_t m = 0;
+  static_for<0, N>([&](auto J) __attribute__((always_inline)) {
+    constexpr int j = J;
+    if (a[j] != T(0)) m |= 1u << (j / kChunk);
+  });
+  return m;
+}
 
-template <int N, typename T, int Bcast = kBcastReadlane>
+// One chunk of the FP32 elimination update a[j] = fma(-u_j, l, a[j]) for
+// j in [J0, J0 + CN), u_j = a[j] of lane p, skipped as a whole when bit C of
+// the pivot row's chunk mask mp is clear.  The test and the skip are inside
+// one asm statement, so the compiler sees straight-line code (an update
+// branch per chunk written in C++ splits the unrolled elimination into
+// hundreds of blocks, and the register allocator then spills the rows).
+// The CN broadcasts are issued before the CN FMAs, which covers the
+// SGPR-write-to-VALU-read wait of each; chunks of fewer than three columns
+// add it explicitly.  The leading s_nop covers a VALU write of mp just
+// before the SALU test.
+#define SPX_RL(i) "v_readlane_b32 %[t" #i "], %[a" #i "], %[p]\n\t"
+#define SPX_FM(i) "v_fma_f32 %[a" #i "], -%[t" #i "], %[l], %[a" #i "]\n\t"
+#define SPX_AO(i) [a##i] "+v"(a[J0 + i])
+#define SPX_TO(i) [t##i] "=&s"(t[i])
+#define SPX_HEAD "s_nop 1\n\ts_bitcmp0_b32 %[mp], %[c]\n\ts_cbranch_scc1 .Lspx_chunk_skip%=\n\t"
+#define SPX_TAIL ".Lspx_chunk_skip%=:\n"
+#define SPX_IN [mp] "s"(mp), [c] "i"(C), [p] "s"(p), [l] "v"(l)
+template <int J0, int CN, int C, int N>
+__device__ __forceinline__ void chunk_update_f32(float (&a)[N], uint32_t mp, int p, float l) {
+  static_assert(CN >= 1 && CN <= 8 && J0 + CN <= N, "chunk of 1..8 columns");
+  float t[8];
+  if constexpr (CN == 1) {
+    asm volatile(SPX_HEAD SPX_RL(0) "s_nop 1\n\t" SPX_FM(0) SPX_TAIL
+                 : SPX_AO(0), SPX_TO(0) : SPX_IN : "scc");
+  } else if constexpr (CN == 2) {
+    asm volatile(SPX_HEAD SPX_RL(0) SPX_RL(1) "s_nop 1\n\t" SPX_FM(0) SPX_FM(1) SPX_TAIL
+                 : SPX_AO(0), SPX_AO(1), SPX_TO(0), SPX_TO(1) : SPX_IN : "scc");
+  } else if constexpr (CN == 3) {
+    asm volatile(SPX_HEAD SPX_RL(0) SPX_RL(1) SPX_RL(2) SPX_FM(0) SPX_FM(1) SPX_FM(2) SPX_TAIL
+                 : SPX_AO(0), SPX_AO(1), SPX_AO(2), SPX_TO(0), SPX_TO(1), SPX_TO(2)
+                 : SPX_IN : "scc");
+  } else if constexpr (CN == 4) {
+    asm volatile(SPX_HEAD SPX_RL(0) SPX_RL(1) SPX_RL(2) SPX_RL(3) SPX_FM(0) SPX_FM(1) SPX_FM(2)
+                     SPX_FM(3) SPX_TAIL
+                 : SPX_AO(0), SPX_AO(1), SPX_AO(2), SPX_AO(3), SPX_TO(0), SPX_TO(1), SPX_TO(2),
+                   SPX_TO(3)
+                 : SPX_IN : "scc");
+  } else if constexpr (CN == 5) {
+    asm volatile(SPX_HEAD SPX_RL(0) SPX_RL(1) SPX_RL(2) SPX_RL(3) SPX_RL(4) SPX_FM(0) SPX_FM(1)
+                     SPX_FM(2) SPX_FM(3) SPX_FM(4) SPX_TAIL
+                 : SPX_AO(0), SPX_AO(1), SPX_AO(2), SPX_AO(3), SPX_AO(4), SPX_TO(0), SPX_TO(1),
+                   SPX_TO(2), SPX_TO(3), SPX_TO(4)
+                 : SPX_IN : "scc");
+  } else if constexpr (CN == 6) {
+    asm volatile(SPX_HEAD SPX_RL(0) SPX_RL(1) SPX_RL(2) SPX_RL(3) SPX_RL(4) SPX_RL(5) SPX_FM(0)
+                     SPX_FM(1) SPX_FM(2) SPX_FM(3) SPX_FM(4) SPX_FM(5) SPX_TAIL
+                 : SPX_AO(0), SPX_AO(1), SPX_AO(2), SPX_AO(3), SPX_AO(4), SPX_AO(5), SPX_TO(0),
+                   SPX_TO(1), SPX_TO(2), SPX_TO(3), SPX_TO(4), SPX_TO(5)
+                 : SPX_IN : "scc");
+  } else if constexpr (CN == 7) {
+    asm volatile(SPX_HEAD SPX_RL(0) SPX_RL(1) SPX_RL(2) SPX_RL(3) SPX_RL(4) SPX_RL(5) SPX_RL(6)
+                     SPX_FM(0) SPX_FM(1) SPX_FM(2) SPX_FM(3) SPX_FM(4) SPX_FM(5) SPX_FM(6)
+                         SPX_TAIL
+                 : SPX_AO(0), SPX_AO(1), SPX_AO(2), SPX_AO(3), SPX_AO(4), SPX_AO(5), SPX_AO(6),
+                   SPX_TO(0), SPX_TO(1), SPX_TO(2), SPX_TO(3), SPX_TO(4), SPX_TO(5), SPX_TO(6)
+                 : SPX_IN : "scc");
+  } else {
+    asm volatile(SPX_HEAD SPX_RL(0) SPX_RL(1) SPX_RL(2) SPX_RL(3) SPX_RL(4) SPX_RL(5) SPX_RL(6)
+                     SPX_RL(7) SPX_FM(0) SPX_FM(1) SPX_FM(2) SPX_FM(3) SPX_FM(4) SPX_FM(5)
+                         SPX_FM(6) SPX_FM(7) SPX_TAIL
+                 : SPX_AO(0), SPX_AO(1), SPX_AO(2), SPX_AO(3), SPX_AO(4), SPX_AO(5), SPX_AO(6),
+                   SPX_AO(7), SPX_TO(0), SPX_TO(1), SPX_TO(2), SPX_TO(3), SPX_TO(4), SPX_TO(5),
+                   SPX_TO(6), SPX_TO(7)
+                 : SPX_IN : "scc");
+  }
+}
+#undef SPX_RL
+#undef SPX_FM
+#undef SPX_AO
+#undef SPX_TO
+#undef SPX_HEAD
+#undef SPX_TAIL
+#undef SPX_IN
+
+template <int N, typename T>
 struct RowLU {
   T a[N];
   int pos;     // position of this lane's row (== lane once factored)
   int orig;    // row of the input whose factor row lane `lane` holds
   T piv;       // U[lane][lane] once factored
   T dinv;      // 1 / U[lane][lane]
-  T* row;      // N elements of LDS: broadcast buffer of the LDS variant
   int ipiv;    // factor<true>: lane k holds the position swapped with k at step k
   bool inexact;  // FP64 fast pass: a near-tie needs factor<*, true> on reloaded rows
 
   // Factor in place.  Returns false if a pivot column is zero (or NaN): the
-  // reference's LU then reports the system singular.  Every step runs over
-  // all N columns with no per-column guard, so the update is straight-line
-  // code; rows and columns past n are identity padding, whose multipliers
-  // are zero.
+  // reference's LU then reports the system singular.  Rows and columns past
+  // n are identity padding, whose multipliers are zero.
+  //
+  // Structural skip: `cmask` holds, per lane, a bit for every chunk of
+  // kChunk columns in which this lane's row may be non-zero (a superset is
+  // fine; ~0u skips nothing).  Step k reads the pivot row's mask, skips the
+  // update of every chunk in which the pivot row is zero — those updates are
+  // fma(-0, l, a) = a, so the factors are the same values (up to the sign of
+  // a zero, which the pivot search ignores) — and adds the pivot row's
+  // chunks to every row that takes a non-zero multiplier (the fill).  Rows
+  // of an MNA Jacobian hold a few entries, so most chunks are skipped
+  // (a 64-unknown inverter chain: 432 of 2,016 column updates run).
   template <bool Record = false, bool Exact = false>
-  __device__ __forceinline__ bool factor(int lane) {
+  __device__ __forceinline__ bool factor(int lane, uint32_t cmask = ~0u) {
     constexpr bool kF64 = std::is_same<T, double>::value;
     pos = lane;
     if constexpr (Record) ipiv = lane;
@@ -125,11 +212,8 @@ struct RowLU {
     bool ok = true;
     uint32_t nearby = 0;  // FP64 fast pass: a near-tie the exact rule might decide otherwise
 
-    // One basic block for all N steps (no early exit, no data-dependent
-    // branch), so the scheduler can start step k+1's pivot search while step
-    // k's trailing updates are still in flight.  A zero pivot column clears
-    // `ok`; its step then runs with a zero reciprocal and the result is
-    // discarded by the caller.
+    // No early exit: a zero pivot column clears `ok`; its step then runs
+    // with a zero reciprocal and the result is discarded by the caller.
     static_for<0, N>([&](auto K) __attribute__((always_inline)) {
       constexpr int k = K;
       bool nz;
@@ -147,48 +231,40 @@ struct RowLU {
           (int)__ffsll((unsigned long long)__ballot(pos == bp)) - 1);
       if (pos == k) pos = bp;  // row at position k moves to the pivot's position
       if (lane == p) pos = k;
-      if constexpr (Bcast == kBcastLds) {
-        if (lane == p) {
-          static_for<k, N>([&](auto J) __attribute__((always_inline)) {
+      const T pv = bcast(a[k], p);
+      if constexpr (kF64 && !Exact) nearby |= pivot_inexact(a[k], pv, pos >= k);
+      // FP32 factors only precondition FP64 refinement: the hardware
+      // reciprocal (1 ulp) serves; FP64 keeps the correctly rounded 1/pivot
+      T inv;
+      if constexpr (kF64)
+        inv = pv != T(0) ? T(1) / pv : T(0);
+      else
+        inv = pv != T(0) ? __builtin_amdgcn_rcpf(pv) : T(0);
+      if (lane == p) {
+        dinv = inv;
+        piv = pv;
+      }
+      const T l = pos > k ? a[k] * inv : T(0);
+      if (pos > k) a[k] = l;
+      if constexpr (k + 1 < N) {
+        if constexpr (kF64) {
+          // FP64: every column (in C++ the skip's branches cost more in
+          // spills than they save)
+          static_for<k + 1, N>([&](auto J) __attribute__((always_inline)) {
             constexpr int j = J;
-            row[j] = a[j];
+            a[j] = fma(-bcast(a[j], p), l, a[j]);
           });
+        } else {
+          const uint32_t mp = (uint32_t)__builtin_amdgcn_readlane((int)cmask, p);
+          if (l != T(0)) cmask |= mp;
+          static_for<(k + 1) / kChunk, (N + kChunk - 1) / kChunk>(
+              [&](auto C) __attribute__((always_inline)) {
+                constexpr int c = C;
+                constexpr int j0 = c * kChunk > k + 1 ? c * kChunk : k + 1;
+                constexpr int j1 = (c + 1) * kChunk < N ? (c + 1) * kChunk : N;
+                chunk_update_f32<j0, j1 - j0, c>(a, mp, p, l);
+              });
         }
-        __syncthreads();
-        const T pv = row[k];
-        if constexpr (kF64 && !Exact) nearby |= pivot_inexact(a[k], pv, pos >= k);
-        const T inv = pv != T(0) ? T(1) / pv : T(0);
-        if (lane == p) {
-          dinv = inv;
-          piv = pv;
-        }
-        const T l = pos > k ? a[k] * inv : T(0);
-        if (pos > k) a[k] = l;
-        static_for<k + 1, N>([&](auto J) __attribute__((always_inline)) {
-          constexpr int j = J;
-          a[j] = fma(-row[j], l, a[j]);
-        });
-        __syncthreads();
-      } else {
-        const T pv = bcast(a[k], p);
-        if constexpr (kF64 && !Exact) nearby |= pivot_inexact(a[k], pv, pos >= k);
-        // FP32 factors only precondition FP64 refinement: the hardware
-        // reciprocal (1 ulp) serves; FP64 keeps the correctly rounded 1/pivot
-        T inv;
-        if constexpr (kF64)
-          inv = pv != T(0) ? T(1) / pv : T(0);
-        else
-          inv = pv != T(0) ? __builtin_amdgcn_rcpf(pv) : T(0);
-        if (lane == p) {
-          dinv = inv;
-          piv = pv;
-        }
-        const T l = pos > k ? a[k] * inv : T(0);
-        if (pos > k) a[k] = l;
-        static_for<k + 1, N>([&](auto J) __attribute__((always_inline)) {
-          constexpr int j = J;
-          a[j] = fma(-bcast(a[j], p), l, a[j]);
-        });
       }
     });
     inexact = kF64 && !Exact && __any(nearby != 0u);

@@ -27,11 +27,9 @@
 namespace spx {
 
 // LDS per instance: pattern values + column indices, per-row column
-// bitmasks, rhs, x, scratch x, element values, parameters, junction states,
-// pivot-row buffer.
+// bitmasks, rhs, x, scratch x, element values, parameters, junction states.
 inline size_t newton_smem_doubles(const DevProgram& g, int N) {
-  return (size_t)g.num_pos + (g.num_pos + 1) / 2 + N + 3 * N + g.num_vals + g.P + g.num_elems +
-         row_buffer_doubles(N);
+  return (size_t)g.num_pos + (g.num_pos + 1) / 2 + N + 3 * N + g.num_vals + g.P + g.num_elems;
 }
 
 // Exact: the FP64 LU compares every bit of the pivot candidates (the exact
@@ -51,7 +49,6 @@ struct Instance {
   double* vals;   // num_vals
   double* pv;     // P parameters
   double* vst;    // num_elems pnjlim junction voltages
-  double* prow;   // row_buffer_doubles(N): LU pivot-row broadcast / position-map scratch
   int q0 = 0, q1 = 0;    // this lane's row: pattern entries [q0, q1)
   int nfact = 0;  // LU factorizations performed
   int nref = 0;   // refinement steps performed
@@ -68,7 +65,6 @@ struct Instance {
 
   // publishes the row masks (a barrier follows)
   __device__ __forceinline__ void init_row() {
-    lu32.row = reinterpret_cast<float*>(prow);
     uint32_t m0 = 0, m1 = 0;
     if (lane < g.n) {
       q0 = g.row_ptr[lane];
@@ -113,8 +109,10 @@ struct Instance {
   // J), and is discarded when the column bit is clear.
   // The read of J[q] is kept unconditional (the empty asm pins it) so the
   // expansion is straight-line selects, not a branch per column.
+  // Returns the row's chunk mask (RowLU::factor): the chunks of the pattern
+  // columns, or of the identity entry.
   template <typename T>
-  __device__ __forceinline__ void load_rows(T (&a)[N]) const {
+  __device__ __forceinline__ uint32_t load_rows(T (&a)[N]) const {
     const bool pad = lane >= g.n;
     const int r = pad ? 0 : lane;  // rows past n (and lanes past N) take no pattern entry
     uint32_t mlo = pad ? 0u : rmask[2 * r], mhi = pad ? 0u : rmask[2 * r + 1];
@@ -135,6 +133,15 @@ struct Instance {
       // first and hold 2N VGPRs of loaded values beside the row
       if constexpr ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     });
+    static_assert(32 % kChunk == 0, "a chunk lies within one mask word");
+    uint32_t cm = 0;
+    static_for<0, (N + kChunk - 1) / kChunk>([&](auto C) __attribute__((always_inline)) {
+      constexpr int c = C;
+      const uint32_t w = (c * kChunk < 32 ? mlo : mhi) >> ((c * kChunk) & 31);
+      if (w & ((1u << kChunk) - 1u)) cm |= 1u << c;
+    });
+    if (diag >= 0 && diag < N) cm = 1u << (diag / kChunk);
+    return cm;
   }
 
   // J x for this lane's row (pattern order = column order)
@@ -148,9 +155,8 @@ struct Instance {
   __device__ __forceinline__ int solve_f64(double& xnew) {
     ++nfact;
     RowLU<N, double> lu;
-    lu.row = prow;
-    load_rows(lu.a);
-    if (!lu.template factor<false, Exact>(lane)) return (!Exact && lu.inexact) ? 2 : 0;
+    const uint32_t cm = load_rows(lu.a);
+    if (!lu.template factor<false, Exact>(lane, cm)) return (!Exact && lu.inexact) ? 2 : 0;
     xnew = lu.template solve<true>(lane < g.n ? rhs[lane] : 0.0, lane);
     return __all(lane >= g.n || isfinite(xnew)) ? 1 : 0;
   }
@@ -209,8 +215,7 @@ struct Instance {
           bool fresh = !have32 || attempt > 0;
           if (fresh) {
             ++nfact;
-            load_rows(lu32.a);
-            have32 = lu32.factor(lane);
+            have32 = lu32.factor(lane, load_rows(lu32.a));
             if (!have32) { rc = 2; break; }
           }
           rc = refine(lu32, fresh, xnew);
@@ -277,9 +282,8 @@ struct Instance {
       for (int attempt = 0; attempt < 2 && r == 1; ++attempt) {
         const bool fresh = !(reuse32 && have32) || attempt > 0;
         if (fresh) {
-          load_rows(lu32.a);
           ++nfact;
-          have32 = lu32.factor(lane);
+          have32 = lu32.factor(lane, load_rows(lu32.a));
           if (!have32) {
             r = 2;
             break;
@@ -413,8 +417,7 @@ __global__ __launch_bounds__(64, IR ? 4 : 3) void k_newton_dense(DevProgram g, D
   double* vals = xw + N;
   double* pv = vals + g.num_vals;
   double* vst = pv + g.P;
-  double* prow = vst + g.num_elems;
-  Instance<N, IR, Exact> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
+  Instance<N, IR, Exact> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst};
   const int n = g.n;
 
   for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
@@ -451,11 +454,10 @@ __global__ __launch_bounds__(64, IR ? 4 : 3) void k_transient_dense(DevProgram g
   double* vals = xw + N;
   double* pv = vals + g.num_vals;
   double* vst = pv + g.P;
-  double* prow = vst + g.num_elems;
-  double* vcap = prow + row_buffer_doubles(N);
+  double* vcap = vst + g.num_elems;
   double* icap = GENERAL ? vcap + g.num_elems : nullptr;
   double* vcap2 = GENERAL ? icap + g.num_elems : nullptr;
-  Instance<N, IR, Exact> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst, prow};
+  Instance<N, IR, Exact> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst};
   TranCtx tc{0, 0.0, tr.h, vcap, icap, vcap2, 0, 0.0, 0.0, 0.0};
   I.tc = &tc;
   for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
