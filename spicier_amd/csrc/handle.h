@@ -35,6 +35,13 @@ struct spx_handle {
   // for, and the device addresses it baked in
   hipGraphExec_t mixed_exec = nullptr;
   std::vector<uint64_t> mixed_key;
+  // side streams (default priority) and fork/join events that let the size
+  // classes of a mixed batch run concurrently, eagerly or as parallel
+  // branches of the captured graph; created on first use
+  static constexpr int kSideStreams = 3;
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev[kSideStreams] = {};
   // autotuned kernel kind per circuit program (transient program blob)
   std::map<std::vector<int>, int> tuned;
   // batched sparse linear solve: the pattern (n, nnz, rows, cols) whose plan

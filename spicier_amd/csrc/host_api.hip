@@ -54,7 +54,8 @@ hipError_t launch_transient_sparse(const DevProgram& g, const DevSparse& sp, con
                                    const DevTran& tr, const double* params, int64_t B,
                                    const DevOut& out, const DevTranOut& tout, double* ws,
                                    int grid, int64_t slot, size_t smem, int lu_in_lds,
-                                   int threads, hipStream_t st);
+                                   int threads, hipStream_t st,
+                                   int* next = nullptr);
 hipError_t lanes_transient_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
                                  int64_t* slot);
 hipError_t launch_transient_lanes(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
@@ -966,6 +967,7 @@ struct MixedClass {
   int64_t slot = 0;
   size_t smem = 0;
   double* ws = nullptr;
+  int* queue = nullptr;  // sparse kernels: the instance queue (shared GPU, dynamic balance)
   const void* prog = nullptr;
 };
 
@@ -988,6 +990,7 @@ int shape_class(spx_handle* h, MixedClass& mc, int kernel, int64_t B, const DevN
             SPX_ERR_BACKEND);
     mc.grid = (int)B;
     mc.ws = nullptr;
+    mc.queue = nullptr;
     return SPX_OK;
   }
   if (kernel == SPX_KERNEL_LANES) {
@@ -1004,6 +1007,12 @@ int shape_class(spx_handle* h, MixedClass& mc, int kernel, int64_t B, const DevN
   if ((rc = workspace(h, wsname.c_str(), sizeof(double) * (size_t)mc.grid * mc.slot, &ws)))
     return rc;
   mc.ws = static_cast<double*>(ws);
+  mc.queue = nullptr;
+  if (kernel != SPX_KERNEL_LANES) {
+    void* q = nullptr;
+    if ((rc = workspace(h, (wsname + "_q").c_str(), sizeof(int), &q))) return rc;
+    mc.queue = static_cast<int*>(q);
+  }
   return SPX_OK;
 }
 
@@ -1016,7 +1025,7 @@ hipError_t enqueue_class(const MixedClass& mc, const DevNrCfg& dc, const DevTran
                                   mc.slot, st);
   const int threads = mc.kernel == SPX_KERNEL_SPARSE64 ? 64 : 256;
   return launch_transient_sparse(mc.g, mc.sp, dc, tr, mc.params, B, o, to, mc.ws, mc.grid,
-                                 mc.slot, mc.smem, mc.lu_lds, threads, st);
+                                 mc.slot, mc.smem, mc.lu_lds, threads, st, mc.queue);
 }
 
 // Times each candidate kernel on a sample of the class; returns the fastest.
@@ -1073,6 +1082,17 @@ int autotune_class(spx_handle* h, MixedClass& mc, const DevNrCfg& dc, int* best)
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
+}
+
+// The handle's side streams and fork/join events (mixed batches).
+int ensure_side_streams(spx_handle* h) {
+  if (h->fork_ev) return SPX_OK;
+  for (int s = 0; s < spx_handle::kSideStreams; ++s) {
+    SPX_HIP(hipStreamCreateWithFlags(&h->side[s], hipStreamNonBlocking), SPX_ERR_BACKEND_INIT);
+    SPX_HIP(hipEventCreateWithFlags(&h->join_ev[s], hipEventDisableTiming), SPX_ERR_BACKEND_INIT);
+  }
+  SPX_HIP(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming), SPX_ERR_BACKEND_INIT);
+  return SPX_OK;
 }
 
 int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_config* cfg,
@@ -1187,7 +1207,7 @@ int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_
     const double dwords[] = {mc.tr.h,      mc.tr.tstop,  mc.tr.h_init, mc.tr.h_min,
                              mc.tr.h_max,  mc.tr.reltol, mc.tr.abstol};
     kput(dwords, sizeof(dwords));
-    const void* ptrs[] = {mc.params, mc.sample_dst, mc.d_specs, dprobe, mc.ws, mc.prog,
+    const void* ptrs[] = {mc.params, mc.sample_dst, mc.d_specs, dprobe, mc.ws, mc.queue, mc.prog,
                           mc.sp.perm, c.result.x, mc.o.status, c.result.iterations,
                           c.result.flags, c.result.residual, c.result.waveforms,
                           c.result.fail_step, mc.to.times, mc.to.num_points,
@@ -1195,15 +1215,55 @@ int mixed_locked(spx_handle* h, const spx_tran_class* cls, int nc, const spx_nr_
                           mc.to.max_step};
     kput(ptrs, sizeof(ptrs));
   }
+  // Classes share nothing, so the register-LU (dense) classes run beside the
+  // others: the sparse and lane classes, whose grids fill the GPU, stay in
+  // order on the main stream, and the dense classes (a workgroup per
+  // instance) go to up to kSideStreams side streams (longest first onto the
+  // least-loaded one, cost ~ B n^1.5), forked from and joined back into the
+  // main stream with events.  Their waves fill the slots the persistent
+  // kernels leave between instances.  Captured, the streams become parallel
+  // branches of one graph.
+  int nstreams = 1;
+  std::vector<int> lane_of(nc, 0);
+  std::vector<int> dense;
+  for (int i = 0; i < nc; ++i)
+    if (mcs[i].kernel == SPX_KERNEL_DENSE) dense.push_back(i);
+  if (nc > 1 && !dense.empty()) {
+    if ((rc = ensure_side_streams(h))) return rc;
+    const bool others = (int)dense.size() < nc;
+    const int nside = std::min((int)dense.size(), spx_handle::kSideStreams);
+    nstreams = 1 + nside;
+    auto cost = [&](int i) { return (double)mcs[i].B * std::pow((double)mcs[i].hp.n, 1.5); };
+    std::stable_sort(dense.begin(), dense.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    // without sparse classes the main stream takes a dense share too
+    std::vector<double> load(nstreams, 0.0);
+    if (others) load[0] = 1e300;
+    for (int i : dense) {
+      const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      lane_of[i] = s;
+      load[s] += cost(i);
+    }
+  }
   auto enqueue_all = [&](hipStream_t st) -> hipError_t {
+    hipError_t e;
+    auto stream_of = [&](int lane) { return lane == 0 ? st : h->side[lane - 1]; };
+    if (nstreams > 1) {
+      if ((e = hipEventRecord(h->fork_ev, st)) != hipSuccess) return e;
+      for (int s = 1; s < nstreams; ++s)
+        if ((e = hipStreamWaitEvent(h->side[s - 1], h->fork_ev, 0)) != hipSuccess) return e;
+    }
     for (int i = 0; i < nc; ++i) {
       const MixedClass& mc = mcs[i];
-      hipError_t e;
+      hipStream_t cs = stream_of(lane_of[i]);
       if (mc.sample_dst && mc.hp.P > 0 &&
           (e = launch_sample(static_cast<spx_param_spec*>(mc.d_specs), mc.hp.P, mc.gen, mc.seed,
-                             mc.B, mc.offset, mc.nvar, mc.sample_dst, st)) != hipSuccess)
+                             mc.B, mc.offset, mc.nvar, mc.sample_dst, cs)) != hipSuccess)
         return e;
-      if ((e = enqueue_class(mc, dc, mc.tr, mc.o, mc.to, mc.B, st)) != hipSuccess) return e;
+      if ((e = enqueue_class(mc, dc, mc.tr, mc.o, mc.to, mc.B, cs)) != hipSuccess) return e;
+    }
+    for (int s = 1; s < nstreams; ++s) {
+      if ((e = hipEventRecord(h->join_ev[s - 1], h->side[s - 1])) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(st, h->join_ev[s - 1], 0)) != hipSuccess) return e;
     }
     return hipSuccess;
   };
@@ -1283,6 +1343,12 @@ int spx_destroy(spx_handle* h) {
     DeviceGuard dg(h->device);
     hipStreamSynchronize(h->stream);
     if (h->mixed_exec) hipGraphExecDestroy(h->mixed_exec);
+    for (int s = 0; s < spx_handle::kSideStreams; ++s) {
+      if (h->side[s]) hipStreamSynchronize(h->side[s]);
+      if (h->side[s]) hipStreamDestroy(h->side[s]);
+      if (h->join_ev[s]) hipEventDestroy(h->join_ev[s]);
+    }
+    if (h->fork_ev) hipEventDestroy(h->fork_ev);
     for (auto& kv : h->ws)
       if (kv.second.first) hipFree(kv.second.first);
     hipStreamDestroy(h->stream);

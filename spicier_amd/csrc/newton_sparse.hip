@@ -49,11 +49,20 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
   // multipliers, then four targets, at a time and issues their index and
   // value loads together (the values are in L2 or LDS, the plan in L2): a
   // level is a handful of dependent load rounds, not one per entry.  Each
-  // target still sums its contributions in column order.
+  // target still sums its contributions in column order.  A level with at
+  // most one entry per thread (a chain's) takes the plain path, which
+  // carries no unused slots.
   auto factor_level = [&](int L, int t0, int stride, bool sync) {
     constexpr int U = 4;
-    const int a1 = sp.pa_ptr[L + 1];
-    for (int t = sp.pa_ptr[L] + t0; t < a1; t += U * stride) {
+    const int a0 = sp.pa_ptr[L], a1 = sp.pa_ptr[L + 1];
+    if (a1 - a0 <= stride) {  // at most one entry per thread (chains): no unrolling
+      const int t = a0 + t0;
+      if (t < a1) {
+        const int li = sp.pa_l[t];
+        lu[li] = lu[li] * (1.0 / lu[sp.pa_d[t]]);
+      }
+    } else
+    for (int t = a0 + t0; t < a1; t += U * stride) {
       int li[U], di[U];
       double lv[U], dv[U];
 #pragma unroll
@@ -73,8 +82,18 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
         if (li[u] >= 0) lu[li[u]] = lv[u] * (1.0 / dv[u]);
     }
     if (sync) __syncthreads();
-    const int b1 = sp.tb_ptr[L + 1];
-    for (int t = sp.tb_ptr[L] + t0; t < b1; t += U * stride) {
+    const int b0 = sp.tb_ptr[L], b1 = sp.tb_ptr[L + 1];
+    if (b1 - b0 <= stride) {
+      const int t = b0 + t0;
+      if (t < b1) {
+        const int ti = sp.tb_t[t];
+        double v = lu[ti];
+        const int c1 = sp.tb_c[t + 1];
+        for (int c = sp.tb_c[t]; c < c1; ++c) v = fma(-lu[sp.cb_l[c]], lu[sp.cb_u[c]], v);
+        lu[ti] = v;
+      }
+    } else
+    for (int t = b0 + t0; t < b1; t += U * stride) {
       int ti[U], c0[U], c1[U];
       double v[U];
       int cmax = 0;
@@ -434,14 +453,19 @@ inline hipError_t grid_for(const void* kernel, size_t smem, int64_t B, int* grid
   return hipSuccess;
 }
 
+__global__ void k_queue_reset(int* q) { *q = 0; }
+
 // Batched Backward-Euler transient on the sparse path; the workspace slot
 // adds the capacitor states after the sparse Newton slot.
+// At least three waves per SIMD (the register budget the unrolled level
+// loops fit in without spilling; at two, ladders run 1.4x slower).
 template <int T, bool GENERAL>
-__global__ __launch_bounds__(T) void k_transient_sparse(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_transient_sparse(
     DevProgram g, DevSparse sp, DevNrCfg cfg, DevTran tr, const double* __restrict__ params,
     int64_t B, DevOut out, DevTranOut tout, double* ws, int64_t slot, int lu_in_lds,
-    int plan_in_lds) {
+    int plan_in_lds, int* next) {
   extern __shared__ double smem[];
+  __shared__ int64_t s_inst;
   const int n = g.n, tid = threadIdx.x;
   double* rhs = smem;
   double* xs = rhs + n;
@@ -465,7 +489,18 @@ __global__ __launch_bounds__(T) void k_transient_sparse(
     sp = rebase_sparse(sp, lp);
     __syncthreads();
   }
-  for (int64_t inst = blockIdx.x; inst < B; inst += gridDim.x) {
+  // instances: blockIdx.x, + gridDim.x, ... or, with a queue, the next
+  // unclaimed one (a workgroup that starts late takes fewer, so a class that
+  // shares the GPU with other launches stays balanced); every workgroup
+  // leaves once the instances are exhausted
+  for (int64_t inst = blockIdx.x;; inst += gridDim.x) {
+    if (next) {
+      if (tid == 0) s_inst = atomicAdd(next, 1);
+      __syncthreads();
+      inst = s_inst;
+      __syncthreads();
+    }
+    if (inst >= B) break;
     if (tid == 0) {
       vals[kValOne] = 1.0;
       vals[kValGmin] = 0.0;
@@ -520,23 +555,28 @@ hipError_t launch_transient_sparse(const DevProgram& g, const DevSparse& sp, con
                                    const DevTran& tr, const double* params, int64_t B,
                                    const DevOut& out, const DevTranOut& tout, double* ws,
                                    int grid, int64_t slot, size_t smem, int lu_in_lds,
-                                   int threads, hipStream_t st) {
+                                   int threads, hipStream_t st, int* next) {
   if (B == 0) return hipSuccess;
+  if (next) {  // the queue starts at 0 (a kernel, not a memset: graph-safe)
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, st, next);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   const int plan = (smem & kPlanInLdsBit) ? 1 : 0;
   smem &= ~kPlanInLdsBit;
   const bool general = tr.adaptive || tr.method != SPX_TRAN_BE || g.has_inductor;
   if (threads == 64 && general)
     hipLaunchKernelGGL((k_transient_sparse<64, true>), dim3((unsigned)grid), dim3(64), smem, st,
-                       g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
+                       g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan, next);
   else if (threads == 64)
     hipLaunchKernelGGL((k_transient_sparse<64, false>), dim3((unsigned)grid), dim3(64), smem, st,
-                       g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
+                       g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan, next);
   else if (general)
     hipLaunchKernelGGL((k_transient_sparse<256, true>), dim3((unsigned)grid), dim3(256), smem,
-                       st, g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
+                       st, g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan, next);
   else
     hipLaunchKernelGGL((k_transient_sparse<256, false>), dim3((unsigned)grid), dim3(256), smem,
-                       st, g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan);
+                       st, g, sp, cfg, tr, params, B, out, tout, ws, slot, lu_in_lds, plan, next);
   return hipGetLastError();
 }
 
