@@ -42,17 +42,25 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # reading for a 1 GiB device copy.  The runs, the scaling and the figures are
 # in this file, whose "traffic_bytes_per_launch" line bench.py reads.
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "newton_dense_64_traffic_inverter_chain_65536.txt")
+# VALU instructions per instance of the same kernels (rocprofv3 --pmc SQ_INSTS_VALU; the runs are in
+# this file, whose "valu_per_instance" line bench.py reads)
+COUNTER_PROFILE = os.path.join(ROOT, "profiles", "newton_dense_64_counters_inverter_chain_65536.txt")
 
 
-def measured_traffic():
-    """{form: HBM bytes per launch} from the committed counter summary."""
+def _profile_line(path, key):
+    """{form: value} from a "key form=value ..." line of a committed counter summary."""
     try:
-        for line in open(TRAFFIC_PROFILE):
-            if line.startswith("traffic_bytes_per_launch"):
+        for line in open(path):
+            if line.startswith(key):
                 return {k: float(v) for k, v in (t.split("=") for t in line.split()[1:])}
     except OSError:
         pass
     return {}
+
+
+def measured_traffic():
+    """{form: HBM bytes per launch} from the committed counter summary."""
+    return _profile_line(TRAFFIC_PROFILE, "traffic_bytes_per_launch")
 
 
 def host_cpu():
@@ -483,13 +491,27 @@ def main():
     # f32 / 157.3 + f64 / 78.6 TFLOP/s, and frac is that time over the
     # kernel's measured time (achieved / peak with peak the blend of the two).
     traffic = measured_traffic()
+    valu = _profile_line(COUNTER_PROFILE, "valu_per_instance")
     f32, f64_ = h["flops_fp32"], h["flops_fp64"]
     k_s = h["newton_kernel_ms"] * 1e-3
     peak_s = f32 / (PEAK_TFLOPS["fp32"] * 1e12) + f64_ / (PEAK_TFLOPS["fp64"] * 1e12)
+    # VALU issue: instructions per instance (committed counter run) x 4 cycles per wave64
+    # instruction, over the SIMD-cycles the kernel had (1,024 SIMDs at 2.4 GHz)
+    vi = valu.get(head)
+    valu_issue = None if vi is None else {
+        "instructions_per_instance": vi,
+        "frac": args.batch * vi * 4 / (k_s * 2.4e9 * 1024),
+        "source": os.path.relpath(COUNTER_PROFILE, ROOT),
+        "note": "the binding resource: SIMD VALU issue slots (a fraction past 1 means some "
+                "VALU instructions, v_readlane and v_cndmask among them, take under 4 cycles)"}
     roofline = {
         "bound": "valu",
         "compute": "LU, refinement and residuals on vector FMA (VGPR-resident rows); packed "
                    "FP32 is disabled, so FP32 issues at the FP64 vector rate",
+        "flop_count": "LAPACK dense-equivalent: getrf + getrs flops per factorization, 2n^2 per "
+                      "refinement step; the FP32 LU skips the zero chunks of each pivot row, so "
+                      "it performs fewer flops than this count",
+        "valu_issue": valu_issue,
         "achieved": (f32 + f64_) / k_s / 1e12, "peak": (f32 + f64_) / peak_s / 1e12,
         "unit": "TFLOP/s", "frac": peak_s / k_s,
         "by_precision": {
