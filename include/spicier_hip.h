@@ -529,7 +529,9 @@ int spx_surrogate_set_normalization(spx_surrogate* s, const float* in_mean, cons
  * weighted by (std_j / (tol_rel |mean_j| + tol_abs))^2 (normalised to mean 1),
  * i.e. measured in units of the Newton tolerance the warm start has to meet.
  * loss_history (host, may be NULL) receives each epoch's mean loss.
- * Synchronous. */
+ * The 32 -> 256 -> 256 -> 64 network trains in three fused launches per step
+ * (row pass, weight-gradient tiles, reduce + Adam); other shapes take the
+ * layered GEMM path.  Synchronous. */
 int spx_surrogate_train_device(spx_surrogate* s, const double* d_params, const double* d_x,
                                int32_t n_x, const uint8_t* d_mask, int64_t B, int32_t epochs,
                                int32_t batch, float lr, float tol_rel, float tol_abs,
