@@ -179,6 +179,66 @@ __device__ __forceinline__ void chunk_update_f32(float (&a)[N], uint32_t mp, int
 #undef SPX_TAIL
 #undef SPX_IN
 
+// The FP64 form of chunk_update_f32, four columns per asm statement: the
+// pivot row's 32-bit halves are broadcast into the fixed SGPR pairs
+// s[88:95] (declared clobbered) and each FMA reads its pair.  The halves
+// enter as 32-bit operands (the register allocator hands over the
+// sub-registers of each row element), since an asm operand cannot name the
+// halves of a 64-bit register pair.
+#define SPX_RL64(i, s0, s1) "v_readlane_b32 " s0 ", %[lo" #i "], %[p]\n\tv_readlane_b32 " s1 ", %[hi" #i "], %[p]\n\t"
+#define SPX_FM64(i, sp) "v_fma_f64 %[a" #i "], -" sp ", %[l], %[a" #i "]\n\t"
+#define SPX_AO64(i) [a##i] "+v"(a[J0 + i])
+#define SPX_HALVES(i) [lo##i] "v"((uint32_t)(uint64_t)__double_as_longlong(a[J0 + i])), \
+                      [hi##i] "v"((uint32_t)((uint64_t)__double_as_longlong(a[J0 + i]) >> 32))
+#define SPX_HEAD64 "s_nop 1\n\ts_bitcmp0_b32 %[mp], %[c]\n\ts_cbranch_scc1 .Lspx_chunk64_skip%=\n\t"
+#define SPX_TAIL64 ".Lspx_chunk64_skip%=:\n"
+#define SPX_IN64 [mp] "s"(mp), [c] "i"(C), [p] "s"(p), [l] "v"(l)
+template <int J0, int CN, int C, int N>
+__device__ __forceinline__ void chunk_update_f64_part(double (&a)[N], uint32_t mp, int p, double l) {
+  static_assert(CN >= 1 && CN <= 4 && J0 + CN <= N, "one to four columns");
+  if constexpr (CN == 1) {
+    asm volatile(SPX_HEAD64 SPX_RL64(0, "s88", "s89") "s_nop 1\n\t" SPX_FM64(0, "s[88:89]") SPX_TAIL64
+                 : SPX_AO64(0) : SPX_HALVES(0), SPX_IN64 : "scc", "s88", "s89");
+  } else if constexpr (CN == 2) {
+    asm volatile(SPX_HEAD64 SPX_RL64(0, "s88", "s89") SPX_RL64(1, "s90", "s91")
+                     SPX_FM64(0, "s[88:89]") SPX_FM64(1, "s[90:91]") SPX_TAIL64
+                 : SPX_AO64(0), SPX_AO64(1) : SPX_HALVES(0), SPX_HALVES(1), SPX_IN64
+                 : "scc", "s88", "s89", "s90", "s91");
+  } else if constexpr (CN == 3) {
+    asm volatile(SPX_HEAD64 SPX_RL64(0, "s88", "s89") SPX_RL64(1, "s90", "s91")
+                     SPX_RL64(2, "s92", "s93") SPX_FM64(0, "s[88:89]") SPX_FM64(1, "s[90:91]")
+                         SPX_FM64(2, "s[92:93]") SPX_TAIL64
+                 : SPX_AO64(0), SPX_AO64(1), SPX_AO64(2)
+                 : SPX_HALVES(0), SPX_HALVES(1), SPX_HALVES(2), SPX_IN64
+                 : "scc", "s88", "s89", "s90", "s91", "s92", "s93");
+  } else {
+    asm volatile(SPX_HEAD64 SPX_RL64(0, "s88", "s89") SPX_RL64(1, "s90", "s91")
+                     SPX_RL64(2, "s92", "s93") SPX_RL64(3, "s94", "s95") SPX_FM64(0, "s[88:89]")
+                         SPX_FM64(1, "s[90:91]") SPX_FM64(2, "s[92:93]") SPX_FM64(3, "s[94:95]")
+                             SPX_TAIL64
+                 : SPX_AO64(0), SPX_AO64(1), SPX_AO64(2), SPX_AO64(3)
+                 : SPX_HALVES(0), SPX_HALVES(1), SPX_HALVES(2), SPX_HALVES(3), SPX_IN64
+                 : "scc", "s88", "s89", "s90", "s91", "s92", "s93", "s94", "s95");
+  }
+}
+#undef SPX_RL64
+#undef SPX_FM64
+#undef SPX_AO64
+#undef SPX_HALVES
+#undef SPX_HEAD64
+#undef SPX_TAIL64
+#undef SPX_IN64
+// A chunk of up to eight FP64 columns: two asm statements of up to four.
+template <int J0, int CN, int C, int N>
+__device__ __forceinline__ void chunk_update_f64(double (&a)[N], uint32_t mp, int p, double l) {
+  if constexpr (CN <= 4) {
+    chunk_update_f64_part<J0, CN, C, N>(a, mp, p, l);
+  } else {
+    chunk_update_f64_part<J0, 4, C, N>(a, mp, p, l);
+    chunk_update_f64_part<J0 + 4, CN - 4, C, N>(a, mp, p, l);
+  }
+}
+
 template <int N, typename T>
 struct RowLU {
   T a[N];
@@ -247,24 +307,18 @@ struct RowLU {
       const T l = pos > k ? a[k] * inv : T(0);
       if (pos > k) a[k] = l;
       if constexpr (k + 1 < N) {
-        if constexpr (kF64) {
-          // FP64: every column (in C++ the skip's branches cost more in
-          // spills than they save)
-          static_for<k + 1, N>([&](auto J) __attribute__((always_inline)) {
-            constexpr int j = J;
-            a[j] = fma(-bcast(a[j], p), l, a[j]);
-          });
-        } else {
-          const uint32_t mp = (uint32_t)__builtin_amdgcn_readlane((int)cmask, p);
-          if (l != T(0)) cmask |= mp;
-          static_for<(k + 1) / kChunk, (N + kChunk - 1) / kChunk>(
-              [&](auto C) __attribute__((always_inline)) {
-                constexpr int c = C;
-                constexpr int j0 = c * kChunk > k + 1 ? c * kChunk : k + 1;
-                constexpr int j1 = (c + 1) * kChunk < N ? (c + 1) * kChunk : N;
+        const uint32_t mp = (uint32_t)__builtin_amdgcn_readlane((int)cmask, p);
+        if (l != T(0)) cmask |= mp;
+        static_for<(k + 1) / kChunk, (N + kChunk - 1) / kChunk>(
+            [&](auto C) __attribute__((always_inline)) {
+              constexpr int c = C;
+              constexpr int j0 = c * kChunk > k + 1 ? c * kChunk : k + 1;
+              constexpr int j1 = (c + 1) * kChunk < N ? (c + 1) * kChunk : N;
+              if constexpr (kF64)
+                chunk_update_f64<j0, j1 - j0, c>(a, mp, p, l);
+              else
                 chunk_update_f32<j0, j1 - j0, c>(a, mp, p, l);
-              });
-        }
+            });
       }
     });
     inexact = kF64 && !Exact && __any(nearby != 0u);
