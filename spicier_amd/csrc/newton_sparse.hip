@@ -14,6 +14,7 @@
 
 #include "assembly.h"
 #include "dcop_driver.h"
+#include "lu_wave.h"
 #include "program.h"
 #include "sparse_plan.h"
 
@@ -35,6 +36,130 @@ __device__ __forceinline__ bool block_all_t(bool p, int tid, int* red) {
   return r;
 }
 
+// v - sum_t val[t] * y[idx[t]] for t = 0 .. len-1, accumulated in that order
+// (one fma per term).  The terms' index and value loads (global or LDS) go
+// out eight at a time, then their y reads, before the eight fmas: a long row
+// of a triangular solve is a few load rounds, not one per four terms.
+__device__ __forceinline__ double row_dot(const double* val, const int* idx, int len,
+                                          const double* y, double v) {
+  constexpr int U = 8;
+  int t = 0;
+  for (; t + U <= len; t += U) {
+    int j[U];
+    double a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      j[u] = idx[t + u];
+      a[u] = val[t + u];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) b[u] = y[j[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v = fma(-a[u], b[u], v);
+  }
+#pragma unroll 4
+  for (; t < len; ++t) v = fma(-val[t], y[idx[t]], v);
+  return v;
+}
+
+// Register row width of the dense tail (the plan caps D at this).
+constexpr int kTailN = 48;
+
+// The plan's dense tail block (rows and columns n-D .. n-1, D <= kTailN), one
+// row per lane of the calling wave: entries the plan does not store are
+// zero, lanes past D hold zero rows.  Every index is clamped before its
+// load, so no lane reads outside the tables.
+__device__ __forceinline__ void tail_load(const DevSparse& sp, const double* lu, int lane,
+                                          double (&a)[kTailN]) {
+  // opaque per call: otherwise the compiler hoists the column addresses and
+  // tests out of the instance loop as live registers, which spill
+  int D = sp.tail;
+  asm volatile("" : "+s"(D));
+  asm volatile("" : "+v"(lane));
+  const int* ti = sp.td_idx + (lane < D ? lane : 0) * D;
+  static_for<0, kTailN>([&](auto C) __attribute__((always_inline)) {
+    constexpr int c = C;
+    const int q0 = ti[c < D ? c : 0];
+    const int q = (lane < D && c < D) ? q0 : -1;
+    const double v = lu[q >= 0 ? q : 0];
+    a[c] = q >= 0 ? v : 0.0;
+    // eight columns' loads in flight at a time: the scheduler would
+    // otherwise issue all 64 index and value loads first and hold them in
+    // registers beside the row
+    if constexpr ((c & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+  });
+}
+
+// Factors the dense tail in one wave, after the sparse levels have applied
+// every update from earlier columns, and stores the factors back.  Step k:
+// the pivot row is lane k; rows below take l = a * (1 / pivot) and the
+// update fma(-u, l, a) of each column right of k.  These are the
+// multipliers and updates the one-column levels would
+// form, in the same order (rows above the pivot take l = 0, which leaves
+// them unchanged).  A zero pivot stays in the stored diagonal, where the
+// caller's pivot test finds it.
+__device__ __forceinline__ void tail_factor(const DevSparse& sp, double* lu, int lane) {
+  int D = sp.tail;  // opaque, as in tail_load: the step tests stay in the steps
+  asm volatile("" : "+s"(D));
+  asm volatile("" : "+v"(lane));
+  double a[kTailN];
+  tail_load(sp, lu, lane, a);
+  // steps past D meet zero rows (pivot 0, multipliers 0) and change
+  // nothing, so every step runs without a test
+  static_for<0, kTailN>([&](auto K) __attribute__((always_inline)) {
+    constexpr int k = K;
+    const double pv = bcast(a[k], k);
+    const double inv = pv != 0.0 ? 1.0 / pv : 0.0;
+    const double l = lane > k ? a[k] * inv : 0.0;
+    if (lane > k) a[k] = l;
+    static_for<k + 1, kTailN>([&](auto J) __attribute__((always_inline)) {
+      constexpr int j = J;
+      a[j] = fma(-bcast(a[j], k), l, a[j]);
+      // eight broadcasts (sixteen SGPRs) in flight at a time
+      if constexpr ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    });
+  });
+  int Ds = D, ls = lane;
+  asm volatile("" : "+s"(Ds));
+  asm volatile("" : "+v"(ls));
+  const int* ti = sp.td_idx + (ls < Ds ? ls : 0) * Ds;
+  static_for<0, kTailN>([&](auto C) __attribute__((always_inline)) {
+    constexpr int c = C;
+    const int q0 = ti[c < Ds ? c : 0];
+    if (ls < Ds && c < Ds && q0 >= 0) lu[q0] = a[c];
+    if constexpr ((c & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+  });
+}
+
+// Both triangular solves of the dense tail in one wave: y of the tail rows
+// holds b minus their terms in earlier columns on entry and x on return.
+// Forward, column k: rows below take fma(-L(r, k), y_k, y_r); backward,
+// column k from the last: x_k = y_k * (1 / U(k, k)), rows above take
+// fma(-U(r, k), x_k, y_r).
+__device__ __forceinline__ void tail_solve(const DevSparse& sp, const double* lu, double* y, int n,
+                                           int lane) {
+  int D = sp.tail;  // opaque, as in tail_load
+  asm volatile("" : "+s"(D));
+  asm volatile("" : "+v"(lane));
+  const int t0 = n - D;
+  double a[kTailN];
+  tail_load(sp, lu, lane, a);
+  double v = lane < D ? y[t0 + lane] : 0.0;
+  const double dinv = 1.0 / (lane < D ? lu[sp.diag[t0 + lane]] : 1.0);
+  // steps past D broadcast a zero and change nothing
+  static_for<0, kTailN>([&](auto K) __attribute__((always_inline)) {
+    constexpr int k = K;
+    const double yk = bcast(v, k);
+    if (lane > k) v = fma(-a[k], yk, v);
+  });
+  static_for_down<0, kTailN>([&](auto K) __attribute__((always_inline)) {
+    constexpr int k = K;
+    const double xk = bcast(v * dinv, k);
+    if (lane < k) v = fma(-a[k], xk, v);
+  });
+  if (lane < D) y[t0 + lane] = v * dinv;
+}
+
 // Factors lu in place with the plan sp, then solves for the right-hand side
 // b(i) (i in unknown order); the solution is left in y (LDS, n) in
 // elimination order.  False if a pivot is exactly zero or the solution is
@@ -42,9 +167,12 @@ __device__ __forceinline__ bool block_all_t(bool p, int tid, int* red) {
 // updates target by target (contributions in column order); the triangular
 // solves are row-oriented over the forward levels and the levels counted
 // from the root.
-template <int T, class RHS>
+template <int T, bool kTail, class RHS>
 __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, double* lu, RHS b,
                                                     double* y, int tid, int* red) {
+  // with kTail, the plan's dense tail (if any) is factored and solved in
+  // wave 0's registers in place of its one-column levels
+  const int D = kTail ? sp.tail : 0;
   // One level's factor work, threads [t0, t0 + stride).  A thread takes four
   // multipliers, then four targets, at a time and issues their index and
   // value loads together (the values are in L2 or LDS, the plan in L2): a
@@ -132,23 +260,18 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
     for (int r = sp.lev_ptr[L] + t0; r < r1; r += stride) {
       const int i = sp.lev_rows[r];
       const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
-      const double* li = lu + sp.diag[i] - nl;
-      double v = y[i];
-#pragma unroll 4
-      for (int t = 0; t < nl; ++t) v = fma(-li[t], y[sp.ut_row[u0 + t]], v);
-      y[i] = v;
+      y[i] = row_dot(lu + sp.diag[i] - nl, sp.ut_row + u0, nl, y, y[i]);
     }
   };
+  const int* bptr = D > 0 ? sp.tblev_ptr : sp.blev_ptr;
+  const int* brows = D > 0 ? sp.tblev_rows : sp.blev_rows;
   auto backward_level = [&](int L, int t0, int stride) {
-    const int r1 = sp.blev_ptr[L + 1];
-    for (int r = sp.blev_ptr[L] + t0; r < r1; r += stride) {
-      const int i = sp.blev_rows[r];
+    const int r1 = bptr[L + 1];
+    for (int r = bptr[L] + t0; r < r1; r += stride) {
+      const int i = brows[r];
       const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
       const double* ui = lu + sp.diag[i];
-      double v = y[i];
-#pragma unroll 4
-      for (int t = 0; t < ns; ++t) v = fma(-ui[1 + t], y[sp.s_row[s0 + t]], v);
-      y[i] = v * (1.0 / ui[0]);
+      y[i] = row_dot(ui + 1, sp.s_row + s0, ns, y, y[i]) * (1.0 / ui[0]);
     }
   };
   // Runs levels [0, count) of one pass: a run of levels flagged narrow goes
@@ -169,19 +292,34 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
       }
     }
   };
-  pass(sp.levels, sp.lev_flags, 1, [&](int l) { factor_level(l, 0, 1, false); },
+  pass(sp.levels - D, sp.lev_flags, 1, [&](int l) { factor_level(l, 0, 1, false); },
        [&](int l) { factor_level(l, tid, T, true); });
+  if (D > 0) {
+    if (tid < kWave) tail_factor(sp, lu, tid);
+    __syncthreads();
+  }
   bool ok = true;
   for (int k = tid; k < n; k += T) ok = ok && lu[sp.diag[k]] != 0.0;
   if (!block_all_t<T>(ok, tid, red)) return false;
   for (int k = tid; k < n; k += T) y[k] = b(sp.perm[k]);
   __syncthreads();
-  pass(sp.levels, sp.lev_flags, 2, [&](int l) { forward_level(l, 0, 1); },
+  pass(sp.levels - D, sp.lev_flags, 2, [&](int l) { forward_level(l, 0, 1); },
        [&](int l) {
          forward_level(l, tid, T);
          __syncthreads();
        });
-  pass(sp.blevels, sp.blev_flags, 1, [&](int l) { backward_level(l, 0, 1); },
+  if (D > 0) {
+    for (int r = tid; r < D; r += T) {  // the tail rows' terms in earlier columns
+      const int i = n - D + r;
+      const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
+      y[i] = row_dot(lu + sp.diag[i] - nl, sp.ut_row + u0, sp.tl_cnt[r], y, y[i]);
+    }
+    __syncthreads();
+    if (tid < kWave) tail_solve(sp, lu, y, n, tid);
+    __syncthreads();
+  }
+  pass(D > 0 ? sp.tblevels : sp.blevels, D > 0 ? sp.tblev_flags : sp.blev_flags, 1,
+       [&](int l) { backward_level(l, 0, 1); },
        [&](int l) {
          backward_level(l, tid, T);
          __syncthreads();
@@ -191,9 +329,12 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
   return block_all_t<T>(fin, tid, red);
 }
 
-template <int T_>
+// kTail_: the factorisation takes the plan's dense tail in registers (the
+// DC kernel; the transient kernels keep the level form and its registers)
+template <int T_, bool kTail_ = false>
 struct SparseInstance {
   static constexpr int T = T_;
+  static constexpr bool kTail = kTail_;
   const DevProgram& g;
   const DevSparse& sp;
   const DevNrCfg& cfg;
@@ -243,7 +384,7 @@ struct SparseInstance {
 
   __device__ __forceinline__ bool factor_solve() {
     ++nfact;
-    return sparse_factor_solve<T>(sp, g.n, lu, [&](int i) { return rhs[i]; }, y, tid, red);
+    return sparse_factor_solve<T, kTail>(sp, g.n, lu, [&](int i) { return rhs[i]; }, y, tid, red);
   }
 
   __device__ __forceinline__ int newton(double gmin, double sf, int& iters, int& fp64_escalations) {
@@ -401,8 +542,10 @@ inline size_t sparse_smem_bytes(const DevProgram& g, const DevSparse& sp, bool l
   return b;
 }
 
+// Three waves per SIMD: the dense tail's register row (128 VGPRs) would
+// otherwise set the allocation for the whole kernel.
 template <int T>
-__global__ __launch_bounds__(T) void k_newton_sparse(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_newton_sparse(
     DevProgram g, DevSparse sp, DevNrCfg cfg, const double* __restrict__ params,
     const double* __restrict__ x0, int64_t B, DevOut out, double* ws, int64_t slot,
     int lu_in_lds) {
@@ -428,7 +571,7 @@ __global__ __launch_bounds__(T) void k_newton_sparse(
     }
     for (int i = tid; i < n; i += T) xs[i] = x0 ? x0[inst * n + i] : 0.0;
     __syncthreads();
-    SparseInstance<T> I{g, sp, cfg, tid, params + inst * g.P, vals, vst, av, lu, rhs, xs, xw, y,
+    SparseInstance<T, true> I{g, sp, cfg, tid, params + inst * g.P, vals, vst, av, lu, rhs, xs, xw, y,
                      red, redd};
     run_dcop(I, g, cfg, x0 != nullptr, out, inst, 0);
     __syncthreads();
@@ -585,7 +728,7 @@ hipError_t launch_transient_sparse(const DevProgram& g, const DevSparse& sp, con
 // crates/spicier-batched-sweep/src/faer_sparse_solver.rs:196-362): system
 // `inst` has triplet values values[inst][nnz]; pattern entry q sums the
 // triplets pos_trip[pos_off[q] .. pos_off[q+1]) in triplet order.
-__global__ __launch_bounds__(kSparseThreads) void k_sparse_solve(
+__global__ __launch_bounds__(kSparseThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_sparse_solve(
     DevSparse sp, int n, int num_pos, const int* __restrict__ pos_off,
     const int* __restrict__ pos_trip, int64_t nnz, const double* __restrict__ values,
     const double* __restrict__ rhs, int64_t B, double* __restrict__ x, uint8_t* singular,
@@ -608,7 +751,7 @@ __global__ __launch_bounds__(kSparseThreads) void k_sparse_solve(
     for (int f = tid; f < sp.num_fill; f += kSparseThreads) lu[sp.fill[f]] = 0.0;
     __syncthreads();
     const double* b = rhs + inst * n;
-    const bool ok = sparse_factor_solve<kSparseThreads>(
+    const bool ok = sparse_factor_solve<kSparseThreads, true>(
         sp, n, lu, [&](int i) { return b[i]; }, y, tid, red);
     for (int k = tid; k < n; k += kSparseThreads) x[inst * n + sp.perm[k]] = ok ? y[k] : 0.0;
     if (tid == 0 && singular) singular[inst] = ok ? 0 : 1;

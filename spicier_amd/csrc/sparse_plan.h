@@ -33,6 +33,19 @@
 //      levels.  A level whose work is a handful of operations (a chain in
 //      the tree: a ladder's is all chain) is marked narrow; a run of narrow
 //      levels is done by one thread in order, with no barrier inside.
+//   4. dense tail: when the last D >= 24 levels (D <= 48) are one column
+//      each and those columns are the last D of the order, the trailing
+//      D x D block is dense (the root separator of a grid: 48 columns for a
+//      32x32 grid) and closed — every entry below or right of a tail pivot
+//      lies in the block.  Kernels that take it factor the block in one
+//      wave's registers after the sparse levels (row r of the block in lane
+//      r, pivot row broadcast by v_readlane: the same multipliers and the
+//      same fma(-l, u, a) updates in the same column order as the levels
+//      would apply) and run the block's triangular solves there too, in
+//      place of D levels of the factorisation and of each solve; the tail
+//      rows' terms in earlier columns (a prefix of each L row) are summed
+//      first.  The backward levels of the remaining rows are then counted
+//      with the tail already solved (tblev_*).
 // The numeric factorisation is Gaussian elimination without pivoting in
 // this order; multipliers are formed as a * (1 / pivot) and updates as
 // fma(-l, u, a).
@@ -71,6 +84,13 @@ struct DevSparse {
   const int* blev_flags;  // blevels: bit 0 the level's backward rows are narrow
   const int* pos_lu;   // num_pos: LU index of pattern entry q (circuit program order)
   const int* fill;     // num_fill LU indices to zero before scattering
+  int tail;            // D <= 48: dense trailing block size (0: none); rows n-D .. n-1
+  int tblevels;        // backward levels of rows 0 .. n-D-1 with the tail solved
+  const int* td_idx;   // D*D: LU index of block entry (n-D+r, n-D+c), -1 if not stored
+  const int* tl_cnt;   // D: L terms of tail row r in columns < n-D (a prefix of its L row)
+  const int* tblev_ptr;    // tblevels + 1 into tblev_rows
+  const int* tblev_rows;
+  const int* tblev_flags;  // bit 0: the level's rows are narrow
   const int* blob;     // the plan's one int block: every pointer above points into it
   int blob_ints;
 };
@@ -102,6 +122,11 @@ __host__ __device__ inline DevSparse rebase_sparse(const DevSparse& s, const int
   d.blev_flags = mv(s.blev_flags);
   d.pos_lu = mv(s.pos_lu);
   d.fill = mv(s.fill);
+  d.td_idx = mv(s.td_idx);
+  d.tl_cnt = mv(s.tl_cnt);
+  d.tblev_ptr = mv(s.tblev_ptr);
+  d.tblev_rows = mv(s.tblev_rows);
+  d.tblev_flags = mv(s.tblev_flags);
   d.blob = to;
   return d;
 }
@@ -111,13 +136,15 @@ __host__ __device__ inline DevSparse rebase_sparse(const DevSparse& s, const int
 // else the reason the pattern cannot use the sparse path.
 struct HostSparsePlan {
   int nnz_lu = 0, max_s = 0, num_fill = 0, levels = 0, blevels = 0, max_level_rows = 0;
+  int tail = 0, tblevels = 0;
   int ordering = 0;  // 0 minimum degree, 1 nested dissection
   int64_t num_upd = 0;
   std::vector<int> blob;
   size_t o_perm = 0, o_diag = 0, o_colptr = 0, o_srow = 0, o_utptr = 0, o_utrow = 0,
          o_levptr = 0, o_levrows = 0, o_paptr = 0, o_pal = 0, o_pad = 0, o_tbptr = 0,
          o_tbt = 0, o_tbc = 0, o_cbl = 0, o_cbu = 0, o_blevptr = 0, o_blevrows = 0,
-         o_poslu = 0, o_fill = 0, o_levflags = 0, o_blevflags = 0;
+         o_poslu = 0, o_fill = 0, o_levflags = 0, o_blevflags = 0, o_tdidx = 0, o_tlcnt = 0,
+         o_tblevptr = 0, o_tblevrows = 0, o_tblevflags = 0;
 };
 
 // n unknowns, the first num_nodes of them node voltages; pattern entries

@@ -368,6 +368,7 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
   // narrow levels: serial cost (operations one thread would do) small enough
   // that doing them in order beats two workgroup barriers
   constexpr int kNarrow = 12;
+  constexpr int kTailMin = 24, kTailMax = 48;
   std::vector<int> lev_flags(levels, 0), blev_flags(blevels, 0);
   for (int L = 0; L < levels; ++L) {
     const int fa = (pa_ptr[L + 1] - pa_ptr[L]) + (tb_c[tb_ptr[L + 1]] - tb_c[tb_ptr[L]]);
@@ -379,6 +380,45 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
     int bw = 0;
     for (int r = blev_ptr[L]; r < blev_ptr[L + 1]; ++r) bw += 1 + (int)S[blev_rows[r]].size();
     blev_flags[L] = bw <= kNarrow ? 1 : 0;
+  }
+  // dense tail: the last D levels one column each, those columns n-D .. n-1
+  // in order, and the block closed under elimination
+  int tail = 0;
+  while (tail < kTailMax && tail < levels && by_lev[levels - 1 - tail].size() == 1 &&
+         by_lev[levels - 1 - tail][0] == n - 1 - tail)
+    ++tail;
+  for (int k = n - tail; k < n && tail > 0; ++k)
+    for (int i : S[k])
+      if (i < n - tail) tail = 0;  // not closed (cannot happen for a level chain)
+  if (tail < kTailMin) tail = 0;
+  const int t0 = n - tail;
+  std::vector<int> td_idx, tl_cnt, tblev_ptr(1, 0), tblev_rows, tblev_flags;
+  int tblevels = 0;
+  if (tail > 0) {
+    for (int r = 0; r < tail; ++r)
+      for (int c = 0; c < tail; ++c) td_idx.push_back(idx(t0 + r, t0 + c));
+    for (int r = 0; r < tail; ++r) {
+      int cnt = 0;
+      for (int j : Lrow[t0 + r]) cnt += j < t0 ? 1 : 0;  // Lrow ascending: a prefix
+      tl_cnt.push_back(cnt);
+    }
+    std::vector<int> tbl(n, 0);
+    for (int k = t0 - 1; k >= 0; --k) {
+      for (int j : S[k])
+        if (j < t0) tbl[k] = std::max(tbl[k], tbl[j] + 1);
+      tblevels = std::max(tblevels, tbl[k] + 1);
+    }
+    std::vector<std::vector<int>> by_tbl(tblevels);
+    for (int k = 0; k < t0; ++k) by_tbl[tbl[k]].push_back(k);
+    for (int L = 0; L < tblevels; ++L) {
+      int bw = 0;
+      for (int k : by_tbl[L]) {
+        tblev_rows.push_back(k);
+        bw += 1 + (int)S[k].size();
+      }
+      tblev_ptr.push_back((int)tblev_rows.size());
+      tblev_flags.push_back(bw <= kNarrow ? 1 : 0);
+    }
   }
   std::vector<char> covered(nnz_lu, 0);
   for (int rc : pos_rc) {
@@ -397,6 +437,8 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
   plan.levels = levels;
   plan.blevels = blevels;
   plan.max_level_rows = max_level_rows;
+  plan.tail = tail;
+  plan.tblevels = tblevels;
   plan.blob.clear();
   auto put = [&](const std::vector<int>& v) {
     size_t off = plan.blob.size();
@@ -426,6 +468,11 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
   plan.o_fill = put(fill);
   plan.o_levflags = put(lev_flags);
   plan.o_blevflags = put(blev_flags);
+  plan.o_tdidx = put(td_idx);
+  plan.o_tlcnt = put(tl_cnt);
+  plan.o_tblevptr = put(tblev_ptr);
+  plan.o_tblevrows = put(tblev_rows);
+  plan.o_tblevflags = put(tblev_flags);
   plan.blob.push_back(0);  // keeps every offset a valid address
   return "";
 }
@@ -458,6 +505,13 @@ DevSparse device_sparse(const HostSparsePlan& p, const int* base) {
   d.fill = base + p.o_fill;
   d.lev_flags = base + p.o_levflags;
   d.blev_flags = base + p.o_blevflags;
+  d.tail = p.tail;
+  d.tblevels = p.tblevels;
+  d.td_idx = base + p.o_tdidx;
+  d.tl_cnt = base + p.o_tlcnt;
+  d.tblev_ptr = base + p.o_tblevptr;
+  d.tblev_rows = base + p.o_tblevrows;
+  d.tblev_flags = base + p.o_tblevflags;
   d.blob = base;
   d.blob_ints = (int)p.blob.size();
   return d;
