@@ -669,16 +669,30 @@ hipError_t sparse_transient_shape_t(const DevProgram& g, const DevSparse& sp, in
   if (*lu_in_lds && sparse_smem_bytes(g, sp, true, T) > 160 * 1024) *lu_in_lds = 0;
   *smem = sparse_smem_bytes(g, sp, *lu_in_lds, T);
   // the plan's tables follow in LDS while the whole stays within the budget
+  // and costs no resident workgroup: a level then reads its index tables
+  // from LDS instead of L2, but a one-wave workgroup is limited by LDS, and
+  // fewer instances in flight cost more than the L2 round trips save (a
+  // 256-unknown ladder: 3 instead of 8 workgroups per CU, 204 vs 116 ms)
   const size_t plan_bytes = (size_t)sp.blob_ints * sizeof(int);
-  if (*smem + plan_bytes <= lds_budget) *smem = (*smem + plan_bytes) | kPlanInLdsBit;
-  *slot = sparse_slot_doubles(g, sp, *lu_in_lds) + ((3 * g.num_elems + 31) & ~31);
-  const size_t bytes = *smem & ~kPlanInLdsBit;
+  const int cap = (int)(*smem > lds_budget ? *smem : lds_budget);
   hipError_t e = hipFuncSetAttribute((const void*)k_transient_sparse<T, false>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, cap);
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_transient_sparse<T, true>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, cap);
   if (e != hipSuccess) return e;
+  if (*smem + plan_bytes <= lds_budget) {
+    int without = 0, with = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &without, (const void*)k_transient_sparse<T, false>, T, *smem);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &with, (const void*)k_transient_sparse<T, false>, T, *smem + plan_bytes);
+    if (e != hipSuccess) return e;
+    if (with >= without) *smem = (*smem + plan_bytes) | kPlanInLdsBit;
+  }
+  *slot = sparse_slot_doubles(g, sp, *lu_in_lds) + ((3 * g.num_elems + 31) & ~31);
+  const size_t bytes = *smem & ~kPlanInLdsBit;
   // sized for the lean kernel; the persistent loop covers any grid
   return grid_for((const void*)k_transient_sparse<T, false>, bytes, B, grid, T);
 }
