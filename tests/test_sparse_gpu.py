@@ -70,6 +70,19 @@ def test_grid32_reference_limiter_fixed_point_is_flagged(oracle):
     assert np.all(got.flags & A.SPX_FLAG_KCL_FAIL)
 
 
+def test_grid24_dense_root_block_matches_oracle(oracle):
+    # n = 580: the order's last 39 levels are single columns forming the dense
+    # root block, which the kernel factors and solves in one wave's registers
+    c = CK.grid_diodes(24)
+    assert c.size == 580
+    B = 8
+    got = E.sweep_dcop(c, B, seed=5, config=_pnj())
+    want = oracle.sweep_dcop(c, got.params, A.default_nr_config(limiter=A.SPX_LIMIT_PNJLIM),
+                             nthreads=16)
+    _check(got, want, "grid24")
+    assert np.all(got.accepted_mask())
+
+
 def test_linear_grid_single_solve(oracle):
     # no diodes: one sparse solve per instance (sweep.rs:159-217)
     c = CK.grid_diodes(20, every=10**9)
@@ -151,6 +164,26 @@ def test_sparse_solve_matches_lapack():
     # about cond * eps; the residual pins the engine's own solution
     err = np.abs(x - want).max(axis=1) / np.abs(want).max(axis=1)
     assert np.all(err < 1e-10), err.max()
+    resid = np.abs(b - np.einsum("bij,bj->bi", m, x)).max(axis=1)
+    assert np.all(resid < 1e-12 * (np.abs(m).sum(axis=2).max(axis=1) * np.abs(x).max(axis=1))), resid.max()
+
+
+@pytest.mark.parametrize("N", [24, 32, 40])
+def test_sparse_solve_dense_root_block_matches_lapack(N):
+    # grids whose order ends in a dense root block (39 and 48 columns; at
+    # N = 40 the last 48 of a longer one-column chain): the block's
+    # factorization and both substitutions run in registers
+    rng = np.random.default_rng(N)
+    B = 6
+    n, rows, cols, vals = _grid_mna(N, rng, B)
+    b = rng.standard_normal((B, n))
+    r = E.HipSparseBatchedSolver().solve_batch_triplets(rows, cols, vals, b, n)
+    m = _dense(n, rows, cols, vals)
+    want = np.linalg.solve(m, b[..., None])[..., 0]
+    assert r.singular_indices == []
+    x = r.solutions.reshape(B, n)
+    err = np.abs(x - want).max(axis=1) / np.abs(want).max(axis=1)
+    assert np.all(err < 1e-9), err.max()
     resid = np.abs(b - np.einsum("bij,bj->bi", m, x)).max(axis=1)
     assert np.all(resid < 1e-12 * (np.abs(m).sum(axis=2).max(axis=1) * np.abs(x).max(axis=1))), resid.max()
 
