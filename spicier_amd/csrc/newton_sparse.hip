@@ -3,10 +3,10 @@
 // the shared-symbolic, level-scheduled LU of sparse_plan.h, FP64
 // throughout.
 //
-// Per instance the workgroup keeps x, rhs, the saved stepping solution and
-// the triangular-solve vector in LDS; element values, pattern values and the
-// filled L+U values live in the workgroup's slot of a global workspace (in
-// LDS instead when they fit), which stays in L2 while the instance runs.
+// Per instance the workgroup keeps x, rhs and the triangular-solve vector in
+// LDS; element values, pattern values, the stepping stages' saved solution
+// and the filled L+U values live in the workgroup's slot of a global
+// workspace (the L+U values in LDS instead when they fit).
 // The grid is persistent: each workgroup walks instances blockIdx.x,
 // blockIdx.x + gridDim.x, ..., so the workspace scales with the grid, not
 // with the batch.
@@ -346,7 +346,7 @@ struct SparseInstance {
   double* lu;        // filled L+U values
   double* rhs;       // LDS n
   double* xs;        // LDS n
-  double* xw;        // LDS n
+  double* xw;        // global slot n: the stepping stages' saved solution
   double* y;         // LDS n, elimination order
   int* red;          // LDS T/64 ints
   double* redd;      // LDS T/64 doubles
@@ -528,14 +528,17 @@ struct SparseInstance {
 };
 
 // Per-slot global workspace, in doubles.
+// The stepping stages' saved solution (xw, n) is here too: it is touched
+// once per stepping level, and in LDS it would cost a resident workgroup of
+// the one-wave transient kernels.
 inline int64_t sparse_slot_doubles(const DevProgram& g, const DevSparse& sp, bool lu_in_lds) {
-  int64_t d = (int64_t)g.num_vals + g.num_elems + g.num_pos + (lu_in_lds ? 0 : sp.nnz_lu);
+  int64_t d = (int64_t)g.num_vals + g.num_elems + g.num_pos + g.n + (lu_in_lds ? 0 : sp.nnz_lu);
   return (d + 31) & ~int64_t(31);  // 256-byte slots
 }
 
 inline size_t sparse_smem_bytes(const DevProgram& g, const DevSparse& sp, bool lu_in_lds,
                                 int threads = kSparseThreads) {
-  size_t d = 4 * (size_t)g.n + threads / kWave;
+  size_t d = 3 * (size_t)g.n + threads / kWave;
   size_t b = d * sizeof(double) + (threads / kWave) * sizeof(int);
   b = (b + 7) & ~size_t(7);
   if (lu_in_lds) b += (size_t)sp.nnz_lu * sizeof(double);
@@ -553,8 +556,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_n
   const int n = g.n, tid = threadIdx.x;
   double* rhs = smem;
   double* xs = rhs + n;
-  double* xw = xs + n;
-  double* y = xw + n;
+  double* y = xs + n;
   double* redd = y + n;
   int* red = reinterpret_cast<int*>(redd + T / kWave);
   double* lds_lu = reinterpret_cast<double*>(
@@ -563,7 +565,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_n
   double* vals = w;
   double* vst = vals + g.num_vals;
   double* av = vst + g.num_elems;
-  double* lu = lu_in_lds ? lds_lu : av + g.num_pos;
+  double* xw = av + g.num_pos;
+  double* lu = lu_in_lds ? lds_lu : xw + n;
   for (int64_t inst = blockIdx.x; inst < B; inst += gridDim.x) {
     if (tid == 0) {
       vals[kValOne] = 1.0;
@@ -612,8 +615,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_t
   const int n = g.n, tid = threadIdx.x;
   double* rhs = smem;
   double* xs = rhs + n;
-  double* xw = xs + n;
-  double* y = xw + n;
+  double* y = xs + n;
   double* redd = y + n;
   int* red = reinterpret_cast<int*>(redd + T / kWave);
   double* lds_lu = reinterpret_cast<double*>(
@@ -622,7 +624,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_t
   double* vals = w;
   double* vst = vals + g.num_vals;
   double* av = vst + g.num_elems;
-  double* vcap = av + g.num_pos;  // vprev, iprev, vpp: num_elems each
+  double* xw = av + g.num_pos;
+  double* vcap = xw + n;  // vprev, iprev, vpp: num_elems each
   double* lu = lu_in_lds ? lds_lu : vcap + 3 * g.num_elems;
   // small plans are read from LDS: every elimination level reads its index
   // tables, and from global memory each level would cost L2 round trips
