@@ -143,7 +143,11 @@ def _spectrum(samples, sample_rate, window, fft_size, zero_padding):
     out = empty_spectrum()
     for i in range(n_pos):
         out["frequencies"].append(float(i) * res)
-        re, im = X[i].real / norm, X[i].imag / norm
+        # IEEE division as the reference's f64: a zero coherent gain (a
+        # two-point Hanning window weighs both points zero) gives 0 / 0 = NaN
+        with np.errstate(divide="ignore", invalid="ignore"):
+            re = float(np.float64(X[i].real) / norm)
+            im = float(np.float64(X[i].imag) / norm)
         unscaled = math.hypot(re, im)
         scale = 1.0 if (i == 0 or (n_fft % 2 == 0 and i == n_pos - 1)) else 2.0
         mag = unscaled * scale

@@ -18,7 +18,6 @@
 // chirp's transform) depends only on the time grid and the configuration,
 // and is built once per grid on the host and kept in the handle.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <climits>
@@ -601,33 +600,91 @@ __global__ void k_fill_rows(T* p, int64_t count, T v) {
     p[i] = v;
 }
 
-// ThdStatistics over sorted values (spectral.rs:117-159): sums in a fixed
-// tree, so the result does not depend on the launch.
+// ThdStatistics (spectral.rs:117-159) in one workgroup, without sorting:
+// sums in a fixed tree (the result does not depend on the launch), min and
+// max by reduction, and the median's order statistics (n/2 - 1 and n/2, or
+// n/2) by radix selection on order-preserving 64-bit keys (sign bit set for
+// positives, all bits inverted for negatives: -0 ranks below +0, as in a
+// radix sort of the values): eight passes of eight bits, each counting the
+// candidates that still match the selected prefix into 256 LDS bins.
 constexpr int kStatThreads = 1024;
-__global__ __launch_bounds__(kStatThreads) void k_thd_stats(const double* __restrict__ sorted,
+
+__device__ __forceinline__ uint64_t order_key(double x) {
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+__device__ __forceinline__ double key_value(uint64_t k) {
+  return __longlong_as_double((long long)((k >> 63) ? (k & ~(1ull << 63)) : ~k));
+}
+
+// The k-th smallest (0-based) of v[0..n); every thread returns it.
+__device__ double select_kth(const double* __restrict__ v, int64_t n, int64_t k,
+                             unsigned* hist, uint64_t* s_prefix, int64_t* s_k) {
+  const int t = threadIdx.x;
+  uint64_t prefix = 0, mask = 0;
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    if (t < 256) hist[t] = 0;
+    __syncthreads();
+    for (int64_t i = t; i < n; i += kStatThreads) {
+      const uint64_t key = order_key(v[i]);
+      if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (t == 0) {
+      int64_t below = 0;
+      int d = 0;
+      for (; d < 255; ++d) {
+        if (below + (int64_t)hist[d] > k) break;
+        below += hist[d];
+      }
+      *s_prefix = prefix | ((uint64_t)d << shift);
+      *s_k = k - below;
+    }
+    __syncthreads();
+    prefix = *s_prefix;
+    k = *s_k;
+    mask |= 0xFFull << shift;
+    __syncthreads();
+  }
+  return key_value(prefix);
+}
+
+__global__ __launch_bounds__(kStatThreads) void k_thd_stats(const double* __restrict__ v,
                                                             int64_t n, double spec,
                                                             double* __restrict__ out) {
-  __shared__ double s0[kStatThreads], s1[kStatThreads];
+  __shared__ double s0[kStatThreads], s1[kStatThreads], s2[kStatThreads], s3[kStatThreads];
+  __shared__ unsigned hist[256];
+  __shared__ uint64_t s_prefix;
+  __shared__ int64_t s_k;
   const int t = threadIdx.x;
-  double sum = 0.0, pass = 0.0, nan = 0.0;
+  double sum = 0.0, pass = 0.0, nan = 0.0, lo = INFINITY, hi = -INFINITY;
   for (int64_t i = t; i < n; i += kStatThreads) {
-    const double x = sorted[i];
+    const double x = v[i];
     sum += x;
     pass += x <= spec ? 1.0 : 0.0;
     nan += x != x ? 1.0 : 0.0;
+    lo = fmin(lo, x);
+    hi = fmax(hi, x);
   }
   s0[t] = sum;
   s1[t] = pass;
+  s2[t] = lo;
+  s3[t] = hi;
   __syncthreads();
   for (int w = kStatThreads / 2; w > 0; w >>= 1) {
-    if (t < w) { s0[t] += s0[t + w]; s1[t] += s1[t + w]; }
+    if (t < w) {
+      s0[t] += s0[t + w];
+      s1[t] += s1[t + w];
+      s2[t] = fmin(s2[t], s2[t + w]);
+      s3[t] = fmax(s3[t], s3[t + w]);
+    }
     __syncthreads();
   }
-  const double mean = s0[0] / (double)n, passing = s1[0];
+  const double mean = s0[0] / (double)n, passing = s1[0], vmin = s2[0], vmax = s3[0];
   __syncthreads();
   double var = 0.0;
   for (int64_t i = t; i < n; i += kStatThreads) {
-    const double d = sorted[i] - mean;
+    const double d = v[i] - mean;
     var += d * d;
   }
   s0[t] = var;
@@ -637,14 +694,22 @@ __global__ __launch_bounds__(kStatThreads) void k_thd_stats(const double* __rest
     if (t < w) { s0[t] += s0[t + w]; s1[t] += s1[t + w]; }
     __syncthreads();
   }
+  const double sq = s0[0], nans = s1[0];
+  __syncthreads();
+  double med = 0.0;
+  if (nans == 0.0) {  // NaNs have no order; the host reports them
+    const double upper = select_kth(v, n, n / 2, hist, &s_prefix, &s_k);
+    med = (n % 2 == 0) ? (select_kth(v, n, n / 2 - 1, hist, &s_prefix, &s_k) + upper) / 2.0
+                       : upper;
+  }
   if (t == 0) {
     out[0] = mean;
-    out[1] = sqrt(s0[0] / (double)n);
-    out[2] = sorted[0];
-    out[3] = sorted[n - 1];
-    out[4] = (n % 2 == 0) ? (sorted[n / 2 - 1] + sorted[n / 2]) / 2.0 : sorted[n / 2];
+    out[1] = sqrt(sq / (double)n);
+    out[2] = vmin;
+    out[3] = vmax;
+    out[4] = med;
     out[5] = (passing / (double)n) * 100.0;
-    out[6] = s1[0];
+    out[6] = nans;
   }
 }
 
@@ -1227,7 +1292,7 @@ int spx_thd_statistics_device(spx_handle* h, const double* d_thd_percent, int64_
   if (int rc = check_handle(h)) return rc;
   if (count < 0) return fail(SPX_ERR_INVALID_DIMENSION, "negative count");
   if (!stats || (count > 0 && !d_thd_percent)) return fail(SPX_ERR_INVALID_ARGUMENT, "null buffer");
-  if (count > INT32_MAX) return fail(SPX_ERR_BATCH_TOO_LARGE, "too many THD values to sort");
+  if (count > INT32_MAX) return fail(SPX_ERR_BATCH_TOO_LARGE, "too many THD values (32-bit selection counts)");
   *stats = spx_thd_statistics{};
   stats->thd_spec = thd_spec;
   if (count == 0) {  // spectral.rs:104-115
@@ -1236,21 +1301,10 @@ int spx_thd_statistics_device(spx_handle* h, const double* d_thd_percent, int64_
   }
   std::lock_guard<std::mutex> lk(h->mu);
   DeviceGuard dg(h->device);
-  size_t tmp = 0;
-  SPX_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp, d_thd_percent, (double*)nullptr,
-                                            (int)count, 0, 64, h->stream),
-          SPX_ERR_BACKEND);
-  void *sorted = nullptr, *scratch = nullptr, *res = nullptr;
-  if (int rc = workspace(h, "sp_sorted", sizeof(double) * count, &sorted)) return rc;
-  if (int rc = workspace(h, "sp_sort_tmp", tmp, &scratch)) return rc;
+  void* res = nullptr;
   if (int rc = workspace(h, "sp_stats", 8 * sizeof(double), &res)) return rc;
-  SPX_HIP(hipcub::DeviceRadixSort::SortKeys(scratch, tmp, d_thd_percent,
-                                            static_cast<double*>(sorted), (int)count, 0, 64,
-                                            h->stream),
-          SPX_ERR_BACKEND);
-  hipLaunchKernelGGL(k_thd_stats, dim3(1), dim3(kStatThreads), 0, h->stream,
-                     static_cast<const double*>(sorted), count, thd_spec,
-                     static_cast<double*>(res));
+  hipLaunchKernelGGL(k_thd_stats, dim3(1), dim3(kStatThreads), 0, h->stream, d_thd_percent,
+                     count, thd_spec, static_cast<double*>(res));
   SPX_HIP(hipGetLastError(), SPX_ERR_BACKEND);
   double v[8];
   SPX_HIP(hipMemcpyAsync(v, res, sizeof(double) * 7, hipMemcpyDeviceToHost, h->stream),
