@@ -84,27 +84,63 @@ __global__ void k_sample_lcg(const spx_param_spec* __restrict__ ps, int P, uint6
 }
 
 // ------------------------------------------------------------- statistics
-// Pass 1: block b reduces instances [b*chunk, (b+1)*chunk) for every node;
-// thread j owns node j, so each step reads one contiguous row.
-__global__ void k_stats_partial(const double* __restrict__ v, const uint8_t* __restrict__ mask,
-                                const int* __restrict__ status, int64_t B, int n, int64_t chunk,
-                                double* __restrict__ part) {
+// Pass 1: block b reduces instances [b*chunk, (b+1)*chunk) for every node.
+// Four waves take the rows in turn (wave w: rows w, w+4, ...), lane j the
+// nodes j, j+64, ...: each step of a wave reads one contiguous row, and four
+// rows' loads go out before their sums (the mask and status tests are
+// wave-uniform scalar loads).  The waves' sums are then combined in wave
+// order, so the result does not depend on timing.
+constexpr int kStatWaves = 4;
+__global__ __launch_bounds__(kStatWaves * kWave) void k_stats_partial(
+    const double* __restrict__ v, const uint8_t* __restrict__ mask,
+    const int* __restrict__ status, int64_t B, int n, int64_t chunk,
+    double* __restrict__ part) {
+  __shared__ double red[kStatWaves][5][kWave];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int64_t i0 = (int64_t)blockIdx.x * chunk;
   const int64_t i1 = i0 + chunk < B ? i0 + chunk : B;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  for (int j0 = 0; j0 < n; j0 += kWave) {
+    const int j = j0 + lane;
+    const bool col = j < n;
     double cnt = 0, sum = 0, sq = 0, mn = INFINITY, mx = -INFINITY;
-    for (int64_t i = i0; i < i1; ++i) {
-      if (mask && !mask[i]) continue;
-      if (status && status[i] != SPX_CONVERGED) continue;
-      double x = v[i * n + j];
-      cnt += 1;
-      sum += x;
-      sq += x * x;
-      mn = fmin(mn, x);
-      mx = fmax(mx, x);
+    constexpr int U = 4;
+    for (int64_t i = i0 + wave; i < i1; i += U * kStatWaves) {
+      double x[U];
+      bool take[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t r = i + (int64_t)u * kStatWaves;
+        take[u] = r < i1 && (!mask || mask[r]) && (!status || status[r] == SPX_CONVERGED);
+        x[u] = (take[u] && col) ? v[r * n + j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (take[u]) {
+          cnt += 1;
+          sum += x[u];
+          sq += x[u] * x[u];
+          mn = fmin(mn, x[u]);
+          mx = fmax(mx, x[u]);
+        }
     }
-    double* o = part + ((int64_t)blockIdx.x * n + j) * 5;
-    o[0] = cnt; o[1] = sum; o[2] = sq; o[3] = mn; o[4] = mx;
+    red[wave][0][lane] = cnt;
+    red[wave][1][lane] = sum;
+    red[wave][2][lane] = sq;
+    red[wave][3][lane] = mn;
+    red[wave][4][lane] = mx;
+    __syncthreads();
+    if (wave == 0 && col) {
+      for (int w = 1; w < kStatWaves; ++w) {
+        cnt += red[w][0][lane];
+        sum += red[w][1][lane];
+        sq += red[w][2][lane];
+        mn = fmin(mn, red[w][3][lane]);
+        mx = fmax(mx, red[w][4][lane]);
+      }
+      double* o = part + ((int64_t)blockIdx.x * n + j) * 5;
+      o[0] = cnt; o[1] = sum; o[2] = sq; o[3] = mn; o[4] = mx;
+    }
+    __syncthreads();
   }
 }
 
@@ -197,9 +233,8 @@ hipError_t launch_stats(const double* v, const uint8_t* mask, const int* status,
   int64_t nb = stats_num_blocks(B);
   int64_t chunk = (B + nb - 1) / nb;
   if (chunk < 1) chunk = 1;
-  int threads = n < 256 ? ((n + 63) / 64) * 64 : 256;
-  hipLaunchKernelGGL(k_stats_partial, dim3((unsigned)nb), dim3(threads), 0, st, v, mask, status, B,
-                     n, chunk, part);
+  hipLaunchKernelGGL(k_stats_partial, dim3((unsigned)nb), dim3(kStatWaves * kWave), 0, st, v, mask,
+                     status, B, n, chunk, part);
   hipLaunchKernelGGL(k_stats_merge, dim3((unsigned)n), dim3(256), 0, st, part, (int)nb, n, out);
   return hipGetLastError();
 }

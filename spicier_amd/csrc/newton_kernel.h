@@ -399,15 +399,26 @@ namespace spx {
 // four, measured best on the 64-unknown sweep (two and five were slower).  The
 // register budget this sets spills a few hundred bytes per wave outside the
 // LU's inner steps; latency hiding across the extra waves is worth more.
+//
+// The exact-pivot pass (Exact) runs after a first pass that leaves the
+// instances it hands over at SPX_ACTIVE: each wave reads the statuses of
+// `scan` instances in one load and runs the active ones in turn, on a grid
+// of B / scan waves.  After FP32+IR (escalated_only 1) hand-overs are rare
+// and scan is 8: a grid of B one-wave workgroups that exit at once costs
+// more to launch than the pass has work.  After the FP64 fast pass (2) the
+// near-tie hand-overs are frequent enough that scan is 1, one instance per
+// wave as before (8 serialised them, 4.85 -> 5.04 ms).
+__host__ __device__ inline int exact_scan(int escalated_only) {
+  return escalated_only == 1 ? 8 : 1;
+}
+
 template <int N, bool IR, bool Exact>
 __global__ __launch_bounds__(64, IR ? 4 : 3) void k_newton_dense(DevProgram g, DevNrCfg cfg,
                                                      const double* __restrict__ params,
                                                      const double* __restrict__ x0, int64_t B,
                                                      DevOut out, int escalated_only) {
   extern __shared__ double smem[];
-  const int64_t inst = blockIdx.x;
   const int lane = threadIdx.x;
-  if (escalated_only && out.status[inst] != SPX_ACTIVE) return;
   double* J = smem;
   int* jc = reinterpret_cast<int*>(J + g.num_pos);
   uint32_t* rmask = reinterpret_cast<uint32_t*>(J + g.num_pos + (g.num_pos + 1) / 2);
@@ -417,17 +428,36 @@ __global__ __launch_bounds__(64, IR ? 4 : 3) void k_newton_dense(DevProgram g, D
   double* vals = xw + N;
   double* pv = vals + g.num_vals;
   double* vst = pv + g.P;
-  Instance<N, IR, Exact> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst};
   const int n = g.n;
-
-  for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
-  for (int p = lane; p < g.P; p += kWave) pv[p] = params[inst * g.P + p];
-  if (lane == 0) { vals[kValOne] = 1.0; vals[kValGmin] = 0.0; }
-  if (lane < N) xs[lane] = (x0 && lane < n) ? x0[inst * n + lane] : 0.0;
-  __syncthreads();
-  I.init_row();
-  __syncthreads();
-  run_dcop(I, g, cfg, x0 != nullptr, out, inst, escalated_only);
+  auto run_one = [&](int64_t inst) {
+    Instance<N, IR, Exact> I{g, cfg, lane, J, jc, rmask, rhs, xs, xw, vals, pv, vst};
+    for (int q = lane; q < g.num_pos; q += kWave) jc[q] = g.pos_rc[q] >> 16;
+    for (int p = lane; p < g.P; p += kWave) pv[p] = params[inst * g.P + p];
+    if (lane == 0) { vals[kValOne] = 1.0; vals[kValGmin] = 0.0; }
+    if (lane < N) xs[lane] = (x0 && lane < n) ? x0[inst * n + lane] : 0.0;
+    __syncthreads();
+    I.init_row();
+    __syncthreads();
+    run_dcop(I, g, cfg, x0 != nullptr, out, inst, escalated_only);
+    if constexpr (Exact) __syncthreads();  // the next instance reuses the LDS
+  };
+  if constexpr (Exact) {
+    const int scan = exact_scan(escalated_only);
+    const int64_t base = (int64_t)blockIdx.x * scan;
+    const int64_t mine = base + lane;
+    const bool act = lane < scan && mine < B &&
+                     (!escalated_only || out.status[mine] == SPX_ACTIVE);
+    uint64_t m = (uint64_t)__ballot(act);
+    while (m) {
+      const int k = __ffsll((unsigned long long)m) - 1;
+      m &= m - 1;
+      run_one(base + k);
+    }
+  } else {
+    const int64_t inst = blockIdx.x;
+    if (escalated_only && out.status[inst] != SPX_ACTIVE) return;
+    run_one(inst);
+  }
 }
 
 // Batched transient, one wave per instance: LDS as k_newton_dense plus the
@@ -538,7 +568,9 @@ hipError_t launch_newton_impl(const DevProgram& g, const DevNrCfg& cfg, const do
   hipError_t e = hipFuncSetAttribute((const void*)k_newton_dense<N, IR, Exact>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_newton_dense<N, IR, Exact>), dim3((unsigned)B), dim3(kWave), sm, st, g,
+  const int scan = Exact ? exact_scan(escalated_only) : 1;
+  const int64_t grid = (B + scan - 1) / scan;
+  hipLaunchKernelGGL((k_newton_dense<N, IR, Exact>), dim3((unsigned)grid), dim3(kWave), sm, st, g,
                      cfg, params, x0, B, out, escalated_only);
   return hipGetLastError();
 }

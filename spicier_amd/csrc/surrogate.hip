@@ -332,13 +332,35 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(
   const float* om = norm + 2 * n_in;
   const float* os = om + n_out;
   for (int64_t r0 = (int64_t)blockIdx.x * kInfRows; r0 < B; r0 += (int64_t)gridDim.x * kInfRows) {
-    // inputs (k_gather_inputs' arithmetic), padded columns zero
-    for (int e = tid; e < kInfRows * kp; e += 256) {
-      const int r = e / kp, i = e - r * kp;
-      const int64_t row = r0 + r;
-      float v = 0.f;
-      if (i < n_in && row < B) v = (float)((params[row * P + slots[i]] - norm[i]) / norm[n_in + i]);
-      Xs[r * ldx + i] = to_bf16(v);
+    // inputs (k_gather_inputs' arithmetic), padded columns zero.  Thread t
+    // takes column t % kp of rows t / kp, + 256 / kp, ...: its slot and
+    // normalisation are loaded once and its rows' parameter loads go out
+    // together (one round trip, not one per element)
+    {
+      constexpr int RS = 256 / KP, M = (kInfRows + RS - 1) / RS;
+      const int i = tid % kp, rr = tid / kp;
+      if (rr < RS) {
+        const bool in = i < n_in;
+        const int sl = in ? slots[i] : 0;
+        const float mu = in ? norm[i] : 0.f, sd = in ? norm[n_in + i] : 1.f;
+        double pv[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const int r = rr + m * RS;
+          const int64_t row = r0 + r;
+          pv[m] = (in && r < kInfRows && row < B) ? params[row * P + sl] : 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const int r = rr + m * RS;
+          if (r < kInfRows) {
+            const int64_t row = r0 + r;
+            float v = 0.f;
+            if (in && row < B) v = (float)((pv[m] - mu) / sd);
+            Xs[r * ldx + i] = to_bf16(v);
+          }
+        }
+      }
     }
     __syncthreads();
     infer_layer<HT, KP>(Xs, ldx, W1t, wave * 16 * HT, lane, [&](int r, int c, float a) {
