@@ -76,10 +76,10 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// Round to nearest even: gfx950's v_cvt_pk_bf16_f32 (one instruction, the
+// same bits as adding 0x7fff + lsb and truncating, for every finite value).
 __device__ __forceinline__ unsigned short to_bf16(float v) {
-  uint32_t b = __float_as_uint(v);
-  b += 0x7fffu + ((b >> 16) & 1u);  // round to nearest even
-  return (unsigned short)(b >> 16);
+  return __builtin_bit_cast(unsigned short, (__bf16)v);
 }
 
 __device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
