@@ -371,6 +371,71 @@ def grid_diodes(N: int = GRID_N, every: int = GRID_DIODE_EVERY) -> Circuit:
     return c
 
 
+# Thermal voltage at 300.15 K (crates/spicier-devices/src/diode.rs:41-45)
+THERMAL_VOLTAGE = 1.380649e-23 * 300.15 / 1.602176634e-19
+
+
+def newton_system(circuit: Circuit, params, x):
+    """The linear systems Newton steps solve at operating points x:
+    resistors, DC voltage and current sources, and diodes linearised at x
+    without junction limiting (the companion conductance
+    gd = max(IS e^(vd/(N Vt)) / (N Vt), 1e-12) and current
+    ieq = IS (e^(vd/(N Vt)) - 1) - gd vd of diode.rs:91-131, as the device
+    kernels stamp them), no Gmin.  params [B][P] and x [B][n] are a sweep's
+    parameters and solutions; the solution of system b is x[b] up to the
+    Newton tolerance when x[b] converged.  Returns (n, rows, cols,
+    vals[B][nnz], rhs[B][n]): one triplet pattern for the batch, duplicates
+    kept (the solvers sum them).  Other element kinds raise ValueError."""
+    params = np.asarray(params, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    B, n = x.shape[0], circuit.size
+    rows, cols, vals = [], [], []
+    rhs = np.zeros((B, n))
+
+    def v(node):  # node voltage per instance (ground 0)
+        return x[:, node - 1] if node > 0 else np.zeros(B)
+
+    def add(i, j, val):  # MNA entry (i, j) of node indices, ground skipped
+        if i > 0 and j > 0:
+            rows.append(i - 1); cols.append(j - 1); vals.append(val)
+
+    branch = circuit.num_nodes
+    for kind, nd, p in circuit.elements:
+        a, b = nd[0], nd[1]
+        if kind == A.SPX_EL_RESISTOR:
+            g = 1.0 / params[:, p[0]]
+            add(a, a, g); add(b, b, g); add(a, b, -g); add(b, a, -g)
+        elif kind == A.SPX_EL_VSOURCE:
+            one = np.ones(B)
+            if a > 0:
+                rows += [a - 1, branch]; cols += [branch, a - 1]; vals += [one, one]
+            if b > 0:
+                rows += [b - 1, branch]; cols += [branch, b - 1]; vals += [-one, -one]
+            rhs[:, branch] = params[:, p[0]]
+            branch += 1
+        elif kind == A.SPX_EL_ISOURCE:
+            i = params[:, p[0]]
+            if a > 0:
+                rhs[:, a - 1] -= i
+            if b > 0:
+                rhs[:, b - 1] += i
+        elif kind == A.SPX_EL_DIODE:
+            is_, nvt = params[:, p[0]], params[:, p[1]] * THERMAL_VOLTAGE
+            vd = v(a) - v(b)
+            e = np.exp(vd / nvt)
+            gd = np.maximum(is_ * e / nvt, 1e-12)
+            ieq = is_ * (e - 1.0) - gd * vd
+            add(a, a, gd); add(b, b, gd); add(a, b, -gd); add(b, a, -gd)
+            if a > 0:
+                rhs[:, a - 1] -= ieq
+            if b > 0:
+                rhs[:, b - 1] += ieq
+        else:
+            raise ValueError(f"newton_system: element kind {kind} is not linearised here")
+    return n, np.array(rows, dtype=np.int32), np.array(cols, dtype=np.int32), \
+        np.stack(vals, axis=1), rhs
+
+
 def grid_linear_batch(N: int = GRID_N, batch: int = 4096, seed: int = 0, g_diode: float = 0.04):
     """Triplets of `batch` linear systems of the N x N grid workload's
     Jacobian family: edge conductances 1/R with R~U(0.9, 1.1) kOhm, the four
