@@ -41,6 +41,10 @@ __device__ __forceinline__ float wave_sum_t(float v) {
 }
 __device__ __forceinline__ double wave_sum_t(double v) { return wave_sum(v); }
 
+// Elements of w and v_j per thread held in registers by the Arnoldi step
+// (systems up to kGmresThreads * kMgsRegs unknowns).
+constexpr int kMgsRegs = 8;
+
 template <typename T>
 __device__ __forceinline__ T block_sum(T v, T* red, int tid) {
   const T w = wave_sum_t(v);
@@ -121,26 +125,78 @@ __device__ int gmres_run(const DevGmres& p, const T* av, const T* b, T* x, const
       spmv(p, av, zk, s.w, tid);
       __syncthreads();
       T* hk = s.H + k * (m + 1);
-      for (int j = 0; j <= k; ++j) {  // modified Gram-Schmidt
-        const T* vj = s.V + (int64_t)j * n;
-        part = T(0);
-        for (int i = tid; i < n; i += kGmresThreads) part += vj[i] * s.w[i];
-        const T hij = block_sum(part, s.red, tid);
-        if (tid == 0) hk[j] = hij;
-        for (int i = tid; i < n; i += kGmresThreads) s.w[i] -= hij * vj[i];
-      }
-      part = T(0);
-      for (int i = tid; i < n; i += kGmresThreads) part += s.w[i] * s.w[i];
-      const T wn = sqrt(block_sum(part, s.red, tid));
-      if (tid == 0) hk[k + 1] = wn;
-      if (wn < T(1e-30)) {  // lucky breakdown
-        ++k;
-        __syncthreads();
-        break;
-      }
       T* vk1 = s.V + (int64_t)(k + 1) * n;
-      const T iw = T(1) / wn;
-      for (int i = tid; i < n; i += kGmresThreads) vk1[i] = s.w[i] * iw;
+      T wn;
+      if (n <= kGmresThreads * kMgsRegs) {
+        // modified Gram-Schmidt with this thread's elements of w and of
+        // v_j in registers: v_j is read once (the projection reuses it) and
+        // v_{j+1} is loaded while h_j is reduced.  The same operations in
+        // the same order as the loop below.
+        T wr[kMgsRegs], vr[kMgsRegs];
+#pragma unroll
+        for (int r = 0; r < kMgsRegs; ++r) {
+          const int i = tid + r * kGmresThreads;
+          wr[r] = i < n ? s.w[i] : T(0);
+          vr[r] = i < n ? s.V[i] : T(0);
+        }
+        for (int j = 0; j <= k; ++j) {
+          T vn[kMgsRegs];
+          const T* vj1 = s.V + (int64_t)(j < k ? j + 1 : j) * n;
+#pragma unroll
+          for (int r = 0; r < kMgsRegs; ++r) {
+            const int i = tid + r * kGmresThreads;
+            vn[r] = i < n ? vj1[i] : T(0);
+          }
+          part = T(0);
+#pragma unroll
+          for (int r = 0; r < kMgsRegs; ++r)
+            if (tid + r * kGmresThreads < n) part += vr[r] * wr[r];
+          const T hij = block_sum(part, s.red, tid);
+          if (tid == 0) hk[j] = hij;
+#pragma unroll
+          for (int r = 0; r < kMgsRegs; ++r) {
+            wr[r] -= hij * vr[r];
+            vr[r] = vn[r];
+          }
+        }
+        part = T(0);
+#pragma unroll
+        for (int r = 0; r < kMgsRegs; ++r)
+          if (tid + r * kGmresThreads < n) part += wr[r] * wr[r];
+        wn = sqrt(block_sum(part, s.red, tid));
+        if (tid == 0) hk[k + 1] = wn;
+        if (wn < T(1e-30)) {  // lucky breakdown
+          ++k;
+          __syncthreads();
+          break;
+        }
+        const T iw = T(1) / wn;
+#pragma unroll
+        for (int r = 0; r < kMgsRegs; ++r) {
+          const int i = tid + r * kGmresThreads;
+          if (i < n) vk1[i] = wr[r] * iw;
+        }
+      } else {
+        for (int j = 0; j <= k; ++j) {  // modified Gram-Schmidt
+          const T* vj = s.V + (int64_t)j * n;
+          part = T(0);
+          for (int i = tid; i < n; i += kGmresThreads) part += vj[i] * s.w[i];
+          const T hij = block_sum(part, s.red, tid);
+          if (tid == 0) hk[j] = hij;
+          for (int i = tid; i < n; i += kGmresThreads) s.w[i] -= hij * vj[i];
+        }
+        part = T(0);
+        for (int i = tid; i < n; i += kGmresThreads) part += s.w[i] * s.w[i];
+        wn = sqrt(block_sum(part, s.red, tid));
+        if (tid == 0) hk[k + 1] = wn;
+        if (wn < T(1e-30)) {  // lucky breakdown
+          ++k;
+          __syncthreads();
+          break;
+        }
+        const T iw = T(1) / wn;
+        for (int i = tid; i < n; i += kGmresThreads) vk1[i] = s.w[i] * iw;
+      }
       if (tid == 0) {
         for (int j = 0; j < k; ++j) {
           const T t = s.cs[j] * hk[j] + s.sn[j] * hk[j + 1];
