@@ -99,8 +99,8 @@ __device__ __forceinline__ void tail_load(const DevSparse& sp, const double* lu,
 // them unchanged).  A zero pivot stays in the stored diagonal, where the
 // caller's pivot test finds it.
 __device__ __forceinline__ void tail_factor(const DevSparse& sp, double* lu, int lane) {
-  int D = sp.tail;  // opaque, as in tail_load: the step tests stay in the steps
-  asm volatile("" : "+s"(D));
+  // lane opaque, as in tail_load: the 48 lane tests of the steps are not
+  // hoisted out of the caller's loops as live masks
   asm volatile("" : "+v"(lane));
   double a[kTailN];
   tail_load(sp, lu, lane, a);
@@ -119,7 +119,7 @@ __device__ __forceinline__ void tail_factor(const DevSparse& sp, double* lu, int
       if constexpr ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     });
   });
-  int Ds = D, ls = lane;
+  int Ds = sp.tail, ls = lane;
   asm volatile("" : "+s"(Ds));
   asm volatile("" : "+v"(ls));
   const int* ti = sp.td_idx + (ls < Ds ? ls : 0) * Ds;

@@ -382,14 +382,13 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
     blev_flags[L] = bw <= kNarrow ? 1 : 0;
   }
   // dense tail: the last D levels one column each, those columns n-D .. n-1
-  // in order, and the block closed under elimination
+  // in order.  The block is then closed under elimination without a check:
+  // S_k lies right of k, so a tail column's entries are tail rows, and every
+  // other column sits in an earlier level, its updates applied first.
   int tail = 0;
   while (tail < kTailMax && tail < levels && by_lev[levels - 1 - tail].size() == 1 &&
          by_lev[levels - 1 - tail][0] == n - 1 - tail)
     ++tail;
-  for (int k = n - tail; k < n && tail > 0; ++k)
-    for (int i : S[k])
-      if (i < n - tail) tail = 0;  // not closed (cannot happen for a level chain)
   if (tail < kTailMin) tail = 0;
   const int t0 = n - tail;
   std::vector<int> td_idx, tl_cnt, tblev_ptr(1, 0), tblev_rows, tblev_flags;
