@@ -255,9 +255,28 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
     }
     if (sync) __syncthreads();
   };
+  // A level with no more rows than the workgroup has waves (the narrow end
+  // of a grid's levels: a few rows of tens of terms each) takes one wave per
+  // row in the DC kernels: the lanes load a row's terms at once and the
+  // wave sums them in a fixed tree, one round trip instead of one per eight
+  // terms (the sum is then ordered by the tree, not by column).
+  auto wave_rows = [&](int r0, int r1) { return kTail && T > kWave && r1 - r0 <= T / kWave; };
   auto forward_level = [&](int L, int t0, int stride) {
-    const int r1 = sp.lev_ptr[L + 1];
-    for (int r = sp.lev_ptr[L] + t0; r < r1; r += stride) {
+    const int r0 = sp.lev_ptr[L], r1 = sp.lev_ptr[L + 1];
+    if (stride > 1 && wave_rows(r0, r1)) {
+      const int w = tid / kWave, lane = tid % kWave;
+      if (r0 + w < r1) {
+        const int i = sp.lev_rows[r0 + w];
+        const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
+        const double* li = lu + sp.diag[i] - nl;
+        double part = 0.0;
+        for (int t = lane; t < nl; t += kWave) part = fma(li[t], y[sp.ut_row[u0 + t]], part);
+        part = wave_sum(part);
+        if (lane == 0) y[i] = y[i] - part;
+      }
+      return;
+    }
+    for (int r = r0 + t0; r < r1; r += stride) {
       const int i = sp.lev_rows[r];
       const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
       y[i] = row_dot(lu + sp.diag[i] - nl, sp.ut_row + u0, nl, y, y[i]);
@@ -266,8 +285,21 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
   const int* bptr = D > 0 ? sp.tblev_ptr : sp.blev_ptr;
   const int* brows = D > 0 ? sp.tblev_rows : sp.blev_rows;
   auto backward_level = [&](int L, int t0, int stride) {
-    const int r1 = bptr[L + 1];
-    for (int r = bptr[L] + t0; r < r1; r += stride) {
+    const int r0 = bptr[L], r1 = bptr[L + 1];
+    if (stride > 1 && wave_rows(r0, r1)) {
+      const int w = tid / kWave, lane = tid % kWave;
+      if (r0 + w < r1) {
+        const int i = brows[r0 + w];
+        const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
+        const double* ui = lu + sp.diag[i];
+        double part = 0.0;
+        for (int t = lane; t < ns; t += kWave) part = fma(ui[1 + t], y[sp.s_row[s0 + t]], part);
+        part = wave_sum(part);
+        if (lane == 0) y[i] = (y[i] - part) * (1.0 / ui[0]);
+      }
+      return;
+    }
+    for (int r = r0 + t0; r < r1; r += stride) {
       const int i = brows[r];
       const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
       const double* ui = lu + sp.diag[i];
