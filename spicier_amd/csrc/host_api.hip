@@ -1029,8 +1029,13 @@ hipError_t enqueue_class(const MixedClass& mc, const DevNrCfg& dc, const DevTran
 }
 
 // Times each candidate kernel on a sample of the class; returns the fastest.
+// The sample is the class's own instances up to 8,192 (20 steps): the
+// candidates' occupancy differs (one-wave workgroups against four-wave ones
+// against 64 instances per wave), and a sample smaller than the GPU ranks
+// them as it would not at full size (1,024 instances picked the 256-thread
+// sparse kernel for 256-unknown ladders, 20 % slower on the full class).
 int autotune_class(spx_handle* h, MixedClass& mc, const DevNrCfg& dc, int* best) {
-  const int64_t Bs = std::min<int64_t>(mc.B, 1024);
+  const int64_t Bs = std::min<int64_t>(mc.B, 8192);
   DevTran trs = mc.tr;
   if (trs.adaptive) {  // an adaptive class is timed on fixed trapezoidal steps of h_init
     trs.adaptive = 0;
