@@ -147,6 +147,29 @@ def other_workloads(E, A, CK, torch, np, dev, stream):
         linear_solver="shared-symbolic sparse LU, one 256-thread workgroup per instance",
         limiter="SPICE pnjlim (the memoryless reference limiter lands diodes on its "
                 "spurious fixed point above vcrit here; the KCL test flags those)")
+    # the same sweep in the refinement form: FP32 L+U values in LDS, FP64
+    # refinement against the FP64 Jacobian, FP64 rerun of handed-over instances
+    b32 = dict(x=torch.empty_like(bufs["x"]), status=torch.empty_like(bufs["status"]),
+               iterations=torch.empty_like(bufs["iterations"]),
+               flags=torch.empty_like(bufs["flags"]),
+               factorizations=torch.empty_like(bufs["iterations"]),
+               refinements=torch.empty_like(bufs["iterations"]))
+    cfg32 = E.nr_config(limiter=E.Limiter.Pnjlim, precision=E.Precision.F32_IR)
+    ms32 = _event_ms(stream, torch, lambda: E.sweep_dcop_device(g, B, b32, seed=42, config=cfg32))
+    st32, fl32 = b32["status"].cpu().numpy(), b32["flags"].cpu().numpy()
+    acc32 = int(np.count_nonzero((st32 == A.SPX_CONVERGED) & ((fl32 & A.SPX_FLAG_KCL_FAIL) == 0)))
+    x64 = bufs["x"].cpu().numpy()
+    rel = np.abs(b32["x"].cpu().numpy() - x64) / np.maximum(np.abs(x64), 1e-12)
+    out["grid_diodes_dcop"]["fp32_ir"] = dict(
+        ms_per_sweep=ms32, solves_per_s=acc32 / ms32 * 1e3, accepted=acc32,
+        mean_iterations=float(b32["iterations"].float().mean().item()),
+        same_iterations_as_fp64=float((b32["iterations"] == bufs["iterations"]).float().mean()),
+        mean_factorizations=float(b32["factorizations"].float().mean().item()),
+        mean_refinements=float(b32["refinements"].float().mean().item()),
+        escalated_to_fp64=int(np.count_nonzero(fl32 & A.SPX_FLAG_FP64_ESCALATED)),
+        max_rel_distance_to_fp64=float(rel.max()),
+        linear_solver="FP32 L+U values in LDS (one workgroup per CU), FP64 iterative "
+                      "refinement against the FP64 Jacobian")
     # the sweep's own Newton systems, linearised at its solutions (what the
     # last Newton step of each instance solves): GMRES-Jacobi(30) vs the
     # batched sparse LU

@@ -306,7 +306,10 @@ int spx_sample_params_device(spx_handle* h, const spx_sweep_spec* spec, int32_t 
  * plus the engine's precision / limiter / warm-start choices. */
 typedef enum spx_precision {
     SPX_PREC_F64 = 0,    /* FP64 LU every Newton iteration */
-    SPX_PREC_F32_IR = 1  /* FP32 LU (reused across iterations) + FP64 iterative refinement */
+    SPX_PREC_F32_IR = 1  /* FP32 LU + FP64 iterative refinement; n <= 64 reuses its FP32 factors
+                            across iterations, n > 64 (sparse LU, FP32 L+U values in LDS)
+                            refactors each iteration; instances FP32 cannot resolve rerun in
+                            FP64, flagged SPX_FLAG_FP64_ESCALATED */
 } spx_precision;
 
 typedef enum spx_limiter {

@@ -62,12 +62,13 @@ hipError_t launch_transient_lanes(const DevProgram& g, const DevSparse& sp, cons
                                   const DevTran& tr, const double* params, int64_t B,
                                   const DevOut& out, const DevTranOut& tout, double* ws, int grid,
                                   int64_t slot, hipStream_t st);
-hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
-                               int64_t* slot, size_t* smem, int* lu_in_lds);
+hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int ir,
+                               int* grid, int64_t* slot, size_t* smem, int* lu_in_lds);
 hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
                                 const double* params, const double* x0, int64_t B,
                                 const DevOut& out, double* ws, int grid, int64_t slot,
-                                size_t smem, int lu_in_lds, hipStream_t st);
+                                size_t smem, int lu_in_lds, int ir, int escalated_only,
+                                hipStream_t st);
 size_t solve_generic_workspace(int n, int64_t chunk);
 hipError_t launch_factor_batch(const double* M, int n, int64_t B, double* F, int32_t* piv,
                                uint8_t* sing, hipStream_t st);
@@ -593,9 +594,6 @@ int sweep_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
   if ((rc = compile_circuit(ckt, hp, kSparseMaxN))) return rc;
   if ((rc = check_spec(spec, hp.P))) return rc;
   const bool sparse = hp.n > kDenseMaxN;
-  if (sparse && cfg->precision != SPX_PREC_F64)
-    return fail(SPX_ERR_INVALID_ARGUMENT,
-                "circuits with more than 64 unknowns run in FP64 (precision SPX_PREC_F64)");
   if (cfg->warm_start == SPX_WARM_GIVEN && !d_x0)
     return fail(SPX_ERR_INVALID_ARGUMENT, "warm start given without x0");
   if (cfg->warm_start == SPX_WARM_SURROGATE && !h->surrogate)
@@ -641,14 +639,25 @@ int sweep_device_locked(spx_handle* h, const spx_circuit* ckt, const spx_sweep_s
     x0 = static_cast<double*>(p);
   }
   if (sparse) {
-    int grid = 0, lu_lds = 0;
-    int64_t slot = 0;
-    size_t smem = 0;
-    SPX_HIP(sparse_launch_shape(g, sp, B, &grid, &slot, &smem, &lu_lds), SPX_ERR_BACKEND);
+    // FP64: one pass.  FP32+IR: the refinement form, then the FP64 kernel on
+    // the instances it handed over (the same workspace, used in turn).
+    const bool ir = cfg->precision == SPX_PREC_F32_IR;
+    int grid = 0, lu_lds = 0, grid32 = 0, lu_lds32 = 0;
+    int64_t slot = 0, slot32 = 0;
+    size_t smem = 0, smem32 = 0;
+    SPX_HIP(sparse_launch_shape(g, sp, B, 0, &grid, &slot, &smem, &lu_lds), SPX_ERR_BACKEND);
+    if (ir)
+      SPX_HIP(sparse_launch_shape(g, sp, B, 1, &grid32, &slot32, &smem32, &lu_lds32),
+              SPX_ERR_BACKEND);
+    const size_t ws_doubles = std::max((size_t)grid * slot, (size_t)grid32 * slot32);
     void* ws = nullptr;
-    if ((rc = workspace(h, "sparse_ws", sizeof(double) * (size_t)grid * slot, &ws))) return rc;
+    if ((rc = workspace(h, "sparse_ws", sizeof(double) * ws_doubles, &ws))) return rc;
+    if (ir)
+      SPX_HIP(launch_newton_sparse(g, sp, dc, params, x0, B, o, static_cast<double*>(ws), grid32,
+                                   slot32, smem32, lu_lds32, 1, 0, h->stream),
+              SPX_ERR_BACKEND);
     SPX_HIP(launch_newton_sparse(g, sp, dc, params, x0, B, o, static_cast<double*>(ws), grid,
-                                 slot, smem, lu_lds, h->stream),
+                                 slot, smem, lu_lds, 0, ir ? 1 : 0, h->stream),
             SPX_ERR_BACKEND);
   } else {
     SPX_HIP(launch_newton(g, dc, params, x0, B, o, h->stream), SPX_ERR_BACKEND);

@@ -1,7 +1,7 @@
 // newton_sparse.hip — Newton DC operating point for systems past the
-// register LU (65 <= n <= 4096): one 256-thread workgroup per instance,
-// the shared-symbolic, level-scheduled LU of sparse_plan.h, FP64
-// throughout.
+// register LU (65 <= n <= 4096): one workgroup per instance, the
+// shared-symbolic, level-scheduled LU of sparse_plan.h, in FP64, or in the
+// refinement form (FP32 factors, FP64 iterative refinement).
 //
 // Per instance the workgroup keeps x, rhs and the triangular-solve vector in
 // LDS; element values, pattern values, the stepping stages' saved solution
@@ -12,6 +12,8 @@
 // with the batch.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "assembly.h"
 #include "dcop_driver.h"
 #include "lu_wave.h"
@@ -21,6 +23,12 @@
 namespace spx {
 
 constexpr int kSparseThreads = 256;
+// The refinement form's workgroup: its FP32 L+U values take most of a CU's
+// LDS, so one workgroup per CU, and 16 waves give a level's entries four
+// times the load chains 256 threads would (a 32x32 grid with diodes:
+// 286 / 237 / 215 ms per 4,096-instance sweep at 256 / 512 / 1,024
+// threads, measured with the factors reused across iterations).
+constexpr int kSparseIrThreads = 1024;
 // Set in a shape's LDS byte count when the plan's tables are staged in LDS
 // (LDS sizes stay far below this bit).
 constexpr size_t kPlanInLdsBit = size_t(1) << 40;
@@ -40,13 +48,13 @@ __device__ __forceinline__ bool block_all_t(bool p, int tid, int* red) {
 // (one fma per term).  The terms' index and value loads (global or LDS) go
 // out eight at a time, then their y reads, before the eight fmas: a long row
 // of a triangular solve is a few load rounds, not one per four terms.
-__device__ __forceinline__ double row_dot(const double* val, const int* idx, int len,
-                                          const double* y, double v) {
+template <typename V>
+__device__ __forceinline__ V row_dot(const V* val, const int* idx, int len, const V* y, V v) {
   constexpr int U = 8;
   int t = 0;
   for (; t + U <= len; t += U) {
     int j[U];
-    double a[U], b[U];
+    V a[U], b[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       j[u] = idx[t + u];
@@ -69,8 +77,9 @@ constexpr int kTailN = 48;
 // row per lane of the calling wave: entries the plan does not store are
 // zero, lanes past D hold zero rows.  Every index is clamped before its
 // load, so no lane reads outside the tables.
-__device__ __forceinline__ void tail_load(const DevSparse& sp, const double* lu, int lane,
-                                          double (&a)[kTailN]) {
+template <typename V>
+__device__ __forceinline__ void tail_load(const DevSparse& sp, const V* lu, int lane,
+                                          V (&a)[kTailN]) {
   // opaque per call: otherwise the compiler hoists the column addresses and
   // tests out of the instance loop as live registers, which spill
   int D = sp.tail;
@@ -81,8 +90,8 @@ __device__ __forceinline__ void tail_load(const DevSparse& sp, const double* lu,
     constexpr int c = C;
     const int q0 = ti[c < D ? c : 0];
     const int q = (lane < D && c < D) ? q0 : -1;
-    const double v = lu[q >= 0 ? q : 0];
-    a[c] = q >= 0 ? v : 0.0;
+    const V v = lu[q >= 0 ? q : 0];
+    a[c] = q >= 0 ? v : V(0);
     // eight columns' loads in flight at a time: the scheduler would
     // otherwise issue all 64 index and value loads first and hold them in
     // registers beside the row
@@ -98,19 +107,20 @@ __device__ __forceinline__ void tail_load(const DevSparse& sp, const double* lu,
 // form, in the same order (rows above the pivot take l = 0, which leaves
 // them unchanged).  A zero pivot stays in the stored diagonal, where the
 // caller's pivot test finds it.
-__device__ __forceinline__ void tail_factor(const DevSparse& sp, double* lu, int lane) {
+template <typename V>
+__device__ __forceinline__ void tail_factor(const DevSparse& sp, V* lu, int lane) {
   // lane opaque, as in tail_load: the 48 lane tests of the steps are not
   // hoisted out of the caller's loops as live masks
   asm volatile("" : "+v"(lane));
-  double a[kTailN];
+  V a[kTailN];
   tail_load(sp, lu, lane, a);
   // steps past D meet zero rows (pivot 0, multipliers 0) and change
   // nothing, so every step runs without a test
   static_for<0, kTailN>([&](auto K) __attribute__((always_inline)) {
     constexpr int k = K;
-    const double pv = bcast(a[k], k);
-    const double inv = pv != 0.0 ? 1.0 / pv : 0.0;
-    const double l = lane > k ? a[k] * inv : 0.0;
+    const V pv = bcast(a[k], k);
+    const V inv = pv != V(0) ? V(1) / pv : V(0);
+    const V l = lane > k ? a[k] * inv : V(0);
     if (lane > k) a[k] = l;
     static_for<k + 1, kTailN>([&](auto J) __attribute__((always_inline)) {
       constexpr int j = J;
@@ -136,42 +146,71 @@ __device__ __forceinline__ void tail_factor(const DevSparse& sp, double* lu, int
 // Forward, column k: rows below take fma(-L(r, k), y_k, y_r); backward,
 // column k from the last: x_k = y_k * (1 / U(k, k)), rows above take
 // fma(-U(r, k), x_k, y_r).
-__device__ __forceinline__ void tail_solve(const DevSparse& sp, const double* lu, double* y, int n,
+template <typename V>
+__device__ __forceinline__ void tail_solve(const DevSparse& sp, const V* lu, V* y, int n,
                                            int lane) {
   int D = sp.tail;  // opaque, as in tail_load
   asm volatile("" : "+s"(D));
   asm volatile("" : "+v"(lane));
   const int t0 = n - D;
-  double a[kTailN];
+  V a[kTailN];
   tail_load(sp, lu, lane, a);
-  double v = lane < D ? y[t0 + lane] : 0.0;
-  const double dinv = 1.0 / (lane < D ? lu[sp.diag[t0 + lane]] : 1.0);
+  V v = lane < D ? y[t0 + lane] : V(0);
+  const V dinv = V(1) / (lane < D ? lu[sp.diag[t0 + lane]] : V(1));
   // steps past D broadcast a zero and change nothing
   static_for<0, kTailN>([&](auto K) __attribute__((always_inline)) {
     constexpr int k = K;
-    const double yk = bcast(v, k);
+    const V yk = bcast(v, k);
     if (lane > k) v = fma(-a[k], yk, v);
   });
   static_for_down<0, kTailN>([&](auto K) __attribute__((always_inline)) {
     constexpr int k = K;
-    const double xk = bcast(v * dinv, k);
+    const V xk = bcast(v * dinv, k);
     if (lane < k) v = fma(-a[k], xk, v);
   });
   if (lane < D) y[t0 + lane] = v * dinv;
 }
 
-// Factors lu in place with the plan sp, then solves for the right-hand side
-// b(i) (i in unknown order); the solution is left in y (LDS, n) in
-// elimination order.  False if a pivot is exactly zero or the solution is
-// not finite.  Level by level: multipliers of the level's columns, then its
-// updates target by target (contributions in column order); the triangular
-// solves are row-oriented over the forward levels and the levels counted
-// from the root.
-template <int T, bool kTail, class RHS>
-__device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, double* lu, RHS b,
-                                                    double* y, int tid, int* red) {
-  // with kTail, the plan's dense tail (if any) is factored and solved in
-  // wave 0's registers in place of its one-column levels
+// Runs levels [0, count) of one pass: a run of levels flagged narrow goes
+// to thread 0 in order (one barrier after the run), any other level to
+// the whole workgroup (barriers inside `wide_fn`).
+template <class NarrowFn, class WideFn>
+__device__ __forceinline__ void run_levels(int count, const int* flags, int bit, int tid,
+                                           NarrowFn narrow_fn, WideFn wide_fn) {
+  for (int L = 0; L < count;) {
+    if (flags[L] & bit) {
+      int L1 = L + 1;
+      while (L1 < count && (flags[L1] & bit)) ++L1;
+      if (tid == 0)
+        for (int l = L; l < L1; ++l) narrow_fn(l);
+      __syncthreads();
+      L = L1;
+    } else {
+      wide_fn(L);
+      ++L;
+    }
+  }
+}
+
+// A level with no more rows than the workgroup has waves (the narrow end of
+// a grid's levels: a few rows of tens of terms each) takes one wave per row
+// in the DC kernels: the lanes load a row's terms at once and the wave sums
+// them in a fixed tree, one round trip instead of one per eight terms (the
+// sum is then ordered by the tree, not by column).
+template <int T, bool kTail>
+__device__ __forceinline__ bool wave_rows(int r0, int r1) {
+  return kTail && T > kWave && r1 - r0 <= T / kWave;
+}
+
+// Factors lu (values of type V: double, or float for the FP32 factors of
+// the refinement form) in place with the plan sp.  False if a pivot is
+// exactly zero.  Level by level: multipliers of the level's columns, then
+// its updates target by target (contributions in column order).  With
+// kTail, the plan's dense tail (if any) is factored in wave 0's registers in
+// place of its one-column levels.
+template <int T, bool kTail, typename V>
+__device__ __forceinline__ bool sparse_factor(const DevSparse& sp, int n, V* lu, int tid,
+                                              int* red) {
   const int D = kTail ? sp.tail : 0;
   // One level's factor work, threads [t0, t0 + stride).  A thread takes four
   // multipliers, then four targets, at a time and issues their index and
@@ -187,12 +226,12 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
       const int t = a0 + t0;
       if (t < a1) {
         const int li = sp.pa_l[t];
-        lu[li] = lu[li] * (1.0 / lu[sp.pa_d[t]]);
+        lu[li] = lu[li] * (V(1) / lu[sp.pa_d[t]]);
       }
     } else
     for (int t = a0 + t0; t < a1; t += U * stride) {
       int li[U], di[U];
-      double lv[U], dv[U];
+      V lv[U], dv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int tt = t + u * stride;
@@ -207,7 +246,7 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
         }
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        if (li[u] >= 0) lu[li[u]] = lv[u] * (1.0 / dv[u]);
+        if (li[u] >= 0) lu[li[u]] = lv[u] * (V(1) / dv[u]);
     }
     if (sync) __syncthreads();
     const int b0 = sp.tb_ptr[L], b1 = sp.tb_ptr[L + 1];
@@ -215,7 +254,7 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
       const int t = b0 + t0;
       if (t < b1) {
         const int ti = sp.tb_t[t];
-        double v = lu[ti];
+        V v = lu[ti];
         const int c1 = sp.tb_c[t + 1];
         for (int c = sp.tb_c[t]; c < c1; ++c) v = fma(-lu[sp.cb_l[c]], lu[sp.cb_u[c]], v);
         lu[ti] = v;
@@ -223,7 +262,7 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
     } else
     for (int t = b0 + t0; t < b1; t += U * stride) {
       int ti[U], c0[U], c1[U];
-      double v[U];
+      V v[U];
       int cmax = 0;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -255,21 +294,34 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
     }
     if (sync) __syncthreads();
   };
-  // A level with no more rows than the workgroup has waves (the narrow end
-  // of a grid's levels: a few rows of tens of terms each) takes one wave per
-  // row in the DC kernels: the lanes load a row's terms at once and the
-  // wave sums them in a fixed tree, one round trip instead of one per eight
-  // terms (the sum is then ordered by the tree, not by column).
-  auto wave_rows = [&](int r0, int r1) { return kTail && T > kWave && r1 - r0 <= T / kWave; };
+  run_levels(sp.levels - D, sp.lev_flags, 1, tid, [&](int l) { factor_level(l, 0, 1, false); },
+             [&](int l) { factor_level(l, tid, T, true); });
+  if (D > 0) {
+    if (tid < kWave) tail_factor(sp, lu, tid);
+    __syncthreads();
+  }
+  bool ok = true;
+  for (int k = tid; k < n; k += T) ok = ok && lu[sp.diag[k]] != V(0);
+  return block_all_t<T>(ok, tid, red);
+}
+
+// Both triangular solves with the factors of sparse_factor (the same kTail):
+// y (LDS, n, elimination order) holds the right-hand side on entry and the
+// solution on return.  False if the solution is not finite.  The solves are
+// row-oriented over the forward levels and the levels counted from the root.
+template <int T, bool kTail, typename V>
+__device__ __forceinline__ bool sparse_solve(const DevSparse& sp, int n, const V* lu, V* y,
+                                             int tid, int* red) {
+  const int D = kTail ? sp.tail : 0;
   auto forward_level = [&](int L, int t0, int stride) {
     const int r0 = sp.lev_ptr[L], r1 = sp.lev_ptr[L + 1];
-    if (stride > 1 && wave_rows(r0, r1)) {
+    if (stride > 1 && wave_rows<T, kTail>(r0, r1)) {
       const int w = tid / kWave, lane = tid % kWave;
       if (r0 + w < r1) {
         const int i = sp.lev_rows[r0 + w];
         const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
-        const double* li = lu + sp.diag[i] - nl;
-        double part = 0.0;
+        const V* li = lu + sp.diag[i] - nl;
+        V part = V(0);
         for (int t = lane; t < nl; t += kWave) part = fma(li[t], y[sp.ut_row[u0 + t]], part);
         part = wave_sum(part);
         if (lane == 0) y[i] = y[i] - part;
@@ -286,60 +338,31 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
   const int* brows = D > 0 ? sp.tblev_rows : sp.blev_rows;
   auto backward_level = [&](int L, int t0, int stride) {
     const int r0 = bptr[L], r1 = bptr[L + 1];
-    if (stride > 1 && wave_rows(r0, r1)) {
+    if (stride > 1 && wave_rows<T, kTail>(r0, r1)) {
       const int w = tid / kWave, lane = tid % kWave;
       if (r0 + w < r1) {
         const int i = brows[r0 + w];
         const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
-        const double* ui = lu + sp.diag[i];
-        double part = 0.0;
+        const V* ui = lu + sp.diag[i];
+        V part = V(0);
         for (int t = lane; t < ns; t += kWave) part = fma(ui[1 + t], y[sp.s_row[s0 + t]], part);
         part = wave_sum(part);
-        if (lane == 0) y[i] = (y[i] - part) * (1.0 / ui[0]);
+        if (lane == 0) y[i] = (y[i] - part) * (V(1) / ui[0]);
       }
       return;
     }
     for (int r = r0 + t0; r < r1; r += stride) {
       const int i = brows[r];
       const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
-      const double* ui = lu + sp.diag[i];
-      y[i] = row_dot(ui + 1, sp.s_row + s0, ns, y, y[i]) * (1.0 / ui[0]);
+      const V* ui = lu + sp.diag[i];
+      y[i] = row_dot(ui + 1, sp.s_row + s0, ns, y, y[i]) * (V(1) / ui[0]);
     }
   };
-  // Runs levels [0, count) of one pass: a run of levels flagged narrow goes
-  // to thread 0 in order (one barrier after the run), any other level to
-  // the whole workgroup (barriers inside `wide`).
-  auto pass = [&](int count, const int* flags, int bit, auto narrow_fn, auto wide_fn) {
-    for (int L = 0; L < count;) {
-      if (flags[L] & bit) {
-        int L1 = L + 1;
-        while (L1 < count && (flags[L1] & bit)) ++L1;
-        if (tid == 0)
-          for (int l = L; l < L1; ++l) narrow_fn(l);
-        __syncthreads();
-        L = L1;
-      } else {
-        wide_fn(L);
-        ++L;
-      }
-    }
-  };
-  pass(sp.levels - D, sp.lev_flags, 1, [&](int l) { factor_level(l, 0, 1, false); },
-       [&](int l) { factor_level(l, tid, T, true); });
-  if (D > 0) {
-    if (tid < kWave) tail_factor(sp, lu, tid);
-    __syncthreads();
-  }
-  bool ok = true;
-  for (int k = tid; k < n; k += T) ok = ok && lu[sp.diag[k]] != 0.0;
-  if (!block_all_t<T>(ok, tid, red)) return false;
-  for (int k = tid; k < n; k += T) y[k] = b(sp.perm[k]);
-  __syncthreads();
-  pass(sp.levels - D, sp.lev_flags, 2, [&](int l) { forward_level(l, 0, 1); },
-       [&](int l) {
-         forward_level(l, tid, T);
-         __syncthreads();
-       });
+  run_levels(sp.levels - D, sp.lev_flags, 2, tid, [&](int l) { forward_level(l, 0, 1); },
+             [&](int l) {
+               forward_level(l, tid, T);
+               __syncthreads();
+             });
   if (D > 0) {
     for (int r = tid; r < D; r += T) {  // the tail rows' terms in earlier columns
       const int i = n - D + r;
@@ -350,23 +373,42 @@ __device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, 
     if (tid < kWave) tail_solve(sp, lu, y, n, tid);
     __syncthreads();
   }
-  pass(D > 0 ? sp.tblevels : sp.blevels, D > 0 ? sp.tblev_flags : sp.blev_flags, 1,
-       [&](int l) { backward_level(l, 0, 1); },
-       [&](int l) {
-         backward_level(l, tid, T);
-         __syncthreads();
-       });
+  run_levels(D > 0 ? sp.tblevels : sp.blevels, D > 0 ? sp.tblev_flags : sp.blev_flags, 1, tid,
+             [&](int l) { backward_level(l, 0, 1); },
+             [&](int l) {
+               backward_level(l, tid, T);
+               __syncthreads();
+             });
   bool fin = true;
   for (int k = tid; k < n; k += T) fin = fin && isfinite(y[k]);
   return block_all_t<T>(fin, tid, red);
 }
 
+// Factors lu in place with the plan sp, then solves for the right-hand side
+// b(i) (i in unknown order); the solution is left in y (LDS, n) in
+// elimination order.  False if a pivot is exactly zero or the solution is
+// not finite.
+template <int T, bool kTail, class RHS>
+__device__ __forceinline__ bool sparse_factor_solve(const DevSparse& sp, int n, double* lu, RHS b,
+                                                    double* y, int tid, int* red) {
+  if (!sparse_factor<T, kTail>(sp, n, lu, tid, red)) return false;
+  for (int k = tid; k < n; k += T) y[k] = b(sp.perm[k]);
+  __syncthreads();
+  return sparse_solve<T, kTail>(sp, n, lu, y, tid, red);
+}
+
 // kTail_: the factorisation takes the plan's dense tail in registers (the
-// DC kernel; the transient kernels keep the level form and its registers)
-template <int T_, bool kTail_ = false>
+// DC kernel; the transient kernels keep the level form and its registers).
+// IR_: the refinement form -- FP32 L+U factors of each Newton Jacobian and
+// FP64 iterative refinement against the FP64 assembled Jacobian (the
+// instance hands itself to the FP64 kernel when FP32 cannot resolve its
+// system).
+template <int T_, bool kTail_ = false, bool IR_ = false>
 struct SparseInstance {
   static constexpr int T = T_;
   static constexpr bool kTail = kTail_;
+  static constexpr bool IR = IR_;
+  using LV = typename std::conditional<IR_, float, double>::type;
   const DevProgram& g;
   const DevSparse& sp;
   const DevNrCfg& cfg;
@@ -375,13 +417,14 @@ struct SparseInstance {
   double* vals;      // element value slots
   double* vst;       // pnjlim junction states
   double* av;        // assembled pattern values (circuit program order)
-  double* lu;        // filled L+U values
+  LV* lu;            // filled L+U values
   double* rhs;       // LDS n
   double* xs;        // LDS n
   double* xw;        // global slot n: the stepping stages' saved solution
-  double* y;         // LDS n, elimination order
+  LV* y;             // LDS n, elimination order
   int* red;          // LDS T/64 ints
   double* redd;      // LDS T/64 doubles
+  double* xi = nullptr;  // LDS n, refinement form: the refinement iterate (unknown order)
   int nfact = 0;
   int nref = 0;
   const TranCtx* tc = nullptr;  // transient context (null: DC sweep)
@@ -406,9 +449,10 @@ struct SparseInstance {
     for (int q = tid; q < g.num_pos; q += T) {
       const double v = gather_sum(g.pos_off, g.contrib, q, vals);
       av[q] = v;
-      lu[sp.pos_lu[q]] = v;
+      if constexpr (!IR) lu[sp.pos_lu[q]] = v;
     }
-    for (int f = tid; f < sp.num_fill; f += T) lu[sp.fill[f]] = 0.0;
+    if constexpr (!IR)
+      for (int f = tid; f < sp.num_fill; f += T) lu[sp.fill[f]] = 0.0;
     for (int r = tid; r < g.n; r += T) rhs[r] = gather_sum(g.rhs_off, g.rhs_contrib, r, vals);
     __syncthreads();
     return !block_all(!lim);
@@ -419,27 +463,140 @@ struct SparseInstance {
     return sparse_factor_solve<T, kTail>(sp, g.n, lu, [&](int i) { return rhs[i]; }, y, tid, red);
   }
 
+  // Refinement form: the FP32 factors of the assembled Jacobian (fill
+  // entries zero).  False if a pivot is exactly zero in FP32.
+  __device__ __forceinline__ bool factor32() {
+    ++nfact;
+    for (int q = tid; q < g.num_pos; q += T) lu[sp.pos_lu[q]] = (LV)av[q];
+    for (int f = tid; f < sp.num_fill; f += T) lu[sp.fill[f]] = LV(0);
+    __syncthreads();
+    return sparse_factor<T, kTail>(sp, g.n, lu, tid, red);
+  }
+
+  // Largest of a >= 0 and of b >= 0 over the workgroup, one barrier pair
+  // (refinement form: redd holds 2 T/64 doubles).
+  __device__ __forceinline__ void block_max2(double& a, double& b) {
+    constexpr int W = T / kWave;
+    const double wa = wave_max(a), wb = wave_max(b);
+    if ((tid & (kWave - 1)) == 0) {
+      redd[tid / kWave] = wa;
+      redd[W + tid / kWave] = wb;
+    }
+    __syncthreads();
+    double ra = redd[0], rb = redd[W];
+    for (int i = 1; i < W; ++i) {
+      ra = fmax(ra, redd[i]);
+      rb = fmax(rb, redd[W + i]);
+    }
+    __syncthreads();
+    a = ra;
+    b = rb;
+  }
+
+  // Refinement form: FP64 iterative refinement of J x = rhs with the FP32
+  // factors in lu, from x = xs; the iterate is left in xi.  Stops on LAPACK
+  // dsgesv's backward-error test
+  //   ||rhs - J x||_inf <= tol * ||J||_inf * ||x||_inf,
+  //   tol = ir_tol, or sqrt(n) * 2^-53 if that is 0,
+  // the test and the contraction rule of the register-LU kernels'
+  // refinement.  The residual of row u sums fma(-J(u, c), x_c, .) over the
+  // row's pattern entries in column order, from rhs_u.  Returns 0
+  // converged, 2 failed (stopped contracting, or not finite).
+  __device__ __forceinline__ int refine() {
+    const int n = g.n;
+    double rs = 0.0;
+    for (int u = tid; u < n; u += T) {
+      double a = 0.0;
+      for (int e = g.row_ptr[u]; e < g.row_ptr[u + 1]; ++e) a += fabs(av[e]);
+      rs = fmax(rs, a);
+      xi[u] = xs[u];
+    }
+    const double anrm = block_max(rs);
+    const double tol = cfg.ir_tol > 0.0 ? cfg.ir_tol : sqrt((double)n) * 1.1102230246251565e-16;
+    double prev = INFINITY;
+    for (int s = 0; s <= cfg.ir_max_steps; ++s) {
+      double rn = 0.0, xn = 0.0;
+      for (int k = tid; k < n; k += T) {  // residual, row perm[k] into y[k]
+        const int u = sp.perm[k];
+        double r = rhs[u];
+        for (int e = g.row_ptr[u]; e < g.row_ptr[u + 1]; ++e)
+          r = fma(-av[e], xi[g.pos_rc[e] >> 16], r);
+        y[k] = (LV)r;
+        rn = fmax(rn, fabs(r));
+        xn = fmax(xn, fabs(xi[u]));
+      }
+      block_max2(rn, xn);
+      if (!(rn > tol * anrm * xn)) {  // converged (also catches r == 0)
+        bool fin = true;
+        for (int u = tid; u < n; u += T) fin = fin && isfinite(xi[u]);
+        return block_all(fin) ? 0 : 2;
+      }
+      if (s == cfg.ir_max_steps) break;
+      if (!sparse_solve<T, kTail>(sp, n, lu, y, tid, red)) break;
+      ++nref;
+      double dn = 0.0;
+      for (int k = tid; k < n; k += T) {
+        const double d = (double)y[k];
+        xi[sp.perm[k]] += d;
+        dn = fmax(dn, fabs(d));
+      }
+      dn = block_max(dn);
+      if (!(dn < 0.5 * prev) && s > 0) break;  // stopped contracting
+      prev = dn;
+    }
+    return 2;
+  }
+
+  // Refinement form: x = J^{-1} rhs into xi from FP32 factors of this J.
+  // The factors are not carried to the next Newton iteration, unlike the
+  // register-LU kernels': on a 32x32 grid with diodes, factors one iteration
+  // old need twice the refinement steps, which cost more than the sparse
+  // refactorisation they save (286 vs 143 ms per 4,096-instance sweep).
+  // 0 solved, 2 hand over to FP64.
+  __device__ __forceinline__ int solve32() {
+    if (!factor32()) return 2;
+    return refine();
+  }
+
   __device__ __forceinline__ int newton(double gmin, double sf, int& iters, int& fp64_escalations) {
     const int n = g.n, nn = g.num_nodes;
     for (int i = tid; i < g.num_elems; i += T) vst[i] = 0.0;
     __syncthreads();
     for (int it = 0; it < cfg.max_iterations; ++it) {
       const bool lim = assemble(gmin, sf, cfg.limiter);
-      if (!factor_solve()) {
-        iters = it + 1;
-        return -1;
-      }
       bool pass = true;
-      for (int k = tid; k < n; k += T) {  // check_convergence, newton.rs:134-159
-        const int u = sp.perm[k];
-        const double xn = y[k], xo = xs[u];
-        const double delta = fabs(xn - xo);
-        const double tol = cfg.v_reltol * fmax(fabs(xn), fabs(xo)) +
-                           (u < nn ? cfg.v_abstol : cfg.i_abstol);
-        pass = pass && !(delta > tol);
+      if constexpr (IR) {
+        if (solve32() != 0) {  // FP32 cannot resolve this system: hand the instance to FP64
+          iters = it + 1;
+          return -2;
+        }
+        for (int u = tid; u < n; u += T) {  // check_convergence, newton.rs:134-159
+          const double xn = xi[u], xo = xs[u];
+          const double delta = fabs(xn - xo);
+          const double tol = cfg.v_reltol * fmax(fabs(xn), fabs(xo)) +
+                             (u < nn ? cfg.v_abstol : cfg.i_abstol);
+          pass = pass && !(delta > tol);
+        }
+      } else {
+        if (!factor_solve()) {
+          iters = it + 1;
+          return -1;
+        }
+        for (int k = tid; k < n; k += T) {  // check_convergence, newton.rs:134-159
+          const int u = sp.perm[k];
+          const double xn = y[k], xo = xs[u];
+          const double delta = fabs(xn - xo);
+          const double tol = cfg.v_reltol * fmax(fabs(xn), fabs(xo)) +
+                             (u < nn ? cfg.v_abstol : cfg.i_abstol);
+          pass = pass && !(delta > tol);
+        }
       }
       const bool conv = block_all(pass) && !lim;
-      for (int k = tid; k < n; k += T) xs[sp.perm[k]] = y[k];
+      if constexpr (IR) {
+        for (int u = tid; u < n; u += T) xs[u] = xi[u];
+      } else {
+        for (int k = tid; k < n; k += T) xs[sp.perm[k]] = y[k];
+      }
       __syncthreads();
       if (conv) {
         iters = it + 1;
@@ -452,10 +609,17 @@ struct SparseInstance {
 
   __device__ __forceinline__ int linear_solve() {
     assemble(0.0, 1.0, cfg.limiter);
-    const bool ok = factor_solve();
-    for (int k = tid; k < g.n; k += T) xs[sp.perm[k]] = ok ? y[k] : 0.0;
-    __syncthreads();
-    return ok ? 0 : 1;
+    if constexpr (IR) {
+      const int rc = solve32();
+      for (int u = tid; u < g.n; u += T) xs[u] = rc == 0 ? xi[u] : 0.0;
+      __syncthreads();
+      return rc;
+    } else {
+      const bool ok = factor_solve();
+      for (int k = tid; k < g.n; k += T) xs[sp.perm[k]] = ok ? y[k] : 0.0;
+      __syncthreads();
+      return ok ? 0 : 1;
+    }
   }
 
   __device__ __forceinline__ void save_x() {
@@ -562,53 +726,76 @@ struct SparseInstance {
 // Per-slot global workspace, in doubles.
 // The stepping stages' saved solution (xw, n) is here too: it is touched
 // once per stepping level, and in LDS it would cost a resident workgroup of
-// the one-wave transient kernels.
-inline int64_t sparse_slot_doubles(const DevProgram& g, const DevSparse& sp, bool lu_in_lds) {
-  int64_t d = (int64_t)g.num_vals + g.num_elems + g.num_pos + g.n + (lu_in_lds ? 0 : sp.nnz_lu);
+// the one-wave transient kernels.  lu_bytes: bytes per L+U value (8, or 4
+// for the FP32 factors of the refinement form).
+inline int64_t sparse_slot_doubles(const DevProgram& g, const DevSparse& sp, bool lu_in_lds,
+                                   int lu_bytes = 8) {
+  int64_t d = (int64_t)g.num_vals + g.num_elems + g.num_pos + g.n +
+              (lu_in_lds ? 0 : ((int64_t)sp.nnz_lu * lu_bytes + 7) / 8);
   return (d + 31) & ~int64_t(31);  // 256-byte slots
 }
 
+// LDS bytes of the DC and transient kernels: rhs, x and the solve vector
+// (n doubles each), the reductions, and the L+U values when in LDS.  The
+// refinement form (ir) keeps the refinement iterate as its third n doubles,
+// the solve vector as n floats, two reduction slots per wave and FP32 L+U.
 inline size_t sparse_smem_bytes(const DevProgram& g, const DevSparse& sp, bool lu_in_lds,
-                                int threads = kSparseThreads) {
-  size_t d = 3 * (size_t)g.n + threads / kWave;
-  size_t b = d * sizeof(double) + (threads / kWave) * sizeof(int);
+                                int threads = kSparseThreads, bool ir = false) {
+  const int w = threads / kWave;
+  size_t d = 3 * (size_t)g.n + (ir ? 2 : 1) * w;
+  size_t b = d * sizeof(double) + w * sizeof(int);
   b = (b + 7) & ~size_t(7);
-  if (lu_in_lds) b += (size_t)sp.nnz_lu * sizeof(double);
+  if (ir) b = (b + (size_t)g.n * sizeof(float) + 7) & ~size_t(7);
+  if (lu_in_lds) b += (size_t)sp.nnz_lu * (ir ? sizeof(float) : sizeof(double));
   return b;
 }
 
 // Three waves per SIMD: the dense tail's register row (128 VGPRs) would
 // otherwise set the allocation for the whole kernel.
-template <int T>
+// IR: the refinement form (FP32 factors, FP64 refinement); it leaves an
+// instance FP32 cannot resolve at SPX_ACTIVE.  escalated_only: run only the
+// instances left at SPX_ACTIVE (the FP64 pass after the refinement form).
+template <int T, bool IR>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_newton_sparse(
     DevProgram g, DevSparse sp, DevNrCfg cfg, const double* __restrict__ params,
     const double* __restrict__ x0, int64_t B, DevOut out, double* ws, int64_t slot,
-    int lu_in_lds) {
+    int lu_in_lds, int escalated_only) {
+  using LV = typename std::conditional<IR, float, double>::type;
   extern __shared__ double smem[];
   const int n = g.n, tid = threadIdx.x;
   double* rhs = smem;
   double* xs = rhs + n;
-  double* y = xs + n;
-  double* redd = y + n;
-  int* red = reinterpret_cast<int*>(redd + T / kWave);
-  double* lds_lu = reinterpret_cast<double*>(
+  double* x3 = xs + n;  // FP64: the solve vector; refinement form: the iterate
+  double* redd = x3 + n;
+  int* red = reinterpret_cast<int*>(redd + (IR ? 2 : 1) * (T / kWave));
+  LV* after = reinterpret_cast<LV*>(
       (reinterpret_cast<uintptr_t>(red + T / kWave) + 7) & ~uintptr_t(7));
+  LV* y;
+  LV* lds_lu;
+  if constexpr (IR) {
+    y = after;
+    lds_lu = reinterpret_cast<LV*>((reinterpret_cast<uintptr_t>(after + n) + 7) & ~uintptr_t(7));
+  } else {
+    y = x3;
+    lds_lu = after;
+  }
   double* w = ws + blockIdx.x * slot;
   double* vals = w;
   double* vst = vals + g.num_vals;
   double* av = vst + g.num_elems;
   double* xw = av + g.num_pos;
-  double* lu = lu_in_lds ? lds_lu : xw + n;
+  LV* lu = lu_in_lds ? lds_lu : reinterpret_cast<LV*>(xw + n);
   for (int64_t inst = blockIdx.x; inst < B; inst += gridDim.x) {
+    if (escalated_only && out.status[inst] != SPX_ACTIVE) continue;
     if (tid == 0) {
       vals[kValOne] = 1.0;
       vals[kValGmin] = 0.0;
     }
     for (int i = tid; i < n; i += T) xs[i] = x0 ? x0[inst * n + i] : 0.0;
     __syncthreads();
-    SparseInstance<T, true> I{g, sp, cfg, tid, params + inst * g.P, vals, vst, av, lu, rhs, xs, xw, y,
-                     red, redd};
-    run_dcop(I, g, cfg, x0 != nullptr, out, inst, 0);
+    SparseInstance<T, true, IR> I{g,  sp, cfg, tid, params + inst * g.P, vals, vst, av, lu,
+                                  rhs, xs, xw, y, red, redd, x3};
+    run_dcop(I, g, cfg, x0 != nullptr, out, inst, escalated_only);
     __syncthreads();
   }
 }
@@ -833,28 +1020,43 @@ hipError_t launch_sparse_solve(const DevSparse& sp, int n, int num_pos, const in
   return hipGetLastError();
 }
 
-// Grid and workspace shape of a sparse launch: `grid` workgroups, each with
-// a slot of `slot` doubles of global workspace and `smem` bytes of LDS.
-hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int* grid,
-                               int64_t* slot, size_t* smem, int* lu_in_lds) {
-  const size_t lds_budget = 64 * 1024;
-  *lu_in_lds = sparse_smem_bytes(g, sp, true) <= lds_budget ? 1 : 0;
-  *smem = sparse_smem_bytes(g, sp, *lu_in_lds);
-  *slot = sparse_slot_doubles(g, sp, *lu_in_lds);
-  hipError_t e = hipFuncSetAttribute((const void*)k_newton_sparse<kSparseThreads>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)*smem);
+// Grid and workspace shape of a sparse DC launch: `grid` workgroups, each
+// with a slot of `slot` doubles of global workspace and `smem` bytes of LDS.
+// FP64 (ir = 0): the L+U values in LDS when everything fits in 64 KB (more
+// resident workgroups beat LDS-resident values past that).  Refinement form
+// (ir = 1): the FP32 L+U values in LDS whenever they fit in a workgroup's
+// 160 KB -- the factorisation and the refinement's solves then never leave
+// LDS -- else in the slot.
+hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t B, int ir,
+                               int* grid, int64_t* slot, size_t* smem, int* lu_in_lds) {
+  const int T = ir ? kSparseIrThreads : kSparseThreads;
+  const void* kern = ir ? (const void*)k_newton_sparse<kSparseIrThreads, true>
+                        : (const void*)k_newton_sparse<kSparseThreads, false>;
+  const size_t lds_budget = ir ? 160 * 1024 : 64 * 1024;
+  *lu_in_lds = sparse_smem_bytes(g, sp, true, T, ir) <= lds_budget ? 1 : 0;
+  *smem = sparse_smem_bytes(g, sp, *lu_in_lds, T, ir);
+  *slot = sparse_slot_doubles(g, sp, *lu_in_lds, ir ? 4 : 8);
+  hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)*smem);
   if (e != hipSuccess) return e;
-  return grid_for((const void*)k_newton_sparse<kSparseThreads>, *smem, B, grid);
+  return grid_for(kern, *smem, B, grid, T);
 }
 
+// ir = 1 runs the refinement form; escalated_only = 1 the FP64 pass over the
+// instances a refinement-form launch left at SPX_ACTIVE (needs out.status).
 hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const DevNrCfg& cfg,
                                 const double* params, const double* x0, int64_t B,
                                 const DevOut& out, double* ws, int grid, int64_t slot,
-                                size_t smem, int lu_in_lds, hipStream_t st) {
+                                size_t smem, int lu_in_lds, int ir, int escalated_only,
+                                hipStream_t st) {
   if (B == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_newton_sparse<kSparseThreads>, dim3((unsigned)grid), dim3(kSparseThreads),
-                     smem, st, g,
-                     sp, cfg, params, x0, B, out, ws, slot, lu_in_lds);
+  if (ir)
+    hipLaunchKernelGGL((k_newton_sparse<kSparseIrThreads, true>), dim3((unsigned)grid),
+                       dim3(kSparseIrThreads), smem, st, g, sp, cfg, params, x0, B, out, ws,
+                       slot, lu_in_lds, 0);
+  else
+    hipLaunchKernelGGL((k_newton_sparse<kSparseThreads, false>), dim3((unsigned)grid),
+                       dim3(kSparseThreads), smem, st, g, sp, cfg, params, x0, B, out, ws, slot,
+                       lu_in_lds, escalated_only);
   return hipGetLastError();
 }
 

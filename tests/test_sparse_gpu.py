@@ -114,10 +114,77 @@ def test_grid32_full_batch_properties():
     assert x[:, dn].max() < 0.8
 
 
-def test_sparse_path_rejects_fp32():
+# ------------------------------ FP32 factors + FP64 refinement on the sparse path
+def _ir(**kw):
+    return E.nr_config(limiter=E.Limiter.Pnjlim, precision=E.Precision.F32_IR, **kw)
+
+
+def _parity_ir(got, want, tag):
+    """The refinement form against an FP64 result: same statuses; the same
+    Newton iteration count on >= 99.9 % of instances, whose solutions agree
+    to 1e-9 / 1e-12, the others at the SPICE tolerance; the same flags apart
+    from FP64_ESCALATED, which at most 1 % carry."""
+    np.testing.assert_array_equal(got.status, want["status"], err_msg=tag)
+    same_it = got.iterations == want["iterations"]
+    assert same_it.mean() >= 0.999, (tag, np.count_nonzero(~same_it))
+    ok = close(got.solutions, want["x"])
+    assert np.all(ok[same_it]), (tag, np.argwhere(~ok[same_it])[:5])
+    if (~same_it).any():
+        assert np.all(close(got.solutions[~same_it], want["x"][~same_it], rtol=1e-3, atol=1e-6))
+    np.testing.assert_array_equal(got.flags & ~A.SPX_FLAG_FP64_ESCALATED, want["flags"],
+                                  err_msg=tag)
+    esc = np.count_nonzero(got.flags & A.SPX_FLAG_FP64_ESCALATED)
+    assert esc <= 0.01 * len(got.flags), (tag, "escalated to FP64 on", esc)
+
+
+@pytest.mark.parametrize("N,B", [(16, 96), (24, 16), (32, 8)])
+def test_grid_fp32_ir_matches_oracle(oracle, N, B):
+    # FP32 L+U values in LDS (grid32: 23,668 values, 95 KB), refined in FP64
+    # against the FP64 assembled Jacobian
+    c = CK.grid_diodes(N)
+    got = E.sweep_dcop(c, B, seed=42, config=_ir())
+    want = oracle.sweep_dcop(c, got.params, A.default_nr_config(limiter=A.SPX_LIMIT_PNJLIM),
+                             nthreads=16)
+    _parity_ir(got, want, f"grid{N} FP32+IR")
+    assert np.all(got.accepted_mask())
+    assert np.all(got.refinements > 0)
+    assert np.all(got.factorizations <= got.iterations)
+
+
+def test_grid32_fp32_ir_full_batch_matches_fp64():
+    c = CK.grid_diodes(32)
+    B = 4096
+    f64 = E.sweep_dcop(c, B, seed=42, config=_pnj())
+    ir = E.sweep_dcop(c, B, seed=42, config=_ir())
+    assert np.all(f64.accepted_mask()) and np.all(ir.accepted_mask())
+    want = {"status": f64.status, "iterations": f64.iterations, "flags": f64.flags,
+            "x": f64.solutions}
+    _parity_ir(ir, want, "grid32 FP32+IR vs FP64, 4096")
+    assert np.all(ir.residual <= 1.0)
+
+
+def test_grid_fp32_ir_handover_reruns_in_fp64():
+    # one refinement step cannot meet the backward-error test, so every
+    # instance is handed to the FP64 kernel: its results are the FP64 run's,
+    # bit for bit, flagged FP64_ESCALATED
     c = CK.grid_diodes(16)
-    with pytest.raises(E.InvalidArgument):
-        E.sweep_dcop(c, 4, config=E.nr_config(precision=E.Precision.F32_IR))
+    B = 64
+    f64 = E.sweep_dcop(c, B, seed=3, config=_pnj())
+    ir = E.sweep_dcop(c, B, seed=3, config=_ir(ir_max_steps=1))
+    np.testing.assert_array_equal(ir.status, f64.status)
+    np.testing.assert_array_equal(ir.iterations, f64.iterations)
+    np.testing.assert_array_equal(ir.solutions, f64.solutions)
+    np.testing.assert_array_equal(ir.flags, f64.flags | A.SPX_FLAG_FP64_ESCALATED)
+
+
+def test_linear_grid_fp32_ir_single_solve(oracle):
+    # no diodes: one refined solve per instance
+    c = CK.grid_diodes(20, every=10**9)
+    B = 64
+    got = E.sweep_dcop(c, B, seed=7, config=E.nr_config(precision=E.Precision.F32_IR))
+    want = oracle.sweep_dcop(c, got.params, A.default_nr_config(), nthreads=16)
+    _parity_ir(got, want, "linear grid FP32+IR")
+    assert np.all(got.iterations == 1) and np.all(got.accepted_mask())
 
 
 # ------------------------------------------------- batched sparse linear solve
