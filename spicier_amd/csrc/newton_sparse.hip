@@ -255,8 +255,30 @@ __device__ __forceinline__ bool sparse_factor(const DevSparse& sp, int n, V* lu,
       if (t < b1) {
         const int ti = sp.tb_t[t];
         V v = lu[ti];
+        int c = sp.tb_c[t];
         const int c1 = sp.tb_c[t + 1];
-        for (int c = sp.tb_c[t]; c < c1; ++c) v = fma(-lu[sp.cb_l[c]], lu[sp.cb_u[c]], v);
+        // DC kernels: eight contributions' index loads, then their value
+        // loads, then the eight fmas in column order (the transient kernels'
+        // register budget has no room for it)
+        constexpr int C = 8;
+        if constexpr (kTail)
+        for (; c + C <= c1; c += C) {
+          int cl[C], cu[C];
+          V a[C], b[C];
+#pragma unroll
+          for (int u = 0; u < C; ++u) {
+            cl[u] = sp.cb_l[c + u];
+            cu[u] = sp.cb_u[c + u];
+          }
+#pragma unroll
+          for (int u = 0; u < C; ++u) {
+            a[u] = lu[cl[u]];
+            b[u] = lu[cu[u]];
+          }
+#pragma unroll
+          for (int u = 0; u < C; ++u) v = fma(-a[u], b[u], v);
+        }
+        for (; c < c1; ++c) v = fma(-lu[sp.cb_l[c]], lu[sp.cb_u[c]], v);
         lu[ti] = v;
       }
     } else
@@ -276,17 +298,46 @@ __device__ __forceinline__ bool sparse_factor(const DevSparse& sp, int n, V* lu,
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (ti[u] >= 0) v[u] = lu[ti[u]];
-      for (int q = 0; q < cmax; ++q) {
-        int cl[U], cu[U];
+      // DC kernels: two contributions of each target per round, their index
+      // loads, then their value loads, then the fmas in column order
+      if constexpr (!kTail) {
+        for (int q = 0; q < cmax; ++q) {
+          int cl[U], cu[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const bool in = c0[u] + q < c1[u];
-          cl[u] = in ? sp.cb_l[c0[u] + q] : -1;
-          cu[u] = in ? sp.cb_u[c0[u] + q] : 0;
+          for (int u = 0; u < U; ++u) {
+            const bool in = c0[u] + q < c1[u];
+            cl[u] = in ? sp.cb_l[c0[u] + q] : -1;
+            cu[u] = in ? sp.cb_u[c0[u] + q] : 0;
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (cl[u] >= 0) v[u] = fma(-lu[cl[u]], lu[cu[u]], v[u]);
         }
+      } else
+      for (int q = 0; q < cmax; q += 2) {
+        int cl[2][U], cu[2][U];
+        V a[2][U], b[2][U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (cl[u] >= 0) v[u] = fma(-lu[cl[u]], lu[cu[u]], v[u]);
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const bool in = c0[u] + q + h < c1[u];
+            cl[h][u] = in ? sp.cb_l[c0[u] + q + h] : -1;
+            cu[h][u] = in ? sp.cb_u[c0[u] + q + h] : 0;
+          }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (cl[h][u] >= 0) {
+              a[h][u] = lu[cl[h][u]];
+              b[h][u] = lu[cu[h][u]];
+            }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (cl[h][u] >= 0) v[u] = fma(-a[h][u], b[h][u], v[u]);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -305,6 +356,11 @@ __device__ __forceinline__ bool sparse_factor(const DevSparse& sp, int n, V* lu,
   return block_all_t<T>(ok, tid, red);
 }
 
+// Four ints of a plan descriptor table (16-byte aligned rows).
+__device__ __forceinline__ int4 desc4(const int* p, int r) {
+  return *reinterpret_cast<const int4*>(p + 4 * r);
+}
+
 // Both triangular solves with the factors of sparse_factor (the same kTail):
 // y (LDS, n, elimination order) holds the right-hand side on entry and the
 // solution on return.  False if the solution is not finite.  The solves are
@@ -313,49 +369,65 @@ template <int T, bool kTail, typename V>
 __device__ __forceinline__ bool sparse_solve(const DevSparse& sp, int n, const V* lu, V* y,
                                              int tid, int* red) {
   const int D = kTail ? sp.tail : 0;
+  // the DC kernels (kTail) take a row's index, L offsets and length from
+  // its descriptor in one load; the transient kernels from the row tables
+  auto fwd_row = [&](int r) -> int4 {
+    if constexpr (kTail) {
+      return desc4(sp.fdesc, r);
+    } else {
+      const int i = sp.lev_rows[r];
+      const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
+      return make_int4(i, u0, nl, sp.diag[i] - nl);
+    }
+  };
   auto forward_level = [&](int L, int t0, int stride) {
     const int r0 = sp.lev_ptr[L], r1 = sp.lev_ptr[L + 1];
     if (stride > 1 && wave_rows<T, kTail>(r0, r1)) {
       const int w = tid / kWave, lane = tid % kWave;
       if (r0 + w < r1) {
-        const int i = sp.lev_rows[r0 + w];
-        const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
-        const V* li = lu + sp.diag[i] - nl;
+        const int4 d = fwd_row(r0 + w);
+        const V* li = lu + d.w;
         V part = V(0);
-        for (int t = lane; t < nl; t += kWave) part = fma(li[t], y[sp.ut_row[u0 + t]], part);
+        for (int t = lane; t < d.z; t += kWave) part = fma(li[t], y[sp.ut_row[d.y + t]], part);
         part = wave_sum(part);
-        if (lane == 0) y[i] = y[i] - part;
+        if (lane == 0) y[d.x] = y[d.x] - part;
       }
       return;
     }
     for (int r = r0 + t0; r < r1; r += stride) {
-      const int i = sp.lev_rows[r];
-      const int u0 = sp.ut_ptr[i], nl = sp.ut_ptr[i + 1] - u0;
-      y[i] = row_dot(lu + sp.diag[i] - nl, sp.ut_row + u0, nl, y, y[i]);
+      const int4 d = fwd_row(r);
+      y[d.x] = row_dot(lu + d.w, sp.ut_row + d.y, d.z, y, y[d.x]);
     }
   };
   const int* bptr = D > 0 ? sp.tblev_ptr : sp.blev_ptr;
   const int* brows = D > 0 ? sp.tblev_rows : sp.blev_rows;
+  auto bwd_row = [&](int r) -> int4 {
+    if constexpr (kTail) {
+      return desc4(sp.bkdesc, r);
+    } else {
+      const int i = brows[r];
+      const int s0 = sp.col_ptr[i];
+      return make_int4(i, s0, sp.col_ptr[i + 1] - s0, sp.diag[i]);
+    }
+  };
   auto backward_level = [&](int L, int t0, int stride) {
     const int r0 = bptr[L], r1 = bptr[L + 1];
     if (stride > 1 && wave_rows<T, kTail>(r0, r1)) {
       const int w = tid / kWave, lane = tid % kWave;
       if (r0 + w < r1) {
-        const int i = brows[r0 + w];
-        const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
-        const V* ui = lu + sp.diag[i];
+        const int4 d = bwd_row(r0 + w);
+        const V* ui = lu + d.w;
         V part = V(0);
-        for (int t = lane; t < ns; t += kWave) part = fma(ui[1 + t], y[sp.s_row[s0 + t]], part);
+        for (int t = lane; t < d.z; t += kWave) part = fma(ui[1 + t], y[sp.s_row[d.y + t]], part);
         part = wave_sum(part);
-        if (lane == 0) y[i] = (y[i] - part) * (V(1) / ui[0]);
+        if (lane == 0) y[d.x] = (y[d.x] - part) * (V(1) / ui[0]);
       }
       return;
     }
     for (int r = r0 + t0; r < r1; r += stride) {
-      const int i = brows[r];
-      const int s0 = sp.col_ptr[i], ns = sp.col_ptr[i + 1] - s0;
-      const V* ui = lu + sp.diag[i];
-      y[i] = row_dot(ui + 1, sp.s_row + s0, ns, y, y[i]) * (V(1) / ui[0]);
+      const int4 d = bwd_row(r);
+      const V* ui = lu + d.w;
+      y[d.x] = row_dot(ui + 1, sp.s_row + d.y, d.z, y, y[d.x]) * (V(1) / ui[0]);
     }
   };
   run_levels(sp.levels - D, sp.lev_flags, 2, tid, [&](int l) { forward_level(l, 0, 1); },
@@ -750,6 +822,48 @@ inline size_t sparse_smem_bytes(const DevProgram& g, const DevSparse& sp, bool l
   return b;
 }
 
+// Ints of the DC kernels' staged level tables (stage_level_tables), with
+// the alignment pad.
+inline size_t sparse_stage_ints(const DevSparse& sp) {
+  auto r4 = [](size_t c) { return (c + 3) & ~size_t(3); };
+  const size_t bl = sp.tail > 0 ? sp.tblevels : sp.blevels;
+  return 4 + 4 * (size_t)sp.n_fdesc + 4 * (size_t)sp.n_bkdesc + 3 * r4(sp.levels + 1) +
+         r4(sp.levels) + r4(bl + 1) + r4(bl);
+}
+
+// Copies the tables every level of the DC kernels' factorisation and solves
+// reads -- the solve descriptors, the level pointers and flags -- to LDS at
+// `at` and points `to` at the copies (followed by a barrier).
+template <int T>
+__device__ __forceinline__ void stage_level_tables(const DevSparse& sp, DevSparse& to,
+                                                   const void* at, int tid) {
+  int* q = reinterpret_cast<int*>((reinterpret_cast<uintptr_t>(at) + 15) & ~uintptr_t(15));
+  auto put = [&](const int* src, int cnt) {
+    int* dst = q;
+    for (int i = tid; i < cnt; i += T) dst[i] = src[i];
+    q += (cnt + 3) & ~3;
+    return static_cast<const int*>(dst);
+  };
+  const bool tl = sp.tail > 0;
+  const int bl = tl ? sp.tblevels : sp.blevels;
+  to.fdesc = put(sp.fdesc, 4 * sp.n_fdesc);
+  to.bkdesc = put(sp.bkdesc, 4 * sp.n_bkdesc);
+  to.lev_ptr = put(sp.lev_ptr, sp.levels + 1);
+  to.pa_ptr = put(sp.pa_ptr, sp.levels + 1);
+  to.tb_ptr = put(sp.tb_ptr, sp.levels + 1);
+  to.lev_flags = put(sp.lev_flags, sp.levels);
+  const int* bp = put(tl ? sp.tblev_ptr : sp.blev_ptr, bl + 1);
+  const int* bf = put(tl ? sp.tblev_flags : sp.blev_flags, bl);
+  if (tl) {
+    to.tblev_ptr = bp;
+    to.tblev_flags = bf;
+  } else {
+    to.blev_ptr = bp;
+    to.blev_flags = bf;
+  }
+  __syncthreads();
+}
+
 // Three waves per SIMD: the dense tail's register row (128 VGPRs) would
 // otherwise set the allocation for the whole kernel.
 // IR: the refinement form (FP32 factors, FP64 refinement); it leaves an
@@ -759,7 +873,7 @@ template <int T, bool IR>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_newton_sparse(
     DevProgram g, DevSparse sp, DevNrCfg cfg, const double* __restrict__ params,
     const double* __restrict__ x0, int64_t B, DevOut out, double* ws, int64_t slot,
-    int lu_in_lds, int escalated_only) {
+    int lu_in_lds, int escalated_only, int stage) {
   using LV = typename std::conditional<IR, float, double>::type;
   extern __shared__ double smem[];
   const int n = g.n, tid = threadIdx.x;
@@ -785,6 +899,9 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_n
   double* av = vst + g.num_elems;
   double* xw = av + g.num_pos;
   LV* lu = lu_in_lds ? lds_lu : reinterpret_cast<LV*>(xw + n);
+  // stage: the solve descriptors and per-level tables follow in LDS
+  DevSparse spl = sp;
+  if (stage) stage_level_tables<T>(sp, spl, lds_lu + (lu_in_lds ? sp.nnz_lu : 0), tid);
   for (int64_t inst = blockIdx.x; inst < B; inst += gridDim.x) {
     if (escalated_only && out.status[inst] != SPX_ACTIVE) continue;
     if (tid == 0) {
@@ -793,7 +910,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_n
     }
     for (int i = tid; i < n; i += T) xs[i] = x0 ? x0[inst * n + i] : 0.0;
     __syncthreads();
-    SparseInstance<T, true, IR> I{g,  sp, cfg, tid, params + inst * g.P, vals, vst, av, lu,
+    SparseInstance<T, true, IR> I{g,  spl, cfg, tid, params + inst * g.P, vals, vst, av, lu,
                                   rhs, xs, xw, y, red, redd, x3};
     run_dcop(I, g, cfg, x0 != nullptr, out, inst, escalated_only);
     __syncthreads();
@@ -1036,9 +1153,20 @@ hipError_t sparse_launch_shape(const DevProgram& g, const DevSparse& sp, int64_t
   *lu_in_lds = sparse_smem_bytes(g, sp, true, T, ir) <= lds_budget ? 1 : 0;
   *smem = sparse_smem_bytes(g, sp, *lu_in_lds, T, ir);
   *slot = sparse_slot_doubles(g, sp, *lu_in_lds, ir ? 4 : 8);
-  hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)*smem);
+  hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (e != hipSuccess) return e;
-  return grid_for(kern, *smem, B, grid, T);
+  // the level tables follow in LDS when they fit and cost no resident
+  // workgroup
+  const size_t stage_bytes = sparse_stage_ints(sp) * sizeof(int);
+  if (*smem + stage_bytes <= 160 * 1024) {
+    int without = 0, with = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&without, kern, T, *smem);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&with, kern, T, *smem + stage_bytes);
+    if (e != hipSuccess) return e;
+    if (with >= without) *smem = (*smem + stage_bytes) | kPlanInLdsBit;
+  }
+  return grid_for(kern, *smem & ~kPlanInLdsBit, B, grid, T);
 }
 
 // ir = 1 runs the refinement form; escalated_only = 1 the FP64 pass over the
@@ -1049,14 +1177,16 @@ hipError_t launch_newton_sparse(const DevProgram& g, const DevSparse& sp, const 
                                 size_t smem, int lu_in_lds, int ir, int escalated_only,
                                 hipStream_t st) {
   if (B == 0) return hipSuccess;
+  const int stage = (smem & kPlanInLdsBit) ? 1 : 0;
+  smem &= ~kPlanInLdsBit;
   if (ir)
     hipLaunchKernelGGL((k_newton_sparse<kSparseIrThreads, true>), dim3((unsigned)grid),
                        dim3(kSparseIrThreads), smem, st, g, sp, cfg, params, x0, B, out, ws,
-                       slot, lu_in_lds, 0);
+                       slot, lu_in_lds, 0, stage);
   else
     hipLaunchKernelGGL((k_newton_sparse<kSparseThreads, false>), dim3((unsigned)grid),
                        dim3(kSparseThreads), smem, st, g, sp, cfg, params, x0, B, out, ws, slot,
-                       lu_in_lds, escalated_only);
+                       lu_in_lds, escalated_only, stage);
   return hipGetLastError();
 }
 

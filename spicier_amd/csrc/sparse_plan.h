@@ -91,8 +91,18 @@ struct DevSparse {
   const int* tblev_ptr;    // tblevels + 1 into tblev_rows
   const int* tblev_rows;
   const int* tblev_flags;  // bit 0: the level's rows are narrow
+  // Row descriptors of the DC kernels' solves, four ints per position, after
+  // the staged prefix of the blob: fdesc[r] = {i, ut_ptr[i], L terms of row
+  // i, LU index of its first L term} for i = lev_rows[r]; bkdesc[r] = {i,
+  // col_ptr[i], U terms right of the diagonal, diag[i]} for the r-th row of
+  // the backward levels the DC kernels run (tblev_rows with a dense tail,
+  // else blev_rows).  A level's rows are then one load each.
+  const int* fdesc;
+  const int* bkdesc;
+  int n_fdesc;   // positions of the forward levels before the dense tail
+  int n_bkdesc;  // positions of the DC kernels' backward levels
   const int* blob;     // the plan's one int block: every pointer above points into it
-  int blob_ints;
+  int blob_ints;       // the prefix the transient kernels stage in LDS (excludes the descriptors)
 };
 
 // The plan with every table moved from `blob` to `to` (a copy of the block,
@@ -127,6 +137,7 @@ __host__ __device__ inline DevSparse rebase_sparse(const DevSparse& s, const int
   d.tblev_ptr = mv(s.tblev_ptr);
   d.tblev_rows = mv(s.tblev_rows);
   d.tblev_flags = mv(s.tblev_flags);
+  // the descriptors lie past the staged prefix: they stay where they are
   d.blob = to;
   return d;
 }
@@ -144,7 +155,8 @@ struct HostSparsePlan {
          o_levptr = 0, o_levrows = 0, o_paptr = 0, o_pal = 0, o_pad = 0, o_tbptr = 0,
          o_tbt = 0, o_tbc = 0, o_cbl = 0, o_cbu = 0, o_blevptr = 0, o_blevrows = 0,
          o_poslu = 0, o_fill = 0, o_levflags = 0, o_blevflags = 0, o_tdidx = 0, o_tlcnt = 0,
-         o_tblevptr = 0, o_tblevrows = 0, o_tblevflags = 0;
+         o_tblevptr = 0, o_tblevrows = 0, o_tblevflags = 0, o_fdesc = 0, o_bkdesc = 0;
+  int n_fdesc = 0, n_bkdesc = 0;
 };
 
 // n unknowns, the first num_nodes of them node voltages; pattern entries

@@ -472,6 +472,21 @@ const char* build_sparse_plan(int n, int num_nodes, const std::vector<int>& pos_
   plan.o_tblevptr = put(tblev_ptr);
   plan.o_tblevrows = put(tblev_rows);
   plan.o_tblevflags = put(tblev_flags);
+  // the DC kernels' solve descriptors (past the prefix the transient kernels
+  // stage in LDS)
+  {
+    std::vector<int> fd, bd;
+    plan.n_fdesc = lev_ptr[levels - tail];
+    for (int r = 0; r < (int)lev_rows.size(); ++r) {
+      const int i = lev_rows[r], nl = ut_ptr[i + 1] - ut_ptr[i];
+      fd.insert(fd.end(), {i, ut_ptr[i], nl, diag[i] - nl});
+    }
+    const std::vector<int>& br = tail > 0 ? tblev_rows : blev_rows;
+    plan.n_bkdesc = (int)br.size();
+    for (int i : br) bd.insert(bd.end(), {i, col_ptr[i], col_ptr[i + 1] - col_ptr[i], diag[i]});
+    plan.o_fdesc = put(fd);
+    plan.o_bkdesc = put(bd);
+  }
   plan.blob.push_back(0);  // keeps every offset a valid address
   return "";
 }
@@ -511,8 +526,12 @@ DevSparse device_sparse(const HostSparsePlan& p, const int* base) {
   d.tblev_ptr = base + p.o_tblevptr;
   d.tblev_rows = base + p.o_tblevrows;
   d.tblev_flags = base + p.o_tblevflags;
+  d.fdesc = base + p.o_fdesc;
+  d.bkdesc = base + p.o_bkdesc;
+  d.n_fdesc = p.n_fdesc;
+  d.n_bkdesc = p.n_bkdesc;
   d.blob = base;
-  d.blob_ints = (int)p.blob.size();
+  d.blob_ints = (int)p.o_fdesc;
   return d;
 }
 
