@@ -163,6 +163,20 @@ def test_grid32_fp32_ir_full_batch_matches_fp64():
     assert np.all(ir.residual <= 1.0)
 
 
+def test_grid40_fp32_ir_global_factors_match_fp64():
+    # n = 1,604: the FP32 L+U values (41,436, 166 KB) exceed a workgroup's LDS
+    # and live in the global slot; against the FP64 sweep (oracle-checked at
+    # the smaller grids)
+    c = CK.grid_diodes(40)
+    B = 64
+    f64 = E.sweep_dcop(c, B, seed=9, config=_pnj())
+    ir = E.sweep_dcop(c, B, seed=9, config=_ir())
+    assert np.all(f64.accepted_mask()) and np.all(ir.accepted_mask())
+    want = {"status": f64.status, "iterations": f64.iterations, "flags": f64.flags,
+            "x": f64.solutions}
+    _parity_ir(ir, want, "grid40 FP32+IR vs FP64")
+
+
 def test_grid_fp32_ir_handover_reruns_in_fp64():
     # one refinement step cannot meet the backward-error test, so every
     # instance is handed to the FP64 kernel: its results are the FP64 run's,
@@ -175,6 +189,35 @@ def test_grid_fp32_ir_handover_reruns_in_fp64():
     np.testing.assert_array_equal(ir.iterations, f64.iterations)
     np.testing.assert_array_equal(ir.solutions, f64.solutions)
     np.testing.assert_array_equal(ir.flags, f64.flags | A.SPX_FLAG_FP64_ESCALATED)
+
+
+@pytest.mark.parametrize("prec", [E.Precision.F64, E.Precision.F32_IR])
+def test_grid_stepping_fallbacks_match_oracle(oracle, prec):
+    # a 4-iteration budget makes plain NR fail on the grid, so Gmin (then
+    # source) stepping runs through the sparse kernels' saved-solution slot
+    c = CK.grid_diodes(16)
+    B = 16
+    for kw in ({"gmin_stepping": True, "source_stepping": True},
+               {"gmin_stepping": False, "source_stepping": True}):
+        cfg = E.nr_config(E.ConvergenceCriteria(max_iterations=4), limiter=E.Limiter.Pnjlim,
+                          precision=prec, **kw)
+        got = E.sweep_dcop(c, B, seed=11, config=cfg)
+        ocfg = A.default_nr_config(max_iterations=4, limiter=A.SPX_LIMIT_PNJLIM,
+                                   gmin_stepping=int(kw["gmin_stepping"]))
+        want = oracle.sweep_dcop(c, got.params, ocfg, nthreads=16)
+        _parity_ir(got, want, f"grid16 stepping {prec.name} {kw}")
+        assert np.all(got.flags & (A.SPX_FLAG_GMIN_STEPPING | A.SPX_FLAG_SOURCE_STEPPING))
+    # every level fails: each failed level counts all its attempts
+    # (newton.rs:300-315, 411-426), 1 + 5 + 5 x 5 = 31 iterations
+    cfg = E.nr_config(E.ConvergenceCriteria(max_iterations=1), limiter=E.Limiter.Pnjlim,
+                      precision=prec)
+    got = E.sweep_dcop(c, B, seed=11, config=cfg)
+    want = oracle.sweep_dcop(c, got.params,
+                             A.default_nr_config(max_iterations=1, limiter=A.SPX_LIMIT_PNJLIM),
+                             nthreads=16)
+    np.testing.assert_array_equal(got.status, want["status"])
+    np.testing.assert_array_equal(got.iterations, want["iterations"])
+    np.testing.assert_array_equal(got.iterations, 31)
 
 
 def test_linear_grid_fp32_ir_single_solve(oracle):
