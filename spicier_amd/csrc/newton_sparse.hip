@@ -864,13 +864,17 @@ __device__ __forceinline__ void stage_level_tables(const DevSparse& sp, DevSpars
   __syncthreads();
 }
 
-// Three waves per SIMD: the dense tail's register row (128 VGPRs) would
-// otherwise set the allocation for the whole kernel.
+// Four waves per SIMD (128 VGPRs): the FP64 kernel then spills the dense
+// tail's register row in part (464 B per lane), and four resident
+// workgroups per CU instead of three still carry more instances through the
+// level chains' round trips (a 32x32 grid with diodes: 122.4 -> 112.6 ms per
+// 4,096-instance sweep; five waves spill 624 B and take 155 ms).  The
+// 1,024-thread refinement form runs four waves per SIMD in any case.
 // IR: the refinement form (FP32 factors, FP64 refinement); it leaves an
 // instance FP32 cannot resolve at SPX_ACTIVE.  escalated_only: run only the
 // instances left at SPX_ACTIVE (the FP64 pass after the refinement form).
 template <int T, bool IR>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(3))) void k_newton_sparse(
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4))) void k_newton_sparse(
     DevProgram g, DevSparse sp, DevNrCfg cfg, const double* __restrict__ params,
     const double* __restrict__ x0, int64_t B, DevOut out, double* ws, int64_t slot,
     int lu_in_lds, int escalated_only, int stage) {
