@@ -623,7 +623,8 @@ struct SparseInstance {
   // The factors are not carried to the next Newton iteration, unlike the
   // register-LU kernels': on a 32x32 grid with diodes, factors one iteration
   // old need twice the refinement steps, which cost more than the sparse
-  // refactorisation they save (286 vs 143 ms per 4,096-instance sweep).
+  // refactorisation they save (215 vs 143 ms per 4,096-instance sweep at
+  // 1,024 threads).
   // 0 solved, 2 hand over to FP64.
   __device__ __forceinline__ int solve32() {
     if (!factor32()) return 2;
