@@ -177,6 +177,31 @@ def test_grid40_fp32_ir_global_factors_match_fp64():
     _parity_ir(ir, want, "grid40 FP32+IR vs FP64")
 
 
+@pytest.mark.parametrize("prec", [E.Precision.F64, E.Precision.F32_IR])
+def test_grid_singular_jacobian_takes_gmin_stepping(oracle, prec):
+    # a node reached only through a capacitor (open at DC) leaves the plain
+    # Newton Jacobian singular: a zero pivot, then Gmin stepping converges.
+    # The refinement form hands every instance to the FP64 kernel at its
+    # zero FP32 pivot, so its results are the FP64 path's.
+    c = CK.grid_diodes(16)
+    c.node("fl")
+    c.capacitor("fl", "n3_3", c.const(1e-12, "Cf"))
+    assert c.size == 261
+    B = 16
+    got = E.sweep_dcop(c, B, seed=5, config=E.nr_config(limiter=E.Limiter.Pnjlim,
+                                                         precision=prec))
+    want = oracle.sweep_dcop(c, got.params, A.default_nr_config(limiter=A.SPX_LIMIT_PNJLIM),
+                             nthreads=16)
+    np.testing.assert_array_equal(got.status, want["status"])
+    np.testing.assert_array_equal(got.iterations, want["iterations"])
+    np.testing.assert_array_equal(got.flags & ~A.SPX_FLAG_FP64_ESCALATED, want["flags"])
+    assert np.all(got.flags & A.SPX_FLAG_GMIN_STEPPING)
+    ok = close(got.solutions, want["x"])
+    assert np.all(ok), np.abs(got.solutions - want["x"]).max()
+    if prec == E.Precision.F32_IR:
+        assert np.all(got.flags & A.SPX_FLAG_FP64_ESCALATED)
+
+
 def test_grid_fp32_ir_handover_reruns_in_fp64():
     # one refinement step cannot meet the backward-error test, so every
     # instance is handed to the FP64 kernel: its results are the FP64 run's,
