@@ -27,7 +27,8 @@ constexpr int kSparseThreads = 256;
 // LDS, so one workgroup per CU, and 16 waves give a level's entries four
 // times the load chains 256 threads would (a 32x32 grid with diodes:
 // 286 / 237 / 215 ms per 4,096-instance sweep at 256 / 512 / 1,024
-// threads, measured with the factors reused across iterations).
+// threads, measured with the factors reused across iterations; 160 / 143 ms
+// at 512 / 1,024 threads with fresh factors).
 constexpr int kSparseIrThreads = 1024;
 // Set in a shape's LDS byte count when the plan's tables are staged in LDS
 // (LDS sizes stay far below this bit).
