@@ -203,6 +203,24 @@ __device__ __forceinline__ bool wave_rows(int r0, int r1) {
   return kTail && T > kWave && r1 - r0 <= T / kWave;
 }
 
+// Lanes per row of a DC kernel's solve level: a wave per row when the
+// workgroup has a wave for each, else 32 or 16 lanes per row while that
+// covers the level in one round; 0 = a thread per row.
+template <int T, bool kTail>
+__device__ __forceinline__ int lanes_per_row(int r0, int r1) {
+  if (!(kTail && T > kWave)) return 0;
+  const int rows = r1 - r0;
+  return rows <= T / 64 ? 64 : rows <= T / 32 ? 32 : rows <= T / 16 ? 16 : 0;
+}
+
+// Sum over aligned groups of g lanes (g = 16, 32 or 64), every lane of a
+// group active; for g = 64 the same tree as wave_sum.
+template <typename V>
+__device__ __forceinline__ V group_sum(V v, int g) {
+  for (int off = g >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
 // Factors lu (values of type V: double, or float for the FP32 factors of
 // the refinement form) in place with the plan sp.  False if a pivot is
 // exactly zero.  Level by level: multipliers of the level's columns, then
@@ -383,14 +401,15 @@ __device__ __forceinline__ bool sparse_solve(const DevSparse& sp, int n, const V
   };
   auto forward_level = [&](int L, int t0, int stride) {
     const int r0 = sp.lev_ptr[L], r1 = sp.lev_ptr[L + 1];
-    if (stride > 1 && wave_rows<T, kTail>(r0, r1)) {
-      const int w = tid / kWave, lane = tid % kWave;
+    const int G = stride > 1 ? lanes_per_row<T, kTail>(r0, r1) : 0;
+    if (G) {
+      const int w = tid / G, lane = tid % G;
       if (r0 + w < r1) {
         const int4 d = fwd_row(r0 + w);
         const V* li = lu + d.w;
         V part = V(0);
-        for (int t = lane; t < d.z; t += kWave) part = fma(li[t], y[sp.ut_row[d.y + t]], part);
-        part = wave_sum(part);
+        for (int t = lane; t < d.z; t += G) part = fma(li[t], y[sp.ut_row[d.y + t]], part);
+        part = group_sum(part, G);
         if (lane == 0) y[d.x] = y[d.x] - part;
       }
       return;
@@ -413,14 +432,15 @@ __device__ __forceinline__ bool sparse_solve(const DevSparse& sp, int n, const V
   };
   auto backward_level = [&](int L, int t0, int stride) {
     const int r0 = bptr[L], r1 = bptr[L + 1];
-    if (stride > 1 && wave_rows<T, kTail>(r0, r1)) {
-      const int w = tid / kWave, lane = tid % kWave;
+    const int G = stride > 1 ? lanes_per_row<T, kTail>(r0, r1) : 0;
+    if (G) {
+      const int w = tid / G, lane = tid % G;
       if (r0 + w < r1) {
         const int4 d = bwd_row(r0 + w);
         const V* ui = lu + d.w;
         V part = V(0);
-        for (int t = lane; t < d.z; t += kWave) part = fma(ui[1 + t], y[sp.s_row[d.y + t]], part);
-        part = wave_sum(part);
+        for (int t = lane; t < d.z; t += G) part = fma(ui[1 + t], y[sp.s_row[d.y + t]], part);
+        part = group_sum(part, G);
         if (lane == 0) y[d.x] = (y[d.x] - part) * (V(1) / ui[0]);
       }
       return;
